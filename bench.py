@@ -1,0 +1,238 @@
+"""Benchmark: FP64 P2-tetrahedral anisotropic-diffusion assembly on one MI355X.
+
+Workload (one "step" = one full assembly of the CSR values):
+  64^3 Kuhn-split unit cube (1,572,864 tets, 2,146,689 P2 DOFs), interior
+  vertices jittered uniformly by +-0.2h (seed 0), per-element SPD tensor
+  A = Q diag(lambda) Q^T with Q a Haar rotation and lambda log-uniform in
+  [1e-2, 1e2] (seed 1), 4-point quadrature, form dot(mul(A, grad u), grad v).
+Inputs (vertex coordinates, connectivity, A, gather plan) are resident in
+HBM before timing; the timed region is K back-to-back assemblies.
+
+Prints one JSON line.  `value` = assembled elements/s of the FP64 path.
+`roofline` uses the algorithmic bytes of one assembly (825.7 B per element:
+vertices, int32 connectivity, 72 B coefficient, int32 element->CSR map,
+FP64 CSR values) over the kernel's mean duration measured with HIP events on
+the launch stream.  `cpu_baseline` times the reference's own compiled C++
+CrossBilinearAssembler (oracle/_ref) on a bounded 16^3 sample of the same
+workload with all host cores.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "scikit-fem-native_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "assembled elements/s + % HBM roofline; learned-FSAI vs Jacobi PCG time to 1e-8"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_ELEMENT_C2 = 825.7  # algorithmic bytes per tet of one FP64 assembly
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def spd_tensors(E, seed):
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((E, 3, 3))
+    q, r = np.linalg.qr(g)
+    q = q * np.sign(np.diagonal(r, axis1=1, axis2=2))[:, None, :]
+    det = np.linalg.det(q)
+    q[:, :, 0] *= det[:, None]  # proper rotation
+    lam = 10. ** rng.uniform(-2, 2, size=(E, 3))
+    return np.einsum("eij,ej,ekj->eik", q, lam, q)
+
+
+def jittered_p2_cube(n, amount, seed):
+    import skfemntv as sk
+    ax = np.linspace(0., 1., n + 1)
+    mesh = sk.MeshTet.init_tensor(ax, ax, ax)
+    p = mesh.p.copy()
+    interior = np.all((p > 1e-12) & (p < 1 - 1e-12), axis=0)
+    rng = np.random.default_rng(seed)
+    p[:, interior] += rng.uniform(-amount / n, amount / n, size=(3, int(interior.sum())))
+    return sk.MeshTet2.from_mesh(sk.MeshTet(p, mesh.t))
+
+
+def cpu_baseline(n_sample=16, repeats=7, warmup=2):
+    """Reference C++ (compiled from its sources into oracle/_ref) on a
+    bounded sample; falls back to the C restatement when _ref is absent."""
+    import oracle
+    from oracle import host, pipeline
+    ax = np.linspace(0., 1., n_sample + 1)
+    p, t = host.kuhn_mesh(ax, ax, ax)
+    p = host.jitter_interior(p, 1. / n_sample, 0.2, 0)
+    p2, t2 = host.p2_mesh(p, t)
+    X, W = host.tet_rule_4()
+    E = t.shape[1]
+    A = spd_tensors(E, 1)
+    A_eq = np.ascontiguousarray(np.broadcast_to(A[:, None], (E, 4, 3, 3)))
+    cores = len(os.sched_getaffinity(0))
+    ref = oracle.ref_module()
+    (shape, grads, dx, x, det, tol, cond), _ = pipeline.basis_geometry(p2, t2, X, W, True)
+    N, dofs = pipeline.dofs_of(t2, p2.shape[1], 10, 1)
+    out = {}
+    if ref is not None:
+        coef = np.ascontiguousarray(A_eq[:, :, None, :, None, :])
+        asm_ = ref.CrossBilinearAssembler(dofs, dofs, shape, shape, dx, grads, grads)
+        for threads in (cores, 1):
+            ref.set_num_threads(threads)
+            for _ in range(warmup):
+                asm_.assemble(coef, "gradient", "gradient", 0)
+            ts = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                asm_.assemble(coef, "gradient", "gradient", 0)
+                ts.append(time.perf_counter() - t0)
+            out[threads] = statistics.median(ts)
+        ref.set_num_threads(1)
+        kind = "reference"
+    else:
+        ip, ix, sc = oracle.pattern(dofs.reshape(E, -1), nrows=N)
+        colour, _ = oracle.colouring(dofs.reshape(E, -1), N)
+        for threads in (cores, 1):
+            for _ in range(warmup):
+                oracle.assemble_tensor(grads, dx, A_eq, sc, len(ix), threads=threads, colour=colour)
+            ts = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                oracle.assemble_tensor(grads, dx, A_eq, sc, len(ix), threads=threads, colour=colour)
+                ts.append(time.perf_counter() - t0)
+            out[threads] = statistics.median(ts)
+        kind = "port"
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": E / out[cores],
+        "unit": "elements/s",
+        "cores": cores,
+        "kind": kind,
+        "sample": (f"P2 anisotropic {n_sample}^3 Kuhn cube ({E} tets), 4-pt rule, warm median of {repeats} "
+                   f"assemble() calls; CrossBilinearAssembler; 1-thread {E / out[1]:.4g} elements/s; {cpu_model}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64, help="cells per cube edge")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build's assembly path runs on a single MI355X; "
+              "multi-GPU sharding is a design note only", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import skfemntv as sk
+    from skfemntv import _native
+    from skfemntv._assembly import NativeCrossBilinearForm, upload
+
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    mesh = jittered_p2_cube(args.n, 0.2, 0)
+    E = mesh.nelements
+    log(f"mesh {E} tets, {mesh.p.shape[1]} nodes ({time.perf_counter() - t0:.1f}s)")
+    X = np.array([[.5854101966249685, .1381966011250105, .1381966011250105, .1381966011250105],
+                  [.1381966011250105, .5854101966249685, .1381966011250105, .1381966011250105],
+                  [.1381966011250105, .1381966011250105, .5854101966249685, .1381966011250105]])
+    W = np.full(4, 1. / 24.)
+    t1 = time.perf_counter()
+    basis = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP2(), dim=1), quadrature=(X, W))
+    log(f"basis + geometry check {time.perf_counter() - t1:.1f}s")
+    t1 = time.perf_counter()
+    form = NativeCrossBilinearForm(basis, basis)
+    torch.cuda.synchronize()
+    setup_plan = time.perf_counter() - t1
+    log(f"plan (pattern + gather map on GPU) {setup_plan:.2f}s nnz={form.plan.nnz}")
+    A = upload(spd_tensors(E, 1))
+    L = _native.lib()
+    stream = _native.stream()
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    _native.check(L.skfa_event_create(ctypes.byref(ev0)))
+    _native.check(L.skfa_event_create(ctypes.byref(ev1)))
+
+    def step(precision="fp64"):
+        form.assemble_tensor_device(A, precision=precision)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    _native.check(L.skfa_event_record(ev0, stream))
+    for _ in range(args.steps):
+        step()
+    _native.check(L.skfa_event_record(ev1, stream))
+    ms = ctypes.c_float()
+    _native.check(L.skfa_event_elapsed_ms(ev0, ev1, ctypes.byref(ms)))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    kernel_s = ms.value / 1e3 / args.steps
+    # FP32-products form on the same inputs
+    for _ in range(args.warmup):
+        step("fp32")
+    _native.check(L.skfa_event_record(ev0, stream))
+    for _ in range(args.steps):
+        step("fp32")
+    _native.check(L.skfa_event_record(ev1, stream))
+    _native.check(L.skfa_event_elapsed_ms(ev0, ev1, ctypes.byref(ms)))
+    kernel32_s = ms.value / 1e3 / args.steps
+    v64 = form.assemble_tensor_device(A, precision="fp64").clone()
+    v32 = form.assemble_tensor_device(A, precision="fp32").clone()
+    err32 = float(torch.linalg.norm(v32 - v64) / torch.linalg.norm(v64))
+    torch.cuda.synchronize()
+
+    algorithmic = BYTES_PER_ELEMENT_C2 * E
+    achieved_gbs = algorithmic / kernel_s / 1e9
+    result = {
+        "metric": METRIC,
+        "value": E / kernel_s,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": kernel_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"P2 tet anisotropic diffusion, {args.n}^3 Kuhn cube jittered 0.2h, "
+                               f"{E} tets, 4-pt quadrature, FP64",
+                   "elements": E, "nnz": form.plan.nnz, "dofs": basis.N, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None},
+        "fp32_products": {"ms_per_step": kernel32_s * 1e3, "speedup_vs_fp64": kernel_s / kernel32_s,
+                          "rel_frobenius_vs_fp64": err32},
+        "nnz_per_s": form.plan.nnz / kernel_s,
+        "wall_ms_per_step": wall / args.steps * 1e3,
+        "plan_setup_s": setup_plan,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

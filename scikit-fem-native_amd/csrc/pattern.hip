@@ -1,0 +1,607 @@
+// Device construction of the CSR pattern and of the element->CSR gather
+// plan for continuous nodal spaces, plus first-seen edge numbering.
+//
+// Replaces, with identical integer outputs:
+//  * the pattern + scatter map of BilinearFormAssembler / CrossBilinearAssembler
+//    (cpp/src/assembly/bilinear_form_assembler.cpp:111-120,
+//     cross_bilinear_assembler.cpp:307-320) and build_edge_csr_pattern
+//    (cpp/src/assembly/edge_topology.cpp:86-154): per row, the sorted unique
+//    set of column DOFs of all elements touching that row;
+//  * build_oriented_edge_topology (cpp/src/assembly/edge_topology.cpp:30-84)
+//    and the edge-node numbering of MeshTet2.from_mesh
+//    (python/skfemntv/basis.py:644-658): edges numbered in order of first
+//    occurrence over (element, local edge).
+//
+// The reference's greedy element colouring has no counterpart: assembly is
+// a row-owner gather (see assemble.hip), so no colouring is needed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "skfa_internal.h"
+#include "skfa_plan.h"
+#include "../../include/skfem_amd.h"
+
+namespace skfa {
+
+// ------------------------------------------------------------------ scan
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+template <class TIn>
+__global__ void __launch_bounds__(kScanThreads) k_scan_tile(const TIn* __restrict__ in, int64_t* __restrict__ out,
+                                                            int64_t n, int64_t* __restrict__ tile_sums) {
+    __shared__ int64_t sh[kScanThreads];
+    const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
+    int64_t v[kScanItems];
+    int64_t local = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k;
+        v[k] = i < n ? int64_t(in[i]) : 0;
+        local += v[k];
+    }
+    sh[threadIdx.x] = local;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over thread totals.
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        int64_t add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int64_t run = sh[threadIdx.x] - local;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == kScanThreads - 1) tile_sums[blockIdx.x] = sh[threadIdx.x];
+}
+
+__global__ void k_scan_add(int64_t* __restrict__ out, int64_t n, const int64_t* __restrict__ tile_offsets) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += tile_offsets[i / kScanTile];
+}
+
+__global__ void k_write_total(int64_t* __restrict__ out, int64_t n, const int64_t* __restrict__ last_in_is_i64,
+                              const int32_t* __restrict__ last_in_is_i32) {
+    // out[n] = out[n-1] + in[n-1]
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int64_t last = last_in_is_i64 ? last_in_is_i64[n - 1] : int64_t(last_in_is_i32[n - 1]);
+        out[n] = out[n - 1] + last;
+    }
+}
+
+template <class TIn>
+static int scan_impl(const TIn* in, int64_t* out, int64_t n, hipStream_t s) {
+    if (n == 0) {
+        SKFA_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), s));
+        return kOk;
+    }
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    DevBuf<int64_t> sums, offs;
+    if (int rc = sums.alloc(tiles)) return rc;
+    if (int rc = offs.alloc(tiles + 1)) return rc;
+    k_scan_tile<TIn><<<dim3(unsigned(tiles)), dim3(kScanThreads), 0, s>>>(in, out, n, sums.p);
+    SKFA_LAUNCH_CHECK();
+    if (tiles > 1) {
+        if (int rc = scan_impl<int64_t>(sums.p, offs.p, tiles, s)) return rc;
+        k_scan_add<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, s>>>(out, n, offs.p);
+        SKFA_LAUNCH_CHECK();
+    }
+    if (std::is_same<TIn, int64_t>::value)
+        k_write_total<<<1, 1, 0, s>>>(out, n, reinterpret_cast<const int64_t*>(in), nullptr);
+    else
+        k_write_total<<<1, 1, 0, s>>>(out, n, nullptr, reinterpret_cast<const int32_t*>(in));
+    SKFA_LAUNCH_CHECK();
+    // the temporaries are freed on return: wait for the kernels that use them
+    SKFA_HIP(hipStreamSynchronize(s));
+    return kOk;
+}
+
+int exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t s) { return scan_impl(in, out, n, s); }
+int exclusive_scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, hipStream_t s) {
+    return scan_impl(in, out, n, s);
+}
+
+// ------------------------------------------------------------------ incidence
+__global__ void k_inc_count(const int32_t* __restrict__ conn, int64_t total, int64_t nnodes,
+                            int32_t* __restrict__ count, int32_t* __restrict__ status) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x) {
+        const int32_t v = conn[i];
+        if (v < 0 || v >= nnodes) {
+            atomicOr(status, 1);
+            continue;
+        }
+        atomicAdd(&count[v], 1);
+    }
+}
+
+__global__ void k_inc_fill(const int32_t* __restrict__ conn, int64_t E, int n, int64_t nnodes,
+                           const int64_t* __restrict__ inc_ptr, int32_t* __restrict__ cursor,
+                           int32_t* __restrict__ inc) {
+    const int64_t total = E * n;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x) {
+        const int32_t v = conn[i];
+        if (v < 0 || v >= nnodes) continue;
+        const int64_t e = i / n;
+        const int a = int(i - e * n);
+        const int32_t slot = atomicAdd(&cursor[v], 1);
+        inc[inc_ptr[v] + slot] = int32_t((e << 4) | a);
+    }
+}
+
+// Sort each node's incidence segment (atomic fill order is arbitrary; the
+// packed key (e<<4)|a orders by element then local node).
+__global__ void k_inc_sort(const int64_t* __restrict__ inc_ptr, int32_t* __restrict__ inc, int64_t nnodes) {
+    for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nnodes; r += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t b = inc_ptr[r], e = inc_ptr[r + 1];
+        for (int64_t i = b + 1; i < e; ++i) {
+            const int32_t key = inc[i];
+            int64_t j = i - 1;
+            while (j >= b && inc[j] > key) {
+                inc[j + 1] = inc[j];
+                --j;
+            }
+            inc[j + 1] = key;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ rows
+// One 64-lane wave per node row.  Candidates = columns of every element
+// touching the row; a candidate is "first" if no earlier candidate has the
+// same column; its rank is the number of first candidates with a smaller
+// column, which is both its CSR position within the row (sorted unique
+// order, as std::sort+std::unique gives) and the element->CSR offset.
+__global__ void __launch_bounds__(64) k_row_analyze(const int32_t* __restrict__ conn, int n,
+                                                    const int64_t* __restrict__ inc_ptr,
+                                                    const int32_t* __restrict__ inc, int64_t nnodes, int phase,
+                                                    int32_t* __restrict__ row_len, const int64_t* __restrict__ indptr,
+                                                    int32_t* __restrict__ indices,
+                                                    const int64_t* __restrict__ wave_ptr, int R,
+                                                    uint8_t* __restrict__ records, int rec_bytes, int off_bytes,
+                                                    int32_t* __restrict__ status) {
+    __shared__ int32_t cand[kMaxCandidates];
+    __shared__ uint8_t first[kMaxCandidates];
+    const int lane = threadIdx.x;
+    for (int64_t r = blockIdx.x; r < nnodes; r += gridDim.x) {
+        const int64_t k0 = inc_ptr[r], k1 = inc_ptr[r + 1];
+        const int64_t m64 = (k1 - k0) * n;
+        if (m64 > kMaxCandidates) {
+            if (lane == 0) atomicOr(status, 2);
+            continue;
+        }
+        const int m = int(m64);
+        for (int i = lane; i < m; i += 64) {
+            const int k = i / n, b = i - (i / n) * n;
+            const int32_t e = inc[k0 + k] >> 4;
+            cand[i] = conn[int64_t(e) * n + b];
+        }
+        __syncthreads();
+        int count = 0;
+        for (int i = lane; i < m; i += 64) {
+            const int32_t c = cand[i];
+            bool f = true;
+            for (int j = 0; j < i; ++j)
+                if (cand[j] == c) { f = false; break; }
+            first[i] = f;
+            count += f;
+        }
+        __syncthreads();
+        if (phase == 0) {
+            // wave reduction of the first-candidate count
+            for (int off = 32; off > 0; off >>= 1) count += __shfl_down(count, off, 64);
+            if (lane == 0) row_len[r] = count;
+        } else {
+            const int64_t base = indptr[r];
+            const int64_t w = r / R;
+            const int64_t ri = r - w * R;
+            for (int i = lane; i < m; i += 64) {
+                const int32_t c = cand[i];
+                int rank = 0;
+                for (int j = 0; j < m; ++j) rank += (first[j] && cand[j] < c);
+                if (first[i]) indices[base + rank] = c;
+                const int k = i / n, b = i - (i / n) * n;
+                uint8_t* rec = records + ((wave_ptr[w] + k) * R + ri) * rec_bytes;
+                if (off_bytes == 1)
+                    rec[4 + b] = uint8_t(rank);
+                else
+                    reinterpret_cast<uint16_t*>(rec + 4)[b] = uint16_t(rank);
+                if (b == 0) *reinterpret_cast<int32_t*>(rec) = inc[k0 + k];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_wave_steps(const int64_t* __restrict__ inc_ptr, int64_t nnodes, int R, int64_t nwaves,
+                             int64_t* __restrict__ steps, const int64_t* __restrict__ indptr,
+                             int64_t* __restrict__ span) {
+    for (int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; w < nwaves; w += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t r0 = w * R, r1 = min(r0 + R, nnodes);
+        int64_t mx = 0;
+        for (int64_t r = r0; r < r1; ++r) mx = max(mx, inc_ptr[r + 1] - inc_ptr[r]);
+        steps[w] = mx;
+        if (span) span[w] = indptr[r1] - indptr[r0];
+    }
+}
+
+__global__ void k_max_diff_i64(const int64_t* __restrict__ ptr, int64_t n, int32_t* __restrict__ out) {
+    int32_t mx = 0;
+    for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < n; r += int64_t(gridDim.x) * blockDim.x)
+        mx = max(mx, int32_t(ptr[r + 1] - ptr[r]));
+    atomicMax(out, mx);
+}
+
+static unsigned grid_for(int64_t items, int threads, int64_t cap = 65536) {
+    int64_t g = (items + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return unsigned(g);
+}
+
+static int plan_build(skfa_plan* P, const int32_t* conn, hipStream_t s) {
+    const int64_t E = P->E, nn = P->nnodes;
+    const int n = P->n;
+    DevBuf<int32_t> cnt, status;
+    if (int rc = cnt.alloc(nn + 1)) return rc;
+    if (int rc = status.alloc(2)) return rc;
+    SKFA_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (nn + 1), s));
+    SKFA_HIP(hipMemsetAsync(status.p, 0, sizeof(int32_t) * 2, s));
+    k_inc_count<<<grid_for(E * n, 256), 256, 0, s>>>(conn, E * n, nn, cnt.p, status.p);
+    SKFA_LAUNCH_CHECK();
+    if (int rc = P->inc_ptr.alloc(nn + 1)) return rc;
+    if (int rc = exclusive_scan_i32_to_i64(cnt.p, P->inc_ptr.p, nn, s)) return rc;
+    int32_t st = 0;
+    SKFA_HIP(hipMemcpyAsync(&st, status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    if (st & 1) {
+        set_error("element node index is out of bounds");
+        return kInvalid;
+    }
+    if (int rc = P->inc.alloc(size_t(E * n))) return rc;
+    SKFA_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (nn + 1), s));
+    k_inc_fill<<<grid_for(E * n, 256), 256, 0, s>>>(conn, E, n, nn, P->inc_ptr.p, cnt.p, P->inc.p);
+    SKFA_LAUNCH_CHECK();
+    k_inc_sort<<<grid_for(nn, 256), 256, 0, s>>>(P->inc_ptr.p, P->inc.p, nn);
+    SKFA_LAUNCH_CHECK();
+
+    // row lengths -> indptr
+    DevBuf<int32_t> rowlen;
+    if (int rc = rowlen.alloc(nn + 1)) return rc;
+    const unsigned rows_grid = unsigned(std::min<int64_t>(std::max<int64_t>(nn, 1), 1 << 20));
+    k_row_analyze<<<rows_grid, 64, 0, s>>>(conn, n, P->inc_ptr.p, P->inc.p, nn, 0, rowlen.p, nullptr, nullptr,
+                                           nullptr, 1, nullptr, 0, 1, status.p);
+    SKFA_LAUNCH_CHECK();
+    SKFA_HIP(hipMemcpyAsync(&st, status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    if (st & 2) {
+        set_error("a node row touches more than 4096 element columns");
+        return kCapacity;
+    }
+    if (int rc = P->indptr.alloc(nn + 1)) return rc;
+    if (int rc = exclusive_scan_i32_to_i64(rowlen.p, P->indptr.p, nn, s)) return rc;
+    SKFA_HIP(hipMemcpyAsync(&P->nnz_node, P->indptr.p + nn, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    // longest row / largest incidence (for the offset width and for reporting)
+    SKFA_HIP(hipMemsetAsync(status.p, 0, sizeof(int32_t) * 2, s));
+    k_max_diff_i64<<<grid_for(nn, 256, 1024), 256, 0, s>>>(P->indptr.p, nn, status.p);
+    k_max_diff_i64<<<grid_for(nn, 256, 1024), 256, 0, s>>>(P->inc_ptr.p, nn, status.p + 1);
+    SKFA_LAUNCH_CHECK();
+    int32_t mx[2];
+    SKFA_HIP(hipMemcpyAsync(mx, status.p, sizeof(mx), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    P->max_row_len = mx[0];
+    P->max_inc = mx[1];
+    if (P->nnz_node > INT32_MAX) {
+        set_error("node-level pattern exceeds 2^31 nonzeros");
+        return kCapacity;
+    }
+    if (int rc = P->indices.alloc(size_t(P->nnz_node))) return rc;
+
+    // wave layout
+    const int R = P->rows_per_wave;
+    P->nwaves = (nn + R - 1) / R;
+    DevBuf<int64_t> steps, span;
+    if (int rc = steps.alloc(P->nwaves + 1)) return rc;
+    if (int rc = span.alloc(P->nwaves + 1)) return rc;
+    if (P->nwaves > 0) {
+        k_wave_steps<<<grid_for(P->nwaves, 256), 256, 0, s>>>(P->inc_ptr.p, nn, R, P->nwaves, steps.p, P->indptr.p,
+                                                                span.p);
+        SKFA_LAUNCH_CHECK();
+    }
+    if (int rc = P->wave_ptr.alloc(P->nwaves + 1)) return rc;
+    if (int rc = exclusive_scan_i64(steps.p, P->wave_ptr.p, P->nwaves, s)) return rc;
+    SKFA_HIP(hipMemcpyAsync(&P->total_steps, P->wave_ptr.p + P->nwaves, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    P->off_bytes = P->max_row_len <= 256 ? 1 : 2;
+    if (P->max_row_len > 65536) {
+        set_error("a node row has more than 65536 columns");
+        return kCapacity;
+    }
+    const int raw = 4 + n * P->off_bytes;
+    P->rec_bytes = raw <= 8 ? 8 : raw <= 16 ? 16 : raw <= 32 ? 32 : 64;
+    const size_t rec_total = size_t(P->total_steps) * R * P->rec_bytes;
+    if (int rc = P->records.alloc(rec_total ? rec_total : 1)) return rc;
+    SKFA_HIP(hipMemsetAsync(P->records.p, 0xFF, rec_total ? rec_total : 1, s));
+    k_row_analyze<<<rows_grid, 64, 0, s>>>(conn, n, P->inc_ptr.p, P->inc.p, nn, 1, nullptr, P->indptr.p,
+                                           P->indices.p, P->wave_ptr.p, R, P->records.p, P->rec_bytes,
+                                           P->off_bytes, status.p);
+    SKFA_LAUNCH_CHECK();
+
+    // bucket waves by LDS footprint (bytes = span * C*C * 8)
+    std::vector<int64_t> hspan(size_t(P->nwaves));
+    if (P->nwaves)
+        SKFA_HIP(hipMemcpyAsync(hspan.data(), span.p, sizeof(int64_t) * P->nwaves, hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    static const int64_t kBounds[] = {8 * 1024, 16 * 1024, 24 * 1024, 32 * 1024, 40 * 1024, 48 * 1024, 64 * 1024};
+    const int nb = int(sizeof(kBounds) / sizeof(kBounds[0]));
+    std::vector<std::vector<int32_t>> lists(nb);
+    const int64_t cc = int64_t(P->C) * P->C;
+    for (int64_t w = 0; w < P->nwaves; ++w) {
+        const int64_t bytes = hspan[size_t(w)] * cc * 8;
+        int b = 0;
+        while (b < nb && bytes > kBounds[b]) ++b;
+        if (b == nb) {
+            set_error("the CSR span of one wave exceeds 64 KiB of LDS");
+            return kCapacity;
+        }
+        lists[b].push_back(int32_t(w));
+    }
+    std::vector<int32_t> all;
+    P->nbuckets = 0;
+    for (int b = 0; b < nb; ++b) {
+        if (lists[b].empty()) continue;
+        P->bucket_lds[P->nbuckets] = kBounds[b];
+        P->bucket_start[P->nbuckets] = int64_t(all.size());
+        all.insert(all.end(), lists[b].begin(), lists[b].end());
+        ++P->nbuckets;
+    }
+    P->bucket_start[P->nbuckets] = int64_t(all.size());
+    if (int rc = P->bucket_waves.alloc(all.size() ? all.size() : 1)) return rc;
+    if (!all.empty())
+        SKFA_HIP(hipMemcpyAsync(P->bucket_waves.p, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice,
+                                s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    return kOk;
+}
+
+// ------------------------------------------------------------------ CSR export
+__global__ void k_export_csr(const int64_t* __restrict__ nptr, const int32_t* __restrict__ nidx, int64_t nnodes, int C,
+                             int64_t* __restrict__ indptr, int32_t* __restrict__ indices) {
+    // row 3n+c has columns 3m+d for each m in node row n, m-major d-minor
+    for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nnodes * C;
+         r += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t node = r / C;
+        const int c = int(r - node * C);
+        const int64_t len = nptr[node + 1] - nptr[node];
+        const int64_t start = int64_t(C) * C * nptr[node] + int64_t(c) * C * len;
+        indptr[r] = start;
+        for (int64_t j = 0; j < len; ++j) {
+            const int32_t m = nidx[nptr[node] + j];
+            for (int d = 0; d < C; ++d) indices[start + j * C + d] = m * C + d;
+        }
+        if (r == nnodes * C - 1) indptr[r + 1] = start + len * C;
+    }
+}
+
+// ------------------------------------------------------------------ edges
+// Open-addressing table of (lo,hi) keys; slot keeps the smallest item id
+// (item = element*6 + local edge) that carries the key.
+constexpr unsigned long long kEmpty = ~0ull;
+
+__device__ inline unsigned long long edge_hash(unsigned long long key) {
+    key ^= key >> 33;
+    key *= 0xff51afd7ed558ccdull;
+    key ^= key >> 33;
+    key *= 0xc4ceb9fe1a85ec53ull;
+    key ^= key >> 33;
+    return key;
+}
+
+__constant__ int kTetEdge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+
+__global__ void k_edge_insert(const int32_t* __restrict__ vconn, int64_t E, int64_t nvert,
+                              unsigned long long* __restrict__ keys, unsigned long long* __restrict__ minitem,
+                              int64_t cap, int64_t* __restrict__ slot_of, unsigned long long* __restrict__ bad_cell,
+                              int32_t* __restrict__ range_error) {
+    const int64_t total = E * 6;
+    for (int64_t it = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; it < total; it += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t e = it / 6;
+        const int l = int(it - e * 6);
+        const int32_t* c = vconn + e * 4;
+        if (l == 0) {
+            bool bad = false;
+            for (int i = 0; i < 4; ++i) {
+                if (c[i] < 0 || c[i] >= nvert) atomicOr(range_error, 1);
+                for (int j = i + 1; j < 4; ++j)
+                    if (c[i] == c[j]) bad = true;
+            }
+            if (bad) atomicMin(bad_cell, (unsigned long long)e);
+        }
+        const int64_t f = c[kTetEdge[l][0]], s = c[kTetEdge[l][1]];
+        if (f < 0 || f >= nvert || s < 0 || s >= nvert || f == s) {
+            slot_of[it] = -1;
+            continue;
+        }
+        const unsigned long long lo = f < s ? f : s, hi = f < s ? s : f;
+        const unsigned long long key = lo * (unsigned long long)nvert + hi;
+        unsigned long long slot = edge_hash(key) & (unsigned long long)(cap - 1);
+        while (true) {
+            const unsigned long long prev = atomicCAS(&keys[slot], kEmpty, key);
+            if (prev == kEmpty || prev == key) break;
+            slot = (slot + 1) & (unsigned long long)(cap - 1);
+        }
+        atomicMin(&minitem[slot], (unsigned long long)it);
+        slot_of[it] = int64_t(slot);
+    }
+}
+
+__global__ void k_edge_first(int64_t total, const int64_t* __restrict__ slot_of,
+                             const unsigned long long* __restrict__ minitem, int32_t* __restrict__ flag) {
+    for (int64_t it = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; it < total; it += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t sl = slot_of[it];
+        flag[it] = (sl >= 0 && minitem[sl] == (unsigned long long)it) ? 1 : 0;
+    }
+}
+
+__global__ void k_edge_assign(const int32_t* __restrict__ vconn, int64_t E, const int64_t* __restrict__ slot_of,
+                              const unsigned long long* __restrict__ minitem, const int64_t* __restrict__ rank,
+                              int32_t* __restrict__ elem_edges, int8_t* __restrict__ signs,
+                              int64_t* __restrict__ edge_lo, int64_t* __restrict__ edge_hi) {
+    const int64_t total = E * 6;
+    for (int64_t it = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; it < total; it += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t sl = slot_of[it];
+        if (sl < 0) continue;
+        const int64_t e = it / 6;
+        const int l = int(it - e * 6);
+        const int32_t* c = vconn + e * 4;
+        const int32_t f = c[kTetEdge[l][0]], s = c[kTetEdge[l][1]];
+        const int64_t first = int64_t(minitem[sl]);
+        const int64_t id = rank[first];
+        elem_edges[it] = int32_t(id);
+        signs[it] = f < s ? 1 : -1;
+        if (first == it && edge_lo) {
+            edge_lo[id] = f < s ? f : s;
+            edge_hi[id] = f < s ? s : f;
+        }
+    }
+}
+
+}  // namespace skfa
+
+using namespace skfa;
+
+extern "C" {
+
+int skfa_plan_create(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
+                     void* stream, skfa_plan** out) {
+    SKFA_REQUIRE(out != nullptr, "plan output pointer is null");
+    *out = nullptr;
+    SKFA_REQUIRE(E >= 0 && nnodes >= 0, "negative sizes");
+    SKFA_REQUIRE(n >= 1 && n <= kMaxLocal, "nodes per element must be in [1, 16]");
+    SKFA_REQUIRE(E < kMaxElements, "element count exceeds 2^27");
+    SKFA_REQUIRE(components == 1 || components == 2 || components == 3, "components must be 1, 2 or 3");
+    SKFA_REQUIRE(E == 0 || node_conn != nullptr, "connectivity pointer is null");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    skfa_plan* P = new skfa_plan();
+    P->E = E;
+    P->n = n;
+    P->nnodes = nnodes;
+    P->C = components;
+    P->lanes_per_row = components * components;
+    P->rows_per_wave = 64 / P->lanes_per_row;
+    int rc = plan_build(P, node_conn, s);
+    if (rc != kOk) {
+        delete P;
+        return rc;
+    }
+    *out = P;
+    return kOk;
+}
+
+int skfa_plan_destroy(skfa_plan* plan) {
+    delete plan;
+    return kOk;
+}
+
+int skfa_plan_info(const skfa_plan* P, int64_t* info) {
+    SKFA_REQUIRE(P != nullptr && info != nullptr, "null argument");
+    info[0] = P->nrows();
+    info[1] = P->nnz();
+    info[2] = P->nnz_node;
+    info[3] = P->max_row_len;
+    info[4] = P->max_inc;
+    info[5] = P->nwaves;
+    info[6] = P->total_steps;
+    info[7] = P->rec_bytes;
+    info[8] = P->nbuckets;
+    return kOk;
+}
+
+int skfa_plan_export_csr(const skfa_plan* P, int64_t* indptr, int32_t* indices, void* stream) {
+    SKFA_REQUIRE(P != nullptr && indptr != nullptr, "null argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (P->nnodes == 0) {
+        SKFA_HIP(hipMemsetAsync(indptr, 0, sizeof(int64_t), s));
+        return kOk;
+    }
+    k_export_csr<<<grid_for(P->nnodes * P->C, 256), 256, 0, s>>>(P->indptr.p, P->indices.p, P->nnodes, P->C, indptr,
+                                                                  indices);
+    SKFA_LAUNCH_CHECK();
+    return kOk;
+}
+
+int skfa_plan_export_map(const skfa_plan* P, int64_t* inc_ptr, int32_t* inc, void* stream) {
+    SKFA_REQUIRE(P != nullptr && inc_ptr != nullptr && inc != nullptr, "null argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    SKFA_HIP(hipMemcpyAsync(inc_ptr, P->inc_ptr.p, sizeof(int64_t) * (P->nnodes + 1), hipMemcpyDeviceToDevice, s));
+    if (P->E * P->n)
+        SKFA_HIP(hipMemcpyAsync(inc, P->inc.p, sizeof(int32_t) * P->E * P->n, hipMemcpyDeviceToDevice, s));
+    return kOk;
+}
+
+int skfa_edge_topology(const int32_t* vconn, int64_t E, int64_t nvertices, int32_t* elem_edges, int8_t* signs,
+                       int64_t* edge_lo, int64_t* edge_hi, int64_t* nedges, int64_t* bad_cell, void* stream) {
+    SKFA_REQUIRE(nedges != nullptr && bad_cell != nullptr, "null argument");
+    SKFA_REQUIRE(E >= 0 && nvertices >= 0, "negative sizes");
+    SKFA_REQUIRE(E * 6 < (int64_t(1) << 31), "element count too large for edge numbering");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    *nedges = 0;
+    *bad_cell = -1;
+    if (E == 0) return kOk;
+    const int64_t total = E * 6;
+    int64_t cap = 1;
+    while (cap < 2 * total) cap <<= 1;
+    DevBuf<unsigned long long> keys, minitem;
+    DevBuf<int64_t> slot_of, rank;
+    DevBuf<int32_t> flag;
+    DevBuf<unsigned long long> status;
+    DevBuf<int32_t> range;
+    if (int rc = range.alloc(1)) return rc;
+    if (int rc = keys.alloc(cap)) return rc;
+    if (int rc = minitem.alloc(cap)) return rc;
+    if (int rc = slot_of.alloc(total)) return rc;
+    if (int rc = rank.alloc(total + 1)) return rc;
+    if (int rc = flag.alloc(total)) return rc;
+    if (int rc = status.alloc(1)) return rc;
+    SKFA_HIP(hipMemsetAsync(keys.p, 0xFF, sizeof(unsigned long long) * cap, s));
+    SKFA_HIP(hipMemsetAsync(minitem.p, 0xFF, sizeof(unsigned long long) * cap, s));
+    SKFA_HIP(hipMemsetAsync(status.p, 0xFF, sizeof(unsigned long long), s));
+    SKFA_HIP(hipMemsetAsync(range.p, 0, sizeof(int32_t), s));
+    k_edge_insert<<<grid_for(total, 256), 256, 0, s>>>(vconn, E, nvertices, keys.p, minitem.p, cap, slot_of.p,
+                                                        status.p, range.p);
+    SKFA_LAUNCH_CHECK();
+    unsigned long long bad = 0;
+    int32_t range_bad = 0;
+    SKFA_HIP(hipMemcpyAsync(&bad, status.p, sizeof(bad), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipMemcpyAsync(&range_bad, range.p, sizeof(range_bad), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    if (range_bad) {
+        set_error("connectivity index is out of bounds");
+        return kInvalid;
+    }
+    if (bad != ~0ull) {
+        *bad_cell = int64_t(bad);
+        set_error("cell " + std::to_string(bad) + " repeats a corner vertex");
+        return kInvalid;
+    }
+    k_edge_first<<<grid_for(total, 256), 256, 0, s>>>(total, slot_of.p, minitem.p, flag.p);
+    SKFA_LAUNCH_CHECK();
+    if (int rc = exclusive_scan_i32_to_i64(flag.p, rank.p, total, s)) return rc;
+    SKFA_HIP(hipMemcpyAsync(nedges, rank.p + total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    k_edge_assign<<<grid_for(total, 256), 256, 0, s>>>(vconn, E, slot_of.p, minitem.p, rank.p, elem_edges, signs,
+                                                        edge_lo, edge_hi);
+    SKFA_LAUNCH_CHECK();
+    SKFA_HIP(hipStreamSynchronize(s));
+    return kOk;
+}
+
+}  // extern "C"

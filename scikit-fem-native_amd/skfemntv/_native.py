@@ -1,0 +1,122 @@
+"""ctypes binding of libskfem_amd.so (the MI355X C-ABI, include/skfem_amd.h).
+
+The library is mandatory: every assembly goes through it and there is no
+CPU fallback.  When the library or a HIP device is missing, the first call
+that needs it raises ``NativeUnavailable``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libskfem_amd.so")
+
+# Exported symbols (kept in sync with include/skfem_amd.h by a test).
+SYMBOLS = (
+    "skfa_last_error", "skfa_abi_version", "skfa_device_name",
+    "skfa_plan_create", "skfa_plan_destroy", "skfa_plan_info", "skfa_plan_export_csr", "skfa_plan_export_map",
+    "skfa_assemble_bilinear", "skfa_assemble_linear", "skfa_tabulate_geometry",
+    "skfa_edge_topology", "skfa_assemble_tet_n1",
+    "skfa_event_create", "skfa_event_destroy", "skfa_event_record", "skfa_event_elapsed_ms",
+    "skfa_stream_synchronize", "skfa_spmv",
+)
+
+
+class NativeUnavailable(RuntimeError):
+    """The MI355X library or a HIP device is not available."""
+
+
+class NativeError(RuntimeError):
+    """A C-ABI call returned an error status."""
+
+
+_lib = None
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+
+
+def _declare(lib):
+    sig = {
+        "skfa_last_error": (ctypes.c_char_p, []),
+        "skfa_abi_version": (ctypes.c_int, []),
+        "skfa_device_name": (ctypes.c_int, [ctypes.c_char_p, I32]),
+        "skfa_plan_create": (ctypes.c_int, [P, I64, I32, I64, I32, P, ctypes.POINTER(P)]),
+        "skfa_plan_destroy": (ctypes.c_int, [P]),
+        "skfa_plan_info": (ctypes.c_int, [P, ctypes.POINTER(I64)]),
+        "skfa_plan_export_csr": (ctypes.c_int, [P, P, P, P]),
+        "skfa_plan_export_map": (ctypes.c_int, [P, P, P, P]),
+        "skfa_assemble_bilinear": (ctypes.c_int, [P, P, I64, P, P, I32, P, P, I32, P, P, P, P, P, I32, I32, P, P]),
+        "skfa_assemble_linear": (ctypes.c_int, [P, P, P, P, I32, P, I32, P, I32, P, I32, P, P]),
+        "skfa_tabulate_geometry": (ctypes.c_int, [P, P, P, I32, I64, P, I32, I32, P, P, P, P, P, P, P, P]),
+        "skfa_edge_topology": (ctypes.c_int, [P, I64, I64, P, P, P, P, ctypes.POINTER(I64), ctypes.POINTER(I64), P]),
+        "skfa_assemble_tet_n1": (ctypes.c_int, [P, P, P, P, I32, P, P, I32, I32, I32, P, P]),
+        "skfa_event_create": (ctypes.c_int, [ctypes.POINTER(P)]),
+        "skfa_event_destroy": (ctypes.c_int, [P]),
+        "skfa_event_record": (ctypes.c_int, [P, P]),
+        "skfa_event_elapsed_ms": (ctypes.c_int, [P, P, ctypes.POINTER(ctypes.c_float)]),
+        "skfa_stream_synchronize": (ctypes.c_int, [P]),
+        "skfa_spmv": (ctypes.c_int, [P, P, P, P, P, I64, I32, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def load_library():
+    """Load libskfem_amd.so (no device needed)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeUnavailable(
+                f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+                "or `make -C scikit-fem-native_amd/csrc`")
+        lib = ctypes.CDLL(LIB_PATH)
+        _declare(lib)
+        _lib = lib
+    return _lib
+
+
+_device_checked = False
+
+
+def lib():
+    """The library, after checking that a HIP device is usable."""
+    global _device_checked
+    L = load_library()
+    if not _device_checked:
+        import torch
+        if not torch.cuda.is_available():
+            raise NativeUnavailable("skfemntv (MI355X build) requires a HIP device; none is visible")
+        _device_checked = True
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = load_library().skfa_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(msg)
+        raise NativeError(f"libskfem_amd status {rc}: {msg}")
+    return rc
+
+
+def ptr(t):
+    """Raw device pointer of a torch tensor (or None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def device():
+    import torch
+    return torch.device("cuda", torch.cuda.current_device())

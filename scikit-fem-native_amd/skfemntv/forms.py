@@ -1,0 +1,279 @@
+"""Typed forms and ``asm`` (python/skfemntv/forms.py:71-112,161-404,667-1091).
+
+``asm(form, basis, **coefficients)`` traces the user function with symbolic
+trial/test nodes into the reference's term kinds, evaluates coefficients on
+the host exactly as the reference does, and assembles on the GPU.  Forms
+the MI355X build does not assemble raise ``UnsupportedNativeForm``; nothing
+falls back to a CPU path.
+"""
+
+from __future__ import annotations
+
+from typing import Callable
+from weakref import WeakKeyDictionary
+
+import numpy as np
+
+from ._assembly import NativeBilinearForm, NativeCrossBilinearForm, NativeLinearForm
+from ._symbols import (BilinearSum, BilinearTerm, Coefficient, CoefficientComponent, Divergence, LinearSum,
+                       LinearTerm, QuadratureValue, SymmetricGradient, TensorGradient, TestGradient, TestValue,
+                       TrialGradient, TrialValue, UnsupportedNativeForm, evaluate_coefficient, extract_terms,
+                       is_symbolic, trace_expression)
+
+
+class _LinearForm:
+    def __init__(self, function: Callable):
+        self.function = function
+        self._native_cache = WeakKeyDictionary()
+
+    def assemble(self, *bases, **kwargs):
+        return asm(self, *bases, **kwargs)
+
+
+class _BilinearForm:
+    def __init__(self, function: Callable):
+        self.function = function
+        self._native_cache = WeakKeyDictionary()
+        self._cross_cache = WeakKeyDictionary()
+
+    def assemble(self, *bases, **kwargs):
+        return asm(self, *bases, **kwargs)
+
+
+def LinearForm(function=None, **kwargs):
+    if kwargs:
+        return lambda fn: _LinearForm(fn)
+    return _LinearForm(function) if function is not None else _LinearForm
+
+
+def BilinearForm(function=None, **kwargs):
+    if kwargs:
+        return lambda fn: _BilinearForm(fn)
+    return _BilinearForm(function) if function is not None else _BilinearForm
+
+
+def _geometry(basis):
+    # Physical quadrature coordinates, component first; computed on the GPU
+    # only when the form reads w.x.
+    return {"x": QuadratureValue(lambda: np.moveaxis(basis.global_coordinates, -1, 0))}
+
+
+def _compile_linear(form, basis, kwargs):
+    expression = trace_expression(form.function, (TestValue(),), _geometry(basis), kwargs, context="LinearForm")
+    return extract_terms(expression, LinearTerm, LinearSum,
+                         message=("native LinearForm must reduce to dot(coefficient, v), "
+                                  "ddot(coefficient, grad(v)), or a sum of those terms"))
+
+
+def _native_linear_assemble(form, basis, kwargs, num_threads=0):
+    """forms.py:161-203."""
+    terms = _compile_linear(form, basis, kwargs)
+    native = form._native_cache.get(basis)
+    if native is None:
+        native = NativeLinearForm(basis)
+        form._native_cache[basis] = native
+    value = None
+    gradient = None
+    for term in terms:
+        coefficient = evaluate_coefficient(term.coefficient, kwargs, factor=term.factor)
+        if term.kind == "value":
+            if native._shape[-1] == 1 and np.shape(coefficient) == native._shape[:2]:
+                coefficient = np.asarray(coefficient)[..., None]
+            elif np.ndim(coefficient) and np.shape(coefficient)[0] == native._shape[-1]:
+                coefficient = np.moveaxis(coefficient, 0, -1)
+            value = coefficient if value is None else value + coefficient
+        elif term.kind == "gradient":
+            if np.ndim(coefficient) >= 2 and np.shape(coefficient)[:2] == native._gradient_shape[-2:]:
+                coefficient = np.moveaxis(coefficient, (0, 1), (-2, -1))
+            gradient = coefficient if gradient is None else gradient + coefficient
+        else:
+            raise UnsupportedNativeForm
+    result, _ = native.assemble(value=value, gradient=gradient, num_threads=num_threads)
+    if result.shape[0] == basis.N:
+        return result
+    padded = np.zeros(basis.N, dtype=result.dtype)
+    padded[:result.shape[0]] = result
+    return padded
+
+
+def anisotropic_tensor_coefficient(raw, coefficient_shape, dimension):
+    """Normalise a diffusion tensor to (E,Q,1,dim,1,dim) (forms.py:377-404):
+    accepts (dim,dim), the recommended (dim,dim,E,Q) or the native (E,Q,dim,dim)."""
+    tensor = np.asarray(raw, dtype=np.float64)
+    coefficient_shape = tuple(coefficient_shape)
+    if tensor.shape == (dimension, dimension):
+        tensor = np.broadcast_to(tensor, coefficient_shape + (dimension, dimension))
+    elif tensor.shape == (dimension, dimension) + coefficient_shape:
+        tensor = np.moveaxis(tensor, (0, 1), (-2, -1))
+    elif tensor.shape == coefficient_shape + (dimension, dimension):
+        pass
+    else:
+        raise ValueError(
+            "anisotropic tensor must use the recommended scikit-fem shape "
+            f"{(dimension, dimension) + coefficient_shape}, constant shape ({dimension}, {dimension}), "
+            f"or the low-level native compatibility shape {coefficient_shape + (dimension, dimension)}; "
+            f"got {tensor.shape}")
+    return tensor[..., None, :, None, :]
+
+
+def _native_bilinear_assemble(form, basis, kwargs, num_threads=0, *, memory_limit_bytes=None,
+                              memory_safety_factor=1.25, precision="fp64"):
+    """forms.py:276-374."""
+    expression = trace_expression(form.function, (TrialValue(), TestValue()), _geometry(basis), kwargs,
+                                  context="BilinearForm")
+    terms = extract_terms(expression, BilinearTerm, BilinearSum,
+                          message=("native BilinearForm must reduce to dot(u, v), "
+                                   "ddot(grad(u), grad(v)), or a sum of those terms"))
+    kinds = {"value": None, "gradient": None, "symmetric_gradient": None, "divergence": None}
+    tensor_terms = []
+    for term in terms:
+        coefficient = evaluate_coefficient(term.coefficient, kwargs, factor=term.factor, squeeze=True)
+        if term.kind in kinds:
+            prev = kinds[term.kind]
+            kinds[term.kind] = coefficient if prev is None else prev + coefficient
+        elif term.kind == "gradient_tensor":
+            tensor_terms.append((coefficient, 1.))
+        else:
+            raise UnsupportedNativeForm(f"unsupported bilinear term kind {term.kind!r}")
+    result = None
+    if any(v is not None for v in kinds.values()):
+        native = form._native_cache.get(basis)
+        if native is None:
+            native = NativeBilinearForm(basis, memory_limit_bytes=memory_limit_bytes,
+                                        memory_safety_factor=memory_safety_factor)
+            form._native_cache[basis] = native
+        result = native.assemble(num_threads=num_threads, precision=precision, **kinds)
+    if tensor_terms:
+        if basis.elem._dim != 1:
+            raise UnsupportedNativeForm("anisotropic gradient tensor currently requires a scalar field")
+        by_trial = form._cross_cache.setdefault(basis, WeakKeyDictionary())
+        tensor_native = by_trial.get(basis)
+        if tensor_native is None:
+            tensor_native = NativeCrossBilinearForm(basis, basis, memory_limit_bytes=memory_limit_bytes,
+                                                    memory_safety_factor=memory_safety_factor)
+            by_trial[basis] = tensor_native
+        for raw, factor in tensor_terms:
+            tensor = anisotropic_tensor_coefficient(raw, basis.coefficient_shape, 3) * factor
+            matrix = tensor_native.assemble_tensor("gradient", "gradient", tensor, num_threads=num_threads,
+                                                   precision=precision)
+            result = matrix if result is None else result + matrix
+    if result is None:
+        raise UnsupportedNativeForm("bilinear form contains no assembled terms")
+    return result
+
+
+def asm(form, *bases, num_threads=None, memory_limit_bytes=None, memory_safety_factor=1.25, **kwargs):
+    """Assemble strictly with the native (MI355X) backend.
+
+    Unsupported forms raise ``UnsupportedNativeForm``; this function never
+    delegates to a CPU implementation.
+    """
+    requested = 0
+    if num_threads is not None:
+        from .runtime import available_num_threads
+        if isinstance(num_threads, bool) or not isinstance(num_threads, int) or num_threads < 1:
+            raise ValueError("num_threads must be a positive integer")
+        requested = min(num_threads, available_num_threads())
+    if isinstance(form, _LinearForm):
+        if kwargs.pop("integration", None) is not None:
+            raise UnsupportedNativeForm("interface assembly is outside the MI355X build")
+        if len(bases) == 1:
+            return _native_linear_assemble(form, bases[0], kwargs, requested)
+        raise UnsupportedNativeForm("native LinearForm requires one basis, or two bases with an interface "
+                                    "integration")
+    if isinstance(form, _BilinearForm):
+        if kwargs.pop("integration", None) is not None:
+            raise UnsupportedNativeForm("interface assembly is outside the MI355X build")
+        if len(bases) != 1:
+            raise UnsupportedNativeForm("native BilinearForm currently requires one shared basis")
+        return _native_bilinear_assemble(form, bases[0], kwargs, requested, memory_limit_bytes=memory_limit_bytes,
+                                         memory_safety_factor=memory_safety_factor)
+    raise TypeError("skfemntv.asm accepts forms created by skfemntv.LinearForm or skfemntv.BilinearForm")
+
+
+# ------------------------------------------------------------------ operators
+def _is_value_like(x):
+    return isinstance(x, (np.ndarray, QuadratureValue)) or (hasattr(x, "value") and hasattr(x, "grad"))
+
+
+def dot(left, right):
+    """forms.py:788-961 restricted to the volume H1 kinds."""
+    if isinstance(left, TensorGradient) and isinstance(right, (TrialGradient, TestGradient)):
+        if left.gradient.__class__ is right.__class__:
+            raise UnsupportedNativeForm("anisotropic contraction requires trial and test gradients")
+        return BilinearTerm("gradient_tensor", left.coefficient, left.gradient.factor * right.factor)
+    if isinstance(right, TensorGradient) and isinstance(left, (TrialGradient, TestGradient)):
+        if right.gradient.__class__ is left.__class__:
+            raise UnsupportedNativeForm("anisotropic contraction requires trial and test gradients")
+        return BilinearTerm("gradient_tensor", right.coefficient, right.gradient.factor * left.factor)
+    if is_symbolic(left) and isinstance(right, TestValue):
+        return LinearTerm("value", left)
+    if is_symbolic(right) and isinstance(left, TestValue):
+        return LinearTerm("value", right)
+    if isinstance(right, TestValue) and _is_value_like(left):
+        return LinearTerm("value", np.asarray(left))
+    if isinstance(left, TestValue) and _is_value_like(right):
+        return LinearTerm("value", np.asarray(right))
+    if (isinstance(left, TrialValue) and isinstance(right, TestValue)) or (
+            isinstance(right, TrialValue) and isinstance(left, TestValue)):
+        return BilinearTerm("value", factor=left.factor * right.factor)
+    # Superset of the reference: dot(grad(u), grad(v)) is the isotropic
+    # gradient kind (the reference evaluates it with numpy and fails).
+    if (isinstance(left, TrialGradient) and isinstance(right, TestGradient)) or (
+            isinstance(right, TrialGradient) and isinstance(left, TestGradient)):
+        return BilinearTerm("gradient", factor=left.factor * right.factor)
+    if isinstance(left, (TrialValue, TestValue, TrialGradient, TestGradient, SymmetricGradient, Divergence)) or \
+            isinstance(right, (TrialValue, TestValue, TrialGradient, TestGradient, SymmetricGradient, Divergence)):
+        raise UnsupportedNativeForm("unsupported dot() operands in a native form")
+    return np.einsum("i...,i...->...", left, right)
+
+
+def mul(left, right):
+    if isinstance(right, (TrialGradient, TestGradient)) and (
+            isinstance(left, (Coefficient, CoefficientComponent, np.ndarray, QuadratureValue))):
+        coefficient = left if is_symbolic(left) else np.asarray(left)
+        return TensorGradient(right, coefficient)
+    raise UnsupportedNativeForm("native mul currently supports coefficient @ trial/test gradient")
+
+
+def ddot(left, right):
+    if isinstance(left, SymmetricGradient) and isinstance(right, SymmetricGradient) and left.role != right.role:
+        return BilinearTerm("symmetric_gradient", factor=left.factor * right.factor)
+    if is_symbolic(left) and isinstance(right, TestGradient):
+        return LinearTerm("gradient", left)
+    if is_symbolic(right) and isinstance(left, TestGradient):
+        return LinearTerm("gradient", right)
+    if isinstance(left, (np.ndarray, QuadratureValue)) and isinstance(right, TestGradient):
+        return LinearTerm("gradient", np.asarray(left))
+    if isinstance(right, (np.ndarray, QuadratureValue)) and isinstance(left, TestGradient):
+        return LinearTerm("gradient", np.asarray(right))
+    if (isinstance(left, TrialGradient) and isinstance(right, TestGradient)) or (
+            isinstance(right, TrialGradient) and isinstance(left, TestGradient)):
+        return BilinearTerm("gradient", factor=left.factor * right.factor)
+    return np.einsum("ij...,ij...->...", left, right)
+
+
+def grad(value):
+    if isinstance(value, TestValue):
+        return TestGradient(value.factor)
+    if isinstance(value, TrialValue):
+        return TrialGradient(value.factor)
+    try:
+        return value.grad
+    except AttributeError as error:
+        raise UnsupportedNativeForm("grad() requires a form field") from error
+
+
+def div(value):
+    if isinstance(value, TrialValue):
+        return Divergence("trial", value.factor)
+    if isinstance(value, TestValue):
+        return Divergence("test", value.factor)
+    try:
+        return np.einsum("ii...->...", value.grad)
+    except AttributeError as error:
+        raise UnsupportedNativeForm("div() requires a vector field") from error
+
+
+__all__ = ["BilinearForm", "LinearForm", "asm", "dot", "ddot", "mul", "grad", "div", "UnsupportedNativeForm",
+           "anisotropic_tensor_coefficient"]
