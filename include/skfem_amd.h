@@ -51,9 +51,21 @@ int skfa_device_name(char* buf, int32_t len);
  */
 int skfa_plan_create(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
                      void* stream, skfa_plan** out);
+/* Same with the LDS budget (bytes, 2 KiB..64 KiB) that sizes each wave's
+ * CSR span: smaller budgets give more waves per CU, larger ones fewer
+ * column-split lanes.  skfa_plan_create uses 16 KiB. */
+int skfa_plan_create_budget(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
+                            int32_t lds_budget, void* stream, skfa_plan** out);
 int skfa_plan_destroy(skfa_plan* plan);
+/* Launch tuning of an existing plan (results are identical for every
+ * setting; only speed changes).  Keys: */
+#define SKFA_TUNE_XCD_CHUNK 1  /* waves per XCD chunk, 0 = plain dispatch order */
+#define SKFA_TUNE_LDS_ATOMIC 2 /* 1: accumulate with ds_add_f64, 0: read-add-write */
+#define SKFA_TUNE_PLACEMENT 3  /* 0 plain order, 1 one contiguous segment per XCD, 2 XCD chunks */
+#define SKFA_TUNE_STREAMING 4  /* 1: non-temporal record loads and CSR value stores */
+int skfa_plan_set_tuning(skfa_plan* plan, int32_t key, int32_t value);
 /* host info[9]: nrows, nnz, nnz_node, max_row_len, max_incidence, waves,
- * gather steps, record bytes, LDS buckets. */
+ * gather records, record bytes, LDS bytes of the largest wave span. */
 int skfa_plan_info(const skfa_plan* plan, int64_t* info);
 /* CSR pattern (indptr (nrows+1) int64, indices (nnz) int32): the reference's
  * BilinearFormAssembler.indptr / .indices (bilinear_form_assembler.cpp:75-76). */
@@ -70,24 +82,34 @@ int skfa_plan_export_map(const skfa_plan* plan, int64_t* inc_ptr, int32_t* inc, 
  * (cpp/src/assembly/cross_bilinear_assembler.cpp:55-107,207-247 with
  * cpp/include/native_fem/cross_contraction.hpp:42-84), with the geometry of
  * tabulate_basis_geometry (cpp/src/assembly/basis_geometry.cpp:107-176)
- * computed inside the kernel instead of read from (E,Q,n,3) tabulations.
+ * computed on the GPU instead of read from (E,Q,n,3) tabulations.
  *   xyz      (npoints, 4) FP64 x,y,z,pad
  *   vconn    (E,4) int32 corner vertices (affine elements)
  *   gconn    (E,gnodes) int32 geometry nodes for isoparametric elements, or
  *            NULL for affine elements
- *   tables   FP64 reference tables w[Q] | phi[Q][n] | dphi[Q][n][3] |
- *            gphi[Q][gn] | gdphi[Q][gn][3]; tables32 the same in FP32 (only
- *            for precision 1)
+ *   tables   FP64 w[Q] | phi[Q][n] | dphi[Q][n][3] | gphi[Q][gn] | gdphi[Q][gn][3]
+ *            (used with per-point coefficients or isoparametric geometry)
+ *   rtab     FP64 reduced tables wg[Qg] | dphi[Qg][n][3] | wm[Qm] | phi[Qm][n]
+ *            for affine elements with element-constant coefficients (any
+ *            rule whose sums sum_q wg dphi_a dphi_b^T and sum_q wm phi_a phi_b
+ *            equal the basis rule's); rtab32 the same in FP32 (precision 1)
  *   c_*      per element (E) [tensor (E,3,3)] when per_qp == 0, or per
  *            quadrature point (E,Q) [(E,Q,3,3)] when per_qp == 1; NULL = absent
  *   precision 0: FP64 throughout; 1: element products in FP32, geometry and
- *            CSR reduction in FP64.
+ *            CSR reduction in FP64 (element-constant coefficients only).
+ *   flags    SKFA_FLAG_SYMMETRIC_TENSOR: the caller guarantees c_tensor is
+ *            symmetric per element (a smaller per-element payload is used).
+ * The element-constant path runs a per-element payload pass into a
+ * plan-owned buffer (allocated on the first such call) and the row-gather
+ * kernel; the other path runs one fused kernel.
  */
-int skfa_assemble_bilinear(const skfa_plan* plan, const double* xyz, int64_t npoints, const int32_t* vconn,
-                           const int32_t* gconn, int32_t gnodes, const double* tables, const float* tables32,
-                           int32_t nq, const double* c_value, const double* c_gradient, const double* c_symgrad,
+int skfa_assemble_bilinear(skfa_plan* plan, const double* xyz, int64_t npoints, const int32_t* vconn,
+                           const int32_t* gconn, int32_t gnodes, const double* tables, int32_t nq,
+                           const double* rtab, const float* rtab32, int32_t nqg, int32_t nqm,
+                           const double* c_value, const double* c_gradient, const double* c_symgrad,
                            const double* c_divergence, const double* c_tensor, int32_t per_qp, int32_t precision,
-                           double* values, void* stream);
+                           int32_t flags, double* values, void* stream);
+#define SKFA_FLAG_SYMMETRIC_TENSOR 1
 
 /* Linear form f.v + G:grad(v) into out (nrows FP64, overwritten).
  * Replaces LinearFormAssembler::assemble
@@ -124,7 +146,7 @@ int skfa_edge_topology(const int32_t* vconn, int64_t E, int64_t nvertices, int32
  * python/skfemntv/hcurl_assembler.py:94-97,181-212).
  *   tables FP64 w[Q] | phi[Q][6][3] (reference Whitney values) | curl[6][3];
  *   kappa / mu per element (per_qp 0) or per (E,Q). */
-int skfa_assemble_tet_n1(const skfa_plan* plan, const double* xyz, const int32_t* vconn, const double* tables,
+int skfa_assemble_tet_n1(skfa_plan* plan, const double* xyz, const int32_t* vconn, const double* tables,
                          int32_t nq, const double* kappa, const double* mu, int32_t per_qp, int32_t include_mass,
                          int32_t include_curl, double* values, void* stream);
 

@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <type_traits>
+#include <vector>
 
 #include "skfa_internal.h"
 #include "skfa_plan.h"
@@ -105,26 +107,572 @@ struct Coef {
     double A[9];
 };
 
-struct H1Args {
-    const double4* xyz;    // (P) x,y,z,pad
-    const int32_t* vconn;  // (E,4) corner vertices
-    const int32_t* gconn;  // (E,gn) geometry nodes (isoparametric) or null
-    int32_t gn;
-    const double* tables;
-    const float* tables32;  // FP32 copy of the tables (FP32-products form)
-    int32_t Q;
-    const double* cm;  // value coefficient
-    const double* cg;  // gradient coefficient
-    const double* cs;  // symmetric-gradient coefficient
-    const double* cl;  // divergence coefficient
-    const double* cA;  // gradient tensor (3x3)
-    int32_t per_qp;    // coefficients per (e,q) instead of per element
-    // plan
-    const int64_t* indptr;
-    const int64_t* wave_ptr;
+// ------------------------------------------------------------------ plan view
+struct PlanArgs {
+    const int64_t* indptr;     // node-level CSR row pointers
+    const int64_t* wave_r0;    // nwaves+1
+    const int64_t* wave_rec;   // nwaves+1
+    const int32_t* wave_split; // nwaves
     const uint8_t* records;
-    const int32_t* bucket_waves;
     int64_t nnodes;
+    int64_t nwaves;
+    const int64_t* xcd_seg;  // 9 wave offsets (placement mode 1)
+    int32_t placement;
+    int32_t xcd_chunk;
+    int32_t lds_atomic;
+    int32_t streaming;       // non-temporal record loads and value stores
+};
+
+// Lane role inside a wave: node row r0+i, components (c,d) of the CxC node
+// block, column part p of S (the lane owns in-row offsets o with o%S == p).
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
+// XCD b % 8).  Wave placement modes (plan tuning):
+//   0: plain order (wave = block);
+//   1: XCD x owns the contiguous wave segment [seg[x], seg[x+1]) of about
+//      1/8 of the gather work, so each XCD's L2 holds only its part of the
+//      mesh; the grid is 8 * max segment and surplus blocks exit at once;
+//   2: chunks of `chunk` consecutive waves dealt round-robin to the XCDs.
+// Returns -1 for a surplus block.
+__device__ inline int placed_wave(const int64_t* seg, int nwaves, int mode, int chunk) {
+    const int b = blockIdx.x;
+    if (mode == 1) {
+        const int x = b & 7, k = b >> 3;
+        const int64_t w = seg[x] + k;
+        return w < seg[x + 1] ? int(w) : -1;
+    }
+    if (mode == 2 && chunk > 0) {
+        const int x = b & 7, k = b >> 3;
+        const int w = ((k / chunk) * 8 + x) * chunk + (k % chunk);
+        const int full = (nwaves / (8 * chunk)) * (8 * chunk);
+        return b < full ? w : b;
+    }
+    return b;
+}
+
+struct Lane {
+    int i, c, d, p, S, R;
+    bool active;
+    int64_t rec0, steps, lo, span, rb;
+};
+
+template <int C>
+__device__ inline void lane_setup(const PlanArgs& P, int w, int lane, Lane& L) {
+    const int64_t r0 = P.wave_r0[w], r1 = P.wave_r0[w + 1];
+    L.R = int(r1 - r0);
+    L.S = P.wave_split[w];
+    const int lpr = C * C * L.S;
+    L.i = lane / lpr;
+    const int rem = lane - L.i * lpr;
+    const int cd = rem / L.S;
+    L.p = rem & (L.S - 1);
+    L.c = cd / C;
+    L.d = cd - L.c * C;
+    L.active = L.i < L.R;
+    L.rec0 = P.wave_rec[w];
+    L.steps = (P.wave_rec[w + 1] - L.rec0) / L.R;
+    L.lo = P.indptr[r0];
+    L.span = (P.indptr[r1] - L.lo) * C * C;
+    L.rb = 0;
+    if (L.active) {
+        const int64_t rn = r0 + L.i;
+        const int64_t len = P.indptr[rn + 1] - P.indptr[rn];
+        L.rb = int64_t(C) * C * (P.indptr[rn] - L.lo) + int64_t(L.c) * C * len + L.d;
+    }
+}
+
+// Every LDS address of a span has exactly one owning lane, so this add has
+// no concurrent writer: the LDS atomic only saves the read-back (one ds_add
+// instead of ds_read + v_add + ds_write); the summation order per entry is
+// the lane's program order.
+__device__ inline void lds_add(double* p, double v, bool atomic) {
+    if (atomic)
+        (void)atomicAdd(p, v);
+    else
+        *p += v;
+}
+
+__device__ inline void zero_span(double* lds, int64_t span, int lane) {
+    for (int64_t k = lane; k < span; k += 64) lds[k] = 0.;
+}
+
+__device__ inline void write_span(const double* lds, int64_t span, int lane, double* out, bool streaming = false) {
+    if (streaming)
+        for (int64_t k = lane; k < span; k += 64) __builtin_nontemporal_store(lds[k], out + k);
+    else
+        for (int64_t k = lane; k < span; k += 64) out[k] = lds[k];
+}
+
+template <int RB>
+struct Record {
+    int32_t packed;
+    uint8_t off[RB - 4];
+};
+
+template <int RB>
+__device__ inline void load_record(const uint8_t* __restrict__ base, Record<RB>& r, bool streaming = false) {
+    if constexpr (RB == 16) {
+        // records are read once per assembly: a non-temporal load keeps them
+        // from evicting the reused element payloads from L2
+        uint4 v;
+        if (streaming) {
+            const uint32_t* q = reinterpret_cast<const uint32_t*>(base);
+            v.x = __builtin_nontemporal_load(q);
+            v.y = __builtin_nontemporal_load(q + 1);
+            v.z = __builtin_nontemporal_load(q + 2);
+            v.w = __builtin_nontemporal_load(q + 3);
+        } else {
+            v = *reinterpret_cast<const uint4*>(base);
+        }
+        r.packed = int32_t(v.x);
+        uint32_t* o = reinterpret_cast<uint32_t*>(r.off);
+        o[0] = v.y; o[1] = v.z; o[2] = v.w;
+    } else if constexpr (RB == 8) {
+        const uint2 v = *reinterpret_cast<const uint2*>(base);
+        r.packed = int32_t(v.x);
+        *reinterpret_cast<uint32_t*>(r.off) = v.y;
+    } else if constexpr (RB == 16) {
+        const uint4 v = *reinterpret_cast<const uint4*>(base);
+        r.packed = int32_t(v.x);
+        uint32_t* o = reinterpret_cast<uint32_t*>(r.off);
+        o[0] = v.y; o[1] = v.z; o[2] = v.w;
+    } else {
+        const uint4* q = reinterpret_cast<const uint4*>(base);
+        uint32_t* o = reinterpret_cast<uint32_t*>(r.off);
+        const uint4 v0 = q[0];
+        r.packed = int32_t(v0.x);
+        o[0] = v0.y; o[1] = v0.z; o[2] = v0.w;
+#pragma unroll
+        for (int k = 1; k < RB / 16; ++k) {
+            const uint4 v = q[k];
+            o[4 * k - 1] = v.x; o[4 * k] = v.y; o[4 * k + 1] = v.z; o[4 * k + 2] = v.w;
+        }
+    }
+}
+
+template <int RB, int OFFW>
+__device__ inline int rec_offset(const Record<RB>& r, int b) {
+    if constexpr (OFFW == 1) return r.off[b];
+    else return reinterpret_cast<const uint16_t*>(r.off)[b];
+}
+
+// ------------------------------------------------------------------ element payload
+// Fast path (affine elements, coefficients constant per element): one pass
+// over elements stores what every element row needs, so a gather step is
+// record -> payload (two dependent loads) instead of record -> connectivity
+// -> coordinates/coefficients.
+//   C == 1 (stride 10): D = adj X adj^T / |det| with
+//            X = (g + s/2) I + (s/2 + lam) e0 e0^T + A, then |det| m
+//   C == 3 (stride 14): B = adj / sqrt|det|, g + s/2, s/2, lam, |det| m
+//            (lane (c,d) builds D_cd = dcd (g+s/2) B B^T + s/2 B_d B_c^T + lam B_c B_d^T)
+//   N1     (stride 14): kappa adj adj^T/|det| (6), wsum mu J^T J/|det| (6),
+//            edge signs (bits)
+
+struct PayloadArgs {
+    const double4* xyz;
+    const int32_t* vconn;
+    int64_t E;
+    const double *cm, *cg, *cs, *cl, *cA;
+    const double* wtab;  // N1: quadrature weights (their sum scales the curl term)
+    int32_t Q;
+    double* out;
+};
+
+template <int C, bool SYM>
+__global__ void __launch_bounds__(256) k_payload_h1(PayloadArgs A) {
+    constexpr int P = C == 1 ? (SYM ? 8 : 10) : 14;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < A.E; e += int64_t(gridDim.x) * blockDim.x) {
+        double J[9];
+        Geo G;
+        affine_jacobian(A.xyz, A.vconn + e * 4, J);
+        adjugate(J, G);
+        const double ad = fabs(G.det);
+        const double m = A.cm ? A.cm[e] : 0.;
+        const double g = A.cg ? A.cg[e] : 0.;
+        const double s = A.cs ? A.cs[e] : 0.;
+        const double lam = A.cl ? A.cl[e] : 0.;
+        double* o = A.out + e * P;
+        const double* a = G.adj;
+        if (C == 1) {
+            double X[9];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) X[t] = A.cA ? A.cA[e * 9 + t] : 0.;
+            X[0] += g + 0.5 * s + 0.5 * s + lam;
+            X[4] += g + 0.5 * s;
+            X[8] += g + 0.5 * s;
+            double T[9];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    T[i * 3 + j] = a[i * 3 + 0] * X[0 * 3 + j] + a[i * 3 + 1] * X[1 * 3 + j] + a[i * 3 + 2] * X[2 * 3 + j];
+            const double inv = 1. / ad;
+            double Dm[9];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    Dm[i * 3 + j] = inv * (T[i * 3 + 0] * a[j * 3 + 0] + T[i * 3 + 1] * a[j * 3 + 1] + T[i * 3 + 2] * a[j * 3 + 2]);
+            if (SYM) {
+                // X symmetric => D symmetric; store the upper triangle
+                o[0] = Dm[0]; o[1] = Dm[1]; o[2] = Dm[2]; o[3] = Dm[4]; o[4] = Dm[5]; o[5] = Dm[8];
+                o[6] = ad * m;
+                o[7] = 0.;
+            } else {
+#pragma unroll
+                for (int t = 0; t < 9; ++t) o[t] = Dm[t];
+                o[9] = ad * m;
+            }
+        } else {
+            const double sc = 1. / sqrt(ad);
+#pragma unroll
+            for (int t = 0; t < 9; ++t) o[t] = a[t] * sc;
+            o[9] = g + 0.5 * s;
+            o[10] = 0.5 * s;
+            o[11] = lam;
+            o[12] = ad * m;
+            o[13] = 0.;
+        }
+    }
+}
+
+__constant__ int kN1Edge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+
+__global__ void __launch_bounds__(256) k_payload_n1(PayloadArgs A) {
+    double wsum = 0.;
+    for (int q = 0; q < A.Q; ++q) wsum += A.wtab[q];
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < A.E; e += int64_t(gridDim.x) * blockDim.x) {
+        const int32_t* c4 = A.vconn + e * 4;
+        double J[9];
+        Geo G;
+        affine_jacobian(A.xyz, c4, J);
+        adjugate(J, G);
+        const double inv = 1. / fabs(G.det);
+        // element-constant coefficients are folded in; per-point ones
+        // (cs/cl flags set) are applied in the gather kernel
+        const double k = A.cs ? 1. : (A.cm ? A.cm[e] : 0.);
+        const double mu = A.cl ? 1. : (A.cg ? A.cg[e] * wsum : 0.);
+        const double* a = G.adj;
+        double* o = A.out + e * 14;
+        const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+            const int i = ij[t][0], j = ij[t][1];
+            o[t] = k * inv * (a[i * 3 + 0] * a[j * 3 + 0] + a[i * 3 + 1] * a[j * 3 + 1] + a[i * 3 + 2] * a[j * 3 + 2]);
+            o[6 + t] = mu * inv * (J[0 * 3 + i] * J[0 * 3 + j] + J[1 * 3 + i] * J[1 * 3 + j] + J[2 * 3 + i] * J[2 * 3 + j]);
+        }
+        int32_t v[4] = {c4[0], c4[1], c4[2], c4[3]};
+        int64_t bits = 0;
+#pragma unroll
+        for (int l = 0; l < 6; ++l) bits |= int64_t(v[kN1Edge[l][0]] < v[kN1Edge[l][1]] ? 0 : 1) << l;
+        o[12] = __longlong_as_double(bits);
+        o[13] = 0.;
+    }
+}
+
+// ------------------------------------------------------------------ gather kernels
+// Reference tables in LDS behind the span: gradient block wg[Qg] | dg[Qg][n][3],
+// mass block wm[Qm] | pm[Qm][n].
+struct GatherArgs {
+    PlanArgs plan;
+    const double* payload;
+    const double* tab;      // FP64: wg | dg | wm | pm
+    const float* tab32;     // FP32 copy
+    int32_t Qg, Qm;
+    int32_t has_mass;
+    double* values;
+};
+
+// K_ab += w_q (dphi_aq^T D) . dphi_bq for all b: the lane's row a reads the
+// LDS copy of the table, the b loop reads the same entries for all lanes.
+template <int NL, typename T>
+__device__ inline void grad_row(int q, int a, const T* D, const T* __restrict__ dg, const T* dg_l, T* Kr) {
+    // dg_l holds w_q * dphi (weights folded in), dg the unweighted table
+    const T* ga = dg_l + (q * NL + a) * 3;
+    const T g0 = ga[0], g1 = ga[1], g2 = ga[2];
+    const T u0 = fma(g0, D[0], fma(g1, D[3], g2 * D[6]));
+    const T u1 = fma(g0, D[1], fma(g1, D[4], g2 * D[7]));
+    const T u2 = fma(g0, D[2], fma(g1, D[5], g2 * D[8]));
+#pragma unroll
+    for (int b = 0; b < NL; ++b) {
+        const T* gb = dg + (q * NL + b) * 3;
+        Kr[b] = fma(u0, gb[0], fma(u1, gb[1], fma(u2, gb[2], Kr[b])));
+    }
+}
+
+// Payload layouts of the gather kernel (doubles per element).
+//   PL_FULL  (C==1): D (9, row-major), |det| m
+//   PL_SYM   (C==1, symmetric D): D00 D01 D02 D11 D12 D22, |det| m, pad
+//   PL_VEC   (C==3): B (9), g + s/2, s/2, lam, |det| m, pad
+constexpr int PL_FULL = 0, PL_SYM = 1, PL_VEC = 3;
+template <int PL>
+struct PayloadLayout { static constexpr int stride = PL == PL_FULL ? 10 : PL == PL_SYM ? 8 : 14; };
+
+template <int K, int RB>
+__device__ inline void load_records(const GatherArgs& A, const Lane& L, int64_t st, Record<RB>* rec) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        rec[k].packed = -1;
+        if (L.active && st + k < L.steps)
+            load_record<RB>(A.plan.records + (L.rec0 + (st + k) * L.R + L.i) * RB, rec[k], A.plan.streaming);
+    }
+}
+
+template <int K, int RB, int P>
+__device__ inline void load_payloads(const GatherArgs& A, const Record<RB>* rec, double (*pl)[P]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (rec[k].packed >= 0) {
+            const double2* src = reinterpret_cast<const double2*>(A.payload + int64_t(rec[k].packed >> 4) * P);
+#pragma unroll
+            for (int t = 0; t < P / 2; ++t) {
+                const double2 v = src[t];
+                pl[k][2 * t] = v.x;
+                pl[k][2 * t + 1] = v.y;
+            }
+        }
+    }
+}
+
+template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG>
+__device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Record<RB>& rec, const double* pl,
+                                    double* span_lds, int Qg, int Qm, bool mass,
+                                    const typename std::conditional<F32, float, double>::type* dg,
+                                    const typename std::conditional<F32, float, double>::type* pm,
+                                    const typename std::conditional<F32, float, double>::type* dg_l,
+                                    const typename std::conditional<F32, float, double>::type* pm_l) {
+    using T = typename std::conditional<F32, float, double>::type;
+    const int a = rec.packed & 15;
+    T D[9];
+    T mf;
+    if constexpr (PL == PL_FULL) {
+#pragma unroll
+        for (int t = 0; t < 9; ++t) D[t] = T(pl[t]);
+        mf = T(pl[9]);
+    } else if constexpr (PL == PL_SYM) {
+        D[0] = T(pl[0]); D[1] = T(pl[1]); D[2] = T(pl[2]);
+        D[3] = T(pl[1]); D[4] = T(pl[3]); D[5] = T(pl[4]);
+        D[6] = T(pl[2]); D[7] = T(pl[4]); D[8] = T(pl[5]);
+        mf = T(pl[6]);
+    } else {
+        const double* B = pl;
+        const double iso = (L.c == L.d) ? pl[9] : 0.;
+        const double hs = pl[10], lam = pl[11];
+        const int c = L.c, d = L.d;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double bb = B[i * 3 + 0] * B[j * 3 + 0] + B[i * 3 + 1] * B[j * 3 + 1] + B[i * 3 + 2] * B[j * 3 + 2];
+                D[i * 3 + j] = T(iso * bb + hs * B[i * 3 + d] * B[j * 3 + c] + lam * B[i * 3 + c] * B[j * 3 + d]);
+            }
+        mf = (L.c == L.d) ? T(pl[12]) : T(0);
+    }
+    T Kr[NL];
+#pragma unroll
+    for (int b = 0; b < NL; ++b) Kr[b] = T(0);
+    if constexpr (QG > 0) {
+#pragma unroll
+        for (int q = 0; q < QG; ++q) grad_row<NL, T>(q, a, D, dg, dg_l, Kr);
+    } else {
+        for (int q = 0; q < Qg; ++q) grad_row<NL, T>(q, a, D, dg, dg_l, Kr);
+    }
+    if (mass) {
+        for (int q = 0; q < Qm; ++q) {
+            const T v = mf * pm_l[q * NL + a];  // pm_l holds w_q * phi
+#pragma unroll
+            for (int b = 0; b < NL; ++b) Kr[b] = fma(v, pm[q * NL + b], Kr[b]);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < NL; ++b) {
+        const int o = rec_offset<RB, OFFW>(rec, b);
+        if ((o & (L.S - 1)) == L.p) lds_add(span_lds + L.rb + int64_t(C) * o, double(Kr[b]), A.plan.lds_atomic);
+    }
+}
+
+// Software-pipelined gather: while batch b is computed, the payloads of
+// batch b+1 and the records of batch b+2 are in flight, so the two
+// dependent loads of a step (record -> payload) overlap other work.
+template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG>
+__global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
+    extern __shared__ double span_lds[];
+    constexpr int P = PayloadLayout<PL>::stride;
+    constexpr int K = 2;
+    using T = typename std::conditional<F32, float, double>::type;
+    const int lane = threadIdx.x;
+    Lane L;
+    const int wv = placed_wave(A.plan.xcd_seg, int(A.plan.nwaves), A.plan.placement, A.plan.xcd_chunk);
+    if (wv < 0) return;
+    lane_setup<C>(A.plan, wv, lane, L);
+    Record<RB> r0[K], r1[K], r2[K];
+    double p0[K][P], p1[K][P];
+    load_records<K, RB>(A, L, 0, r0);
+    zero_span(span_lds, L.span, lane);
+    // stage the lane-varying tables (rows a of dg and pm) behind the span
+    const int Qg = QG > 0 ? QG : A.Qg;  // compile-time for the 1- and 4-point reduced rules
+    const int Qm = A.Qm;
+    const int tab_len = Qg + Qg * NL * 3 + Qm + Qm * NL;
+    T* tab_lds = reinterpret_cast<T*>(span_lds + L.span);
+    const T* tab_src = F32 ? reinterpret_cast<const T*>(A.tab32) : reinterpret_cast<const T*>(A.tab);
+    for (int k = lane; k < tab_len; k += 64) {
+        T v = tab_src[k];
+        if (k >= Qg && k < Qg + Qg * NL * 3) v *= tab_src[(k - Qg) / (NL * 3)];  // w_q * dphi
+        if (k >= Qg + Qg * NL * 3 + Qm) v *= tab_src[Qg + Qg * NL * 3 + (k - Qg - Qg * NL * 3 - Qm) / NL];  // w_q * phi
+        tab_lds[k] = v;
+    }
+    load_payloads<K, RB, P>(A, r0, p0);
+    load_records<K, RB>(A, L, K, r1);
+    __syncthreads();
+    const T* dg = tab_src + Qg;            // uniform reads (scalar loads)
+    const T* pm = dg + Qg * NL * 3 + Qm;
+    const T* dg_l = tab_lds + Qg;          // lane-varying reads
+    const T* pm_l = tab_lds + Qg + Qg * NL * 3 + Qm;
+    const bool mass = A.has_mass != 0;
+    for (int64_t st = 0; st < L.steps; st += K) {
+        load_payloads<K, RB, P>(A, r1, p1);
+        load_records<K, RB>(A, L, st + 2 * K, r2);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (r0[k].packed >= 0)
+                compute_step<NL, C, PL, F32, RB, OFFW, QG>(A, L, r0[k], p0[k], span_lds, Qg, Qm, mass, dg, pm, dg_l,
+                                                           pm_l);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            r0[k] = r1[k];
+            r1[k] = r2[k];
+#pragma unroll
+            for (int t = 0; t < P; ++t) p0[k][t] = p1[k][t];
+        }
+    }
+    __syncthreads();
+    write_span(span_lds, L.span, lane, A.values + L.lo * C * C, A.plan.streaming);
+}
+
+// N1 rows are global edges; incidence (e, local edge a).
+//   K_ab = s_a s_b [ sum_q w_q phi_aq^T Dm phi_bq + curl_a^T Dc curl_b ]
+// with Dm = kappa adj adj^T/|det| and Dc = (sum_q w_q) mu J^T J/|det|: the
+// covariant Piola map of hcurl_basis.py:355-364 integrated as in
+// integrate_tet_n1 (edge_topology.cpp:196-231).
+struct GatherN1Args {
+    PlanArgs plan;
+    const double* payload;
+    const double* tab;  // w[Q] | phi[Q][6][3] | curl[6][3]
+    int32_t Q;
+    int32_t include_mass, include_curl;
+    const double* kappa_qp;  // (E,Q) when the coefficients vary per point, else null
+    const double* mu_qp;
+    double* values;
+};
+
+template <int RB, int OFFW>
+__global__ void __launch_bounds__(64) k_gather_n1(GatherN1Args A) {
+    extern __shared__ double span_lds[];
+    constexpr int K = 4;
+    const int lane = threadIdx.x;
+    Lane L;
+    const int wv = placed_wave(A.plan.xcd_seg, int(A.plan.nwaves), A.plan.placement, A.plan.xcd_chunk);
+    if (wv < 0) return;
+    lane_setup<1>(A.plan, wv, lane, L);
+    zero_span(span_lds, L.span, lane);
+    const int Q = A.Q;
+    const int tab_len = Q + Q * 18 + 18;
+    double* tab_lds = span_lds + L.span;
+    for (int k = lane; k < tab_len; k += 64) tab_lds[k] = A.tab[k];
+    __syncthreads();
+    const double* W = A.tab;
+    const double* phi = W + Q;
+    const double* curl = phi + Q * 18;
+    const double* phi_l = tab_lds + Q;
+    const double* curl_l = phi_l + Q * 18;
+    for (int64_t st = 0; st < L.steps; st += K) {
+        Record<RB> rec[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            rec[k].packed = -1;
+            if (L.active && st + k < L.steps) load_record<RB>(A.plan.records + (L.rec0 + (st + k) * L.R + L.i) * RB, rec[k]);
+        }
+        double pl[K][14];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (rec[k].packed >= 0) {
+                const double2* src = reinterpret_cast<const double2*>(A.payload + int64_t(rec[k].packed >> 4) * 14);
+#pragma unroll
+                for (int t = 0; t < 7; ++t) {
+                    const double2 v = src[t];
+                    pl[k][2 * t] = v.x;
+                    pl[k][2 * t + 1] = v.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (rec[k].packed < 0) continue;
+            const int a = rec[k].packed & 15;
+            const double* m6 = pl[k];
+            const double* c6 = pl[k] + 6;
+            const int64_t bits = __double_as_longlong(pl[k][12]);
+            double Kr[6];
+#pragma unroll
+            for (int b = 0; b < 6; ++b) Kr[b] = 0.;
+            if (A.include_mass) {
+                const int64_t e = rec[k].packed >> 4;
+                for (int q = 0; q < Q; ++q) {
+                    const double* pa = phi_l + (q * 6 + a) * 3;
+                    const double f = A.kappa_qp ? W[q] * A.kappa_qp[e * Q + q] : W[q];
+                    const double u0 = f * (pa[0] * m6[0] + pa[1] * m6[1] + pa[2] * m6[2]);
+                    const double u1 = f * (pa[0] * m6[1] + pa[1] * m6[3] + pa[2] * m6[4]);
+                    const double u2 = f * (pa[0] * m6[2] + pa[1] * m6[4] + pa[2] * m6[5]);
+#pragma unroll
+                    for (int b = 0; b < 6; ++b) {
+                        const double* pb = phi + (q * 6 + b) * 3;
+                        Kr[b] += u0 * pb[0] + u1 * pb[1] + u2 * pb[2];
+                    }
+                }
+            }
+            if (A.include_curl) {
+                const double* ca = curl_l + a * 3;
+                double cw = 1.;
+                if (A.mu_qp) {
+                    const int64_t e = rec[k].packed >> 4;
+                    cw = 0.;
+                    for (int q = 0; q < Q; ++q) cw += W[q] * A.mu_qp[e * Q + q];
+                }
+                const double u0 = cw * (ca[0] * c6[0] + ca[1] * c6[1] + ca[2] * c6[2]);
+                const double u1 = cw * (ca[0] * c6[1] + ca[1] * c6[3] + ca[2] * c6[4]);
+                const double u2 = cw * (ca[0] * c6[2] + ca[1] * c6[4] + ca[2] * c6[5]);
+#pragma unroll
+                for (int b = 0; b < 6; ++b) {
+                    const double* cb = curl + b * 3;
+                    Kr[b] += u0 * cb[0] + u1 * cb[1] + u2 * cb[2];
+                }
+            }
+            const double sa = ((bits >> a) & 1) ? -1. : 1.;
+#pragma unroll
+            for (int b = 0; b < 6; ++b) {
+                const double sb = ((bits >> b) & 1) ? -1. : 1.;
+                const int o = rec_offset<RB, OFFW>(rec[k], b);
+                if ((o & (L.S - 1)) == L.p) lds_add(span_lds + L.rb + o, sa * sb * Kr[b], A.plan.lds_atomic);
+            }
+        }
+    }
+    __syncthreads();
+    write_span(span_lds, L.span, lane, A.values + L.lo, A.plan.streaming);
+}
+
+// ------------------------------------------------------------------ general H1 kernel
+// Per-quadrature-point coefficients and/or isoparametric (curved P2)
+// geometry: J, det and the coefficient are evaluated per point inside the
+// gather step (basis_geometry.cpp:107-176 fused with the element kernel).
+struct H1Args {
+    PlanArgs plan;
+    const double4* xyz;
+    const int32_t* vconn;
+    const int32_t* gconn;
+    int32_t gn;
+    const double* tables;  // w | phi | dphi | gphi | gdphi
+    int32_t Q;
+    const double *cm, *cg, *cs, *cl, *cA;
+    int32_t per_qp;
     double* values;
 };
 
@@ -133,19 +681,10 @@ __device__ inline void load_coef(const H1Args& A, int64_t idx, Coef& k) {
     k.g = A.cg ? A.cg[idx] : 0.;
     k.s = A.cs ? A.cs[idx] : 0.;
     k.lam = A.cl ? A.cl[idx] : 0.;
-    if (A.cA) {
 #pragma unroll
-        for (int t = 0; t < 9; ++t) k.A[t] = A.cA[idx * 9 + t];
-    } else {
-#pragma unroll
-        for (int t = 0; t < 9; ++t) k.A[t] = 0.;
-    }
+    for (int t = 0; t < 9; ++t) k.A[t] = A.cA ? A.cA[idx * 9 + t] : 0.;
 }
 
-// D = (1/|det|) * [ dcd (g + s/2) adj adj^T + s/2 adj[:,d] adj[:,c]^T
-//                   + lam adj[:,c] adj[:,d]^T + (C==1) adj A adj^T ]
-// so that the (a,c),(b,d) entry per quadrature point is
-//   w_q * dhat_a^T D dhat_b  (+ mass  w_q |det| m dcd phi_a phi_b).
 template <int C>
 __device__ inline void build_D(const Geo& G, const Coef& k, int c, int d, bool has_tensor, double* D) {
     const double inv = 1. / fabs(G.det);
@@ -159,7 +698,7 @@ __device__ inline void build_D(const Geo& G, const Coef& k, int c, int d, bool h
             D[i * 3 + j] = iso * aat + 0.5 * k.s * a[i * 3 + d] * a[j * 3 + c] + k.lam * a[i * 3 + c] * a[j * 3 + d];
         }
     if (C == 1 && has_tensor) {
-        double T[9];  // adj A
+        double T[9];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -175,295 +714,71 @@ __device__ inline void build_D(const Geo& G, const Coef& k, int c, int d, bool h
     for (int t = 0; t < 9; ++t) D[t] *= inv;
 }
 
-template <int RB>
-struct Record {
-    int32_t packed;
-    uint8_t off[RB - 4];
-};
-
-template <int RB>
-__device__ inline void load_record(const uint8_t* __restrict__ base, Record<RB>& r) {
-    if constexpr (RB == 8) {
-        const uint2 v = *reinterpret_cast<const uint2*>(base);
-        r.packed = int32_t(v.x);
-        *reinterpret_cast<uint32_t*>(r.off) = v.y;
-    } else if constexpr (RB == 16) {
-        const uint4 v = *reinterpret_cast<const uint4*>(base);
-        r.packed = int32_t(v.x);
-        uint32_t* o = reinterpret_cast<uint32_t*>(r.off);
-        o[0] = v.y; o[1] = v.z; o[2] = v.w;
-    } else {
-        const uint4* p = reinterpret_cast<const uint4*>(base);
-        uint32_t* o = reinterpret_cast<uint32_t*>(r.off);
-        const uint4 v0 = p[0];
-        r.packed = int32_t(v0.x);
-        o[0] = v0.y; o[1] = v0.z; o[2] = v0.w;
-#pragma unroll
-        for (int k = 1; k < RB / 16; ++k) {
-            const uint4 v = p[k];
-            o[4 * k - 1] = v.x; o[4 * k] = v.y; o[4 * k + 1] = v.z; o[4 * k + 2] = v.w;
-        }
-    }
-}
-
-template <int RB, int OFFW>
-__device__ inline int rec_offset(const Record<RB>& r, int b) {
-    if constexpr (OFFW == 1) return r.off[b];
-    else return reinterpret_cast<const uint16_t*>(r.off)[b];
-}
-
-// ------------------------------------------------------------------ H1 kernel
-// NL: local nodes; C: components; ISO: per-qp Jacobian from gn nodes;
-// F32: element products in FP32 (geometry and accumulation stay FP64).
-template <int NL, int C, bool ISO, bool F32, int RB, int OFFW>
-__global__ void __launch_bounds__(64) k_assemble_h1(H1Args A, int64_t bucket_base) {
+template <int NL, int C, bool ISO, int RB, int OFFW>
+__global__ void __launch_bounds__(64) k_assemble_h1_general(H1Args A) {
     extern __shared__ double span_lds[];
-    constexpr int LPR = C * C;
-    constexpr int R = 64 / LPR;
-    const int w = A.bucket_waves[bucket_base + blockIdx.x];
     const int lane = threadIdx.x;
-    const int i = lane / LPR;
-    const int cd = lane - i * LPR;
-    const int c = cd / C, d = cd - (cd / C) * C;
-    const int64_t r0 = int64_t(w) * R;
-    const int64_t rn = r0 + i;
-    const bool active = (i < R) && (rn < A.nnodes);
-    const int64_t rend = (r0 + R < A.nnodes) ? r0 + R : A.nnodes;
-    const int64_t lo = A.indptr[r0], hi = A.indptr[rend];
-    const int64_t span = (hi - lo) * LPR;
-    for (int64_t k = lane; k < span; k += 64) span_lds[k] = 0.;
-    int64_t rb = 0;
-    if (active) {
-        const int64_t len = A.indptr[rn + 1] - A.indptr[rn];
-        rb = int64_t(LPR) * (A.indptr[rn] - lo) + int64_t(c) * C * len + d;
-    }
+    Lane L;
+    const int wv = placed_wave(A.plan.xcd_seg, int(A.plan.nwaves), A.plan.placement, A.plan.xcd_chunk);
+    if (wv < 0) return;
+    lane_setup<C>(A.plan, wv, lane, L);
+    zero_span(span_lds, L.span, lane);
     __syncthreads();
-
     const int Q = A.Q;
     const Tables T = tables_at(A.tables, Q, NL, A.gn);
-    const float* T32 = A.tables32;
     const bool has_tensor = A.cA != nullptr;
     const bool has_mass = A.cm != nullptr;
-    const int64_t s0 = A.wave_ptr[w], s1 = A.wave_ptr[w + 1];
-    for (int64_t st = s0; st < s1; ++st) {
-        if (!active) continue;
+    for (int64_t st = 0; st < L.steps; ++st) {
+        if (!L.active) continue;
         Record<RB> rec;
-        load_record<RB>(A.records + (st * R + i) * RB, rec);
+        load_record<RB>(A.plan.records + (L.rec0 + st * L.R + L.i) * RB, rec);
         if (rec.packed < 0) continue;
         const int64_t e = rec.packed >> 4;
         const int a = rec.packed & 15;
-        const int32_t* c4 = A.vconn + e * 4;
         Geo G;
         double Jm[9];
         Coef k;
         double D[9];
         double mfac = 0.;
         if (!ISO) {
-            affine_jacobian(A.xyz, c4, Jm);
+            affine_jacobian(A.xyz, A.vconn + e * 4, Jm);
             adjugate(Jm, G);
-            if (!A.per_qp) {
-                load_coef(A, e, k);
-                build_D<C>(G, k, c, d, has_tensor, D);
-                mfac = (c == d) ? fabs(G.det) * k.m : 0.;
+        }
+        double Kr[NL];
+#pragma unroll
+        for (int b = 0; b < NL; ++b) Kr[b] = 0.;
+        for (int q = 0; q < Q; ++q) {
+            if (ISO) {
+                iso_jacobian(A.xyz, A.gconn + e * A.gn, A.gn, T.gdphi + q * A.gn * 3, Jm);
+                adjugate(Jm, G);
+            }
+            load_coef(A, A.per_qp ? e * Q + q : e, k);
+            build_D<C>(G, k, L.c, L.d, has_tensor, D);
+            mfac = (L.c == L.d) ? fabs(G.det) * k.m : 0.;
+            const double wq = T.w[q];
+            const double* ga = T.dphi + (q * NL + a) * 3;
+            const double u0 = wq * (ga[0] * D[0] + ga[1] * D[3] + ga[2] * D[6]);
+            const double u1 = wq * (ga[0] * D[1] + ga[1] * D[4] + ga[2] * D[7]);
+            const double u2 = wq * (ga[0] * D[2] + ga[1] * D[5] + ga[2] * D[8]);
+#pragma unroll
+            for (int b = 0; b < NL; ++b) {
+                const double* gb = T.dphi + (q * NL + b) * 3;
+                Kr[b] += u0 * gb[0] + u1 * gb[1] + u2 * gb[2];
+            }
+            if (has_mass) {
+                const double v = wq * mfac * T.phi[q * NL + a];
+#pragma unroll
+                for (int b = 0; b < NL; ++b) Kr[b] += v * T.phi[q * NL + b];
             }
         }
-        if constexpr (F32) {
-            float K[NL];
 #pragma unroll
-            for (int b = 0; b < NL; ++b) K[b] = 0.f;
-            const float* w32 = T32;
-            const float* phi32 = w32 + Q;
-            const float* dphi32 = phi32 + Q * NL;
-            for (int q = 0; q < Q; ++q) {
-                if (ISO || A.per_qp) {
-                    if (ISO) {
-                        iso_jacobian(A.xyz, A.gconn + e * A.gn, A.gn, T.gdphi + q * A.gn * 3, Jm);
-                        adjugate(Jm, G);
-                    }
-                    load_coef(A, A.per_qp ? e * Q + q : e, k);
-                    build_D<C>(G, k, c, d, has_tensor, D);
-                    mfac = (c == d) ? fabs(G.det) * k.m : 0.;
-                }
-                float Df[9];
-#pragma unroll
-                for (int t = 0; t < 9; ++t) Df[t] = float(D[t]);
-                const float wq = w32[q];
-                const float ga0 = dphi32[(q * NL + a) * 3 + 0], ga1 = dphi32[(q * NL + a) * 3 + 1],
-                            ga2 = dphi32[(q * NL + a) * 3 + 2];
-                const float u0 = wq * (ga0 * Df[0] + ga1 * Df[3] + ga2 * Df[6]);
-                const float u1 = wq * (ga0 * Df[1] + ga1 * Df[4] + ga2 * Df[7]);
-                const float u2 = wq * (ga0 * Df[2] + ga1 * Df[5] + ga2 * Df[8]);
-#pragma unroll
-                for (int b = 0; b < NL; ++b) {
-                    const float* gb = dphi32 + (q * NL + b) * 3;
-                    K[b] += u0 * gb[0] + u1 * gb[1] + u2 * gb[2];
-                }
-                if (has_mass) {
-                    const float v = wq * float(mfac) * phi32[q * NL + a];
-#pragma unroll
-                    for (int b = 0; b < NL; ++b) K[b] += v * phi32[q * NL + b];
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < NL; ++b) span_lds[rb + int64_t(C) * rec_offset<RB, OFFW>(rec, b)] += double(K[b]);
-        } else {
-            double K[NL];
-#pragma unroll
-            for (int b = 0; b < NL; ++b) K[b] = 0.;
-            for (int q = 0; q < Q; ++q) {
-                if (ISO || A.per_qp) {
-                    if (ISO) {
-                        iso_jacobian(A.xyz, A.gconn + e * A.gn, A.gn, T.gdphi + q * A.gn * 3, Jm);
-                        adjugate(Jm, G);
-                    }
-                    load_coef(A, A.per_qp ? e * Q + q : e, k);
-                    build_D<C>(G, k, c, d, has_tensor, D);
-                    mfac = (c == d) ? fabs(G.det) * k.m : 0.;
-                }
-                const double wq = T.w[q];
-                const double* ga = T.dphi + (q * NL + a) * 3;
-                const double u0 = wq * (ga[0] * D[0] + ga[1] * D[3] + ga[2] * D[6]);
-                const double u1 = wq * (ga[0] * D[1] + ga[1] * D[4] + ga[2] * D[7]);
-                const double u2 = wq * (ga[0] * D[2] + ga[1] * D[5] + ga[2] * D[8]);
-#pragma unroll
-                for (int b = 0; b < NL; ++b) {
-                    const double* gb = T.dphi + (q * NL + b) * 3;
-                    K[b] += u0 * gb[0] + u1 * gb[1] + u2 * gb[2];
-                }
-                if (has_mass) {
-                    const double v = wq * mfac * T.phi[q * NL + a];
-#pragma unroll
-                    for (int b = 0; b < NL; ++b) K[b] += v * T.phi[q * NL + b];
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < NL; ++b) span_lds[rb + int64_t(C) * rec_offset<RB, OFFW>(rec, b)] += K[b];
+        for (int b = 0; b < NL; ++b) {
+            const int o = rec_offset<RB, OFFW>(rec, b);
+            if ((o & (L.S - 1)) == L.p) lds_add(span_lds + L.rb + int64_t(C) * o, Kr[b], A.plan.lds_atomic);
         }
     }
     __syncthreads();
-    double* out = A.values + lo * LPR;
-    for (int64_t k2 = lane; k2 < span; k2 += 64) out[k2] = span_lds[k2];
-}
-
-// ------------------------------------------------------------------ N1 kernel
-// Rows are global edges; incidence (e, local edge a); sign of local edge l
-// is +1 iff its first vertex has the smaller global id
-// (edge_topology.cpp:71).  Per element, with adj/det from the corners:
-//   mass  K_ab += s_a s_b sum_q w_q kappa phi_aq^T (adj adj^T/|det|) phi_bq
-//   curl  K_ab += s_a s_b (sum_q w_q mu_q) curl_a^T (J^T J/|det|) curl_b
-// which is the covariant Piola mapping of hcurl_basis.py:355-364 integrated
-// as in integrate_tet_n1 (edge_topology.cpp:196-231).
-struct N1Args {
-    const double4* xyz;
-    const int32_t* vconn;
-    const double* tables;  // w[Q] | phi[Q][6][3] | curl[6][3]
-    int32_t Q;
-    const double* kappa;   // mass coefficient
-    const double* mu;      // curl coefficient
-    int32_t per_qp;
-    int32_t include_mass, include_curl;
-    const int64_t* indptr;
-    const int64_t* wave_ptr;
-    const uint8_t* records;
-    const int32_t* bucket_waves;
-    int64_t nnodes;
-    double* values;
-};
-
-__constant__ int kN1Edge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
-
-template <int RB, int OFFW>
-__global__ void __launch_bounds__(64) k_assemble_n1(N1Args A, int64_t bucket_base) {
-    extern __shared__ double span_lds[];
-    const int w = A.bucket_waves[bucket_base + blockIdx.x];
-    const int lane = threadIdx.x;
-    const int64_t r0 = int64_t(w) * 64;
-    const int64_t rn = r0 + lane;
-    const bool active = rn < A.nnodes;
-    const int64_t rend = (r0 + 64 < A.nnodes) ? r0 + 64 : A.nnodes;
-    const int64_t lo = A.indptr[r0], hi = A.indptr[rend];
-    const int64_t span = hi - lo;
-    for (int64_t k = lane; k < span; k += 64) span_lds[k] = 0.;
-    const int64_t rb = active ? A.indptr[rn] - lo : 0;
-    __syncthreads();
-    const int Q = A.Q;
-    const double* W = A.tables;
-    const double* phi = W + Q;           // [Q][6][3]
-    const double* curl = phi + Q * 18;   // [6][3]
-    double wsum = 0.;
-    for (int q = 0; q < Q; ++q) wsum += W[q];
-    const int64_t s0 = A.wave_ptr[w], s1 = A.wave_ptr[w + 1];
-    for (int64_t st = s0; st < s1; ++st) {
-        if (!active) continue;
-        Record<RB> rec;
-        load_record<RB>(A.records + (st * 64 + lane) * RB, rec);
-        if (rec.packed < 0) continue;
-        const int64_t e = rec.packed >> 4;
-        const int a = rec.packed & 15;
-        const int32_t* c4 = A.vconn + e * 4;
-        int32_t vid[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) vid[t] = c4[t];
-        double Jm[9];
-        Geo G;
-        affine_jacobian(A.xyz, c4, Jm);
-        adjugate(Jm, G);
-        const double inv = 1. / fabs(G.det);
-        double Dm[9], Dc[9];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                Dm[i * 3 + j] = inv * (G.adj[i * 3 + 0] * G.adj[j * 3 + 0] + G.adj[i * 3 + 1] * G.adj[j * 3 + 1] +
-                                       G.adj[i * 3 + 2] * G.adj[j * 3 + 2]);
-                Dc[i * 3 + j] = inv * (Jm[0 * 3 + i] * Jm[0 * 3 + j] + Jm[1 * 3 + i] * Jm[1 * 3 + j] +
-                                       Jm[2 * 3 + i] * Jm[2 * 3 + j]);
-            }
-        double sgn[6];
-#pragma unroll
-        for (int l = 0; l < 6; ++l) sgn[l] = vid[kN1Edge[l][0]] < vid[kN1Edge[l][1]] ? 1. : -1.;
-        double K[6];
-#pragma unroll
-        for (int b = 0; b < 6; ++b) K[b] = 0.;
-        double cw = 0.;
-        if (A.include_mass) {
-            for (int q = 0; q < Q; ++q) {
-                const double kq = A.per_qp ? A.kappa[e * Q + q] : A.kappa[e];
-                const double* pa = phi + (q * 6 + a) * 3;
-                const double f = W[q] * kq;
-                const double u0 = f * (pa[0] * Dm[0] + pa[1] * Dm[3] + pa[2] * Dm[6]);
-                const double u1 = f * (pa[0] * Dm[1] + pa[1] * Dm[4] + pa[2] * Dm[7]);
-                const double u2 = f * (pa[0] * Dm[2] + pa[1] * Dm[5] + pa[2] * Dm[8]);
-#pragma unroll
-                for (int b = 0; b < 6; ++b) {
-                    const double* pb = phi + (q * 6 + b) * 3;
-                    K[b] += u0 * pb[0] + u1 * pb[1] + u2 * pb[2];
-                }
-            }
-        }
-        if (A.include_curl) {
-            if (A.per_qp) {
-                for (int q = 0; q < Q; ++q) cw += W[q] * A.mu[e * Q + q];
-            } else {
-                cw = wsum * A.mu[e];
-            }
-            const double* ca = curl + a * 3;
-            const double u0 = cw * (ca[0] * Dc[0] + ca[1] * Dc[3] + ca[2] * Dc[6]);
-            const double u1 = cw * (ca[0] * Dc[1] + ca[1] * Dc[4] + ca[2] * Dc[7]);
-            const double u2 = cw * (ca[0] * Dc[2] + ca[1] * Dc[5] + ca[2] * Dc[8]);
-#pragma unroll
-            for (int b = 0; b < 6; ++b) {
-                const double* cb = curl + b * 3;
-                K[b] += u0 * cb[0] + u1 * cb[1] + u2 * cb[2];
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < 6; ++b)
-            span_lds[rb + rec_offset<RB, OFFW>(rec, b)] += sgn[a] * sgn[b] * K[b];
-    }
-    __syncthreads();
-    double* out = A.values + lo;
-    for (int64_t k2 = lane; k2 < span; k2 += 64) out[k2] = span_lds[k2];
+    write_span(span_lds, L.span, lane, A.values + L.lo * C * C, A.plan.streaming);
 }
 
 // ------------------------------------------------------------------ linear form
@@ -601,41 +916,99 @@ __global__ void k_geometry(GeoArgs A) {
 }
 
 // ------------------------------------------------------------------ dispatch
-template <int NL, int C, bool ISO, bool F32>
-static int launch_h1_rb(const skfa_plan* P, const H1Args& A, hipStream_t s) {
-    for (int b = 0; b < P->nbuckets; ++b) {
-        const int64_t nw = P->bucket_start[b + 1] - P->bucket_start[b];
-        if (nw == 0) continue;
-        const size_t lds = size_t(P->bucket_lds[b]);
-        if (P->rec_bytes == 8 && P->off_bytes == 1)
-            k_assemble_h1<NL, C, ISO, F32, 8, 1><<<dim3(unsigned(nw)), dim3(64), lds, s>>>(A, P->bucket_start[b]);
-        else if (P->rec_bytes == 16 && P->off_bytes == 1)
-            k_assemble_h1<NL, C, ISO, F32, 16, 1><<<dim3(unsigned(nw)), dim3(64), lds, s>>>(A, P->bucket_start[b]);
-        else if (P->rec_bytes == 16 && P->off_bytes == 2)
-            k_assemble_h1<NL, C, ISO, F32, 16, 2><<<dim3(unsigned(nw)), dim3(64), lds, s>>>(A, P->bucket_start[b]);
-        else if (P->rec_bytes == 32 && P->off_bytes == 2)
-            k_assemble_h1<NL, C, ISO, F32, 32, 2><<<dim3(unsigned(nw)), dim3(64), lds, s>>>(A, P->bucket_start[b]);
-        else {
-            set_error("unsupported gather record layout");
-            return kUnsupported;
-        }
-        SKFA_LAUNCH_CHECK();
+static PlanArgs plan_args(const skfa_plan* P) {
+    PlanArgs A;
+    A.indptr = P->indptr.p;
+    A.wave_r0 = P->wave_r0.p;
+    A.wave_rec = P->wave_rec.p;
+    A.wave_split = P->wave_split.p;
+    A.records = P->records.p;
+    A.nnodes = P->nnodes;
+    A.nwaves = P->nwaves;
+    A.xcd_seg = P->xcd_seg.p;
+    A.placement = P->tune_placement;
+    A.xcd_chunk = P->tune_xcd_chunk;
+    A.lds_atomic = P->tune_lds_atomic;
+    A.streaming = P->tune_streaming;
+    return A;
+}
+
+static dim3 wave_grid(const skfa_plan* P) {
+    if (P->tune_placement == 1) {
+        int64_t mx = 0;
+        for (int x = 0; x < 8; ++x) mx = std::max<int64_t>(mx, P->xcd_seg_host[x + 1] - P->xcd_seg_host[x]);
+        return dim3(unsigned(8 * mx));
     }
+    return dim3(unsigned(P->nwaves));
+}
+
+static unsigned elem_grid(int64_t E) { return unsigned(std::min<int64_t>((E + 255) / 256, 65536)); }
+
+static int ensure_payload(skfa_plan* P, int stride) {
+    const size_t need = size_t(P->E) * stride;
+    if (P->payload.n >= need && need) return kOk;
+    return P->payload.alloc(need ? need : 1);
+}
+
+template <int NL, int C, int PL, bool F32, int QG>
+static int launch_gather_q(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
+    const dim3 grid = wave_grid(P), block(64);
+    if (P->rec_bytes == 8 && P->off_bytes == 1)
+        k_gather_h1<NL, C, PL, F32, 8, 1, QG><<<grid, block, lds, s>>>(A);
+    else if (P->rec_bytes == 16 && P->off_bytes == 1)
+        k_gather_h1<NL, C, PL, F32, 16, 1, QG><<<grid, block, lds, s>>>(A);
+    else if (P->rec_bytes == 16 && P->off_bytes == 2)
+        k_gather_h1<NL, C, PL, F32, 16, 2, QG><<<grid, block, lds, s>>>(A);
+    else if (P->rec_bytes == 32 && P->off_bytes == 2)
+        k_gather_h1<NL, C, PL, F32, 32, 2, QG><<<grid, block, lds, s>>>(A);
+    else {
+        set_error("unsupported gather record layout");
+        return kUnsupported;
+    }
+    SKFA_LAUNCH_CHECK();
     return kOk;
 }
 
-template <int NL>
-static int launch_h1_nl(const skfa_plan* P, const H1Args& A, bool iso, bool f32, hipStream_t s) {
-    if (P->C == 1) {
-        if (iso) return f32 ? launch_h1_rb<NL, 1, true, true>(P, A, s) : launch_h1_rb<NL, 1, true, false>(P, A, s);
-        return f32 ? launch_h1_rb<NL, 1, false, true>(P, A, s) : launch_h1_rb<NL, 1, false, false>(P, A, s);
+template <int NL, int C, int PL, bool F32>
+static int launch_gather_pl(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
+    if (A.Qg == 1) return launch_gather_q<NL, C, PL, F32, 1>(P, A, lds, s);
+    if (A.Qg == 4) return launch_gather_q<NL, C, PL, F32, 4>(P, A, lds, s);
+    return launch_gather_q<NL, C, PL, F32, 0>(P, A, lds, s);
+}
+
+template <int NL, int C, bool F32>
+static int launch_gather(const skfa_plan* P, const GatherArgs& A, bool sym, size_t lds, hipStream_t s) {
+    if constexpr (C == 3) return launch_gather_pl<NL, 3, PL_VEC, F32>(P, A, lds, s);
+    if (sym) return launch_gather_pl<NL, 1, PL_SYM, F32>(P, A, lds, s);
+    return launch_gather_pl<NL, 1, PL_FULL, F32>(P, A, lds, s);
+}
+
+template <int NL, int C, bool ISO>
+static int launch_general(const skfa_plan* P, const H1Args& A, size_t lds, hipStream_t s) {
+    const dim3 grid = wave_grid(P), block(64);
+    if (P->rec_bytes == 8 && P->off_bytes == 1)
+        k_assemble_h1_general<NL, C, ISO, 8, 1><<<grid, block, lds, s>>>(A);
+    else if (P->rec_bytes == 16 && P->off_bytes == 1)
+        k_assemble_h1_general<NL, C, ISO, 16, 1><<<grid, block, lds, s>>>(A);
+    else if (P->rec_bytes == 16 && P->off_bytes == 2)
+        k_assemble_h1_general<NL, C, ISO, 16, 2><<<grid, block, lds, s>>>(A);
+    else if (P->rec_bytes == 32 && P->off_bytes == 2)
+        k_assemble_h1_general<NL, C, ISO, 32, 2><<<grid, block, lds, s>>>(A);
+    else {
+        set_error("unsupported gather record layout");
+        return kUnsupported;
     }
-    if (P->C == 3) {
-        if (iso) return f32 ? launch_h1_rb<NL, 3, true, true>(P, A, s) : launch_h1_rb<NL, 3, true, false>(P, A, s);
-        return f32 ? launch_h1_rb<NL, 3, false, true>(P, A, s) : launch_h1_rb<NL, 3, false, false>(P, A, s);
-    }
-    set_error("bilinear assembly supports 1 or 3 components on tetrahedra");
-    return kUnsupported;
+    SKFA_LAUNCH_CHECK();
+    return kOk;
+}
+
+template <int NL, int C>
+static int dispatch_nl(const skfa_plan* P, bool fast, bool iso, bool f32, bool sym, const GatherArgs& G,
+                       const H1Args& H, size_t lds_fast, size_t lds_general, hipStream_t s) {
+    if (fast)
+        return f32 ? launch_gather<NL, C, true>(P, G, sym, lds_fast, s)
+                   : launch_gather<NL, C, false>(P, G, sym, lds_fast, s);
+    return iso ? launch_general<NL, C, true>(P, H, lds_general, s) : launch_general<NL, C, false>(P, H, lds_general, s);
 }
 
 }  // namespace skfa
@@ -644,20 +1017,29 @@ using namespace skfa;
 
 extern "C" {
 
-int skfa_assemble_bilinear(const skfa_plan* P, const double* xyz, int64_t npoints, const int32_t* vconn,
-                           const int32_t* gconn, int32_t gnodes, const double* tables, const float* tables32,
-                           int32_t nq, const double* c_value, const double* c_gradient, const double* c_symgrad,
+int skfa_assemble_bilinear(skfa_plan* P, const double* xyz, int64_t npoints, const int32_t* vconn,
+                           const int32_t* gconn, int32_t gnodes, const double* tables, int32_t nq,
+                           const double* rtab, const float* rtab32, int32_t nqg, int32_t nqm,
+                           const double* c_value, const double* c_gradient, const double* c_symgrad,
                            const double* c_divergence, const double* c_tensor, int32_t per_qp, int32_t precision,
-                           double* values, void* stream) {
+                           int32_t flags, double* values, void* stream) {
     SKFA_REQUIRE(P != nullptr, "plan is null");
+    SKFA_REQUIRE(P->n == 4 || P->n == 10, "bilinear assembly supports 4- and 10-node tetrahedra");
+    SKFA_REQUIRE(P->C == 1 || P->C == 3, "bilinear assembly supports 1 or 3 components on tetrahedra");
     SKFA_REQUIRE(values != nullptr || P->nnz() == 0, "values pointer is null");
-    SKFA_REQUIRE(xyz != nullptr && vconn != nullptr && tables != nullptr, "geometry pointer is null");
-    SKFA_REQUIRE(nq >= 1 && nq <= 4096, "quadrature point count out of range");
+    SKFA_REQUIRE(xyz != nullptr && vconn != nullptr, "geometry pointer is null");
     SKFA_REQUIRE(c_value || c_gradient || c_symgrad || c_divergence || c_tensor,
                  "at least one coefficient is required");
     SKFA_REQUIRE(!(c_tensor && P->C != 1), "anisotropic gradient tensor currently requires a scalar field");
-    SKFA_REQUIRE(precision == 0 || (precision == 1 && tables32 != nullptr), "invalid precision / FP32 tables");
     const bool iso = gconn != nullptr;
+    const bool fast = !iso && per_qp == 0;
+    SKFA_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (FP32 products)");
+    SKFA_REQUIRE(!(precision == 1 && !fast),
+                 "FP32 products are available for affine elements with element-constant coefficients");
+    SKFA_REQUIRE(!fast || (rtab != nullptr && nqg >= 1 && nqm >= 1 && nqg <= 64 && nqm <= 64),
+                 "reduced tables are required for element-constant coefficients");
+    SKFA_REQUIRE(!(precision == 1 && rtab32 == nullptr), "FP32 tables are null");
+    SKFA_REQUIRE(fast || (tables != nullptr && nq >= 1 && nq <= 4096), "tables / quadrature size invalid");
     SKFA_REQUIRE(!iso || (gnodes >= 4 && gnodes <= 64), "geometry node count out of range");
     (void)npoints;
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -665,72 +1047,109 @@ int skfa_assemble_bilinear(const skfa_plan* P, const double* xyz, int64_t npoint
         if (P->nnz()) SKFA_HIP(hipMemsetAsync(values, 0, sizeof(double) * P->nnz(), s));
         return kOk;
     }
-    H1Args A;
-    A.xyz = reinterpret_cast<const double4*>(xyz);
-    A.vconn = vconn;
-    A.gconn = gconn;
-    A.gn = iso ? gnodes : 4;
-    A.tables = tables;
-    A.tables32 = tables32;
-    A.Q = nq;
-    A.cm = c_value; A.cg = c_gradient; A.cs = c_symgrad; A.cl = c_divergence; A.cA = c_tensor;
-    A.per_qp = per_qp;
-    A.indptr = P->indptr.p;
-    A.wave_ptr = P->wave_ptr.p;
-    A.records = P->records.p;
-    A.bucket_waves = P->bucket_waves.p;
-    A.nnodes = P->nnodes;
-    A.values = values;
-    const bool f32 = precision == 1;
-    switch (P->n) {
-        case 4: return launch_h1_nl<4>(P, A, iso, f32, s);
-        case 10: return launch_h1_nl<10>(P, A, iso, f32, s);
-        default: set_error("bilinear assembly supports 4- and 10-node tetrahedra"); return kUnsupported;
+    GatherArgs G{};
+    H1Args H{};
+    bool sym = false;
+    size_t lds_fast = 0, lds_general = size_t(P->lds_bytes);
+    if (fast) {
+        // D is symmetric without a tensor term, or with a tensor the caller
+        // declares symmetric
+        sym = P->C == 1 && (c_tensor == nullptr || (flags & SKFA_FLAG_SYMMETRIC_TENSOR));
+        const int stride = P->C == 1 ? (sym ? 8 : 10) : 14;
+        if (int rc = ensure_payload(P, stride)) return rc;
+        PayloadArgs pa{};
+        pa.xyz = reinterpret_cast<const double4*>(xyz);
+        pa.vconn = vconn;
+        pa.E = P->E;
+        pa.cm = c_value; pa.cg = c_gradient; pa.cs = c_symgrad; pa.cl = c_divergence; pa.cA = c_tensor;
+        pa.out = P->payload.p;
+        if (P->C == 1 && sym)
+            k_payload_h1<1, true><<<elem_grid(P->E), 256, 0, s>>>(pa);
+        else if (P->C == 1)
+            k_payload_h1<1, false><<<elem_grid(P->E), 256, 0, s>>>(pa);
+        else
+            k_payload_h1<3, false><<<elem_grid(P->E), 256, 0, s>>>(pa);
+        SKFA_LAUNCH_CHECK();
+        G.plan = plan_args(P);
+        G.payload = P->payload.p;
+        G.tab = rtab;
+        G.tab32 = rtab32;
+        G.Qg = nqg;
+        G.Qm = nqm;
+        G.has_mass = c_value != nullptr;
+        G.values = values;
+        const size_t tab_len = size_t(nqg) + size_t(nqg) * P->n * 3 + nqm + size_t(nqm) * P->n;
+        lds_fast = size_t(P->lds_bytes) + tab_len * (precision == 1 ? 4 : 8);
+    } else {
+        H.plan = plan_args(P);
+        H.xyz = reinterpret_cast<const double4*>(xyz);
+        H.vconn = vconn;
+        H.gconn = gconn;
+        H.gn = iso ? gnodes : 4;
+        H.tables = tables;
+        H.Q = nq;
+        H.cm = c_value; H.cg = c_gradient; H.cs = c_symgrad; H.cl = c_divergence; H.cA = c_tensor;
+        H.per_qp = per_qp;
+        H.values = values;
     }
+    SKFA_REQUIRE(lds_fast <= 160 * 1024 && lds_general <= 160 * 1024, "LDS request exceeds 160 KiB");
+    const bool f32 = precision == 1;
+    if (P->n == 4)
+        return P->C == 1 ? dispatch_nl<4, 1>(P, fast, iso, f32, sym, G, H, lds_fast, lds_general, s)
+                         : dispatch_nl<4, 3>(P, fast, iso, f32, sym, G, H, lds_fast, lds_general, s);
+    return P->C == 1 ? dispatch_nl<10, 1>(P, fast, iso, f32, sym, G, H, lds_fast, lds_general, s)
+                     : dispatch_nl<10, 3>(P, fast, iso, f32, sym, G, H, lds_fast, lds_general, s);
 }
 
-int skfa_assemble_tet_n1(const skfa_plan* P, const double* xyz, const int32_t* vconn, const double* tables,
+int skfa_assemble_tet_n1(skfa_plan* P, const double* xyz, const int32_t* vconn, const double* tables,
                          int32_t nq, const double* kappa, const double* mu, int32_t per_qp, int32_t include_mass,
                          int32_t include_curl, double* values, void* stream) {
     SKFA_REQUIRE(P != nullptr && P->n == 6 && P->C == 1, "plan must be an edge plan (6 local edges, 1 component)");
     SKFA_REQUIRE(xyz && vconn && tables, "geometry pointer is null");
     SKFA_REQUIRE(!include_mass || kappa, "mass coefficient is null");
     SKFA_REQUIRE(!include_curl || mu, "curl coefficient is null");
+    SKFA_REQUIRE(nq >= 1 && nq <= 64, "quadrature size out of range");
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (P->nnz() == 0 || P->E == 0) {
         if (P->nnz()) SKFA_HIP(hipMemsetAsync(values, 0, sizeof(double) * P->nnz(), s));
         return kOk;
     }
-    N1Args A;
-    A.xyz = reinterpret_cast<const double4*>(xyz);
-    A.vconn = vconn;
-    A.tables = tables;
+    if (int rc = ensure_payload(P, 14)) return rc;
+    PayloadArgs pa{};
+    pa.xyz = reinterpret_cast<const double4*>(xyz);
+    pa.vconn = vconn;
+    pa.E = P->E;
+    pa.cm = include_mass ? kappa : nullptr;
+    pa.cg = include_curl ? mu : nullptr;
+    // per-point coefficients: flag the payload pass to leave them out
+    pa.cs = per_qp && include_mass ? kappa : nullptr;
+    pa.cl = per_qp && include_curl ? mu : nullptr;
+    pa.wtab = tables;
+    pa.Q = nq;
+    pa.out = P->payload.p;
+    k_payload_n1<<<elem_grid(P->E), 256, 0, s>>>(pa);
+    SKFA_LAUNCH_CHECK();
+    GatherN1Args A{};
+    A.plan = plan_args(P);
+    A.payload = P->payload.p;
+    A.tab = tables;
     A.Q = nq;
-    A.kappa = kappa;
-    A.mu = mu;
-    A.per_qp = per_qp;
     A.include_mass = include_mass;
     A.include_curl = include_curl;
-    A.indptr = P->indptr.p;
-    A.wave_ptr = P->wave_ptr.p;
-    A.records = P->records.p;
-    A.bucket_waves = P->bucket_waves.p;
-    A.nnodes = P->nnodes;
+    A.kappa_qp = per_qp && include_mass ? kappa : nullptr;
+    A.mu_qp = per_qp && include_curl ? mu : nullptr;
     A.values = values;
-    for (int b = 0; b < P->nbuckets; ++b) {
-        const int64_t nw = P->bucket_start[b + 1] - P->bucket_start[b];
-        if (nw == 0) continue;
-        const size_t lds = size_t(P->bucket_lds[b]);
-        if (P->rec_bytes == 16 && P->off_bytes == 1)
-            k_assemble_n1<16, 1><<<dim3(unsigned(nw)), dim3(64), lds, s>>>(A, P->bucket_start[b]);
-        else if (P->rec_bytes == 16 && P->off_bytes == 2)
-            k_assemble_n1<16, 2><<<dim3(unsigned(nw)), dim3(64), lds, s>>>(A, P->bucket_start[b]);
-        else {
-            set_error("unsupported gather record layout");
-            return kUnsupported;
-        }
-        SKFA_LAUNCH_CHECK();
+    const size_t lds = size_t(P->lds_bytes) + size_t(nq + nq * 18 + 18) * 8;
+    const dim3 grid = wave_grid(P), block(64);
+    if (P->rec_bytes == 16 && P->off_bytes == 1)
+        k_gather_n1<16, 1><<<grid, block, lds, s>>>(A);
+    else if (P->rec_bytes == 16 && P->off_bytes == 2)
+        k_gather_n1<16, 2><<<grid, block, lds, s>>>(A);
+    else {
+        set_error("unsupported gather record layout");
+        return kUnsupported;
     }
+    SKFA_LAUNCH_CHECK();
     return kOk;
 }
 

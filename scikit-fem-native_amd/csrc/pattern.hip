@@ -166,7 +166,8 @@ __global__ void __launch_bounds__(64) k_row_analyze(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ inc, int64_t nnodes, int phase,
                                                     int32_t* __restrict__ row_len, const int64_t* __restrict__ indptr,
                                                     int32_t* __restrict__ indices,
-                                                    const int64_t* __restrict__ wave_ptr, int R,
+                                                    const int64_t* __restrict__ row_rec,
+                                                    const int32_t* __restrict__ row_stride,
                                                     uint8_t* __restrict__ records, int rec_bytes, int off_bytes,
                                                     int32_t* __restrict__ status) {
     __shared__ int32_t cand[kMaxCandidates];
@@ -202,15 +203,15 @@ __global__ void __launch_bounds__(64) k_row_analyze(const int32_t* __restrict__ 
             if (lane == 0) row_len[r] = count;
         } else {
             const int64_t base = indptr[r];
-            const int64_t w = r / R;
-            const int64_t ri = r - w * R;
+            const int64_t rrec = row_rec[r];
+            const int64_t rstride = row_stride[r];
             for (int i = lane; i < m; i += 64) {
                 const int32_t c = cand[i];
                 int rank = 0;
                 for (int j = 0; j < m; ++j) rank += (first[j] && cand[j] < c);
                 if (first[i]) indices[base + rank] = c;
                 const int k = i / n, b = i - (i / n) * n;
-                uint8_t* rec = records + ((wave_ptr[w] + k) * R + ri) * rec_bytes;
+                uint8_t* rec = records + (rrec + k * rstride) * rec_bytes;
                 if (off_bytes == 1)
                     rec[4 + b] = uint8_t(rank);
                 else
@@ -219,18 +220,6 @@ __global__ void __launch_bounds__(64) k_row_analyze(const int32_t* __restrict__ 
             }
         }
         __syncthreads();
-    }
-}
-
-__global__ void k_wave_steps(const int64_t* __restrict__ inc_ptr, int64_t nnodes, int R, int64_t nwaves,
-                             int64_t* __restrict__ steps, const int64_t* __restrict__ indptr,
-                             int64_t* __restrict__ span) {
-    for (int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; w < nwaves; w += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t r0 = w * R, r1 = min(r0 + R, nnodes);
-        int64_t mx = 0;
-        for (int64_t r = r0; r < r1; ++r) mx = max(mx, inc_ptr[r + 1] - inc_ptr[r]);
-        steps[w] = mx;
-        if (span) span[w] = indptr[r1] - indptr[r0];
     }
 }
 
@@ -279,7 +268,7 @@ static int plan_build(skfa_plan* P, const int32_t* conn, hipStream_t s) {
     if (int rc = rowlen.alloc(nn + 1)) return rc;
     const unsigned rows_grid = unsigned(std::min<int64_t>(std::max<int64_t>(nn, 1), 1 << 20));
     k_row_analyze<<<rows_grid, 64, 0, s>>>(conn, n, P->inc_ptr.p, P->inc.p, nn, 0, rowlen.p, nullptr, nullptr,
-                                           nullptr, 1, nullptr, 0, 1, status.p);
+                                           nullptr, nullptr, nullptr, 0, 1, status.p);
     SKFA_LAUNCH_CHECK();
     SKFA_HIP(hipMemcpyAsync(&st, status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SKFA_HIP(hipStreamSynchronize(s));
@@ -306,21 +295,88 @@ static int plan_build(skfa_plan* P, const int32_t* conn, hipStream_t s) {
     }
     if (int rc = P->indices.alloc(size_t(P->nnz_node))) return rc;
 
-    // wave layout
-    const int R = P->rows_per_wave;
-    P->nwaves = (nn + R - 1) / R;
-    DevBuf<int64_t> steps, span;
-    if (int rc = steps.alloc(P->nwaves + 1)) return rc;
-    if (int rc = span.alloc(P->nwaves + 1)) return rc;
-    if (P->nwaves > 0) {
-        k_wave_steps<<<grid_for(P->nwaves, 256), 256, 0, s>>>(P->inc_ptr.p, nn, R, P->nwaves, steps.p, P->indptr.p,
-                                                                span.p);
-        SKFA_LAUNCH_CHECK();
-    }
-    if (int rc = P->wave_ptr.alloc(P->nwaves + 1)) return rc;
-    if (int rc = exclusive_scan_i64(steps.p, P->wave_ptr.p, P->nwaves, s)) return rc;
-    SKFA_HIP(hipMemcpyAsync(&P->total_steps, P->wave_ptr.p + P->nwaves, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    // wave layout (host greedy partition over node rows)
+    std::vector<int64_t> hptr(size_t(nn + 1)), hinc(size_t(nn + 1));
+    SKFA_HIP(hipMemcpyAsync(hptr.data(), P->indptr.p, sizeof(int64_t) * (nn + 1), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipMemcpyAsync(hinc.data(), P->inc_ptr.p, sizeof(int64_t) * (nn + 1), hipMemcpyDeviceToHost, s));
     SKFA_HIP(hipStreamSynchronize(s));
+    const int cc = P->C * P->C;
+    std::vector<int64_t> w_r0, w_rec;
+    std::vector<int32_t> w_split;
+    std::vector<int64_t> r_rec(static_cast<size_t>(nn));
+    std::vector<int32_t> r_stride(static_cast<size_t>(nn));
+    int64_t rec_total = 0, steps_total = 0, lds_max = 0;
+    int64_t r = 0;
+    while (r < nn) {
+        int chosen_R = 0, chosen_S = 0;
+        for (int S = 1; S <= 64 / cc; S <<= 1) {
+            // power-of-two splits: a lane owns the in-row offsets o with
+            // (o & (S-1)) == part
+            int R = 64 / (cc * S);
+            if (R < 1) break;
+            const int64_t r1 = std::min<int64_t>(r + R, nn);
+            const int64_t bytes = (hptr[size_t(r1)] - hptr[size_t(r)]) * cc * 8;
+            if (bytes <= P->lds_budget || R == 1) {
+                chosen_R = int(r1 - r);
+                chosen_S = S;
+                if (bytes <= P->lds_budget) break;
+            }
+        }
+        const int64_t r1 = r + chosen_R;
+        const int64_t bytes = (hptr[size_t(r1)] - hptr[size_t(r)]) * cc * 8;
+        if (bytes > kMaxWaveLds) {
+            set_error("the CSR span of one node row exceeds 64 KiB of LDS");
+            return kCapacity;
+        }
+        lds_max = std::max(lds_max, bytes);
+        int64_t steps = 0;
+        for (int64_t q = r; q < r1; ++q) steps = std::max(steps, hinc[size_t(q + 1)] - hinc[size_t(q)]);
+        w_r0.push_back(r);
+        w_rec.push_back(rec_total);
+        w_split.push_back(chosen_S);
+        for (int64_t q = r; q < r1; ++q) {
+            r_rec[size_t(q)] = rec_total + (q - r);
+            r_stride[size_t(q)] = chosen_R;
+        }
+        rec_total += steps * chosen_R;
+        steps_total += steps;
+        r = r1;
+    }
+    w_r0.push_back(nn);
+    w_rec.push_back(rec_total);
+    // XCD segments: 8 contiguous wave ranges of about equal record count
+    {
+        const int64_t nw = int64_t(w_split.size());
+        int64_t x = 0;
+        P->xcd_seg_host[0] = 0;
+        for (int64_t w = 0; w < nw && x < 7; ++w) {
+            if (w_rec[size_t(w + 1)] * 8 >= rec_total * (x + 1)) P->xcd_seg_host[++x] = w + 1;
+        }
+        while (x < 8) P->xcd_seg_host[++x] = nw;
+        P->xcd_seg_host[8] = nw;
+        for (int k = 1; k <= 8; ++k)
+            if (P->xcd_seg_host[k] < P->xcd_seg_host[k - 1]) P->xcd_seg_host[k] = P->xcd_seg_host[k - 1];
+    }
+    P->nwaves = int64_t(w_split.size());
+    P->total_records = rec_total;
+    P->total_steps = steps_total;
+    P->lds_bytes = int32_t(lds_max);
+    if (int rc = P->xcd_seg.alloc(9)) return rc;
+    SKFA_HIP(hipMemcpyAsync(P->xcd_seg.p, P->xcd_seg_host, sizeof(int64_t) * 9, hipMemcpyHostToDevice, s));
+    if (int rc = P->wave_r0.alloc(w_r0.size())) return rc;
+    if (int rc = P->wave_rec.alloc(w_rec.size())) return rc;
+    if (int rc = P->wave_split.alloc(std::max<size_t>(w_split.size(), 1))) return rc;
+    if (int rc = P->row_rec.alloc(std::max<int64_t>(nn, 1))) return rc;
+    if (int rc = P->row_stride.alloc(std::max<int64_t>(nn, 1))) return rc;
+    SKFA_HIP(hipMemcpyAsync(P->wave_r0.p, w_r0.data(), sizeof(int64_t) * w_r0.size(), hipMemcpyHostToDevice, s));
+    SKFA_HIP(hipMemcpyAsync(P->wave_rec.p, w_rec.data(), sizeof(int64_t) * w_rec.size(), hipMemcpyHostToDevice, s));
+    if (!w_split.empty())
+        SKFA_HIP(hipMemcpyAsync(P->wave_split.p, w_split.data(), sizeof(int32_t) * w_split.size(),
+                                hipMemcpyHostToDevice, s));
+    if (nn) {
+        SKFA_HIP(hipMemcpyAsync(P->row_rec.p, r_rec.data(), sizeof(int64_t) * nn, hipMemcpyHostToDevice, s));
+        SKFA_HIP(hipMemcpyAsync(P->row_stride.p, r_stride.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, s));
+    }
     P->off_bytes = P->max_row_len <= 256 ? 1 : 2;
     if (P->max_row_len > 65536) {
         set_error("a node row has more than 65536 columns");
@@ -328,47 +384,13 @@ static int plan_build(skfa_plan* P, const int32_t* conn, hipStream_t s) {
     }
     const int raw = 4 + n * P->off_bytes;
     P->rec_bytes = raw <= 8 ? 8 : raw <= 16 ? 16 : raw <= 32 ? 32 : 64;
-    const size_t rec_total = size_t(P->total_steps) * R * P->rec_bytes;
-    if (int rc = P->records.alloc(rec_total ? rec_total : 1)) return rc;
-    SKFA_HIP(hipMemsetAsync(P->records.p, 0xFF, rec_total ? rec_total : 1, s));
+    const size_t rec_bytes_total = size_t(rec_total) * P->rec_bytes;
+    if (int rc = P->records.alloc(rec_bytes_total ? rec_bytes_total : 1)) return rc;
+    SKFA_HIP(hipMemsetAsync(P->records.p, 0xFF, rec_bytes_total ? rec_bytes_total : 1, s));
     k_row_analyze<<<rows_grid, 64, 0, s>>>(conn, n, P->inc_ptr.p, P->inc.p, nn, 1, nullptr, P->indptr.p,
-                                           P->indices.p, P->wave_ptr.p, R, P->records.p, P->rec_bytes,
-                                           P->off_bytes, status.p);
+                                           P->indices.p, P->row_rec.p, P->row_stride.p, P->records.p,
+                                           P->rec_bytes, P->off_bytes, status.p);
     SKFA_LAUNCH_CHECK();
-
-    // bucket waves by LDS footprint (bytes = span * C*C * 8)
-    std::vector<int64_t> hspan(size_t(P->nwaves));
-    if (P->nwaves)
-        SKFA_HIP(hipMemcpyAsync(hspan.data(), span.p, sizeof(int64_t) * P->nwaves, hipMemcpyDeviceToHost, s));
-    SKFA_HIP(hipStreamSynchronize(s));
-    static const int64_t kBounds[] = {8 * 1024, 16 * 1024, 24 * 1024, 32 * 1024, 40 * 1024, 48 * 1024, 64 * 1024};
-    const int nb = int(sizeof(kBounds) / sizeof(kBounds[0]));
-    std::vector<std::vector<int32_t>> lists(nb);
-    const int64_t cc = int64_t(P->C) * P->C;
-    for (int64_t w = 0; w < P->nwaves; ++w) {
-        const int64_t bytes = hspan[size_t(w)] * cc * 8;
-        int b = 0;
-        while (b < nb && bytes > kBounds[b]) ++b;
-        if (b == nb) {
-            set_error("the CSR span of one wave exceeds 64 KiB of LDS");
-            return kCapacity;
-        }
-        lists[b].push_back(int32_t(w));
-    }
-    std::vector<int32_t> all;
-    P->nbuckets = 0;
-    for (int b = 0; b < nb; ++b) {
-        if (lists[b].empty()) continue;
-        P->bucket_lds[P->nbuckets] = kBounds[b];
-        P->bucket_start[P->nbuckets] = int64_t(all.size());
-        all.insert(all.end(), lists[b].begin(), lists[b].end());
-        ++P->nbuckets;
-    }
-    P->bucket_start[P->nbuckets] = int64_t(all.size());
-    if (int rc = P->bucket_waves.alloc(all.size() ? all.size() : 1)) return rc;
-    if (!all.empty())
-        SKFA_HIP(hipMemcpyAsync(P->bucket_waves.p, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice,
-                                s));
     SKFA_HIP(hipStreamSynchronize(s));
     return kOk;
 }
@@ -496,8 +518,57 @@ int skfa_plan_create(const int32_t* node_conn, int64_t E, int32_t n, int64_t nno
     P->n = n;
     P->nnodes = nnodes;
     P->C = components;
-    P->lanes_per_row = components * components;
-    P->rows_per_wave = 64 / P->lanes_per_row;
+    int rc = plan_build(P, node_conn, s);
+    if (rc != kOk) {
+        delete P;
+        return rc;
+    }
+    *out = P;
+    return kOk;
+}
+
+int skfa_plan_set_tuning(skfa_plan* P, int32_t key, int32_t value) {
+    SKFA_REQUIRE(P != nullptr, "plan is null");
+    switch (key) {
+        case SKFA_TUNE_XCD_CHUNK:
+            SKFA_REQUIRE(value >= 0 && value <= 4096, "xcd chunk out of range");
+            P->tune_xcd_chunk = value;
+            return kOk;
+        case SKFA_TUNE_PLACEMENT:
+            SKFA_REQUIRE(value >= 0 && value <= 2, "placement must be 0, 1 or 2");
+            P->tune_placement = value;
+            return kOk;
+        case SKFA_TUNE_STREAMING:
+            SKFA_REQUIRE(value == 0 || value == 1, "streaming flag must be 0 or 1");
+            P->tune_streaming = value;
+            return kOk;
+        case SKFA_TUNE_LDS_ATOMIC:
+            SKFA_REQUIRE(value == 0 || value == 1, "lds atomic flag must be 0 or 1");
+            P->tune_lds_atomic = value;
+            return kOk;
+        default:
+            set_error("unknown tuning key");
+            return kInvalid;
+    }
+}
+
+int skfa_plan_create_budget(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
+                            int32_t lds_budget, void* stream, skfa_plan** out) {
+    SKFA_REQUIRE(out != nullptr, "plan output pointer is null");
+    *out = nullptr;
+    SKFA_REQUIRE(lds_budget >= 2048 && lds_budget <= int32_t(kMaxWaveLds), "LDS budget out of range");
+    SKFA_REQUIRE(E >= 0 && nnodes >= 0, "negative sizes");
+    SKFA_REQUIRE(n >= 1 && n <= kMaxLocal, "nodes per element must be in [1, 16]");
+    SKFA_REQUIRE(E < kMaxElements, "element count exceeds 2^27");
+    SKFA_REQUIRE(components == 1 || components == 2 || components == 3, "components must be 1, 2 or 3");
+    SKFA_REQUIRE(E == 0 || node_conn != nullptr, "connectivity pointer is null");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    skfa_plan* P = new skfa_plan();
+    P->E = E;
+    P->n = n;
+    P->nnodes = nnodes;
+    P->C = components;
+    P->lds_budget = lds_budget;
     int rc = plan_build(P, node_conn, s);
     if (rc != kOk) {
         delete P;
@@ -520,9 +591,9 @@ int skfa_plan_info(const skfa_plan* P, int64_t* info) {
     info[3] = P->max_row_len;
     info[4] = P->max_inc;
     info[5] = P->nwaves;
-    info[6] = P->total_steps;
+    info[6] = P->total_records;
     info[7] = P->rec_bytes;
-    info[8] = P->nbuckets;
+    info[8] = P->lds_bytes;
     return kOk;
 }
 

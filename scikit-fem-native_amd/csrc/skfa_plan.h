@@ -14,8 +14,6 @@ struct skfa_plan {
     int32_t n = 0;        // nodes per element (local rows of the node-level map)
     int64_t nnodes = 0;   // node-level rows
     int32_t C = 1;        // field components (DOF = node * C + c)
-    int32_t lanes_per_row = 1;  // C*C lanes cooperate on one node row
-    int32_t rows_per_wave = 64; // node rows handled by one 64-lane wave
     int32_t max_row_len = 0;    // longest node row
     int32_t max_inc = 0;        // largest element count around one node
 
@@ -28,21 +26,40 @@ struct skfa_plan {
     skfa::DevBuf<int64_t> inc_ptr; // nnodes+1
     skfa::DevBuf<int32_t> inc;     // E*n, packed (e << 4) | a
 
-    // wave-lockstep gather records: record (wave_ptr[w]+s)*R + i holds the
-    // s-th incidence of node row w*R+i and the in-row offsets of the
-    // element's n columns.  Padding records carry packed = -1.
+    // Wave-lockstep gather records.  Wave w owns node rows
+    // [wave_r0[w], wave_r0[w+1]) (R_w = their count) and split factor S_w:
+    // C*C*S_w lanes cooperate on one node row, lane part p accumulating the
+    // in-row offsets o with o % S_w == p.  R_w and S_w are chosen so the
+    // wave's CSR span fits the LDS budget.  Record wave_rec[w] + s*R_w + i
+    // holds the s-th incidence of row wave_r0[w]+i and the in-row offsets of
+    // the element's n columns; padding records carry packed = -1.
     int64_t nwaves = 0;
-    skfa::DevBuf<int64_t> wave_ptr; // nwaves+1
+    skfa::DevBuf<int64_t> wave_r0;    // nwaves+1
+    skfa::DevBuf<int64_t> wave_rec;   // nwaves+1
+    skfa::DevBuf<int32_t> wave_split; // nwaves
+    skfa::DevBuf<int64_t> row_rec;    // nnodes: record of the row's first incidence
+    skfa::DevBuf<int32_t> row_stride; // nnodes: R of the row's wave
     int32_t rec_bytes = 0;
     int32_t off_bytes = 1;
-    int64_t total_steps = 0;
+    int64_t total_records = 0;
+    int64_t total_steps = 0;          // sum over waves of steps (for reporting)
+    int32_t lds_budget = 16 * 1024;   // bytes of LDS per wave requested at plan time
+    int32_t lds_bytes = 0;            // largest span of any wave, bytes
     skfa::DevBuf<uint8_t> records;
 
-    // waves grouped by the LDS footprint of their CSR span
-    int32_t nbuckets = 0;
-    int64_t bucket_lds[8] = {0};   // bytes of LDS per wave in bucket
-    int64_t bucket_start[9] = {0}; // offsets into bucket_waves
-    skfa::DevBuf<int32_t> bucket_waves;
+    // per-element payload of the element-constant fast path (allocated on
+    // first use, E * stride doubles)
+    skfa::DevBuf<double> payload;
+
+    // launch tuning (skfa_plan_set_tuning): XCD chunk of waves (0 = plain
+    // order) and LDS accumulation by ds_add (1) or read-add-write (0)
+    int32_t tune_placement = 2;   // 0 plain, 1 XCD segments, 2 XCD chunks
+    int32_t tune_xcd_chunk = 16;
+    int32_t tune_lds_atomic = 1;
+    int32_t tune_streaming = 1;   // non-temporal record loads / value stores
+    // contiguous wave segments of equal gather work, one per XCD
+    int64_t xcd_seg_host[9] = {0};
+    skfa::DevBuf<int64_t> xcd_seg;
 
     int64_t nrows() const { return nnodes * C; }
     int64_t nnz() const { return nnz_node * C * C; }

@@ -25,13 +25,15 @@ from ._symbols import UnsupportedNativeForm
 class Plan:
     """Device CSR pattern + gather plan of one (basis, components) pair."""
 
-    def __init__(self, node_conn_dev, E, n, nnodes, components):
+    def __init__(self, node_conn_dev, E, n, nnodes, components, lds_budget=12 * 1024):
         import torch
         L = _native.lib()
         h = ctypes.c_void_p()
-        _native.check(L.skfa_plan_create(_native.ptr(node_conn_dev), E, n, nnodes, components, _native.stream(),
-                                         ctypes.byref(h)))
+        _native.check(L.skfa_plan_create_budget(_native.ptr(node_conn_dev), E, n, nnodes, components,
+                                                int(lds_budget), _native.stream(), ctypes.byref(h)))
         self._h = h
+        self.lds_budget = int(lds_budget)
+        self.tuning = {"xcd_chunk": 16, "lds_atomic": 1, "placement": 2, "streaming": 1}
         info = (ctypes.c_int64 * 9)()
         _native.check(L.skfa_plan_info(h, info))
         self.nrows, self.nnz, self.nnz_node = int(info[0]), int(info[1]), int(info[2])
@@ -47,6 +49,22 @@ class Plan:
     @property
     def handle(self):
         return self._h
+
+    def set_tuning(self, xcd_chunk=None, lds_atomic=None, placement=None, streaming=None):
+        """Launch tuning; every setting gives bit-identical values."""
+        L = _native.lib()
+        if placement is not None:
+            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_PLACEMENT, int(placement)))
+            self.tuning["placement"] = int(placement)
+        if streaming is not None:
+            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_STREAMING, int(bool(streaming))))
+            self.tuning["streaming"] = int(bool(streaming))
+        if xcd_chunk is not None:
+            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_XCD_CHUNK, int(xcd_chunk)))
+            self.tuning["xcd_chunk"] = int(xcd_chunk)
+        if lds_atomic is not None:
+            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_LDS_ATOMIC, int(bool(lds_atomic))))
+            self.tuning["lds_atomic"] = int(bool(lds_atomic))
 
     def host_csr_arrays(self):
         """(indptr, indices) on the host, int32 when they fit (as scipy does)."""
@@ -104,7 +122,10 @@ def field_layout(coefficient, shape, name, trailing=()):
 
 def upload(host):
     import torch
-    return torch.from_numpy(np.ascontiguousarray(host)).to(_native.device())
+    arr = np.ascontiguousarray(host)
+    if not arr.flags.writeable:
+        arr = arr.copy()
+    return torch.from_numpy(arr).to(_native.device())
 
 
 def to_per_qp(dev_tensor, Q):
@@ -114,20 +135,23 @@ def to_per_qp(dev_tensor, Q):
 
 
 def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=None, symmetric_gradient=None,
-                             divergence=None, tensor=None, per_qp=False, precision="fp64"):
+                             divergence=None, tensor=None, per_qp=False, precision="fp64", symmetric_tensor=False):
     """Launch skfa_assemble_bilinear with device coefficient tensors."""
     d = basis.device_data()
     L = _native.lib()
     iso = not basis._affine
     tables = d["tables_iso"] if iso else d["tables"]
-    tables32 = d["tables_iso32"] if iso else d["tables32"]
     prec = {"fp64": 0, "fp32": 1}[precision]
+    if prec == 1 and (iso or per_qp):
+        raise UnsupportedNativeForm("FP32 products are available for affine elements with element-constant "
+                                    "coefficients")
     _native.check(L.skfa_assemble_bilinear(
         plan.handle, _native.ptr(d["xyz"]), d["xyz"].shape[0], _native.ptr(d["vconn"]),
         _native.ptr(d["gconn"]) if iso else None, d["gnodes"] if iso else 4,
-        _native.ptr(tables), _native.ptr(tables32), len(basis.W),
+        _native.ptr(tables), len(basis.W), _native.ptr(d["rtab"]), _native.ptr(d["rtab32"]), d["nqg"], d["nqm"],
         _native.ptr(value), _native.ptr(gradient), _native.ptr(symmetric_gradient), _native.ptr(divergence),
-        _native.ptr(tensor), int(per_qp), prec, _native.ptr(values_dev), _native.stream()))
+        _native.ptr(tensor), int(per_qp), prec, _native.FLAG_SYMMETRIC_TENSOR if symmetric_tensor else 0,
+        _native.ptr(values_dev), _native.stream()))
 
 
 def _prepare_scalar_kinds(basis, coefficients):
@@ -222,17 +246,20 @@ class NativeCrossBilinearForm:
         except ValueError as error:
             raise ValueError(f"tensor coefficient must broadcast to {expected}") from error
         arr, per_qp = field_layout(coefficient[:, :, 0, :, 0, :], self.coefficient_shape, "tensor", (3, 3))
-        self.assemble_tensor_device(upload(arr), per_qp=per_qp, precision=precision)
+        sym = bool(np.array_equal(arr, np.swapaxes(arr, -1, -2)))
+        self.assemble_tensor_device(upload(arr), per_qp=per_qp, precision=precision, symmetric=sym)
         ip, ix = self.plan.host_csr_arrays()
         data = self._values[:self.plan.nnz].cpu().numpy()
         m = csr_matrix((data, ix.copy(), ip.copy()), shape=(self.plan.nrows, self.plan.nrows), copy=False)
         m.resize(self.shape)
         return m
 
-    def assemble_tensor_device(self, tensor, *, per_qp=False, precision="fp64", out=None):
-        """Tensor (E,3,3) [(E,Q,3,3) when per_qp] already on the device."""
+    def assemble_tensor_device(self, tensor, *, per_qp=False, precision="fp64", out=None, symmetric=False):
+        """Tensor (E,3,3) [(E,Q,3,3) when per_qp] already on the device;
+        ``symmetric`` declares A == A^T per element (smaller payload)."""
         vals = self._values if out is None else out
-        assemble_bilinear_device(self.basis, self.plan, vals, tensor=tensor, per_qp=per_qp, precision=precision)
+        assemble_bilinear_device(self.basis, self.plan, vals, tensor=tensor, per_qp=per_qp, precision=precision,
+                                 symmetric_tensor=symmetric)
         return vals
 
 

@@ -11,17 +11,24 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "libskfem_amd.so")
+LIB_PATH = os.environ.get("SKFEMNTV_LIBRARY") or os.path.join(_HERE, "_lib", "libskfem_amd.so")
 
 # Exported symbols (kept in sync with include/skfem_amd.h by a test).
 SYMBOLS = (
     "skfa_last_error", "skfa_abi_version", "skfa_device_name",
-    "skfa_plan_create", "skfa_plan_destroy", "skfa_plan_info", "skfa_plan_export_csr", "skfa_plan_export_map",
+    "skfa_plan_create", "skfa_plan_create_budget", "skfa_plan_set_tuning", "skfa_plan_destroy", "skfa_plan_info", "skfa_plan_export_csr", "skfa_plan_export_map",
     "skfa_assemble_bilinear", "skfa_assemble_linear", "skfa_tabulate_geometry",
     "skfa_edge_topology", "skfa_assemble_tet_n1",
     "skfa_event_create", "skfa_event_destroy", "skfa_event_record", "skfa_event_elapsed_ms",
     "skfa_stream_synchronize", "skfa_spmv",
 )
+
+
+FLAG_SYMMETRIC_TENSOR = 1
+TUNE_XCD_CHUNK = 1
+TUNE_LDS_ATOMIC = 2
+TUNE_PLACEMENT = 3
+TUNE_STREAMING = 4
 
 
 class NativeUnavailable(RuntimeError):
@@ -46,10 +53,13 @@ def _declare(lib):
         "skfa_device_name": (ctypes.c_int, [ctypes.c_char_p, I32]),
         "skfa_plan_create": (ctypes.c_int, [P, I64, I32, I64, I32, P, ctypes.POINTER(P)]),
         "skfa_plan_destroy": (ctypes.c_int, [P]),
+        "skfa_plan_create_budget": (ctypes.c_int, [P, I64, I32, I64, I32, I32, P, ctypes.POINTER(P)]),
+        "skfa_plan_set_tuning": (ctypes.c_int, [P, I32, I32]),
         "skfa_plan_info": (ctypes.c_int, [P, ctypes.POINTER(I64)]),
         "skfa_plan_export_csr": (ctypes.c_int, [P, P, P, P]),
         "skfa_plan_export_map": (ctypes.c_int, [P, P, P, P]),
-        "skfa_assemble_bilinear": (ctypes.c_int, [P, P, I64, P, P, I32, P, P, I32, P, P, P, P, P, I32, I32, P, P]),
+        "skfa_assemble_bilinear": (ctypes.c_int, [P, P, I64, P, P, I32, P, I32, P, P, I32, I32,
+                                                  P, P, P, P, P, I32, I32, I32, P, P]),
         "skfa_assemble_linear": (ctypes.c_int, [P, P, P, P, I32, P, I32, P, I32, P, I32, P, P]),
         "skfa_tabulate_geometry": (ctypes.c_int, [P, P, P, I32, I64, P, I32, I32, P, P, P, P, P, P, P, P]),
         "skfa_edge_topology": (ctypes.c_int, [P, I64, I64, P, P, P, P, ctypes.POINTER(I64), ctypes.POINTER(I64), P]),

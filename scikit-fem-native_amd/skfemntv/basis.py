@@ -122,6 +122,42 @@ def pack_tables(W, shape, grad, gshape, ggrad):
         np.float64)
 
 
+def reduced_tables(X, W, quadratic):
+    """Quadrature tables of the element-constant fast path.
+
+    For affine elements with coefficients constant per element, the element
+    matrix depends on the rule only through G = sum_q w_q dphi_a(q) dphi_b(q)^T
+    and M = sum_q w_q phi_a(q) phi_b(q).  Any rule reproducing those sums gives
+    the same matrix, so the gradient part uses one point when the reference
+    gradients are constant (P1) or the 4-point rule when its sums equal the
+    basis rule's, and the mass part likewise; otherwise the basis rule.
+    Returns (tables wg|dg|wm|pm FP64, Qg, Qm)."""
+    s, g = tabulate(X, quadratic)
+    X4, W4 = tet_rule_4()
+    s4, g4 = tabulate(X4, quadratic)
+
+    def gsum(w, gg):
+        return np.einsum("q,qai,qbj->aibj", w, gg, gg)
+
+    def msum(w, ss):
+        return np.einsum("q,qa,qb->ab", w, ss, ss)
+
+    G = gsum(W, g)
+    if not quadratic and np.array_equal(g, np.broadcast_to(g[:1], g.shape)):
+        wg, dg = np.array([W.sum()]), g[:1]
+    elif np.allclose(gsum(W4, g4), G, rtol=1e-14, atol=1e-15 * np.abs(G).max()):
+        wg, dg = W4, g4
+    else:
+        wg, dg = W, g
+    M = msum(W, s)
+    if np.allclose(msum(W4, s4), M, rtol=1e-14, atol=1e-15 * np.abs(M).max()):
+        wm, pm = W4, s4
+    else:
+        wm, pm = W, s
+    tab = np.concatenate([wg, dg.ravel(), wm, pm.ravel()]).astype(np.float64)
+    return tab, len(wg), len(wm)
+
+
 def _is_affine_p2(p, t):
     """True when every edge node of a 10-node mesh is the exact midpoint of
     its edge (as MeshTet2.from_mesh creates it), so the isoparametric map is
@@ -205,6 +241,7 @@ class Basis:
             geo_tables = pack_tables(self.W, self._ref_shape, self._ref_grad, self._gshape, self._ggrad)
             p1s, p1g = tabulate(self.X, False)
             affine_tables = pack_tables(self.W, self._ref_shape, self._ref_grad, p1s, p1g)
+            rtab, nqg, nqm = reduced_tables(self.X, self.W, self._quadratic)
             d = {
                 "xyz": torch.from_numpy(xyz).to(dev),
                 "vconn": torch.from_numpy(np.ascontiguousarray(mesh.t[:4].T, dtype=np.int32)).to(dev),
@@ -217,8 +254,10 @@ class Basis:
                 "tables_iso": torch.from_numpy(geo_tables).to(dev),
                 # tables for affine assembly (gphi over the 4 corners)
                 "tables": torch.from_numpy(affine_tables).to(dev),
-                "tables32": torch.from_numpy(affine_tables.astype(np.float32)).to(dev),
-                "tables_iso32": torch.from_numpy(geo_tables.astype(np.float32)).to(dev),
+                "rtab": torch.from_numpy(rtab).to(dev),
+                "rtab32": torch.from_numpy(rtab.astype(np.float32)).to(dev),
+                "nqg": nqg,
+                "nqm": nqm,
             }
             self._device = d
         return self._device

@@ -276,3 +276,33 @@ def test_curved_p2_uses_isoparametric_geometry():
     E = t.shape[1]
     ip, ix, vals, N = pipeline.bilinear(p2, t2, X, W, True, gradient=np.ones((E, 11)))
     check_csr(K, ip, ix, vals)
+
+
+def test_tet_n1_per_point_coefficients_match_oracle():
+    import skfemntv as sk
+    say("tet n1 per-point coefficients")
+    p, t = kuhn(2, 0.1, seed=6)
+    basis = sk.AffineTetN1Basis(sk.MeshTet(p, t))
+    E, Q = t.shape[1], len(basis.W)
+    rng = np.random.default_rng(4)
+    kq = rng.uniform(0.5, 2., (E, Q))
+    mq = rng.uniform(0.5, 2., (E, Q))
+    K = sk.TetN1Assembler(basis).assemble_maxwell(mass_coefficient=kq, curl_coefficient=mq)
+    ip, ix, data, *_ = pipeline.n1_maxwell(p, t, kq, mq)
+    check_csr(K, ip, ix, data)
+
+
+def test_p1_single_element_and_larger_plan_layouts():
+    """Row layouts with column splitting (long vertex rows) and tiny meshes."""
+    import skfemntv as sk
+    from skfemntv.helpers import ddot, grad
+    for n in (1, 6):
+        say(f"p1 laplace layouts n={n}")
+        p, t = kuhn(n, 0.15 if n > 1 else 0.0, seed=n)
+        for dim in (1, 3):
+            basis = sk.Basis(sk.MeshTet(p, t), sk.ElementVector(sk.ElementTetP1(), dim=dim))
+            K = sk.asm(sk.BilinearForm(lambda u, v, w: ddot(grad(u), grad(v))), basis)
+            X, W = host.tet_rule_4()
+            E = t.shape[1]
+            ip, ix, vals, N = pipeline.bilinear(p, t, X, W, False, components=dim, gradient=np.ones((E, 4)))
+            check_csr(K, ip, ix, vals)
