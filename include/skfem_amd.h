@@ -20,6 +20,7 @@
 #ifndef SKFEM_AMD_H
 #define SKFEM_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -156,6 +157,122 @@ int skfa_event_destroy(void* ev);
 int skfa_event_record(void* ev, void* stream);
 int skfa_event_elapsed_ms(void* start, void* stop, float* ms);
 int skfa_stream_synchronize(void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Solving assembled systems (batched, block-diagonal CSR).
+ *
+ * No native counterpart in the reference: its solver policy is external
+ * (python/skfemntv/capabilities.py:67).  The Dirichlet condensation mirrors
+ * the reference's only solve pattern, solve_dirichlet in
+ * examples/scikit-fem-gallery/comparison.py:80-87
+ * (free = setdiff1d(arange(n), boundary); matrix[free][:, free]).
+ * A batch of independent systems is one CSR whose system s owns rows
+ * [sys_ptr[s], sys_ptr[s+1]) and has no couplings outside them.
+ * ------------------------------------------------------------------------- */
+
+/* y = A x (FP64), `lanes_per_row` in {4, 8, 16}. */
+int skfa_spmv(const int64_t* indptr, const int32_t* indices, const double* vals, const double* x, double* y,
+              int64_t nrows, int32_t lanes_per_row, void* stream);
+
+/* Keep rows/columns with keep[i] != 0, in order.  newid (nrows+1) receives
+ * the exclusive scan of keep (new index of kept row i); out_* are sized by
+ * the caller for the full input (nnz and nrows+1 suffice); *out_nrows and
+ * *out_nnz (host) receive the sizes.  Synchronises `stream`. */
+int skfa_csr_condense(const int64_t* indptr, const int32_t* indices, const double* vals, int64_t nrows,
+                      const uint8_t* keep, int64_t* newid, int64_t* out_indptr, int32_t* out_indices,
+                      double* out_vals, int64_t* out_nrows, int64_t* out_nnz, void* stream);
+
+/* Lower-triangular (j <= i) pattern of a CSR: lptr (nrows+1), and when lidx
+ * and lsrc are non-null the column indices and source positions.  *nnz_lower
+ * is a host output.  Call once with lidx = lsrc = NULL to size. */
+int skfa_csr_lower(const int64_t* indptr, const int32_t* indices, int64_t nrows, int64_t* lptr, int32_t* lidx,
+                   int64_t* lsrc, int64_t* nnz_lower, void* stream);
+
+/* Transpose pattern: tptr (ncols+1), tidx (nnz) sorted per row, tsrc (nnz)
+ * = position of each entry in the input.  Deterministic. */
+int skfa_csr_transpose(const int64_t* ptr, const int32_t* idx, int64_t nrows, int64_t ncols, int64_t* tptr,
+                       int32_t* tidx, int64_t* tsrc, void* stream);
+
+/* out[k] = src[map[k]] (FP64 and FP32). */
+int skfa_gather_values(const double* src, const int64_t* map, int64_t n, double* out, void* stream);
+int skfa_gather_values_f32(const float* src, const int64_t* map, int64_t n, float* out, void* stream);
+
+/* dinv[i] = 1 / A_ii. */
+int skfa_diag_inverse(const int64_t* indptr, const int32_t* indices, const double* vals, int64_t n, double* dinv,
+                      void* stream);
+
+/* Per-system sums: out_xy[s] = sum x_i y_i, out_xx[s] = sum x_i^2 (optional). */
+int skfa_sys_dot(const double* x, const double* y, const int64_t* sys_ptr, int32_t nsys, double* out_xy,
+                 double* out_xx, void* stream);
+
+/* Batched preconditioned conjugate gradients, FP64, x0 = 0.  kind 0: Jacobi
+ * (dinv); kind 1: z = G^T (G r) with G lower CSR (FP32 values) and its
+ * transpose.  System s stops when ||r_s|| <= tol ||b_s|| (recursive
+ * residual) or after max_iter.  Workspaces (device): work 6n doubles,
+ * sys_work 5*nsys doubles, int_work nsys+1 int32.  iters (device, nsys)
+ * receives per-system iteration counts, *total_iters (host) the number of
+ * batch iterations run. */
+int skfa_pcg_solve(const int64_t* indptr, const int32_t* indices, const double* vals, int64_t n,
+                   const int64_t* sys_ptr, int32_t nsys, const double* b, double* x, int32_t kind, const double* dinv,
+                   const int64_t* gptr, const int32_t* gidx, const float* gval, const int64_t* gtptr,
+                   const int32_t* gtidx, const float* gtval, double tol, int32_t max_iter, int32_t check_every,
+                   int32_t spmv_lanes, double* work, double* sys_work, int32_t* int_work, int32_t* iters,
+                   int32_t* total_iters, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Learned FSAI factor (no counterpart in the reference).
+ *
+ * At = D^{-1/2} A D^{-1/2}; the factor G = Gt D^{-1/2} has A's lower pattern;
+ * a 16-128-128-1 ReLU MLP maps 16 features of each lower nonzero to an output
+ * o (Gt_ij = o_ij off the diagonal, Gt_ii = exp(o_ii)).  Parameters are one
+ * FP32 vector of SKFA_FSAI_PARAMS floats: W1[16][128] | b1[128] |
+ * W2[128][128] | b2[128] | w3[128] | b3.  MFMA operands are BF16 copies
+ * ("fragments", SKFA_FSAI_FRAG uint16) made by skfa_mlp_pack / skfa_mlp_adam.
+ * ------------------------------------------------------------------------- */
+#define SKFA_FSAI_FEATURES 16
+#define SKFA_FSAI_PARAMS 18817
+#define SKFA_FSAI_FRAG 34816
+
+/* Features (float, nnz_lower x 16) of every lower nonzero; also writes
+ * rsqrt_diag (n) = A_ii^{-1/2} and row_stats (4n doubles).  xyz (n x 3
+ * doubles, coordinates of each unknown) may be NULL (geometric features 0). */
+int skfa_fsai_features(const int64_t* indptr, const int32_t* indices, const double* vals, int64_t n,
+                       const double* xyz, const int64_t* lptr, const int32_t* lidx, double* rsqrt_diag,
+                       double* row_stats, float* features, void* stream);
+
+/* Factor values from network outputs: gtilde (FP64, optional), g = Gt D^{-1/2}
+ * (FP32, optional), logdiag (n, optional). */
+int skfa_fsai_factor(const float* outputs, const int64_t* lptr, const int32_t* lidx, const double* rsqrt_diag,
+                     int64_t n, double* gtilde, float* g, double* logdiag, void* stream);
+
+/* Kaporin objective per system, loss[s] = log(tr(Gt At Gt^T)/n_s)
+ * - (2/n_s) sum log Gt_ii, and (if grad_outputs != NULL) grad_scale * dL/do
+ * per lower nonzero.  work: 3*nnz_lower + n + 3*nsys + 1 doubles; row_sys: n
+ * int32.  Synchronises `stream` once (reads nnz_lower). */
+int skfa_fsai_kaporin(const int64_t* indptr, const int32_t* indices, const double* vals, int64_t n,
+                      const int64_t* sys_ptr, int32_t nsys, const int64_t* lptr, const int32_t* lidx,
+                      const double* rsqrt_diag, const float* outputs, double grad_scale, double* work,
+                      int32_t* row_sys, double* loss, float* grad_outputs, void* stream);
+
+/* BF16 MFMA fragments from FP32 parameters. */
+int skfa_mlp_pack(const float* params, uint16_t* frag, void* stream);
+
+/* out[k] = MLP(features[k]) for k < nnz (BF16 MFMA, FP32 accumulate). */
+int skfa_mlp_forward(const float* params, const uint16_t* frag, const float* features, int64_t nnz, float* out,
+                     void* stream);
+
+/* Workspace bytes skfa_mlp_backward needs for nnz rows. */
+size_t skfa_mlp_workspace_bytes(int64_t nnz);
+
+/* grad (SKFA_FSAI_PARAMS floats) = sum_k grad_out[k] * d MLP(features[k]) / d params.
+ * Deterministic (fixed partial-sum order). */
+int skfa_mlp_backward(const float* params, const uint16_t* frag, const float* features, const float* grad_out,
+                      int64_t nnz, void* workspace, size_t workspace_bytes, float* grad, void* stream);
+
+/* One Adam step on the FP32 parameters (gradient multiplied by clip_scale),
+ * then repack the BF16 fragments. */
+int skfa_mlp_adam(float* params, const float* grad, float* m, float* v, float lr, float beta1, float beta2,
+                  float eps, int32_t step, float clip_scale, uint16_t* frag, void* stream);
 
 #ifdef __cplusplus
 }

@@ -20,8 +20,16 @@ SYMBOLS = (
     "skfa_assemble_bilinear", "skfa_assemble_linear", "skfa_tabulate_geometry",
     "skfa_edge_topology", "skfa_assemble_tet_n1",
     "skfa_event_create", "skfa_event_destroy", "skfa_event_record", "skfa_event_elapsed_ms",
-    "skfa_stream_synchronize", "skfa_spmv",
+    "skfa_stream_synchronize",
+    "skfa_spmv", "skfa_csr_condense", "skfa_csr_lower", "skfa_csr_transpose", "skfa_gather_values",
+    "skfa_gather_values_f32", "skfa_diag_inverse", "skfa_sys_dot", "skfa_pcg_solve",
+    "skfa_fsai_features", "skfa_fsai_factor", "skfa_fsai_kaporin", "skfa_mlp_pack", "skfa_mlp_forward",
+    "skfa_mlp_workspace_bytes", "skfa_mlp_backward", "skfa_mlp_adam",
 )
+
+FSAI_FEATURES = 16
+FSAI_PARAMS = 18817
+FSAI_FRAG = 34816
 
 
 FLAG_SYMMETRIC_TENSOR = 1
@@ -70,6 +78,25 @@ def _declare(lib):
         "skfa_event_elapsed_ms": (ctypes.c_int, [P, P, ctypes.POINTER(ctypes.c_float)]),
         "skfa_stream_synchronize": (ctypes.c_int, [P]),
         "skfa_spmv": (ctypes.c_int, [P, P, P, P, P, I64, I32, P]),
+        "skfa_csr_condense": (ctypes.c_int, [P, P, P, I64, P, P, P, P, P, ctypes.POINTER(I64),
+                                             ctypes.POINTER(I64), P]),
+        "skfa_csr_lower": (ctypes.c_int, [P, P, I64, P, P, P, ctypes.POINTER(I64), P]),
+        "skfa_csr_transpose": (ctypes.c_int, [P, P, I64, I64, P, P, P, P]),
+        "skfa_gather_values": (ctypes.c_int, [P, P, I64, P, P]),
+        "skfa_gather_values_f32": (ctypes.c_int, [P, P, I64, P, P]),
+        "skfa_diag_inverse": (ctypes.c_int, [P, P, P, I64, P, P]),
+        "skfa_sys_dot": (ctypes.c_int, [P, P, P, I32, P, P, P]),
+        "skfa_pcg_solve": (ctypes.c_int, [P, P, P, I64, P, I32, P, P, I32, P, P, P, P, P, P, P,
+                                          ctypes.c_double, I32, I32, I32, P, P, P, P, ctypes.POINTER(I32), P]),
+        "skfa_fsai_features": (ctypes.c_int, [P, P, P, I64, P, P, P, P, P, P, P]),
+        "skfa_fsai_factor": (ctypes.c_int, [P, P, P, P, I64, P, P, P, P]),
+        "skfa_fsai_kaporin": (ctypes.c_int, [P, P, P, I64, P, I32, P, P, P, P, ctypes.c_double, P, P, P, P, P]),
+        "skfa_mlp_pack": (ctypes.c_int, [P, P, P]),
+        "skfa_mlp_forward": (ctypes.c_int, [P, P, P, I64, P, P]),
+        "skfa_mlp_workspace_bytes": (ctypes.c_size_t, [I64]),
+        "skfa_mlp_backward": (ctypes.c_int, [P, P, P, P, I64, P, ctypes.c_size_t, P, P]),
+        "skfa_mlp_adam": (ctypes.c_int, [P, P, P, P, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_float, I32, ctypes.c_float, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
