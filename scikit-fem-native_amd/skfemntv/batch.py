@@ -116,7 +116,7 @@ class AnisotropicBatch:
         full = DeviceCSR(plan.indptr, plan.indices[:plan.nnz],
                          torch.arange(plan.nnz, dtype=torch.float64, device=dev), (plan.nrows, plan.nrows))
         keep_dev = torch.from_numpy(keep.astype(np.uint8)).to(dev)
-        cond, _, newid = full.condense(keep_dev)
+        cond, _ = full.condense(keep_dev)
         self.value_map = cond.data.to(torch.int64)
         self.free = torch.nonzero(keep_dev, as_tuple=True)[0]
         self.A = DeviceCSR(cond.indptr, cond.indices, torch.empty_like(cond.data), cond.shape)

@@ -71,7 +71,8 @@ class DeviceCSR:
 
     def condense(self, keep):
         """Rows and columns with keep != 0 (device uint8/bool, length n), in
-        order.  Returns (DeviceCSR, position map into self.data, newid)."""
+        order.  Returns (DeviceCSR, newid) with newid[i] the new index of
+        kept row i (n+1 entries, exclusive scan of keep)."""
         torch = _torch()
         L = _native.lib()
         n = self.shape[0]
@@ -246,7 +247,7 @@ def solve_dirichlet(matrix, rhs, boundary_dofs, *, preconditioner="jacobi", tol=
     keep = torch.ones(n, dtype=torch.uint8, device=dev)
     bd = torch.as_tensor(np.unique(np.asarray(boundary_dofs, dtype=np.int64)), device=dev)
     keep[bd] = 0
-    Af, _, newid = A.condense(keep)
+    Af, _ = A.condense(keep)
     b = rhs if isinstance(rhs, torch.Tensor) else torch.as_tensor(np.asarray(rhs, dtype=np.float64), device=dev)
     free = torch.nonzero(keep, as_tuple=True)[0]
     bf = b[free].contiguous()

@@ -92,7 +92,7 @@ def test_device_csr_ops_match_scipy(oracle_blocks):
     y = A.matvec(torch.from_numpy(x).cuda()).cpu().numpy()
     assert rel_frob(y, Ao @ x) <= 1e-14
     keep = rng.random(Ao.shape[0]) > 0.3
-    Ac, _, newid = A.condense(torch.from_numpy(keep.astype(np.uint8)).cuda())
+    Ac, newid = A.condense(torch.from_numpy(keep.astype(np.uint8)).cuda())
     Ar, _, _ = of.condense(Ao, np.zeros(Ao.shape[0]), keep)
     Ar.sort_indices()
     Ag = Ac.to_scipy()
