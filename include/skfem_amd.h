@@ -115,8 +115,10 @@ int skfa_assemble_bilinear(skfa_plan* plan, const double* xyz, int64_t npoints, 
 /* Linear form f.v + G:grad(v) into out (nrows FP64, overwritten).
  * Replaces LinearFormAssembler::assemble
  * (cpp/src/assembly/linear_form_assembler.cpp:41-137).  f: (C) constant
- * (f_per_qp 0) or (E,Q,C); G: (C,3) constant or (E,Q,C,3). */
-int skfa_assemble_linear(const skfa_plan* plan, const double* xyz, const int32_t* vconn, const int32_t* gconn,
+ * (f_per_qp 0) or (E,Q,C); G: (C,3) constant or (E,Q,C,3).  A constant f
+ * without G on affine elements uses a per-element |det J| buffer that the
+ * plan allocates on first use. */
+int skfa_assemble_linear(skfa_plan* plan, const double* xyz, const int32_t* vconn, const int32_t* gconn,
                          int32_t gnodes, const double* tables, int32_t nq, const double* f, int32_t f_per_qp,
                          const double* G, int32_t G_per_qp, double* out, void* stream);
 
@@ -193,6 +195,13 @@ int skfa_csr_lower(const int64_t* indptr, const int32_t* indices, int64_t nrows,
 int skfa_csr_transpose(const int64_t* ptr, const int32_t* idx, int64_t nrows, int64_t ncols, int64_t* tptr,
                        int32_t* tidx, int64_t* tsrc, void* stream);
 
+/* Transpose of the lower pattern (lptr, lidx) of a CSR whose pattern is
+ * symmetric (FEM matrices): tptr (nrows+1), tidx, tsrc as skfa_csr_transpose,
+ * without atomics or sorting.  Returns -1 if the pattern is not symmetric.
+ * Synchronises `stream`. */
+int skfa_csr_transpose_lower(const int64_t* indptr, const int32_t* indices, int64_t nrows, const int64_t* lptr,
+                             const int32_t* lidx, int64_t* tptr, int32_t* tidx, int64_t* tsrc, void* stream);
+
 /* out[k] = src[map[k]] (FP64 and FP32). */
 int skfa_gather_values(const double* src, const int64_t* map, int64_t n, double* out, void* stream);
 int skfa_gather_values_f32(const float* src, const int64_t* map, int64_t n, float* out, void* stream);
@@ -208,10 +217,11 @@ int skfa_sys_dot(const double* x, const double* y, const int64_t* sys_ptr, int32
 /* Batched preconditioned conjugate gradients, FP64, x0 = 0.  kind 0: Jacobi
  * (dinv); kind 1: z = G^T (G r) with G lower CSR (FP32 values) and its
  * transpose.  System s stops when ||r_s|| <= tol ||b_s|| (recursive
- * residual) or after max_iter.  Workspaces (device): work 6n doubles,
- * sys_work 5*nsys doubles, int_work nsys+1 int32.  iters (device, nsys)
+ * residual) or after max_iter; converged systems stop costing bandwidth
+ * (their row blocks return at once).  Workspaces (device): work 7n doubles,
+ * sys_work 6*nsys doubles, int_work nsys+1 int32.  iters (device, nsys)
  * receives per-system iteration counts, *total_iters (host) the number of
- * batch iterations run. */
+ * batch iterations run.  Reductions have a fixed order (reproducible). */
 int skfa_pcg_solve(const int64_t* indptr, const int32_t* indices, const double* vals, int64_t n,
                    const int64_t* sys_ptr, int32_t nsys, const double* b, double* x, int32_t kind, const double* dinv,
                    const int64_t* gptr, const int32_t* gidx, const float* gval, const int64_t* gtptr,

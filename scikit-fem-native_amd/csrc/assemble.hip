@@ -845,6 +845,42 @@ __global__ void k_assemble_linear(LinArgs A) {
     }
 }
 
+// Constant load on affine elements: out[node*C + c] = f_c sum over the
+// node's incidences (e,a) of |det J_e| sum_q w_q phi_a(q).
+__global__ void k_elem_absdet(const double4* __restrict__ xyz, const int32_t* __restrict__ vconn, int64_t E,
+                              double* __restrict__ out) {
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < E; e += int64_t(gridDim.x) * blockDim.x) {
+        double Jm[9];
+        Geo G;
+        affine_jacobian(xyz, vconn + e * 4, Jm);
+        adjugate(Jm, G);
+        out[e] = fabs(G.det);
+    }
+}
+
+__global__ void k_linear_const(const int64_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc, int64_t nnodes,
+                               int32_t C, const double* __restrict__ det, const double* __restrict__ tables,
+                               int32_t Q, int32_t n, int32_t gn, const double* __restrict__ f,
+                               double* __restrict__ out) {
+    __shared__ double wphi[16];
+    if (threadIdx.x < n) {
+        const Tables T = tables_at(tables, Q, n, gn);
+        double s = 0.;
+        for (int q = 0; q < Q; ++q) s += T.w[q] * T.phi[q * n + threadIdx.x];
+        wphi[threadIdx.x] = s;
+    }
+    __syncthreads();
+    for (int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; node < nnodes;
+         node += int64_t(gridDim.x) * blockDim.x) {
+        double acc = 0.;
+        for (int64_t k = inc_ptr[node]; k < inc_ptr[node + 1]; ++k) {
+            const int32_t packed = inc[k];
+            acc += det[packed >> 4] * wphi[packed & 15];
+        }
+        for (int c = 0; c < C; ++c) out[node * C + c] = acc * f[c];
+    }
+}
+
 // ------------------------------------------------------------------ geometry
 // Per (e,q): det, tolerance 64*eps*max|J|^3, condition number, dx, physical
 // point, and optionally the tabulated shape/gradients
@@ -1153,7 +1189,7 @@ int skfa_assemble_tet_n1(skfa_plan* P, const double* xyz, const int32_t* vconn, 
     return kOk;
 }
 
-int skfa_assemble_linear(const skfa_plan* P, const double* xyz, const int32_t* vconn, const int32_t* gconn,
+int skfa_assemble_linear(skfa_plan* P, const double* xyz, const int32_t* vconn, const int32_t* gconn,
                          int32_t gnodes, const double* tables, int32_t nq, const double* f, int32_t f_per_qp,
                          const double* G, int32_t G_per_qp, double* out, void* stream) {
     SKFA_REQUIRE(P != nullptr && out != nullptr, "null argument");
@@ -1180,6 +1216,16 @@ int skfa_assemble_linear(const skfa_plan* P, const double* xyz, const int32_t* v
     A.out = out;
     const int64_t total = P->nrows();
     unsigned grid = unsigned(std::min<int64_t>((total + 255) / 256, 65536));
+    if (!gconn && f && !f_per_qp && !G && P->n <= 16) {
+        if (P->elem_det.n < size_t(P->E))
+            if (int rc = P->elem_det.alloc(size_t(std::max<int64_t>(P->E, 1)))) return rc;
+        k_elem_absdet<<<unsigned(std::min<int64_t>((P->E + 255) / 256, 65536)), 256, 0, s>>>(A.xyz, vconn, P->E,
+                                                                                              P->elem_det.p);
+        k_linear_const<<<unsigned(std::min<int64_t>((P->nnodes + 255) / 256, 65536)), 256, 0, s>>>(
+            A.inc_ptr, A.inc, P->nnodes, A.C, P->elem_det.p, tables, nq, A.n, A.gn, f, out);
+        SKFA_LAUNCH_CHECK();
+        return kOk;
+    }
     if (gconn)
         k_assemble_linear<true><<<grid, 256, 0, s>>>(A);
     else

@@ -52,10 +52,8 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
-    // round to nearest even (inputs are finite)
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
+    // round to nearest even (gfx950 v_cvt_pk_bf16_f32)
+    return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
 }
 __host__ __device__ __forceinline__ int unit_of_slot(int kb2, int quad, int s) {
     // k-slot s (0..7) of lane quad `quad` in a 32-deep MFMA step kb2 maps to a
@@ -119,61 +117,75 @@ __global__ void k_rsqrt_diag(const int64_t* __restrict__ ip, const int32_t* __re
 //  4 rowsum(i)/A_ii   5 rowsum(j)/A_jj    6 len(i)/16        7 len(j)/16
 //  8 sum_{j<k<i} At_ik At_kj   9 sum_{k<j} At_ik At_jk   10 sum_k At_ik At_kj
 // 11 log(A_ii/A_jj)/2 12-14 (x_j - x_i)/h_i  15 |x_j - x_i|/h_i
-__global__ void k_features(const int64_t* __restrict__ ip, const int32_t* __restrict__ ix,
-                           const double* __restrict__ av, const double* __restrict__ rs,
-                           const double* __restrict__ stats, const double* __restrict__ xyz,
-                           const int64_t* __restrict__ lptr, const int32_t* __restrict__ lidx, int64_t n,
-                           float* __restrict__ feat) {
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+__device__ __forceinline__ int64_t row_of(const int64_t* __restrict__ lptr, int64_t n, int64_t q) {
+    // largest i with lptr[i] <= q
+    int64_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (lptr[mid] <= q) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_features(const int64_t* __restrict__ ip, const int32_t* __restrict__ ix,
+                                                  const double* __restrict__ av, const double* __restrict__ rs,
+                                                  const double* __restrict__ stats, const double* __restrict__ xyz,
+                                                  const int64_t* __restrict__ lptr, const int32_t* __restrict__ lidx,
+                                                  int64_t n, int64_t nnz_l, float* __restrict__ feat) {
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < nnz_l; q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t i = row_of(lptr, n, q);
+        const int32_t j = lidx[q];
         const int64_t bi = ip[i], ei = ip[i + 1];
-        for (int64_t q = lptr[i]; q < lptr[i + 1]; ++q) {
-            const int32_t j = lidx[q];
-            const int64_t bj = ip[j], ej = ip[j + 1];
-            double aij = 0., t_mid = 0., t_lo = 0., t_all = 0.;
-            // sorted merge of rows i and j
-            int64_t a = bi, b = bj;
-            while (a < ei && b < ej) {
-                const int32_t ca = ix[a], cb = ix[b];
-                if (ca == cb) {
-                    const double p = av[a] * rs[i] * rs[ca] * av[b] * rs[j] * rs[cb];
-                    t_all += p;
-                    if (ca < j) t_lo += p;
-                    else if (ca > j && ca < i) t_mid += p;
-                    ++a;
-                    ++b;
-                } else if (ca < cb) {
-                    ++a;
-                } else {
-                    ++b;
-                }
-            }
-            for (int64_t k = bi; k < ei; ++k)
-                if (ix[k] == j) aij = av[k] * rs[i] * rs[j];
-            float* f = feat + q * kF;
-            f[0] = float(aij);
-            f[1] = (i == j) ? 1.f : 0.f;
-            f[2] = float(stats[4 * i + 0]);
-            f[3] = float(stats[4 * j + 0]);
-            f[4] = float(stats[4 * i + 1]);
-            f[5] = float(stats[4 * j + 1]);
-            f[6] = float(stats[4 * i + 2]);
-            f[7] = float(stats[4 * j + 2]);
-            f[8] = float(t_mid);
-            f[9] = float(t_lo);
-            f[10] = float(t_all);
-            f[11] = float(0.5 * log(rs[j] * rs[j] / (rs[i] * rs[i])));
-            if (xyz && i != j) {
-                const double h = stats[4 * i + 3];
-                const double dx = (xyz[3 * j] - xyz[3 * i]) / h, dy = (xyz[3 * j + 1] - xyz[3 * i + 1]) / h,
-                             dz = (xyz[3 * j + 2] - xyz[3 * i + 2]) / h;
-                f[12] = float(dx);
-                f[13] = float(dy);
-                f[14] = float(dz);
-                f[15] = float(sqrt(dx * dx + dy * dy + dz * dz));
+        const int64_t bj = ip[j], ej = ip[j + 1];
+        const double rsi = rs[i], rsj = rs[j];
+        double aij = 0., t_mid = 0., t_lo = 0., t_all = 0.;
+        // sorted merge of rows i and j
+        int64_t a = bi, b = bj;
+        while (a < ei && b < ej) {
+            const int32_t ca = ix[a], cb = ix[b];
+            if (ca == cb) {
+                const double rk = rs[ca];
+                const double p = (av[a] * rsi * rk) * (av[b] * rsj * rk);
+                t_all += p;
+                if (ca < j) t_lo += p;
+                else if (ca > j && ca < i) t_mid += p;
+                if (ca == j) aij = av[a] * rsi * rsj;
+                ++a;
+                ++b;
+            } else if (ca < cb) {
+                ++a;
             } else {
-                f[12] = f[13] = f[14] = f[15] = 0.f;
+                ++b;
             }
         }
+        float f[kF];
+        f[0] = float(aij);
+        f[1] = (i == j) ? 1.f : 0.f;
+        f[2] = float(stats[4 * i + 0]);
+        f[3] = float(stats[4 * j + 0]);
+        f[4] = float(stats[4 * i + 1]);
+        f[5] = float(stats[4 * j + 1]);
+        f[6] = float(stats[4 * i + 2]);
+        f[7] = float(stats[4 * j + 2]);
+        f[8] = float(t_mid);
+        f[9] = float(t_lo);
+        f[10] = float(t_all);
+        f[11] = float(0.5 * log(rsj * rsj / (rsi * rsi)));
+        if (xyz && i != j) {
+            const double h = stats[4 * i + 3];
+            const double dx = (xyz[3 * j] - xyz[3 * i]) / h, dy = (xyz[3 * j + 1] - xyz[3 * i + 1]) / h,
+                         dz = (xyz[3 * j + 2] - xyz[3 * i + 2]) / h;
+            f[12] = float(dx);
+            f[13] = float(dy);
+            f[14] = float(dz);
+            f[15] = float(sqrt(dx * dx + dy * dy + dz * dz));
+        } else {
+            f[12] = f[13] = f[14] = f[15] = 0.f;
+        }
+        float4* out = reinterpret_cast<float4*>(feat + q * kF);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[v] = make_float4(f[4 * v], f[4 * v + 1], f[4 * v + 2], f[4 * v + 3]);
     }
 }
 
@@ -315,16 +327,11 @@ struct TileFwd {
 };
 
 __device__ __forceinline__ uint4 pack8(const v4f& a, const v4f& b, bool relu) {
-    float f[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
-    uint16_t h[8];
+    const float f[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+    v8bf h;
 #pragma unroll
-    for (int s = 0; s < 8; ++s) h[s] = f2bf(relu ? fmaxf(f[s], 0.f) : f[s]);
-    uint4 u;
-    u.x = uint32_t(h[0]) | (uint32_t(h[1]) << 16);
-    u.y = uint32_t(h[2]) | (uint32_t(h[3]) << 16);
-    u.z = uint32_t(h[4]) | (uint32_t(h[5]) << 16);
-    u.w = uint32_t(h[6]) | (uint32_t(h[7]) << 16);
-    return u;
+    for (int s = 0; s < 8; ++s) h[s] = static_cast<__bf16>(relu ? fmaxf(f[s], 0.f) : f[s]);
+    return __builtin_bit_cast(uint4, h);
 }
 
 __device__ __forceinline__ void tile_forward(const uint2* __restrict__ w1f, const uint4* __restrict__ w2f,
@@ -640,7 +647,12 @@ int skfa_fsai_features(const int64_t* indptr, const int32_t* indices, const doub
     const unsigned g = grid_of(n, 256);
     k_rsqrt_diag<<<g, 256, 0, s>>>(indptr, indices, vals, n, rsqrt_diag);
     k_row_stats<<<g, 256, 0, s>>>(indptr, indices, vals, rsqrt_diag, xyz, n, row_stats);
-    k_features<<<g, 256, 0, s>>>(indptr, indices, vals, rsqrt_diag, row_stats, xyz, lptr, lidx, n, features);
+    int64_t nnz_l = 0;
+    SKFA_HIP(hipMemcpyAsync(&nnz_l, lptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    if (nnz_l)
+        k_features<<<grid_of(nnz_l, 256), 256, 0, s>>>(indptr, indices, vals, rsqrt_diag, row_stats, xyz, lptr, lidx,
+                                                      n, nnz_l, features);
     SKFA_LAUNCH_CHECK();
     return kOk;
 }

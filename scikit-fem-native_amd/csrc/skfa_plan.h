@@ -50,6 +50,8 @@ struct skfa_plan {
     // per-element payload of the element-constant fast path (allocated on
     // first use, E * stride doubles)
     skfa::DevBuf<double> payload;
+    // |det J| per element for the constant-load linear form (first use)
+    skfa::DevBuf<double> elem_det;
 
     // launch tuning (skfa_plan_set_tuning): XCD chunk of waves (0 = plain
     // order) and LDS accumulation by ds_add (1) or read-add-write (0)

@@ -119,6 +119,44 @@ __global__ void k_lower_fill(const int64_t* __restrict__ indptr, const int32_t* 
     }
 }
 
+// Transpose of the lower pattern of a matrix whose pattern is symmetric:
+// row i of L^T holds the columns j >= i of A's row i, and entry (i, j) of
+// L^T sits in row j of L, found by binary search.  No atomics, no sort.
+__global__ void k_upper_count(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, int64_t nrows,
+                              int32_t* __restrict__ count) {
+    for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nrows; r += int64_t(gridDim.x) * blockDim.x) {
+        int32_t c = 0;
+        for (int64_t k = indptr[r]; k < indptr[r + 1]; ++k) c += indices[k] >= r ? 1 : 0;
+        count[r] = c;
+    }
+}
+
+__global__ void k_upper_fill(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, int64_t nrows,
+                             const int64_t* __restrict__ lptr, const int32_t* __restrict__ lidx,
+                             const int64_t* __restrict__ tptr, int32_t* __restrict__ tidx, int64_t* __restrict__ tsrc,
+                             int32_t* __restrict__ missing) {
+    for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nrows; r += int64_t(gridDim.x) * blockDim.x) {
+        int64_t o = tptr[r];
+        for (int64_t k = indptr[r]; k < indptr[r + 1]; ++k) {
+            const int32_t j = indices[k];
+            if (j < r) continue;
+            int64_t lo = lptr[j], hi = lptr[j + 1];
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (lidx[mid] < r) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo == lptr[j + 1] || lidx[lo] != r) {
+                atomicAdd(missing, 1);
+                lo = lptr[j];
+            }
+            tidx[o] = j;
+            tsrc[o] = lo;
+            ++o;
+        }
+    }
+}
+
 __global__ void k_col_count(const int32_t* __restrict__ idx, int64_t nnz, int32_t* __restrict__ count) {
     for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < nnz; k += int64_t(gridDim.x) * blockDim.x)
         atomicAdd(&count[idx[k]], 1);
@@ -197,14 +235,6 @@ __global__ void __launch_bounds__(kRedThreads) k_sys_dot2(const double* __restri
     }
 }
 
-__global__ void k_row_system(const int64_t* __restrict__ sys_ptr, int32_t nsys, int32_t* __restrict__ row_sys) {
-    const int s = blockIdx.y;
-    if (s >= nsys) return;
-    for (int64_t i = sys_ptr[s] + int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < sys_ptr[s + 1];
-         i += int64_t(gridDim.x) * blockDim.x)
-        row_sys[i] = s;
-}
-
 __global__ void k_diag_inv(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
                            const double* __restrict__ vals, int64_t n, double* __restrict__ dinv) {
     for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < n; r += int64_t(gridDim.x) * blockDim.x) {
@@ -215,50 +245,252 @@ __global__ void k_diag_inv(const int64_t* __restrict__ indptr, const int32_t* __
     }
 }
 
-__global__ void k_pcg_init(const double* __restrict__ bb, int32_t nsys, int32_t* __restrict__ active) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nsys) active[s] = bb[s] > 0. ? 1 : 0;
+// ---- fused batched PCG ----------------------------------------------------
+// Rows are grouped in blocks of at most kPcgThreads consecutive rows that never
+// straddle two systems (block table: system, first row, end row).  A sparse
+// product in a block first forms the products of the block's contiguous
+// nonzero range with coalesced loads into LDS, then each thread adds its
+// row's products in order ("stream" layout, good for short FEM rows).  Each
+// heavy kernel returns at once for blocks of converged systems and writes one
+// partial sum per block; k_sys_reduce adds a system's partials in block
+// order, so every result is reproducible bit for bit.
+constexpr int kPcgThreads = 256;
+constexpr int64_t kStreamCap = 6144;   // max nonzeros of one block (48 KiB of LDS)
+
+struct BlockTable {
+    const int32_t* sys;
+    const int64_t* row0;
+    const int64_t* row1;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
+// XCD runs a contiguous range of row blocks and the neighbouring entries a
+// block gathers stay in that XCD's L2 (a bijection for any block count).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int x = b & 7, idx = b >> 3;
+    const int q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + idx;
 }
 
-// x += a p, r -= a q for systems still iterating (a = rz / pq).
-__global__ void k_pcg_update(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p,
-                             const double* __restrict__ q, const double* __restrict__ rz, const double* __restrict__ pq,
-                             const int32_t* __restrict__ active, const int32_t* __restrict__ row_sys, int64_t n) {
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-        const int s = row_sys[i];
-        if (!active[s] || !(pq[s] != 0.)) continue;
-        const double a = rz[s] / pq[s];
-        x[i] = fma(a, p[i], x[i]);
-        r[i] = fma(-a, q[i], r[i]);
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    // sum over the block, fixed order; result valid in thread 0
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < int(blockDim.x >> 6); ++k) t += sh[k];
+    return t;
+}
+
+// Row sums of A x over rows [r0, r1) with x given by `xv(j)`; thread t gets row r0 + t.
+template <typename V, typename XV>
+__device__ __forceinline__ double stream_rows(const int64_t* __restrict__ ip, const int32_t* __restrict__ ix,
+                                              const V* __restrict__ av, int64_t r0, int64_t r1, double* prod,
+                                              XV xv) {
+    const int64_t k0 = ip[r0], k1 = ip[r1];
+    const int64_t row = r0 + threadIdx.x;
+    const bool has_row = row < r1;
+    // the row bounds are loaded early so that they overlap the product phase
+    const int64_t rb = has_row ? ip[row] : 0, re = has_row ? ip[row + 1] : 0;
+    const int cnt = int(k1 - k0);
+    const int nt = int(blockDim.x);
+    // eight independent loads per thread in flight
+    for (int base = int(threadIdx.x); base < cnt; base += 8 * nt) {
+        int32_t j[8];
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = base + u * nt;
+            j[u] = k < cnt ? ix[k0 + k] : 0;
+            v[u] = k < cnt ? double(av[k0 + k]) : 0.;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = base + u * nt;
+            if (k < cnt) prod[k] = v[u] * xv(j[u]);
+        }
+    }
+    __syncthreads();
+    double acc = 0.;
+    if (has_row)
+        for (int64_t k = rb - k0; k < re - k0; ++k) acc += prod[k];
+    return acc;
+}
+
+// p_new = z + beta p_old (beta = 0 on the first step), q = A p_new,
+// partial[blk] = p_new . q
+__global__ void __launch_bounds__(kPcgThreads) k_pcg_direction_spmv(
+    BlockTable bt, const int64_t* __restrict__ ip, const int32_t* __restrict__ ix, const double* __restrict__ av,
+    const double* __restrict__ z, const double* __restrict__ p_old, double* __restrict__ p_new,
+    double* __restrict__ q, const double* __restrict__ rz, const double* __restrict__ rz_old,
+    const int32_t* __restrict__ active, int first, double* __restrict__ partial) {
+    extern __shared__ double prod[];
+    __shared__ double sh[kPcgThreads / 64];
+    const int blk = xcd_block(int(blockIdx.x), int(gridDim.x));
+    const int s = bt.sys[blk];
+    if (!active[s]) return;
+    const double beta = first ? 0. : rz[s] / rz_old[s];
+    const int64_t r0 = bt.row0[blk], r1 = bt.row1[blk];
+    double acc;
+    if (first) acc = stream_rows(ip, ix, av, r0, r1, prod, [&](int32_t j) { return z[j]; });
+    else acc = stream_rows(ip, ix, av, r0, r1, prod, [&](int32_t j) { return fma(beta, p_old[j], z[j]); });
+    const int64_t row = r0 + threadIdx.x;
+    double contrib = 0.;
+    if (row < r1) {
+        const double pn = first ? z[row] : fma(beta, p_old[row], z[row]);
+        p_new[row] = pn;
+        q[row] = acc;
+        contrib = pn * acc;
+    }
+    const double t = block_sum(contrib, sh);
+    if (threadIdx.x == 0) partial[blk] = t;
+}
+
+// out[s] = sum of partial over the blocks of system s (active systems only)
+__global__ void __launch_bounds__(256) k_sys_reduce(const int64_t* __restrict__ bptr, const double* __restrict__ p0,
+                                                    const double* __restrict__ p1, const int32_t* __restrict__ active,
+                                                    double* __restrict__ out0, double* __restrict__ out1) {
+    __shared__ double s0[256], s1[256];
+    const int s = blockIdx.x;
+    if (!active[s]) return;
+    double a = 0., b = 0.;
+    for (int64_t k = bptr[s] + threadIdx.x; k < bptr[s + 1]; k += 256) {
+        a += p0[k];
+        if (p1) b += p1[k];
+    }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = b;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            s0[threadIdx.x] += s0[threadIdx.x + off];
+            s1[threadIdx.x] += s1[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out0[s] = s0[0];
+        if (out1) out1[s] = s1[0];
     }
 }
 
-__global__ void k_jacobi(const double* __restrict__ r, const double* __restrict__ dinv, double* __restrict__ z, int64_t n) {
+// Jacobi step: a = rz/pq; x += a p; r -= a q; z = dinv r;
+// partials r.z and r.r.  One thread per row.
+__global__ void __launch_bounds__(kPcgThreads) k_pcg_update_jacobi(
+    BlockTable bt, double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+    const double* __restrict__ p, const double* __restrict__ q, const double* __restrict__ dinv,
+    const double* __restrict__ rz, const double* __restrict__ pq, const int32_t* __restrict__ active,
+    double* __restrict__ part_rz, double* __restrict__ part_rr) {
+    __shared__ double sh0[kPcgThreads / 64], sh1[kPcgThreads / 64];
+    const int blk = xcd_block(int(blockIdx.x), int(gridDim.x));
+    const int s = bt.sys[blk];
+    if (!active[s]) return;
+    const double a = pq[s] != 0. ? rz[s] / pq[s] : 0.;
+    const int64_t row = bt.row0[blk] + threadIdx.x;
+    double c_rz = 0., c_rr = 0.;
+    if (row < bt.row1[blk]) {
+        x[row] = fma(a, p[row], x[row]);
+        const double rn = fma(-a, q[row], r[row]);
+        r[row] = rn;
+        const double zn = rn * dinv[row];
+        z[row] = zn;
+        c_rz = rn * zn;
+        c_rr = rn * rn;
+    }
+    const double t0 = block_sum(c_rz, sh0);
+    const double t1 = block_sum(c_rr, sh1);
+    if (threadIdx.x == 0) {
+        part_rz[blk] = t0;
+        part_rr[blk] = t1;
+    }
+}
+
+// Factor step, first half: a = rz/pq; r_new = r_old - a q; x += a p;
+// t = G r_new (neighbours' r_new formed on the fly from r_old and q);
+// partial r.r.
+__global__ void __launch_bounds__(kPcgThreads) k_pcg_update_g(
+    BlockTable bt, const int64_t* __restrict__ gp, const int32_t* __restrict__ gi, const float* __restrict__ gv,
+    double* __restrict__ x, const double* __restrict__ r_old, double* __restrict__ r_new, double* __restrict__ t,
+    const double* __restrict__ p, const double* __restrict__ q, const double* __restrict__ rz,
+    const double* __restrict__ pq, const int32_t* __restrict__ active, double* __restrict__ part_rr) {
+    extern __shared__ double prod[];
+    __shared__ double sh[kPcgThreads / 64];
+    const int blk = xcd_block(int(blockIdx.x), int(gridDim.x));
+    const int s = bt.sys[blk];
+    if (!active[s]) return;
+    const double a = pq[s] != 0. ? rz[s] / pq[s] : 0.;
+    const int64_t r0 = bt.row0[blk], r1 = bt.row1[blk];
+    const double acc = stream_rows(gp, gi, gv, r0, r1, prod, [&](int32_t j) { return fma(-a, q[j], r_old[j]); });
+    const int64_t row = r0 + threadIdx.x;
+    double c = 0.;
+    if (row < r1) {
+        const double rn = fma(-a, q[row], r_old[row]);
+        r_new[row] = rn;
+        x[row] = fma(a, p[row], x[row]);
+        t[row] = acc;
+        c = rn * rn;
+    }
+    const double tot = block_sum(c, sh);
+    if (threadIdx.x == 0) part_rr[blk] = tot;
+}
+
+// Factor step, second half: z = G^T t; partial r.z.
+__global__ void __launch_bounds__(kPcgThreads) k_pcg_apply_gt(
+    BlockTable bt, const int64_t* __restrict__ tp, const int32_t* __restrict__ ti, const float* __restrict__ tv,
+    const double* __restrict__ t, const double* __restrict__ r, double* __restrict__ z,
+    const int32_t* __restrict__ active, double* __restrict__ part_rz) {
+    extern __shared__ double prod[];
+    __shared__ double sh[kPcgThreads / 64];
+    const int blk = xcd_block(int(blockIdx.x), int(gridDim.x));
+    const int s = bt.sys[blk];
+    if (!active[s]) return;
+    const int64_t r0 = bt.row0[blk], r1 = bt.row1[blk];
+    const double acc = stream_rows(tp, ti, tv, r0, r1, prod, [&](int32_t j) { return t[j]; });
+    const int64_t row = r0 + threadIdx.x;
+    double c = 0.;
+    if (row < r1) {
+        z[row] = acc;
+        c = r[row] * acc;
+    }
+    const double tot = block_sum(c, sh);
+    if (threadIdx.x == 0) part_rz[blk] = tot;
+}
+
+// Largest nonzero count of one block of rows (for the LDS size).
+__global__ void k_block_nnz_max(BlockTable bt, int64_t nblk, const int64_t* __restrict__ ip,
+                                unsigned long long* __restrict__ out) {
+    for (int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; b < nblk; b += int64_t(gridDim.x) * blockDim.x)
+        atomicMax(out, static_cast<unsigned long long>(ip[bt.row1[b]] - ip[bt.row0[b]]));
+}
+
+// After a step: rz_old <- rz, rz <- rz_new, iteration count, convergence.
+__global__ void k_jacobi_init(const double* __restrict__ r, const double* __restrict__ dinv, double* __restrict__ z,
+                              int64_t n) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
         z[i] = r[i] * dinv[i];
 }
 
-// p = z + (rz_new / rz_old) p for active systems
-__global__ void k_pcg_direction(double* __restrict__ p, const double* __restrict__ z, const double* __restrict__ rz_new,
-                                const double* __restrict__ rz_old, const int32_t* __restrict__ active,
-                                const int32_t* __restrict__ row_sys, int64_t n, int first) {
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-        const int s = row_sys[i];
-        if (!active[s]) continue;
-        p[i] = first ? z[i] : fma(rz_new[s] / rz_old[s], p[i], z[i]);
-    }
+__global__ void k_pcg_advance(const double* __restrict__ rz_new, const double* __restrict__ rr,
+                              const double* __restrict__ bb, double tol2, int32_t nsys, double* __restrict__ rz,
+                              double* __restrict__ rz_old, int32_t* __restrict__ active, int32_t* __restrict__ iters,
+                              int32_t* __restrict__ nactive) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys || !active[s]) return;
+    iters[s] += 1;
+    rz_old[s] = rz[s];
+    rz[s] = rz_new[s];
+    if (!(rr[s] > tol2 * bb[s])) active[s] = 0;
+    else atomicAdd(nactive, 1);
 }
 
-// convergence: active[s] = ||r_s||^2 > tol^2 ||b_s||^2; iteration counters
-__global__ void k_pcg_check(const double* __restrict__ rr, const double* __restrict__ bb, double tol2, int32_t nsys,
-                            int32_t* __restrict__ active, int32_t* __restrict__ iters, int32_t* __restrict__ nactive) {
+__global__ void k_pcg_start(const double* __restrict__ bb, int32_t nsys, int32_t* __restrict__ active,
+                            int32_t* __restrict__ iters) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
-    if (active[s]) {
-        iters[s] += 1;
-        if (!(rr[s] > tol2 * bb[s])) active[s] = 0;
-    }
-    if (active[s]) atomicAdd(nactive, 1);
+    active[s] = bb[s] > 0. ? 1 : 0;
+    iters[s] = 0;
 }
 
 }  // namespace skfa
@@ -342,6 +574,26 @@ int skfa_csr_transpose(const int64_t* ptr, const int32_t* idx, int64_t nrows, in
     return kOk;
 }
 
+int skfa_csr_transpose_lower(const int64_t* indptr, const int32_t* indices, int64_t nrows, const int64_t* lptr,
+                             const int32_t* lidx, int64_t* tptr, int32_t* tidx, int64_t* tsrc, void* stream) {
+    SKFA_REQUIRE(indptr && indices && lptr && lidx && tptr && tidx && tsrc, "null argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevBuf<int32_t> cnt, missing;
+    if (int rc = cnt.alloc(size_t(nrows) + 1)) return rc;
+    if (int rc = missing.alloc(1)) return rc;
+    SKFA_HIP(hipMemsetAsync(missing.p, 0, sizeof(int32_t), s));
+    k_upper_count<<<grid_of(nrows, 256), 256, 0, s>>>(indptr, indices, nrows, cnt.p);
+    SKFA_LAUNCH_CHECK();
+    if (int rc = exclusive_scan_i32_to_i64(cnt.p, tptr, nrows, s)) return rc;
+    k_upper_fill<<<grid_of(nrows, 256), 256, 0, s>>>(indptr, indices, nrows, lptr, lidx, tptr, tidx, tsrc, missing.p);
+    SKFA_LAUNCH_CHECK();
+    int32_t h_missing = 0;
+    SKFA_HIP(hipMemcpyAsync(&h_missing, missing.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    SKFA_REQUIRE(h_missing == 0, "matrix pattern is not symmetric");
+    return kOk;
+}
+
 int skfa_gather_values(const double* src, const int64_t* map, int64_t n, double* out, void* stream) {
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (n == 0) return kOk;
@@ -377,14 +629,14 @@ int skfa_sys_dot(const double* x, const double* y, const int64_t* sys_ptr, int32
     return kOk;
 }
 
-/* Batched PCG on a block-diagonal SPD system (all FP64).  Preconditioner:
- * kind 0 = Jacobi (dinv), kind 1 = factorised approximate inverse
- * z = G^T (G r) with G (lower CSR, FP32 values, applied in FP64) and
- * GT = G^T (CSR).  x is overwritten
- * (start from zero).  Stops each system at ||r||/||b|| <= tol (recursive
- * residual) or at max_iter; iters[s] receives the iteration counts.  The
- * convergence flags are read back every `check_every` iterations.
- * work: 6*n doubles; sys_work: 8*nsys doubles + 3*nsys+1 ints (device). */
+/* Batched PCG on a block-diagonal SPD system (all FP64 arithmetic).
+ * kind 0 = Jacobi (dinv); kind 1 = factorised approximate inverse
+ * z = G^T (G r) with G (lower CSR, FP32 values) and GT = G^T (CSR).
+ * x is overwritten (x0 = 0).  System s stops when ||r|| <= tol ||b||
+ * (recursive residual) or after max_iter; converged systems stop costing
+ * bandwidth at once (their row blocks return early).  The host reads the
+ * number of active systems every `check_every` iterations.
+ * work: 7n doubles; sys_work: 8*nsys doubles; int_work: nsys+1 int32. */
 int skfa_pcg_solve(const int64_t* indptr, const int32_t* indices, const double* vals, int64_t n,
                    const int64_t* sys_ptr, int32_t nsys, const double* b, double* x, int32_t kind, const double* dinv,
                    const int64_t* gptr, const int32_t* gidx, const float* gval, const int64_t* gtptr,
@@ -395,58 +647,146 @@ int skfa_pcg_solve(const int64_t* indptr, const int32_t* indices, const double* 
     SKFA_REQUIRE(kind == 0 ? dinv != nullptr : (gptr && gidx && gval && gtptr && gtidx && gtval),
                  "preconditioner arrays are null");
     SKFA_REQUIRE(check_every >= 1, "check_every must be positive");
+    SKFA_REQUIRE(nsys >= 1, "at least one system");
+    (void)spmv_lanes;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    double* r = work;
-    double* z = work + n;
-    double* p = work + 2 * n;
-    double* q = work + 3 * n;
-    double* t = work + 4 * n;
+    // block table: blocks of `rows_per_block` consecutive rows inside one
+    // system, halved until every block's nonzeros fit the LDS staging area
+    std::vector<int64_t> hsys(size_t(nsys) + 1);
+    SKFA_HIP(hipMemcpyAsync(hsys.data(), sys_ptr, sizeof(int64_t) * (nsys + 1), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    SKFA_REQUIRE(hsys[0] == 0 && hsys[nsys] == n, "sys_ptr must start at 0 and end at n");
+    for (int32_t q = 0; q < nsys; ++q) SKFA_REQUIRE(hsys[q + 1] >= hsys[q], "sys_ptr must be non-decreasing");
+    std::vector<int32_t> bsys;
+    std::vector<int64_t> br0, br1, bptr(size_t(nsys) + 1, 0);
+    DevBuf<int32_t> d_bsys;
+    DevBuf<int64_t> d_br0, d_br1, d_bptr;
+    DevBuf<unsigned long long> d_max;
+    if (int rc = d_max.alloc(3)) return rc;
+    int64_t nblk = 0;
+    int64_t max_block_nnz = 0;
+    unsigned long long hmax[3] = {0, 0, 0};
+    // half as many rows as threads: each thread forms about one row's worth
+    // of products (one round of loads in flight) in the stream kernels
+    for (int rows_per_block = kPcgThreads / 2;; rows_per_block /= 2) {
+        bsys.clear();
+        br0.clear();
+        br1.clear();
+        for (int32_t q = 0; q < nsys; ++q) {
+            for (int64_t r = hsys[q]; r < hsys[q + 1]; r += rows_per_block) {
+                bsys.push_back(q);
+                br0.push_back(r);
+                br1.push_back(std::min<int64_t>(r + rows_per_block, hsys[q + 1]));
+            }
+            bptr[q + 1] = int64_t(bsys.size());
+        }
+        nblk = int64_t(bsys.size());
+        SKFA_REQUIRE(nblk < (int64_t(1) << 31), "too many row blocks");
+        if (int rc = d_bsys.alloc(size_t(std::max<int64_t>(nblk, 1)))) return rc;
+        if (int rc = d_br0.alloc(size_t(std::max<int64_t>(nblk, 1)))) return rc;
+        if (int rc = d_br1.alloc(size_t(std::max<int64_t>(nblk, 1)))) return rc;
+        if (nblk) {
+            SKFA_HIP(hipMemcpyAsync(d_bsys.p, bsys.data(), sizeof(int32_t) * nblk, hipMemcpyHostToDevice, s));
+            SKFA_HIP(hipMemcpyAsync(d_br0.p, br0.data(), sizeof(int64_t) * nblk, hipMemcpyHostToDevice, s));
+            SKFA_HIP(hipMemcpyAsync(d_br1.p, br1.data(), sizeof(int64_t) * nblk, hipMemcpyHostToDevice, s));
+        }
+        SKFA_HIP(hipMemsetAsync(d_max.p, 0, 3 * sizeof(unsigned long long), s));
+        const BlockTable probe{d_bsys.p, d_br0.p, d_br1.p};
+        if (nblk) {
+            k_block_nnz_max<<<grid_of(nblk, 256), 256, 0, s>>>(probe, nblk, indptr, d_max.p);
+            if (kind != 0) {
+                k_block_nnz_max<<<grid_of(nblk, 256), 256, 0, s>>>(probe, nblk, gptr, d_max.p + 1);
+                k_block_nnz_max<<<grid_of(nblk, 256), 256, 0, s>>>(probe, nblk, gtptr, d_max.p + 2);
+            }
+            SKFA_LAUNCH_CHECK();
+        }
+        SKFA_HIP(hipMemcpyAsync(hmax, d_max.p, sizeof(hmax), hipMemcpyDeviceToHost, s));
+        SKFA_HIP(hipStreamSynchronize(s));
+        max_block_nnz = int64_t(std::max(hmax[0], std::max(hmax[1], hmax[2])));
+        if (max_block_nnz <= kStreamCap) break;
+        if (rows_per_block == 1) {
+            set_error("a matrix row has more nonzeros than the PCG staging area holds");
+            return kCapacity;
+        }
+    }
+    // LDS staging per kernel, sized by its own matrix's largest block
+    const size_t lds_bytes = size_t(std::max<unsigned long long>(hmax[0], 1)) * sizeof(double);
+    int rows_per_block_used = 1;
+    for (int64_t b = 0; b < nblk; ++b)
+        rows_per_block_used = std::max<int>(rows_per_block_used, int(br1[size_t(b)] - br0[size_t(b)]));
+    const int upd_threads = std::max(64, (rows_per_block_used + 63) / 64 * 64);
+    const size_t lds_g = size_t(std::max<unsigned long long>(hmax[1], 1)) * sizeof(double);
+    const size_t lds_gt = size_t(std::max<unsigned long long>(hmax[2], 1)) * sizeof(double);
+    DevBuf<double> parts;
+    if (int rc = d_bptr.alloc(size_t(nsys) + 1)) return rc;
+    if (int rc = parts.alloc(size_t(std::max<int64_t>(nblk, 1)) * 3)) return rc;
+    SKFA_HIP(hipMemcpyAsync(d_bptr.p, bptr.data(), sizeof(int64_t) * (nsys + 1), hipMemcpyHostToDevice, s));
+    const BlockTable bt{d_bsys.p, d_br0.p, d_br1.p};
+    double* part_a = parts.p;
+    double* part_b = parts.p + nblk;
+    double* part_c = parts.p + 2 * nblk;
+
+    double* r0 = work;
+    double* r1 = work + n;
+    double* z = work + 2 * n;
+    double* p0 = work + 3 * n;
+    double* p1 = work + 4 * n;
+    double* q = work + 5 * n;
+    double* t = work + 6 * n;
     double* rz = sys_work;
-    double* rz_new = sys_work + nsys;
-    double* pq = sys_work + 2 * nsys;
-    double* rr = sys_work + 3 * nsys;
-    double* bb = sys_work + 4 * nsys;
-    int32_t* row_sys = reinterpret_cast<int32_t*>(work + 5 * n);
+    double* rz_old = sys_work + nsys;
+    double* rz_new = sys_work + 2 * nsys;
+    double* pq = sys_work + 3 * nsys;
+    double* rr = sys_work + 4 * nsys;
+    double* bb = sys_work + 5 * nsys;
     int32_t* active = int_work;
     int32_t* nactive = int_work + nsys;
     const unsigned g = grid_of(n, 256);
+    const unsigned sg = unsigned((nsys + 255) / 256);
     const double tol2 = tol * tol;
-    auto precond = [&](const double* rin, double* zout) -> int {
-        if (kind == 0) {
-            k_jacobi<<<g, 256, 0, s>>>(rin, dinv, zout, n);
-            SKFA_LAUNCH_CHECK();
-            return kOk;
-        }
-        if (int rc = spmv_launch(gptr, gidx, gval, rin, t, n, 8, s)) return rc;
-        return spmv_launch(gtptr, gtidx, gtval, t, zout, n, 8, s);
-    };
-    SKFA_REQUIRE(nsys <= 65535, "at most 65535 systems per batch");
-    SKFA_HIP(hipMemsetAsync(iters, 0, sizeof(int32_t) * nsys, s));
-    k_row_system<<<dim3(64, unsigned(nsys)), 256, 0, s>>>(sys_ptr, nsys, row_sys);
-    SKFA_LAUNCH_CHECK();
+
+    // start: x = 0, r = b, z = M r, rz = r.z, bb = b.b
     SKFA_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
-    SKFA_HIP(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    SKFA_HIP(hipMemcpyAsync(r0, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     k_sys_dot2<<<nsys, kRedThreads, 0, s>>>(b, b, sys_ptr, bb, nullptr);
-    // systems with b = 0 are solved by x = 0 and never iterate
-    k_pcg_init<<<(nsys + 255) / 256, 256, 0, s>>>(bb, nsys, active);
-    if (int rc = precond(r, z)) return rc;
-    k_sys_dot2<<<nsys, kRedThreads, 0, s>>>(r, z, sys_ptr, rz, nullptr);
-    k_pcg_direction<<<g, 256, 0, s>>>(p, z, rz, rz, active, row_sys, n, 1);
+    k_pcg_start<<<sg, 256, 0, s>>>(bb, nsys, active, iters);
+    if (kind == 0) {
+        k_jacobi_init<<<g, 256, 0, s>>>(r0, dinv, z, n);
+    } else {
+        if (int rc = spmv_launch(gptr, gidx, gval, r0, t, n, 4, s)) return rc;
+        if (int rc = spmv_launch(gtptr, gtidx, gtval, t, z, n, 4, s)) return rc;
+    }
+    k_sys_dot2<<<nsys, kRedThreads, 0, s>>>(r0, z, sys_ptr, rz, nullptr);
     SKFA_LAUNCH_CHECK();
     int it = 0;
     int32_t h_active = 1;
+    double* p_old = p1;
+    double* p_new = p0;
+    double* r_cur = r0;
+    double* r_nxt = r1;
+    const unsigned nb = unsigned(std::max<int64_t>(nblk, 1));
     while (it < max_iter && h_active > 0) {
+        SKFA_HIP(hipMemsetAsync(nactive, 0, sizeof(int32_t), s));
         for (int k = 0; k < check_every && it < max_iter; ++k, ++it) {
-            if (int rc = spmv_launch(indptr, indices, vals, p, q, n, spmv_lanes, s)) return rc;
-            k_sys_dot2<<<nsys, kRedThreads, 0, s>>>(p, q, sys_ptr, pq, nullptr);
-            k_pcg_update<<<g, 256, 0, s>>>(x, r, p, q, rz, pq, active, row_sys, n);
-            if (int rc = precond(r, z)) return rc;
-            k_sys_dot2<<<nsys, kRedThreads, 0, s>>>(r, z, sys_ptr, rz_new, rr);
-            SKFA_HIP(hipMemsetAsync(nactive, 0, sizeof(int32_t), s));
-            k_pcg_check<<<(nsys + 255) / 256, 256, 0, s>>>(rr, bb, tol2, nsys, active, iters, nactive);
-            k_pcg_direction<<<g, 256, 0, s>>>(p, z, rz_new, rz, active, row_sys, n, 0);
+            k_pcg_direction_spmv<<<nb, kPcgThreads, lds_bytes, s>>>(
+                bt, indptr, indices, vals, z, p_old, p_new, q, rz, rz_old, active, it == 0 ? 1 : 0, part_a);
+            k_sys_reduce<<<nsys, 256, 0, s>>>(d_bptr.p, part_a, nullptr, active, pq, nullptr);
+            if (kind == 0) {
+                k_pcg_update_jacobi<<<nb, upd_threads, 0, s>>>(bt, x, r_cur, z, p_new, q, dinv, rz, pq, active,
+                                                            part_b, part_c);
+            } else {
+                k_pcg_update_g<<<nb, kPcgThreads, lds_g, s>>>(bt, gptr, gidx, gval, x, r_cur, r_nxt, t, p_new, q,
+                                                                rz, pq, active, part_c);
+                k_pcg_apply_gt<<<nb, kPcgThreads, lds_gt, s>>>(bt, gtptr, gtidx, gtval, t, r_nxt, z, active,
+                                                                part_b);
+                std::swap(r_cur, r_nxt);
+            }
+            k_sys_reduce<<<nsys, 256, 0, s>>>(d_bptr.p, part_b, part_c, active, rz_new, rr);
+            if (k == check_every - 1 || it == max_iter - 1)
+                SKFA_HIP(hipMemsetAsync(nactive, 0, sizeof(int32_t), s));
+            k_pcg_advance<<<sg, 256, 0, s>>>(rz_new, rr, bb, tol2, nsys, rz, rz_old, active, iters, nactive);
             SKFA_LAUNCH_CHECK();
-            std::swap(rz, rz_new);
+            std::swap(p_old, p_new);
         }
         SKFA_HIP(hipMemcpyAsync(&h_active, nactive, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         SKFA_HIP(hipStreamSynchronize(s));

@@ -21,7 +21,7 @@ SYMBOLS = (
     "skfa_edge_topology", "skfa_assemble_tet_n1",
     "skfa_event_create", "skfa_event_destroy", "skfa_event_record", "skfa_event_elapsed_ms",
     "skfa_stream_synchronize",
-    "skfa_spmv", "skfa_csr_condense", "skfa_csr_lower", "skfa_csr_transpose", "skfa_gather_values",
+    "skfa_spmv", "skfa_csr_condense", "skfa_csr_lower", "skfa_csr_transpose", "skfa_csr_transpose_lower", "skfa_gather_values",
     "skfa_gather_values_f32", "skfa_diag_inverse", "skfa_sys_dot", "skfa_pcg_solve",
     "skfa_fsai_features", "skfa_fsai_factor", "skfa_fsai_kaporin", "skfa_mlp_pack", "skfa_mlp_forward",
     "skfa_mlp_workspace_bytes", "skfa_mlp_backward", "skfa_mlp_adam",
@@ -82,6 +82,7 @@ def _declare(lib):
                                              ctypes.POINTER(I64), P]),
         "skfa_csr_lower": (ctypes.c_int, [P, P, I64, P, P, P, ctypes.POINTER(I64), P]),
         "skfa_csr_transpose": (ctypes.c_int, [P, P, I64, I64, P, P, P, P]),
+        "skfa_csr_transpose_lower": (ctypes.c_int, [P, P, I64, P, P, P, P, P, P]),
         "skfa_gather_values": (ctypes.c_int, [P, P, I64, P, P]),
         "skfa_gather_values_f32": (ctypes.c_int, [P, P, I64, P, P]),
         "skfa_diag_inverse": (ctypes.c_int, [P, P, P, I64, P, P]),

@@ -32,15 +32,37 @@ from .element import ElementTetP1, ElementVector
 from .mesh import MeshTet
 
 
+def random_rotations(E, rng):
+    """(E,3,3) Haar-random rotations: the rotation of a uniformly random unit
+    quaternion (a normalised 4-D Gaussian)."""
+    q = rng.standard_normal((4, E))
+    q /= np.sqrt((q * q).sum(axis=0))
+    w, x, y, z = q
+    R = np.empty((E, 3, 3))
+    R[:, 0, 0] = 1. - 2. * (y * y + z * z)
+    R[:, 0, 1] = 2. * (x * y - w * z)
+    R[:, 0, 2] = 2. * (x * z + w * y)
+    R[:, 1, 0] = 2. * (x * y + w * z)
+    R[:, 1, 1] = 1. - 2. * (x * x + z * z)
+    R[:, 1, 2] = 2. * (y * z - w * x)
+    R[:, 2, 0] = 2. * (x * z - w * y)
+    R[:, 2, 1] = 2. * (y * z + w * x)
+    R[:, 2, 2] = 1. - 2. * (x * x + y * y)
+    return R
+
+
 def random_spd_tensors(E, rng, exponent_range=(-2., 2.)):
-    """(E,3,3) SPD tensors Q diag(10^u) Q^T with Haar-random rotations Q."""
-    g = rng.standard_normal((E, 3, 3))
-    q, r = np.linalg.qr(g)
-    q = q * np.sign(np.diagonal(r, axis1=1, axis2=2))[:, None, :]
-    det = np.linalg.det(q)
-    q[:, :, 0] *= det[:, None]
-    lam = 10. ** rng.uniform(exponent_range[0], exponent_range[1], size=(E, 3))
-    return np.einsum("eij,ej,ekj->eik", q, lam, q)
+    """(E,3,3) SPD tensors Q diag(10^u) Q^T, Q Haar-random, u uniform in
+    ``exponent_range`` (exactly symmetric)."""
+    R = random_rotations(E, rng)
+    lam = 10. ** rng.uniform(exponent_range[0], exponent_range[1], size=(3, E))
+    A = np.empty((E, 3, 3))
+    for i in range(3):
+        for j in range(i, 3):
+            v = R[:, i, 0] * R[:, j, 0] * lam[0] + R[:, i, 1] * R[:, j, 1] * lam[1] + R[:, i, 2] * R[:, j, 2] * lam[2]
+            A[:, i, j] = v
+            A[:, j, i] = v
+    return A
 
 
 def box_system(shape, rng, jitter=0.2):

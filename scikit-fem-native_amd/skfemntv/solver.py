@@ -111,7 +111,7 @@ class DeviceCSR:
         lsrc = torch.empty(max(nl.value, 1), dtype=torch.int64, device=dev)
         _native.check(L.skfa_csr_lower(_native.ptr(self.indptr), _native.ptr(self.indices), n, _native.ptr(lptr),
                                        _native.ptr(lidx), _native.ptr(lsrc), ctypes.byref(nl), _native.stream()))
-        return LowerPattern(lptr, lidx[:nl.value], lsrc[:nl.value], n)
+        return LowerPattern(lptr, lidx[:nl.value], lsrc[:nl.value], n, source=self)
 
 
 @dataclass
@@ -121,6 +121,7 @@ class LowerPattern:
     lidx: object
     lsrc: object
     n: int
+    source: object = field(default=None, repr=False)
     _transpose: tuple = field(default=None, repr=False)
 
     @property
@@ -131,13 +132,25 @@ class LowerPattern:
         """(tptr, tidx, tsrc): CSR of the transpose and, per entry, its position in the lower pattern."""
         if self._transpose is None:
             torch = _torch()
+            L = _native.lib()
             dev = self.lptr.device
             tptr = torch.empty(self.n + 1, dtype=torch.int64, device=dev)
             tidx = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
             tsrc = torch.empty(max(self.nnz, 1), dtype=torch.int64, device=dev)
-            _native.check(_native.lib().skfa_csr_transpose(_native.ptr(self.lptr), _native.ptr(self.lidx), self.n,
-                                                           self.n, _native.ptr(tptr), _native.ptr(tidx),
-                                                           _native.ptr(tsrc), _native.stream()))
+            P = _native.ptr
+            done = False
+            if self.source is not None:
+                # structurally symmetric source: upper part of its pattern, no sort
+                A = self.source
+                rc = L.skfa_csr_transpose_lower(P(A.indptr), P(A.indices), self.n, P(self.lptr), P(self.lidx),
+                                                P(tptr), P(tidx), P(tsrc), _native.stream())
+                if rc == 0:
+                    done = True
+                elif rc != -1:
+                    _native.check(rc)
+            if not done:
+                _native.check(L.skfa_csr_transpose(P(self.lptr), P(self.lidx), self.n, self.n, P(tptr), P(tidx),
+                                                   P(tsrc), _native.stream()))
             self._transpose = (tptr, tidx[:self.nnz], tsrc[:self.nnz])
         return self._transpose
 
@@ -188,7 +201,7 @@ class PCGResult:
     seconds: float            # wall time of the solve (device synchronised)
 
 
-def pcg(A, b, sys_ptr, preconditioner, *, tol=1e-8, max_iter=20000, check_every=16, spmv_lanes=8):
+def pcg(A, b, sys_ptr, preconditioner, *, tol=1e-8, max_iter=20000, check_every=8, spmv_lanes=8):
     """Batched FP64 PCG from x0 = 0 until ||r_s|| <= tol ||b_s|| per system."""
     torch = _torch()
     L = _native.lib()
@@ -198,8 +211,8 @@ def pcg(A, b, sys_ptr, preconditioner, *, tol=1e-8, max_iter=20000, check_every=
         sys_ptr = torch.as_tensor(np.asarray(sys_ptr, dtype=np.int64), device=dev)
     nsys = int(sys_ptr.shape[0]) - 1
     x = torch.empty(n, dtype=torch.float64, device=dev)
-    work = torch.empty(6 * n + 1, dtype=torch.float64, device=dev)
-    sys_work = torch.empty(5 * nsys, dtype=torch.float64, device=dev)
+    work = torch.empty(7 * n + 1, dtype=torch.float64, device=dev)
+    sys_work = torch.empty(6 * nsys, dtype=torch.float64, device=dev)
     int_work = torch.empty(nsys + 1, dtype=torch.int32, device=dev)
     iters = torch.empty(nsys, dtype=torch.int32, device=dev)
     total = ctypes.c_int32()

@@ -254,3 +254,33 @@ def test_training_decreases_loss_and_zero_net_is_jacobi(batch):
     net2 = FSAINetwork(seed=0)
     train(net2, groups, epochs=6, lr=3e-3, seed=0)
     assert torch.equal(net.params, net2.params)
+
+
+def test_pretrained_network_gain_over_jacobi():
+    """The shipped network (trained on 2,000 assembled systems with cells
+    per axis in {8..16}) against the same preconditioner without it (zero
+    network output = Jacobi) on held-out, larger systems."""
+    import time
+    import torch
+    from skfemntv.batch import AnisotropicBatch, batch_shapes
+    from skfemntv.fsai import FSAINetwork, FSAISystems, LearnedFSAI
+    from skfemntv.solver import JacobiPreconditioner, pcg, relative_residuals
+    print("case: pretrained FSAI vs Jacobi, 8 held-out systems 14..22 cells/axis", flush=True)
+    b = AnisotropicBatch(batch_shapes(8, 14, 22, 123), seed=4242)
+    A, rhs = b.assemble()
+    net = FSAINetwork.pretrained()
+    rj = pcg(A, rhs, b.sys_ptr, JacobiPreconditioner(A), tol=1e-8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    M = LearnedFSAI(net)(FSAISystems(A, b.sys_ptr, b.xyz))
+    rl = pcg(A, rhs, b.sys_ptr, M, tol=1e-8)
+    learned_s = time.perf_counter() - t0
+    gain = rj.iterations.astype(float) / rl.iterations
+    print(f"jacobi iterations {rj.iterations.tolist()} ({rj.seconds * 1e3:.2f} ms), learned "
+          f"{rl.iterations.tolist()} ({learned_s * 1e3:.2f} ms incl. features + inference); "
+          f"gain {gain.min():.2f}..{gain.max():.2f}", flush=True)
+    assert np.all(gain >= 1.8), gain
+    for res in (rj, rl):
+        assert np.all(relative_residuals(A, rhs, res.x, b.sys_ptr) <= 1.5e-8)
+    # both solve the same systems
+    assert rel_frob(rl.x.cpu().numpy(), rj.x.cpu().numpy()) <= 1e-6
