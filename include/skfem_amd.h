@@ -58,12 +58,22 @@ int skfa_plan_create(const int32_t* node_conn, int64_t E, int32_t n, int64_t nno
 int skfa_plan_create_budget(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
                             int32_t lds_budget, void* stream, skfa_plan** out);
 int skfa_plan_destroy(skfa_plan* plan);
-/* Launch tuning of an existing plan (results are identical for every
- * setting; only speed changes).  Keys: */
+/* Launch tuning of an existing plan (results are bit-identical for every
+ * setting of keys 1-4; key 5 selects between two formulations that differ
+ * by rounding only).  Keys: */
 #define SKFA_TUNE_XCD_CHUNK 1  /* waves per XCD chunk, 0 = plain dispatch order */
 #define SKFA_TUNE_LDS_ATOMIC 2 /* 1: accumulate with ds_add_f64, 0: read-add-write */
 #define SKFA_TUNE_PLACEMENT 3  /* 0 plain order, 1 one contiguous segment per XCD, 2 XCD chunks */
 #define SKFA_TUNE_STREAMING 4  /* 1: non-temporal record loads and CSR value stores */
+#define SKFA_TUNE_ELEMENT_TABLE 5 /* symmetric scalar forms: 1 element-table gather, 0 quadrature gather,
+                                     -1 automatic (table for 4-node elements); rounding differs only */
+#define SKFA_TUNE_ENTRY_SPLIT 6 /* quadrature gather, rows with S > 1 lanes: 0 each lane computes the whole
+                                    element row and adds the columns it owns, 1 lane p computes entries
+                                    p, p+S, ... only (bit-identical results) */
+#define SKFA_TUNE_MFMA 7       /* scalar fields, element-constant coefficients, <= 4-point reduced rules:
+                                  element rows contracted on MFMA (16 rows x 4 points per instruction);
+                                  0 off (default, -1 = default), 1 every wave, 2 waves of <= 16 rows
+                                  only; rounding differs only */
 int skfa_plan_set_tuning(skfa_plan* plan, int32_t key, int32_t value);
 /* host info[9]: nrows, nnz, nnz_node, max_row_len, max_incidence, waves,
  * gather records, record bytes, LDS bytes of the largest wave span. */

@@ -255,6 +255,21 @@ __device__ inline int rec_offset(const Record<RB>& r, int b) {
     else return reinterpret_cast<const uint16_t*>(r.off)[b];
 }
 
+// In-row offset of local node b (runtime b) without dynamic register indexing.
+template <int RB, int OFFW>
+__device__ inline int rec_offset_dyn(const Record<RB>& r, int b) {
+    constexpr int NW = (RB - 4) / 4;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(r.off);
+    constexpr int per = 4 / OFFW;   // offsets per word
+    const int wi = b / per;
+    uint32_t word = w[0];
+#pragma unroll
+    for (int t = 1; t < NW; ++t)
+        if (wi == t) word = w[t];
+    const int sh = (b - wi * per) * 8 * OFFW;
+    return int((word >> sh) & (OFFW == 1 ? 0xffu : 0xffffu));
+}
+
 // ------------------------------------------------------------------ element payload
 // Fast path (affine elements, coefficients constant per element): one pass
 // over elements stores what every element row needs, so a gather step is
@@ -380,6 +395,8 @@ struct GatherArgs {
     int32_t Qg, Qm;
     int32_t has_mass;
     double* values;
+    // only waves with rmin <= rows <= rmax are assembled by this launch
+    int32_t rmin = 0, rmax = 1 << 30;
 };
 
 // K_ab += w_q (dphi_aq^T D) . dphi_bq for all b: the lane's row a reads the
@@ -433,13 +450,15 @@ __device__ inline void load_payloads(const GatherArgs& A, const Record<RB>* rec,
     }
 }
 
-template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG>
+template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, bool ES>
 __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Record<RB>& rec, const double* pl,
                                     double* span_lds, int Qg, int Qm, bool mass,
                                     const typename std::conditional<F32, float, double>::type* dg,
                                     const typename std::conditional<F32, float, double>::type* pm,
                                     const typename std::conditional<F32, float, double>::type* dg_l,
-                                    const typename std::conditional<F32, float, double>::type* pm_l) {
+                                    const typename std::conditional<F32, float, double>::type* pm_l,
+                                    const typename std::conditional<F32, float, double>::type* dgu_l,
+                                    const typename std::conditional<F32, float, double>::type* pmu_l) {
     using T = typename std::conditional<F32, float, double>::type;
     const int a = rec.packed & 15;
     T D[9];
@@ -467,6 +486,48 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
             }
         mf = (L.c == L.d) ? T(pl[12]) : T(0);
     }
+    if constexpr (ES) {
+        // entry split: lane p computes entries b = p, p + S, ... of row a with
+        // the same operation order as the full row below (bit-identical)
+        auto u_of = [&](int q, T* u) {
+            const T* ga = dg_l + (q * NL + a) * 3;
+            const T g0 = ga[0], g1 = ga[1], g2 = ga[2];
+            u[0] = fma(g0, D[0], fma(g1, D[3], g2 * D[6]));
+            u[1] = fma(g0, D[1], fma(g1, D[4], g2 * D[7]));
+            u[2] = fma(g0, D[2], fma(g1, D[5], g2 * D[8]));
+        };
+        auto finish = [&](int b, T k) {
+            if (mass)
+                for (int q = 0; q < Qm; ++q) k = fma(mf * pm_l[q * NL + a], pmu_l[q * NL + b], k);
+            lds_add(span_lds + L.rb + int64_t(C) * rec_offset_dyn<RB, OFFW>(rec, b), double(k), A.plan.lds_atomic);
+        };
+        if constexpr (QG > 0) {
+            T u[QG][3];
+#pragma unroll
+            for (int q = 0; q < QG; ++q) u_of(q, u[q]);
+            for (int b = L.p; b < NL; b += L.S) {
+                T k = T(0);
+#pragma unroll
+                for (int q = 0; q < QG; ++q) {
+                    const T* gb = dgu_l + (q * NL + b) * 3;
+                    k = fma(u[q][0], gb[0], fma(u[q][1], gb[1], fma(u[q][2], gb[2], k)));
+                }
+                finish(b, k);
+            }
+        } else {
+            for (int b = L.p; b < NL; b += L.S) {
+                T k = T(0);
+                for (int q = 0; q < Qg; ++q) {
+                    T u[3];
+                    u_of(q, u);
+                    const T* gb = dgu_l + (q * NL + b) * 3;
+                    k = fma(u[0], gb[0], fma(u[1], gb[1], fma(u[2], gb[2], k)));
+                }
+                finish(b, k);
+            }
+        }
+        return;
+    }
     T Kr[NL];
 #pragma unroll
     for (int b = 0; b < NL; ++b) Kr[b] = T(0);
@@ -493,7 +554,7 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
 // Software-pipelined gather: while batch b is computed, the payloads of
 // batch b+1 and the records of batch b+2 are in flight, so the two
 // dependent loads of a step (record -> payload) overlap other work.
-template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG>
+template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, bool ES>
 __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
     extern __shared__ double span_lds[];
     constexpr int P = PayloadLayout<PL>::stride;
@@ -503,6 +564,10 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
     Lane L;
     const int wv = placed_wave(A.plan.xcd_seg, int(A.plan.nwaves), A.plan.placement, A.plan.xcd_chunk);
     if (wv < 0) return;
+    {
+        const int rows = int(A.plan.wave_r0[wv + 1] - A.plan.wave_r0[wv]);
+        if (rows < A.rmin || rows > A.rmax) return;
+    }
     lane_setup<C>(A.plan, wv, lane, L);
     Record<RB> r0[K], r1[K], r2[K];
     double p0[K][P], p1[K][P];
@@ -520,6 +585,13 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
         if (k >= Qg + Qg * NL * 3 + Qm) v *= tab_src[Qg + Qg * NL * 3 + (k - Qg - Qg * NL * 3 - Qm) / NL];  // w_q * phi
         tab_lds[k] = v;
     }
+    // entry split: the unweighted tables are read at lane-varying b too
+    T* tabu_lds = tab_lds + tab_len;
+    if constexpr (ES) {
+        const int lg = Qg * NL * 3;
+        for (int k = lane; k < lg + Qm * NL; k += 64)
+            tabu_lds[k] = k < lg ? tab_src[Qg + k] : tab_src[Qg + lg + Qm + (k - lg)];
+    }
     load_payloads<K, RB, P>(A, r0, p0);
     load_records<K, RB>(A, L, K, r1);
     __syncthreads();
@@ -527,6 +599,8 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
     const T* pm = dg + Qg * NL * 3 + Qm;
     const T* dg_l = tab_lds + Qg;          // lane-varying reads
     const T* pm_l = tab_lds + Qg + Qg * NL * 3 + Qm;
+    const T* dgu_l = tabu_lds;
+    const T* pmu_l = tabu_lds + Qg * NL * 3;
     const bool mass = A.has_mass != 0;
     for (int64_t st = 0; st < L.steps; st += K) {
         load_payloads<K, RB, P>(A, r1, p1);
@@ -534,8 +608,8 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
             if (r0[k].packed >= 0)
-                compute_step<NL, C, PL, F32, RB, OFFW, QG>(A, L, r0[k], p0[k], span_lds, Qg, Qm, mass, dg, pm, dg_l,
-                                                           pm_l);
+                compute_step<NL, C, PL, F32, RB, OFFW, QG, ES>(A, L, r0[k], p0[k], span_lds, Qg, Qm, mass, dg, pm,
+                                                               dg_l, pm_l, dgu_l, pmu_l);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             r0[k] = r1[k];
@@ -546,6 +620,282 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
     }
     __syncthreads();
     write_span(span_lds, L.span, lane, A.values + L.lo * C * C, A.plan.streaming);
+}
+
+// ------------------------------------------------------------------ MFMA gather
+// Scalar fields (C == 1), element-constant coefficients, up to 4 points in the
+// reduced gradient and mass rules.  A wave's rows are taken 16 at a time:
+// lane l works on row slot l % 16 and quadrature point q = l / 16 and forms
+//   u_q = D^T (w_q grad(phi_a)(q))          (the row's local node a)
+// then one 16x16x4 MFMA per gradient component gives, for the 16 rows,
+//   K^T[b][row] = sum_q grad(phi_b)(q)_x u_q(row)_x  (+ y, z, + mass)
+// with the reference gradients as per-lane constants.  Lane l receives
+// entries b = l/16 + 4r (FP64 layout) or 4(l/16) + r (FP32 layout) of its
+// own row, so every entry is computed once and added by one lane: each LDS
+// address is written by at most one lane per step and the per-address sum
+// runs in the row's element order (deterministic).
+typedef double mf_v4d __attribute__((ext_vector_type(4)));
+typedef float mf_v4f __attribute__((ext_vector_type(4)));
+
+template <bool F32>
+struct MfmaT;
+template <>
+struct MfmaT<false> {
+    using T = double;
+    using V = mf_v4d;
+    static __device__ __forceinline__ V mma(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // entry b of output element r for lane group p
+    static __device__ __forceinline__ int entry(int p, int r) { return p + 4 * r; }
+};
+template <>
+struct MfmaT<true> {
+    using T = float;
+    using V = mf_v4f;
+    static __device__ __forceinline__ V mma(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int entry(int p, int r) { return 4 * p + r; }
+};
+
+template <int NL, int PL, bool F32, int RB, int OFFW>
+__global__ void __launch_bounds__(64) k_gather_mfma(GatherArgs A) {
+    extern __shared__ double span_lds[];
+    using M = MfmaT<F32>;
+    using T = typename M::T;
+    constexpr int P = PayloadLayout<PL>::stride;
+    constexpr int K = 2;   // (step, row group) units per pipeline batch
+    const int lane = threadIdx.x;
+    const int p = lane >> 4;      // quadrature point of this lane
+    const int slot = lane & 15;   // row slot within a group of 16
+    const int wv = placed_wave(A.plan.xcd_seg, int(A.plan.nwaves), A.plan.placement, A.plan.xcd_chunk);
+    if (wv < 0) return;
+    const int64_t r0w = A.plan.wave_r0[wv], r1w = A.plan.wave_r0[wv + 1];
+    const int R = int(r1w - r0w);
+    if (R < A.rmin || R > A.rmax) return;
+    const int G = (R + 15) >> 4;
+    const int64_t rec0 = A.plan.wave_rec[wv];
+    const int64_t steps = (A.plan.wave_rec[wv + 1] - rec0) / R;
+    const int64_t lo = A.plan.indptr[r0w];
+    const int64_t span = A.plan.indptr[r1w] - lo;
+    const int64_t units = steps * G;
+    // tables: weighted rows for u and mass (lane-varying a), behind the span
+    const int Qg = A.Qg, Qm = A.Qm;
+    const T* src = F32 ? reinterpret_cast<const T*>(A.tab32) : reinterpret_cast<const T*>(A.tab);
+    const int lg = Qg * NL * 3;
+    T* wdg = reinterpret_cast<T*>(span_lds + ((span + 1) & ~int64_t(1)));
+    T* wpm = wdg + lg;
+    for (int k = lane; k < lg + Qm * NL; k += 64) {
+        if (k < lg) wdg[k] = src[Qg + k] * src[k / (NL * 3)];
+        else wpm[k - lg] = src[Qg + lg + Qm + (k - lg)] * src[Qg + lg + (k - lg) / NL];
+    }
+    zero_span(span_lds, span, lane);
+    // per-lane MFMA A operands: grad(phi_b)(q) components and phi_b(q), b = slot, q = p
+    T ax = T(0), ay = T(0), az = T(0), am = T(0);
+    if (slot < NL && p < Qg) {
+        const T* g = src + Qg + (p * NL + slot) * 3;
+        ax = g[0];
+        ay = g[1];
+        az = g[2];
+    }
+    if (slot < NL && p < Qm) am = src[Qg + lg + Qm + p * NL + slot];
+    const bool mass = A.has_mass != 0;
+    // unit t = (step t / G, group t % G): the lane's row is 16 (t % G) + slot
+    auto load_unit = [&](int64_t t, Record<RB>& rec) {
+        rec.packed = -1;
+        if (t < units) {
+            const int64_t st = t / G;
+            const int row = int(t - st * G) * 16 + slot;
+            if (row < R) load_record<RB>(A.plan.records + (rec0 + st * R + row) * RB, rec, A.plan.streaming);
+        }
+    };
+    Record<RB> r0[K], r1[K], r2[K];
+    double p0[K][P], p1[K][P];
+#pragma unroll
+    for (int k = 0; k < K; ++k) load_unit(k, r0[k]);
+    load_payloads<K, RB, P>(A, r0, p0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) load_unit(K + k, r1[k]);
+    __syncthreads();
+    for (int64_t t0 = 0; t0 < units; t0 += K) {
+        load_payloads<K, RB, P>(A, r1, p1);
+#pragma unroll
+        for (int k = 0; k < K; ++k) load_unit(t0 + 2 * K + k, r2[k]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int64_t t = t0 + k;
+            if (t >= units) break;   // uniform
+            const Record<RB>& rec = r0[k];
+            const bool live = rec.packed >= 0;
+            const int a = live ? (rec.packed & 15) : 0;
+            const double* pl = p0[k];
+            T D[9];
+            T mf;
+            if constexpr (PL == PL_FULL) {
+#pragma unroll
+                for (int u = 0; u < 9; ++u) D[u] = T(pl[u]);
+                mf = T(pl[9]);
+            } else {
+                D[0] = T(pl[0]); D[1] = T(pl[1]); D[2] = T(pl[2]);
+                D[3] = T(pl[1]); D[4] = T(pl[3]); D[5] = T(pl[4]);
+                D[6] = T(pl[2]); D[7] = T(pl[4]); D[8] = T(pl[5]);
+                mf = T(pl[6]);
+            }
+            T u0 = T(0), u1 = T(0), u2 = T(0), mb = T(0);
+            if (live && p < Qg) {
+                const T* ga = wdg + (p * NL + a) * 3;
+                const T g0 = ga[0], g1 = ga[1], g2 = ga[2];
+                u0 = fma(g0, D[0], fma(g1, D[3], g2 * D[6]));
+                u1 = fma(g0, D[1], fma(g1, D[4], g2 * D[7]));
+                u2 = fma(g0, D[2], fma(g1, D[5], g2 * D[8]));
+            }
+            if (mass && live && p < Qm) mb = mf * wpm[p * NL + a];
+            typename M::V c = {T(0), T(0), T(0), T(0)};
+            c = M::mma(ax, u0, c);
+            c = M::mma(ay, u1, c);
+            c = M::mma(az, u2, c);
+            if (mass) c = M::mma(am, mb, c);
+            if (live) {
+                const int64_t st = t / G;
+                const int row = int(t - st * G) * 16 + slot;
+                const int64_t rb = A.plan.indptr[r0w + row] - lo;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int b = M::entry(p, r);
+                    if (b < NL) lds_add(span_lds + rb + rec_offset_dyn<RB, OFFW>(rec, b), double(c[r]),
+                                        A.plan.lds_atomic);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            r0[k] = r1[k];
+            r1[k] = r2[k];
+#pragma unroll
+            for (int u = 0; u < P; ++u) p0[k][u] = p1[k][u];
+        }
+    }
+    __syncthreads();
+    write_span(span_lds, span, lane, A.values + lo, A.plan.streaming);
+}
+
+// ------------------------------------------------------------------ element-table gather
+// Symmetric scalar path (C == 1, D symmetric): with affine elements and
+// element-constant coefficients the element matrix is linear in the six
+// entries of D and the mass factor,
+//   K_ab = sum_k d_k N[a][b][k],  d = (D00, D01, D02, D11, D12, D22, |det| m),
+//   N[a][b][k] = sum_q wg_q (dphi_ai dphi_bj + [i != j] dphi_aj dphi_bi)   (k = (i,j), i <= j)
+//   N[a][b][6] = sum_q wm_q phi_a phi_b,
+// so a row step costs 6-7 FMAs per entry.  The S lanes of a row divide the
+// entries b = p, p + S, ... among themselves (no lane recomputes another's
+// entries).  Within a step each LDS address is written by at most one lane
+// (distinct b of one element are distinct columns), so the per-address
+// summation order is the element order of the row's incidences.
+// table slots per (a, b): 6 D entries and mass; the odd stride spreads the
+// lanes' random (a, b) rows over the LDS banks
+constexpr int kNK = 7;
+
+template <int NL>
+__global__ void k_build_ntab(const double* __restrict__ rtab, int32_t Qg, int32_t Qm, double* __restrict__ out,
+                             float* __restrict__ out32) {
+    const int t = threadIdx.x;
+    if (t >= NL * NL) return;
+    const int a = t / NL, b = t % NL;
+    const double* wg = rtab;
+    const double* dg = wg + Qg;
+    const double* wm = dg + Qg * NL * 3;
+    const double* pm = wm + Qm;
+    double M[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+    for (int q = 0; q < Qg; ++q)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) M[i][j] += wg[q] * dg[(q * NL + a) * 3 + i] * dg[(q * NL + b) * 3 + j];
+    double mass = 0.;
+    for (int q = 0; q < Qm; ++q) mass += wm[q] * pm[q * NL + a] * pm[q * NL + b];
+    const double v[kNK] = {M[0][0], M[0][1] + M[1][0], M[0][2] + M[2][0], M[1][1], M[1][2] + M[2][1], M[2][2], mass};
+    for (int k = 0; k < kNK; ++k) {
+        out[t * kNK + k] = v[k];
+        out32[t * kNK + k] = float(v[k]);
+    }
+}
+
+template <int NL, bool F32, int RB, int OFFW, bool MASS>
+__device__ inline void table_step(const GatherArgs& A, const Lane& L, const Record<RB>& rec, const double* pl,
+                                  double* span_lds,
+                                  const typename std::conditional<F32, float, double>::type* ntab) {
+    using T = typename std::conditional<F32, float, double>::type;
+    const int a = rec.packed & 15;
+    const T d0 = T(pl[0]), d1 = T(pl[1]), d2 = T(pl[2]), d3 = T(pl[3]), d4 = T(pl[4]), d5 = T(pl[5]);
+    const T d6 = MASS ? T(pl[6]) : T(0);
+    const T* na = ntab + a * NL * kNK;
+    for (int b = L.p; b < NL; b += L.S) {
+        const T* n = na + b * kNK;
+        T k = d0 * n[0];
+        k = fma(d1, n[1], k);
+        k = fma(d2, n[2], k);
+        k = fma(d3, n[3], k);
+        k = fma(d4, n[4], k);
+        k = fma(d5, n[5], k);
+        if (MASS) k = fma(d6, n[6], k);
+        lds_add(span_lds + L.rb + rec_offset_dyn<RB, OFFW>(rec, b), double(k), A.plan.lds_atomic);
+    }
+}
+
+// Workgroups of WPB waves share one copy of the element table; each wave
+// owns a span region of `span_stride` doubles behind it.  Block placement
+// follows the plan's XCD chunking at block granularity.
+template <int NL, bool F32, int RB, int OFFW, bool MASS, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_gather_table(GatherArgs A, int32_t span_stride, int32_t nblocks) {
+    extern __shared__ double lds[];
+    constexpr int P = PayloadLayout<PL_SYM>::stride;
+    constexpr int K = 2;
+    using T = typename std::conditional<F32, float, double>::type;
+    constexpr int kTab = NL * NL * kNK;
+    constexpr int kTabDoubles = ((kTab * int(sizeof(T)) + 15) / 16) * 2;
+    T* ntab = reinterpret_cast<T*>(lds);
+    const T* src = F32 ? reinterpret_cast<const T*>(A.tab32) : reinterpret_cast<const T*>(A.tab);
+    for (int k = threadIdx.x; k < kTab; k += 64 * WPB) ntab[k] = src[k];
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    // block placement (XCD chunks of blocks when the plan asks for chunking)
+    int pb = blockIdx.x;
+    if (A.plan.placement != 0 && A.plan.xcd_chunk > 0) {
+        const int chunk = A.plan.xcd_chunk;
+        const int x = pb & 7, k = pb >> 3;
+        const int full = (nblocks / (8 * chunk)) * (8 * chunk);
+        if (pb < full) pb = ((k / chunk) * 8 + x) * chunk + (k % chunk);
+    }
+    const int64_t wv = int64_t(pb) * WPB + wib;
+    const bool live = wv < A.plan.nwaves;
+    double* span_lds = lds + kTabDoubles + int64_t(wib) * span_stride;
+    Lane L;
+    Record<RB> r0[K], r1[K], r2[K];
+    double p0[K][P], p1[K][P];
+    if (live) {
+        lane_setup<1>(A.plan, int(wv), lane, L);
+        load_records<K, RB>(A, L, 0, r0);
+        zero_span(span_lds, L.span, lane);
+        load_payloads<K, RB, P>(A, r0, p0);
+        load_records<K, RB>(A, L, K, r1);
+    }
+    __syncthreads();
+    if (!live) return;
+    for (int64_t st = 0; st < L.steps; st += K) {
+        load_payloads<K, RB, P>(A, r1, p1);
+        load_records<K, RB>(A, L, st + 2 * K, r2);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (r0[k].packed >= 0) table_step<NL, F32, RB, OFFW, MASS>(A, L, r0[k], p0[k], span_lds, ntab);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            r0[k] = r1[k];
+            r1[k] = r2[k];
+#pragma unroll
+            for (int t = 0; t < P; ++t) p0[k][t] = p1[k][t];
+        }
+    }
+    // the span is this wave's own: its LDS operations complete in order
+    write_span(span_lds, L.span, lane, A.values + L.lo, A.plan.streaming);
 }
 
 // N1 rows are global edges; incidence (e, local edge a).
@@ -986,23 +1336,32 @@ static int ensure_payload(skfa_plan* P, int stride) {
     return P->payload.alloc(need ? need : 1);
 }
 
-template <int NL, int C, int PL, bool F32, int QG>
-static int launch_gather_q(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
+template <int NL, int C, int PL, bool F32, int QG, bool ES>
+static int launch_gather_k(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
     const dim3 grid = wave_grid(P), block(64);
+    if (ES) lds += (size_t(A.Qg) * NL * 3 + size_t(A.Qm) * NL) * (F32 ? 4 : 8);
     if (P->rec_bytes == 8 && P->off_bytes == 1)
-        k_gather_h1<NL, C, PL, F32, 8, 1, QG><<<grid, block, lds, s>>>(A);
+        k_gather_h1<NL, C, PL, F32, 8, 1, QG, ES><<<grid, block, lds, s>>>(A);
     else if (P->rec_bytes == 16 && P->off_bytes == 1)
-        k_gather_h1<NL, C, PL, F32, 16, 1, QG><<<grid, block, lds, s>>>(A);
+        k_gather_h1<NL, C, PL, F32, 16, 1, QG, ES><<<grid, block, lds, s>>>(A);
     else if (P->rec_bytes == 16 && P->off_bytes == 2)
-        k_gather_h1<NL, C, PL, F32, 16, 2, QG><<<grid, block, lds, s>>>(A);
+        k_gather_h1<NL, C, PL, F32, 16, 2, QG, ES><<<grid, block, lds, s>>>(A);
     else if (P->rec_bytes == 32 && P->off_bytes == 2)
-        k_gather_h1<NL, C, PL, F32, 32, 2, QG><<<grid, block, lds, s>>>(A);
+        k_gather_h1<NL, C, PL, F32, 32, 2, QG, ES><<<grid, block, lds, s>>>(A);
     else {
         set_error("unsupported gather record layout");
         return kUnsupported;
     }
     SKFA_LAUNCH_CHECK();
     return kOk;
+}
+
+// column split (every lane of a row computes the whole element row) or
+// entry split (lanes compute disjoint entries), from the plan's tuning
+template <int NL, int C, int PL, bool F32, int QG>
+static int launch_gather_q(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
+    if (P->tune_split) return launch_gather_k<NL, C, PL, F32, QG, true>(P, A, lds, s);
+    return launch_gather_k<NL, C, PL, F32, QG, false>(P, A, lds, s);
 }
 
 template <int NL, int C, int PL, bool F32>
@@ -1012,9 +1371,91 @@ static int launch_gather_pl(const skfa_plan* P, const GatherArgs& A, size_t lds,
     return launch_gather_q<NL, C, PL, F32, 0>(P, A, lds, s);
 }
 
+constexpr int kTableWaves = 4;
+
+template <int NL, bool F32, bool MASS>
+static int launch_table(const skfa_plan* P, const GatherArgs& A, hipStream_t s) {
+    using T = typename std::conditional<F32, float, double>::type;
+    constexpr int W = kTableWaves;
+    const int32_t span_stride = int32_t(((P->lds_bytes / 8) + 1) & ~1);
+    const int32_t nblocks = int32_t((P->nwaves + W - 1) / W);
+    const size_t tab_bytes = size_t((NL * NL * kNK * sizeof(T) + 15) / 16) * 16;
+    const size_t lds = tab_bytes + size_t(W) * span_stride * 8;
+    SKFA_REQUIRE(lds <= 160 * 1024, "LDS request exceeds 160 KiB");
+    const dim3 grid(unsigned(std::max(nblocks, 1))), block(64 * W);
+    if (P->rec_bytes == 8 && P->off_bytes == 1)
+        k_gather_table<NL, F32, 8, 1, MASS, W><<<grid, block, lds, s>>>(A, span_stride, nblocks);
+    else if (P->rec_bytes == 16 && P->off_bytes == 1)
+        k_gather_table<NL, F32, 16, 1, MASS, W><<<grid, block, lds, s>>>(A, span_stride, nblocks);
+    else if (P->rec_bytes == 16 && P->off_bytes == 2)
+        k_gather_table<NL, F32, 16, 2, MASS, W><<<grid, block, lds, s>>>(A, span_stride, nblocks);
+    else if (P->rec_bytes == 32 && P->off_bytes == 2)
+        k_gather_table<NL, F32, 32, 2, MASS, W><<<grid, block, lds, s>>>(A, span_stride, nblocks);
+    else {
+        set_error("unsupported gather record layout");
+        return kUnsupported;
+    }
+    SKFA_LAUNCH_CHECK();
+    return kOk;
+}
+
+template <int NL, int PL, bool F32>
+static int launch_mfma(const skfa_plan* P, const GatherArgs& A, hipStream_t s) {
+    const dim3 grid = wave_grid(P), block(64);
+    const size_t span = size_t((P->lds_bytes / 8 + 1) & ~1) * 8;
+    const size_t lds = span + (size_t(A.Qg) * NL * 3 + size_t(A.Qm) * NL) * (F32 ? 4 : 8);
+    SKFA_REQUIRE(lds <= 160 * 1024, "LDS request exceeds 160 KiB");
+    if (P->rec_bytes == 8 && P->off_bytes == 1)
+        k_gather_mfma<NL, PL, F32, 8, 1><<<grid, block, lds, s>>>(A);
+    else if (P->rec_bytes == 16 && P->off_bytes == 1)
+        k_gather_mfma<NL, PL, F32, 16, 1><<<grid, block, lds, s>>>(A);
+    else if (P->rec_bytes == 16 && P->off_bytes == 2)
+        k_gather_mfma<NL, PL, F32, 16, 2><<<grid, block, lds, s>>>(A);
+    else if (P->rec_bytes == 32 && P->off_bytes == 2)
+        k_gather_mfma<NL, PL, F32, 32, 2><<<grid, block, lds, s>>>(A);
+    else {
+        set_error("unsupported gather record layout");
+        return kUnsupported;
+    }
+    SKFA_LAUNCH_CHECK();
+    return kOk;
+}
+
 template <int NL, int C, bool F32>
-static int launch_gather(const skfa_plan* P, const GatherArgs& A, bool sym, size_t lds, hipStream_t s) {
+static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, size_t lds, hipStream_t s) {
     if constexpr (C == 3) return launch_gather_pl<NL, 3, PL_VEC, F32>(P, A, lds, s);
+    // MFMA row contraction: 1 = every wave; 2 = waves of at most 16 rows (one
+    // MFMA row group, the long-row waves where column-split lanes would
+    // recompute whole element rows), the rest by the column-split gather
+    const int mode = P->tune_mfma < 0 ? 0 : P->tune_mfma;
+    if (mode != 0 && A.Qg <= 4 && A.Qm <= 4) {
+        GatherArgs M = A;
+        if (mode == 2) M.rmax = 16;
+        if (int rc = sym ? launch_mfma<NL, PL_SYM, F32>(P, M, s) : launch_mfma<NL, PL_FULL, F32>(P, M, s)) return rc;
+        if (mode == 1) return kOk;
+        GatherArgs Rest = A;
+        Rest.rmin = 17;
+        return sym ? launch_gather_pl<NL, 1, PL_SYM, F32>(P, Rest, lds, s)
+                   : launch_gather_pl<NL, 1, PL_FULL, F32>(P, Rest, lds, s);
+    }
+    // element tables win for 4-node elements (few table rows, long column
+    // split); the quadrature gather with uniform table reads wins for P2
+    const bool table = P->tune_table < 0 ? NL == 4 : P->tune_table != 0;
+    if (sym && table) {
+        // element tables from the reduced quadrature tables, then the
+        // table-driven gather
+        const size_t nt = size_t(NL) * NL * kNK;
+        if (P->ntab.n < nt)
+            if (int rc = P->ntab.alloc(nt)) return rc;
+        if (P->ntab32.n < nt)
+            if (int rc = P->ntab32.alloc(nt)) return rc;
+        k_build_ntab<NL><<<1, 128, 0, s>>>(A.tab, A.Qg, A.Qm, P->ntab.p, P->ntab32.p);
+        SKFA_LAUNCH_CHECK();
+        GatherArgs T = A;
+        T.tab = P->ntab.p;
+        T.tab32 = P->ntab32.p;
+        return A.has_mass ? launch_table<NL, F32, true>(P, T, s) : launch_table<NL, F32, false>(P, T, s);
+    }
     if (sym) return launch_gather_pl<NL, 1, PL_SYM, F32>(P, A, lds, s);
     return launch_gather_pl<NL, 1, PL_FULL, F32>(P, A, lds, s);
 }
@@ -1039,7 +1480,7 @@ static int launch_general(const skfa_plan* P, const H1Args& A, size_t lds, hipSt
 }
 
 template <int NL, int C>
-static int dispatch_nl(const skfa_plan* P, bool fast, bool iso, bool f32, bool sym, const GatherArgs& G,
+static int dispatch_nl(skfa_plan* P, bool fast, bool iso, bool f32, bool sym, const GatherArgs& G,
                        const H1Args& H, size_t lds_fast, size_t lds_general, hipStream_t s) {
     if (fast)
         return f32 ? launch_gather<NL, C, true>(P, G, sym, lds_fast, s)

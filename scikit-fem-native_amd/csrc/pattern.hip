@@ -546,6 +546,18 @@ int skfa_plan_set_tuning(skfa_plan* P, int32_t key, int32_t value) {
             SKFA_REQUIRE(value == 0 || value == 1, "lds atomic flag must be 0 or 1");
             P->tune_lds_atomic = value;
             return kOk;
+        case SKFA_TUNE_ELEMENT_TABLE:
+            SKFA_REQUIRE(value >= -1 && value <= 1, "element table flag must be -1 (auto), 0 or 1");
+            P->tune_table = value;
+            return kOk;
+        case SKFA_TUNE_MFMA:
+            SKFA_REQUIRE(value >= -1 && value <= 2, "MFMA mode must be -1 (auto), 0, 1 or 2");
+            P->tune_mfma = value;
+            return kOk;
+        case SKFA_TUNE_ENTRY_SPLIT:
+            SKFA_REQUIRE(value == 0 || value == 1, "entry split flag must be 0 or 1");
+            P->tune_split = value;
+            return kOk;
         default:
             set_error("unknown tuning key");
             return kInvalid;

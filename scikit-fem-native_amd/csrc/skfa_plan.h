@@ -52,9 +52,15 @@ struct skfa_plan {
     skfa::DevBuf<double> payload;
     // |det J| per element for the constant-load linear form (first use)
     skfa::DevBuf<double> elem_det;
+    // element tables of the symmetric scalar gather (FP64 and FP32 copies)
+    skfa::DevBuf<double> ntab;
+    skfa::DevBuf<float> ntab32;
 
     // launch tuning (skfa_plan_set_tuning): XCD chunk of waves (0 = plain
     // order) and LDS accumulation by ds_add (1) or read-add-write (0)
+    int32_t tune_table = -1;      // element-table gather for symmetric scalar forms: -1 auto, 0 off, 1 on
+    int32_t tune_split = 0;       // quadrature gather rows with S lanes: 0 column split, 1 entry split
+    int32_t tune_mfma = -1;       // MFMA row contraction for scalar fields: -1/0 off, 1 all waves, 2 waves <= 16 rows
     int32_t tune_placement = 2;   // 0 plain, 1 XCD segments, 2 XCD chunks
     int32_t tune_xcd_chunk = 16;
     int32_t tune_lds_atomic = 1;

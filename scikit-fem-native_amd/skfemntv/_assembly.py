@@ -33,7 +33,7 @@ class Plan:
                                                 int(lds_budget), _native.stream(), ctypes.byref(h)))
         self._h = h
         self.lds_budget = int(lds_budget)
-        self.tuning = {"xcd_chunk": 16, "lds_atomic": 1, "placement": 2, "streaming": 1}
+        self.tuning = {"xcd_chunk": 16, "lds_atomic": 1, "placement": 2, "streaming": 1, "element_table": -1, "entry_split": 0, "mfma": -1}
         info = (ctypes.c_int64 * 9)()
         _native.check(L.skfa_plan_info(h, info))
         self.nrows, self.nnz, self.nnz_node = int(info[0]), int(info[1]), int(info[2])
@@ -50,9 +50,24 @@ class Plan:
     def handle(self):
         return self._h
 
-    def set_tuning(self, xcd_chunk=None, lds_atomic=None, placement=None, streaming=None):
-        """Launch tuning; every setting gives bit-identical values."""
+    def set_tuning(self, xcd_chunk=None, lds_atomic=None, placement=None, streaming=None, element_table=None, entry_split=None, mfma=None):
+        """Launch tuning.  xcd_chunk, lds_atomic, placement and streaming give
+        bit-identical values; element_table selects the element-table (1) or
+        quadrature (0) formulation of symmetric scalar forms, -1 automatic
+        (they differ by rounding only); entry_split makes the S lanes of a
+        row compute disjoint entries of the element row instead of each
+        computing all of them (rounding unchanged); mfma (-1 auto, 0, 1)
+        contracts scalar element rows on MFMA (rounding only)."""
         L = _native.lib()
+        if mfma is not None:
+            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_MFMA, int(mfma)))
+            self.tuning["mfma"] = int(mfma)
+        if entry_split is not None:
+            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_ENTRY_SPLIT, int(bool(entry_split))))
+            self.tuning["entry_split"] = int(bool(entry_split))
+        if element_table is not None:
+            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_ELEMENT_TABLE, int(element_table)))
+            self.tuning["element_table"] = int(element_table)
         if placement is not None:
             _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_PLACEMENT, int(placement)))
             self.tuning["placement"] = int(placement)

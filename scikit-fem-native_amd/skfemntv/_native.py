@@ -37,6 +37,9 @@ TUNE_XCD_CHUNK = 1
 TUNE_LDS_ATOMIC = 2
 TUNE_PLACEMENT = 3
 TUNE_STREAMING = 4
+TUNE_ELEMENT_TABLE = 5
+TUNE_ENTRY_SPLIT = 6
+TUNE_MFMA = 7
 
 
 class NativeUnavailable(RuntimeError):

@@ -138,6 +138,39 @@ def test_p2_anisotropic_fp32_products_within_1e6():
     assert e32 <= FP32_TOL
 
 
+@pytest.mark.parametrize("quadratic", [False, True])
+def test_symmetric_tensor_formulations_match_oracle(quadratic):
+    """Element-table gather and quadrature gather, FP64 and FP32 products,
+    on a symmetric per-element tensor (the benchmark's form)."""
+    import skfemntv as sk
+    from skfemntv._assembly import NativeCrossBilinearForm, upload
+    say(f"symmetric tensor formulations quadratic={quadratic}")
+    p, t = kuhn(4, 0.2, seed=13)
+    X, W = host.tet_rule_4()
+    if quadratic:
+        p, t = host.p2_mesh(p, t)
+        basis = sk.Basis(sk.MeshTet2(p, t), sk.ElementVector(sk.ElementTetP2(), dim=1), quadrature=(X, W))
+    else:
+        basis = sk.Basis(sk.MeshTet(p, t), sk.ElementVector(sk.ElementTetP1(), dim=1), quadrature=(X, W))
+    E = t.shape[1]
+    A = spd_tensors(E, 17)
+    A = 0.5 * (A + np.swapaxes(A, 1, 2))
+    ip, ix, vals, N = pipeline.tensor(p, t, X, W, quadratic,
+                                      np.ascontiguousarray(np.broadcast_to(A[:, None], (E, 4, 3, 3))))
+    form = NativeCrossBilinearForm(basis, basis)
+    Ad = upload(A)
+    out = {}
+    for table, mfma, split in ((1, 0, 0), (0, 0, 0), (0, 0, 1), (0, 1, 0)):
+        form.plan.set_tuning(element_table=table, mfma=mfma, entry_split=split)
+        for prec in ("fp64", "fp32"):
+            v = form.assemble_tensor_device(Ad, precision=prec, symmetric=True).clone().cpu().numpy()[:len(vals)]
+            out[(table, mfma, split, prec)] = pipeline.relative_frobenius(v, vals)
+    form.plan.set_tuning(element_table=-1, mfma=-1, entry_split=0)
+    say("rel errors " + ", ".join(f"table={k[0]} mfma={k[1]} split={k[2]} {k[3]}: {e:.2e}" for k, e in out.items()))
+    for key, e in out.items():
+        assert e <= (FP64_TOL if key[-1] == "fp64" else FP32_TOL), (key, e)
+
+
 def test_vector_p2_elasticity_matches_oracle():
     import skfemntv as sk
     from skfemntv.helpers import ddot, div, sym_grad
