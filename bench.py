@@ -190,6 +190,100 @@ def batched_fsai(systems, lo, hi, steps, warmup, L, ev0, ev1, stream):
     }
 
 
+def timed(fn, steps, warmup, L, ev0, ev1, stream):
+    """Mean seconds per call of fn over `steps` calls, HIP events on the launch stream."""
+    import torch
+    from skfemntv import _native
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    _native.check(L.skfa_event_record(ev0, stream))
+    for _ in range(steps):
+        fn()
+    _native.check(L.skfa_event_record(ev1, stream))
+    ms = ctypes.c_float()
+    _native.check(L.skfa_event_elapsed_ms(ev0, ev1, ctypes.byref(ms)))
+    return ms.value / 1e3 / steps
+
+
+def other_workload(name, steps, warmup, L, ev0, ev1, stream):
+    """The other assembly configurations at full size (one JSON line each):
+    c1 P1 Laplace + load 32^3, c3 vector-P2 elasticity 48^3, c4 N1 curl-curl +
+    kappa mass 96^3.  Algorithmic bytes per element: 106.7, 3255 and 356."""
+    import torch
+    import skfemntv as sk
+    from skfemntv._assembly import NativeBilinearForm, NativeLinearForm, upload
+    from skfemntv.hcurl import AffineTetN1Basis, TetN1Assembler
+    t0 = time.perf_counter()
+    if name == "c1":
+        ax = np.linspace(0., 1., 33)
+        mesh = sk.MeshTet.init_tensor(ax, ax, ax)
+        basis = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP1(), dim=1))
+        E = mesh.nelements
+        form, load = NativeBilinearForm(basis), NativeLinearForm(basis)
+        one = upload(np.ones(E))
+        f = upload(np.array([1.]))
+
+        def step(precision="fp64"):
+            form.assemble_device(gradient=one, precision=precision)
+            load.assemble_device(f, False, None, False)
+        bpe, desc, nnz = 106.7, f"P1 Laplace ddot(grad u, grad v) + load f=1, 32^3 Kuhn cube, {E} tets, FP64", form.plan.nnz
+    elif name == "c3":
+        n = 48
+        ax = np.linspace(0., 1., n + 1)
+        m = sk.MeshTet.init_tensor(ax, ax, ax)
+        p = m.p.copy()
+        interior = np.all((p > 1e-12) & (p < 1 - 1e-12), axis=0)
+        rng = np.random.default_rng(0)
+        p[:, interior] += rng.uniform(-0.15 / n, 0.15 / n, size=(3, int(interior.sum())))
+        mesh = sk.MeshTet2.from_mesh(sk.MeshTet(p, m.t))
+        X, W = sk.basis.tet_rule_4()
+        basis = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP2(), dim=3), quadrature=(X, W))
+        E = mesh.nelements
+        rng = np.random.default_rng(1)
+        Ey = np.exp(rng.standard_normal(E))
+        nu = rng.uniform(0.2, 0.45, E)
+        lam = Ey * nu / ((1 + nu) * (1 - 2 * nu))
+        mu = Ey / (2 * (1 + nu))
+        form = NativeBilinearForm(basis)
+        s_dev, l_dev = upload(2 * mu), upload(lam)
+
+        def step(precision="fp64"):
+            form.assemble_device(symmetric_gradient=s_dev, divergence=l_dev, precision=precision)
+        bpe, nnz = 3255.0, form.plan.nnz
+        desc = f"vector-P2 elasticity 2 mu eps:eps + lam div div, 48^3 Kuhn cube jittered 0.15h, {E} tets, FP64"
+    elif name == "c4":
+        ax = np.linspace(0., 1., 97)
+        mesh = sk.MeshTet.init_tensor(ax, ax, ax)
+        basis = AffineTetN1Basis(mesh)
+        E = mesh.nelements
+        asm_ = TetN1Assembler(basis)
+        kappa = upload(np.random.default_rng(2).uniform(0.1, 10., E))
+        one = upload(np.ones(E))
+
+        def step(precision="fp64"):
+            asm_.assemble_device(kappa, one, False)
+        bpe, nnz = 356.0, asm_.plan.nnz
+        desc = f"N1 curl-curl + kappa mass, 96^3 Kuhn cube, {E} tets, FP64"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    log(f"{name}: setup {setup:.1f}s")
+    sec = timed(step, steps, warmup, L, ev0, ev1, stream)
+    out = {"metric": METRIC, "value": E / sec, "unit": "elements/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": desc, "elements": E, "nnz": nnz, "parallelism": "single GPU"},
+           "roofline": {"bound": "hbm", "achieved": bpe * E / sec / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": bpe * E / sec / 1e9 / HBM_PEAK_GBS, "traffic": None},
+           "nnz_per_s": nnz / sec, "setup_s": setup}
+    if name in ("c1", "c3"):
+        sec32 = timed(lambda: step("fp32"), steps, warmup, L, ev0, ev1, stream)
+        out["fp32_products"] = {"ms_per_step": sec32 * 1e3, "speedup_vs_fp64": sec / sec32}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +295,8 @@ def main():
     ap.add_argument("--cells", type=int, nargs=2, default=(16, 48), help="batched workload: cells per axis range")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (counter calibration)")
+    ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c3", "c4"),
+                    help="c2 (default): the headline line; c1/c3/c4: the other assembly configurations")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's assembly path runs on a single MI355X; "
@@ -213,6 +309,14 @@ def main():
     from skfemntv._assembly import NativeCrossBilinearForm, upload
 
     torch.cuda.set_device(0)
+    if args.workload != "c2":
+        L = _native.lib()
+        stream = _native.stream()
+        ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+        _native.check(L.skfa_event_create(ctypes.byref(ev0)))
+        _native.check(L.skfa_event_create(ctypes.byref(ev1)))
+        print(json.dumps(other_workload(args.workload, args.steps, args.warmup, L, ev0, ev1, stream)), flush=True)
+        return
     t0 = time.perf_counter()
     mesh = jittered_p2_cube(args.n, 0.2, 0)
     E = mesh.nelements
@@ -274,6 +378,14 @@ def main():
 
     algorithmic = BYTES_PER_ELEMENT_C2 * E
     achieved_gbs = algorithmic / kernel_s / 1e9
+    # measured HBM bytes of one assembly (FETCH_SIZE + WRITE_SIZE of the
+    # payload pass and the gather, calibrated on a copy of known size), from
+    # the committed rocprofv3 summary of this workload
+    traffic = None
+    traffic_file = os.path.join(ROOT, "profiles", "c2_p2_aniso_64_hbm_traffic.json")
+    if args.n == 64 and os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            traffic = json.load(f)["per_assembly_bytes"]
     result = {
         "metric": METRIC,
         "value": E / kernel_s,
@@ -291,7 +403,8 @@ def main():
                                f"{E} tets, 4-pt quadrature, FP64",
                    "elements": E, "nnz": form.plan.nnz, "dofs": basis.N, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_payload_h1 + k_gather_h1 (one assembly)"},
         "fp32_products": {"ms_per_step": kernel32_s * 1e3, "speedup_vs_fp64": kernel_s / kernel32_s,
                           "rel_frobenius_vs_fp64": err32},
         "nnz_per_s": form.plan.nnz / kernel_s,
