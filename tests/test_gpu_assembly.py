@@ -159,12 +159,16 @@ def test_symmetric_tensor_formulations_match_oracle(quadratic):
                                       np.ascontiguousarray(np.broadcast_to(A[:, None], (E, 4, 3, 3))))
     form = NativeCrossBilinearForm(basis, basis)
     Ad = upload(A)
-    out = {}
-    for table, mfma, split in ((1, 0, 0), (0, 0, 0), (0, 0, 1), (0, 1, 0)):
+    out, raw = {}, {}
+    for table, mfma, split in ((1, 0, 0), (0, 0, 0), (0, 0, 1), (0, 1, 0), (0, 2, 0)):
         form.plan.set_tuning(element_table=table, mfma=mfma, entry_split=split)
         for prec in ("fp64", "fp32"):
             v = form.assemble_tensor_device(Ad, precision=prec, symmetric=True).clone().cpu().numpy()[:len(vals)]
             out[(table, mfma, split, prec)] = pipeline.relative_frobenius(v, vals)
+            raw[(table, mfma, split, prec)] = v
+    # the entry split changes only which lane computes an entry: same bits
+    for prec in ("fp64", "fp32"):
+        np.testing.assert_array_equal(raw[(0, 0, 1, prec)], raw[(0, 0, 0, prec)])
     form.plan.set_tuning(element_table=-1, mfma=-1, entry_split=0)
     say("rel errors " + ", ".join(f"table={k[0]} mfma={k[1]} split={k[2]} {k[3]}: {e:.2e}" for k, e in out.items()))
     for key, e in out.items():
