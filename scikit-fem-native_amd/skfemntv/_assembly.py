@@ -22,6 +22,10 @@ from ._symbols import UnsupportedNativeForm
 
 
 # ------------------------------------------------------------------ plans
+_TUNE_KEYS = {"xcd_chunk": 1, "lds_atomic": 2, "placement": 3, "streaming": 4, "element_table": 5,
+              "entry_split": 6, "mfma": 7}
+
+
 class Plan:
     """Device CSR pattern + gather plan of one (basis, components) pair."""
 
@@ -33,7 +37,8 @@ class Plan:
                                                 int(lds_budget), _native.stream(), ctypes.byref(h)))
         self._h = h
         self.lds_budget = int(lds_budget)
-        self.tuning = {"xcd_chunk": 16, "lds_atomic": 1, "placement": 2, "streaming": 1, "element_table": -1, "entry_split": 0, "mfma": -1}
+        self.tuning = {"xcd_chunk": 16, "lds_atomic": 1, "placement": 2, "streaming": 1, "element_table": -1,
+                       "entry_split": 0, "mfma": -1}
         info = (ctypes.c_int64 * 9)()
         _native.check(L.skfa_plan_info(h, info))
         self.nrows, self.nnz, self.nnz_node = int(info[0]), int(info[1]), int(info[2])
@@ -50,36 +55,22 @@ class Plan:
     def handle(self):
         return self._h
 
-    def set_tuning(self, xcd_chunk=None, lds_atomic=None, placement=None, streaming=None, element_table=None, entry_split=None, mfma=None):
-        """Launch tuning.  xcd_chunk, lds_atomic, placement and streaming give
-        bit-identical values; element_table selects the element-table (1) or
-        quadrature (0) formulation of symmetric scalar forms, -1 automatic
-        (they differ by rounding only); entry_split makes the S lanes of a
-        row compute disjoint entries of the element row instead of each
-        computing all of them (rounding unchanged); mfma (-1 auto, 0, 1)
-        contracts scalar element rows on MFMA (rounding only)."""
+    def set_tuning(self, **settings):
+        """Launch tuning (library key in parentheses).
+
+        Bit-identical results for every value: ``xcd_chunk`` (1, waves per
+        XCD chunk), ``lds_atomic`` (2), ``placement`` (3: 0 plain, 1 XCD
+        segments, 2 XCD chunks), ``streaming`` (4), ``entry_split`` (6: lanes
+        of a row compute disjoint entries).  Different rounding only:
+        ``element_table`` (5: -1 auto, 0, 1) and ``mfma`` (7: 0 off, 1 all
+        waves, 2 waves of at most 16 rows)."""
         L = _native.lib()
-        if mfma is not None:
-            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_MFMA, int(mfma)))
-            self.tuning["mfma"] = int(mfma)
-        if entry_split is not None:
-            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_ENTRY_SPLIT, int(bool(entry_split))))
-            self.tuning["entry_split"] = int(bool(entry_split))
-        if element_table is not None:
-            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_ELEMENT_TABLE, int(element_table)))
-            self.tuning["element_table"] = int(element_table)
-        if placement is not None:
-            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_PLACEMENT, int(placement)))
-            self.tuning["placement"] = int(placement)
-        if streaming is not None:
-            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_STREAMING, int(bool(streaming))))
-            self.tuning["streaming"] = int(bool(streaming))
-        if xcd_chunk is not None:
-            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_XCD_CHUNK, int(xcd_chunk)))
-            self.tuning["xcd_chunk"] = int(xcd_chunk)
-        if lds_atomic is not None:
-            _native.check(L.skfa_plan_set_tuning(self._h, _native.TUNE_LDS_ATOMIC, int(bool(lds_atomic))))
-            self.tuning["lds_atomic"] = int(bool(lds_atomic))
+        for name, value in settings.items():
+            if name not in _TUNE_KEYS:
+                raise ValueError(f"unknown tuning setting {name!r}")
+            v = int(value)
+            _native.check(L.skfa_plan_set_tuning(self._h, _TUNE_KEYS[name], v))
+            self.tuning[name] = v
 
     def host_csr_arrays(self):
         """(indptr, indices) on the host, int32 when they fit (as scipy does)."""
