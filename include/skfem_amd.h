@@ -67,9 +67,11 @@ int skfa_plan_destroy(skfa_plan* plan);
 #define SKFA_TUNE_STREAMING 4  /* 1: non-temporal record loads and CSR value stores */
 #define SKFA_TUNE_ELEMENT_TABLE 5 /* symmetric scalar forms: 1 element-table gather, 0 quadrature gather,
                                      -1 automatic (table for 4-node elements); rounding differs only */
-#define SKFA_TUNE_ENTRY_SPLIT 6 /* quadrature gather, rows with S > 1 lanes: 0 each lane computes the whole
-                                    element row and adds the columns it owns, 1 lane p computes entries
-                                    p, p+S, ... only (bit-identical results) */
+#define SKFA_TUNE_ROW_SPLIT 6  /* quadrature gather, rows with S > 1 lanes: 0 incidence split (lane p takes
+                                  the row's incidences p, p+S, ...; the S lanes add in turn; default),
+                                  1 entry split (lane p computes entries p, p+S, ... of every element
+                                  row), 2 column split (every lane computes the whole element row and
+                                  adds the columns it owns); bit-identical results */
 #define SKFA_TUNE_MFMA 7       /* scalar fields, element-constant coefficients, <= 4-point reduced rules:
                                   element rows contracted on MFMA (16 rows x 4 points per instruction);
                                   0 off (default, -1 = default), 1 every wave, 2 waves of <= 16 rows

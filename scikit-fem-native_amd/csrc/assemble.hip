@@ -149,8 +149,11 @@ __device__ inline int placed_wave(const int64_t* seg, int nwaves, int mode, int 
     return b;
 }
 
+// steps counts the wave's incidence layers; a lane's k-th step reads layer
+// k * lmul + ladd (lmul = 1, ladd = 0 unless the row's lanes split incidences)
 struct Lane {
     int i, c, d, p, S, R;
+    int lmul, ladd;
     bool active;
     int64_t rec0, steps, lo, span, rb;
 };
@@ -168,6 +171,8 @@ __device__ inline void lane_setup(const PlanArgs& P, int w, int lane, Lane& L) {
     L.c = cd / C;
     L.d = cd - L.c * C;
     L.active = L.i < L.R;
+    L.lmul = 1;
+    L.ladd = 0;
     L.rec0 = P.wave_rec[w];
     L.steps = (P.wave_rec[w + 1] - L.rec0) / L.R;
     L.lo = P.indptr[r0];
@@ -429,8 +434,9 @@ __device__ inline void load_records(const GatherArgs& A, const Lane& L, int64_t 
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         rec[k].packed = -1;
-        if (L.active && st + k < L.steps)
-            load_record<RB>(A.plan.records + (L.rec0 + (st + k) * L.R + L.i) * RB, rec[k], A.plan.streaming);
+        const int64_t layer = (st + k) * L.lmul + L.ladd;
+        if (L.active && layer < L.steps)
+            load_record<RB>(A.plan.records + (L.rec0 + layer * L.R + L.i) * RB, rec[k], A.plan.streaming);
     }
 }
 
@@ -450,7 +456,17 @@ __device__ inline void load_payloads(const GatherArgs& A, const Record<RB>* rec,
     }
 }
 
-template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, bool ES>
+// How the S lanes of one node row share its work (all bit-identical: every
+// CSR entry receives its element contributions in incidence order):
+//   SP_INCIDENCE  lane p takes incidence layers p, p+S, ...; the S lanes
+//                 add their element rows in turn (p = 0, 1, ...)
+//   SP_ENTRY      all S lanes read the same element; lane p computes entries
+//                 b = p, p+S, ... of its row
+//   SP_COLUMN     all S lanes read the same element and compute the whole
+//                 row; lane p adds the in-row offsets o with o % S == p
+constexpr int SP_INCIDENCE = 0, SP_ENTRY = 1, SP_COLUMN = 2;
+
+template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, int SP>
 __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Record<RB>& rec, const double* pl,
                                     double* span_lds, int Qg, int Qm, bool mass,
                                     const typename std::conditional<F32, float, double>::type* dg,
@@ -486,9 +502,9 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
             }
         mf = (L.c == L.d) ? T(pl[12]) : T(0);
     }
-    if constexpr (ES) {
-        // entry split: lane p computes entries b = p, p + S, ... of row a with
-        // the same operation order as the full row below (bit-identical)
+    if constexpr (SP == SP_ENTRY) {
+        // lane p computes entries b = p, p + S, ... of row a with the same
+        // operation order as the full row below (bit-identical)
         auto u_of = [&](int q, T* u) {
             const T* ga = dg_l + (q * NL + a) * 3;
             const T g0 = ga[0], g1 = ga[1], g2 = ga[2];
@@ -544,21 +560,38 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
             for (int b = 0; b < NL; ++b) Kr[b] = fma(v, pm[q * NL + b], Kr[b]);
         }
     }
+    if constexpr (SP == SP_COLUMN) {
 #pragma unroll
-    for (int b = 0; b < NL; ++b) {
-        const int o = rec_offset<RB, OFFW>(rec, b);
-        if ((o & (L.S - 1)) == L.p) lds_add(span_lds + L.rb + int64_t(C) * o, double(Kr[b]), A.plan.lds_atomic);
+        for (int b = 0; b < NL; ++b) {
+            const int o = rec_offset<RB, OFFW>(rec, b);
+            if ((o & (L.S - 1)) == L.p) lds_add(span_lds + L.rb + int64_t(C) * o, double(Kr[b]), A.plan.lds_atomic);
+        }
+    } else {
+        // LDS addresses fit 32 bits; one writer per address in every add
+        double* row = span_lds + int(L.rb);
+        auto add_row = [&]() {
+#pragma unroll
+            for (int b = 0; b < NL; ++b)
+                lds_add(row + C * rec_offset<RB, OFFW>(rec, b), double(Kr[b]), A.plan.lds_atomic);
+        };
+        if (L.S == 1) {
+            add_row();
+        } else {
+            for (int turn = 0; turn < L.S; ++turn)
+                if (L.p == turn) add_row();
+        }
     }
 }
 
 // Software-pipelined gather: while batch b is computed, the payloads of
 // batch b+1 and the records of batch b+2 are in flight, so the two
 // dependent loads of a step (record -> payload) overlap other work.
-template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, bool ES>
+template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, int SP>
 __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
     extern __shared__ double span_lds[];
     constexpr int P = PayloadLayout<PL>::stride;
     constexpr int K = 2;
+    constexpr bool ES = SP == SP_ENTRY;
     using T = typename std::conditional<F32, float, double>::type;
     const int lane = threadIdx.x;
     Lane L;
@@ -569,6 +602,12 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
         if (rows < A.rmin || rows > A.rmax) return;
     }
     lane_setup<C>(A.plan, wv, lane, L);
+    int64_t nsteps = L.steps;
+    if constexpr (SP == SP_INCIDENCE) {
+        L.lmul = L.S;
+        L.ladd = L.p;
+        nsteps = (L.steps + L.S - 1) / L.S;
+    }
     Record<RB> r0[K], r1[K], r2[K];
     double p0[K][P], p1[K][P];
     load_records<K, RB>(A, L, 0, r0);
@@ -602,13 +641,13 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
     const T* dgu_l = tabu_lds;
     const T* pmu_l = tabu_lds + Qg * NL * 3;
     const bool mass = A.has_mass != 0;
-    for (int64_t st = 0; st < L.steps; st += K) {
+    for (int64_t st = 0; st < nsteps; st += K) {
         load_payloads<K, RB, P>(A, r1, p1);
         load_records<K, RB>(A, L, st + 2 * K, r2);
 #pragma unroll
         for (int k = 0; k < K; ++k)
             if (r0[k].packed >= 0)
-                compute_step<NL, C, PL, F32, RB, OFFW, QG, ES>(A, L, r0[k], p0[k], span_lds, Qg, Qm, mass, dg, pm,
+                compute_step<NL, C, PL, F32, RB, OFFW, QG, SP>(A, L, r0[k], p0[k], span_lds, Qg, Qm, mass, dg, pm,
                                                                dg_l, pm_l, dgu_l, pmu_l);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -1336,18 +1375,18 @@ static int ensure_payload(skfa_plan* P, int stride) {
     return P->payload.alloc(need ? need : 1);
 }
 
-template <int NL, int C, int PL, bool F32, int QG, bool ES>
+template <int NL, int C, int PL, bool F32, int QG, int SP>
 static int launch_gather_k(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
     const dim3 grid = wave_grid(P), block(64);
-    if (ES) lds += (size_t(A.Qg) * NL * 3 + size_t(A.Qm) * NL) * (F32 ? 4 : 8);
+    if (SP == SP_ENTRY) lds += (size_t(A.Qg) * NL * 3 + size_t(A.Qm) * NL) * (F32 ? 4 : 8);
     if (P->rec_bytes == 8 && P->off_bytes == 1)
-        k_gather_h1<NL, C, PL, F32, 8, 1, QG, ES><<<grid, block, lds, s>>>(A);
+        k_gather_h1<NL, C, PL, F32, 8, 1, QG, SP><<<grid, block, lds, s>>>(A);
     else if (P->rec_bytes == 16 && P->off_bytes == 1)
-        k_gather_h1<NL, C, PL, F32, 16, 1, QG, ES><<<grid, block, lds, s>>>(A);
+        k_gather_h1<NL, C, PL, F32, 16, 1, QG, SP><<<grid, block, lds, s>>>(A);
     else if (P->rec_bytes == 16 && P->off_bytes == 2)
-        k_gather_h1<NL, C, PL, F32, 16, 2, QG, ES><<<grid, block, lds, s>>>(A);
+        k_gather_h1<NL, C, PL, F32, 16, 2, QG, SP><<<grid, block, lds, s>>>(A);
     else if (P->rec_bytes == 32 && P->off_bytes == 2)
-        k_gather_h1<NL, C, PL, F32, 32, 2, QG, ES><<<grid, block, lds, s>>>(A);
+        k_gather_h1<NL, C, PL, F32, 32, 2, QG, SP><<<grid, block, lds, s>>>(A);
     else {
         set_error("unsupported gather record layout");
         return kUnsupported;
@@ -1356,12 +1395,12 @@ static int launch_gather_k(const skfa_plan* P, const GatherArgs& A, size_t lds, 
     return kOk;
 }
 
-// column split (every lane of a row computes the whole element row) or
-// entry split (lanes compute disjoint entries), from the plan's tuning
+// how the S lanes of a row share its work, from the plan's tuning
 template <int NL, int C, int PL, bool F32, int QG>
 static int launch_gather_q(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
-    if (P->tune_split) return launch_gather_k<NL, C, PL, F32, QG, true>(P, A, lds, s);
-    return launch_gather_k<NL, C, PL, F32, QG, false>(P, A, lds, s);
+    if (P->tune_split == SP_ENTRY) return launch_gather_k<NL, C, PL, F32, QG, SP_ENTRY>(P, A, lds, s);
+    if (P->tune_split == SP_COLUMN) return launch_gather_k<NL, C, PL, F32, QG, SP_COLUMN>(P, A, lds, s);
+    return launch_gather_k<NL, C, PL, F32, QG, SP_INCIDENCE>(P, A, lds, s);
 }
 
 template <int NL, int C, int PL, bool F32>

@@ -331,6 +331,9 @@ static int plan_build(skfa_plan* P, const int32_t* conn, hipStream_t s) {
         lds_max = std::max(lds_max, bytes);
         int64_t steps = 0;
         for (int64_t q = r; q < r1; ++q) steps = std::max(steps, hinc[size_t(q + 1)] - hinc[size_t(q)]);
+        // incidence layers padded to a multiple of S: the S lanes of a row
+        // may take layers st*S + p (incidence split)
+        steps = (steps + chosen_S - 1) / chosen_S * chosen_S;
         w_r0.push_back(r);
         w_rec.push_back(rec_total);
         w_split.push_back(chosen_S);
@@ -554,8 +557,8 @@ int skfa_plan_set_tuning(skfa_plan* P, int32_t key, int32_t value) {
             SKFA_REQUIRE(value >= -1 && value <= 2, "MFMA mode must be -1 (auto), 0, 1 or 2");
             P->tune_mfma = value;
             return kOk;
-        case SKFA_TUNE_ENTRY_SPLIT:
-            SKFA_REQUIRE(value == 0 || value == 1, "entry split flag must be 0 or 1");
+        case SKFA_TUNE_ROW_SPLIT:
+            SKFA_REQUIRE(value >= 0 && value <= 2, "row split must be 0 (incidence), 1 (entry) or 2 (column)");
             P->tune_split = value;
             return kOk;
         default:

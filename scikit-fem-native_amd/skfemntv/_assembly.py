@@ -23,7 +23,7 @@ from ._symbols import UnsupportedNativeForm
 
 # ------------------------------------------------------------------ plans
 _TUNE_KEYS = {"xcd_chunk": 1, "lds_atomic": 2, "placement": 3, "streaming": 4, "element_table": 5,
-              "entry_split": 6, "mfma": 7}
+              "row_split": 6, "mfma": 7}
 
 
 class Plan:
@@ -38,7 +38,7 @@ class Plan:
         self._h = h
         self.lds_budget = int(lds_budget)
         self.tuning = {"xcd_chunk": 16, "lds_atomic": 1, "placement": 2, "streaming": 1, "element_table": -1,
-                       "entry_split": 0, "mfma": -1}
+                       "row_split": 0, "mfma": -1}
         info = (ctypes.c_int64 * 9)()
         _native.check(L.skfa_plan_info(h, info))
         self.nrows, self.nnz, self.nnz_node = int(info[0]), int(info[1]), int(info[2])
@@ -60,8 +60,8 @@ class Plan:
 
         Bit-identical results for every value: ``xcd_chunk`` (1, waves per
         XCD chunk), ``lds_atomic`` (2), ``placement`` (3: 0 plain, 1 XCD
-        segments, 2 XCD chunks), ``streaming`` (4), ``entry_split`` (6: lanes
-        of a row compute disjoint entries).  Different rounding only:
+        segments, 2 XCD chunks), ``streaming`` (4), ``row_split`` (6: how
+        the lanes of one row share it: 0 incidences, 1 entries, 2 columns).  Different rounding only:
         ``element_table`` (5: -1 auto, 0, 1) and ``mfma`` (7: 0 off, 1 all
         waves, 2 waves of at most 16 rows)."""
         L = _native.lib()

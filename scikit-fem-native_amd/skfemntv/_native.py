@@ -38,7 +38,7 @@ TUNE_LDS_ATOMIC = 2
 TUNE_PLACEMENT = 3
 TUNE_STREAMING = 4
 TUNE_ELEMENT_TABLE = 5
-TUNE_ENTRY_SPLIT = 6
+TUNE_ROW_SPLIT = 6
 TUNE_MFMA = 7
 
 

@@ -160,19 +160,75 @@ def test_symmetric_tensor_formulations_match_oracle(quadratic):
     form = NativeCrossBilinearForm(basis, basis)
     Ad = upload(A)
     out, raw = {}, {}
-    for table, mfma, split in ((1, 0, 0), (0, 0, 0), (0, 0, 1), (0, 1, 0), (0, 2, 0)):
-        form.plan.set_tuning(element_table=table, mfma=mfma, entry_split=split)
+    for table, mfma, split in ((1, 0, 0), (0, 0, 0), (0, 0, 1), (0, 0, 2), (0, 1, 0), (0, 2, 0)):
+        form.plan.set_tuning(element_table=table, mfma=mfma, row_split=split)
         for prec in ("fp64", "fp32"):
             v = form.assemble_tensor_device(Ad, precision=prec, symmetric=True).clone().cpu().numpy()[:len(vals)]
             out[(table, mfma, split, prec)] = pipeline.relative_frobenius(v, vals)
             raw[(table, mfma, split, prec)] = v
-    # the entry split changes only which lane computes an entry: same bits
+    # the row splits change only which lane computes or adds an entry: same bits
     for prec in ("fp64", "fp32"):
         np.testing.assert_array_equal(raw[(0, 0, 1, prec)], raw[(0, 0, 0, prec)])
-    form.plan.set_tuning(element_table=-1, mfma=-1, entry_split=0)
+        np.testing.assert_array_equal(raw[(0, 0, 2, prec)], raw[(0, 0, 0, prec)])
+    form.plan.set_tuning(element_table=-1, mfma=-1, row_split=0)
     say("rel errors " + ", ".join(f"table={k[0]} mfma={k[1]} split={k[2]} {k[3]}: {e:.2e}" for k, e in out.items()))
     for key, e in out.items():
         assert e <= (FP64_TOL if key[-1] == "fp64" else FP32_TOL), (key, e)
+
+
+@pytest.mark.parametrize("components", [1, 3])
+def test_row_splits_agree_at_small_lds_budgets(components):
+    """Small span budgets put several lanes on one node row (S = 2..16 for
+    scalar P2, S = 2..4 for 3-component fields): the incidence, entry and
+    column splits give the same bits, and match the oracle."""
+    import skfemntv as sk
+    from skfemntv._assembly import NativeCrossBilinearForm, Plan, upload
+    from skfemntv.helpers import ddot, div, sym_grad
+    say(f"row splits at small LDS budgets, components={components}")
+    p, t = kuhn(3, 0.15, seed=21)
+    p2, t2 = host.p2_mesh(p, t)
+    X, W = host.tet_rule_4()
+    E, Q = t.shape[1], 4
+    basis = sk.Basis(sk.MeshTet2(p2, t2), sk.ElementVector(sk.ElementTetP2(), dim=components), quadrature=(X, W))
+    d = basis.device_data()
+    if components == 1:
+        A = spd_tensors(E, 5)
+        A = 0.5 * (A + np.swapaxes(A, 1, 2))
+        ip, ix, vals, N = pipeline.tensor(p2, t2, X, W, True,
+                                          np.ascontiguousarray(np.broadcast_to(A[:, None], (E, Q, 3, 3))))
+        form = NativeCrossBilinearForm(basis, basis)
+        Ad = upload(A)
+        budgets = (2048, 4096)
+    else:
+        rng = np.random.default_rng(3)
+        mu, lam = np.exp(rng.standard_normal(E)), np.exp(rng.standard_normal(E))
+
+        @sk.BilinearForm
+        def elasticity(u, v, w):
+            return 2. * w.mu * ddot(sym_grad(u), sym_grad(v)) + w.lam * (div(u) * div(v))
+
+        ip, ix, vals, N = pipeline.bilinear(p2, t2, X, W, True, components=3,
+                                            symgrad=np.repeat(2 * mu[:, None], Q, 1),
+                                            divergence=np.repeat(lam[:, None], Q, 1))
+        budgets = (2048, 8192)
+    for budget in budgets:
+        plan = Plan(d["node_conn"], E, 10, d["nnodes"], components, lds_budget=budget)
+        raw = {}
+        for split in (0, 1, 2):
+            plan.set_tuning(row_split=split)
+            if components == 1:
+                form.plan = plan
+                v = form.assemble_tensor_device(Ad, precision="fp64", symmetric=True).clone().cpu().numpy()
+                raw[split] = v[:len(vals)]
+                assert pipeline.relative_frobenius(raw[split], vals) <= FP64_TOL
+            else:
+                basis._plan = plan
+                K = sk.asm(elasticity, basis, mu=mu[:, None], lam=lam[:, None])
+                check_csr(K, ip, ix, vals)
+                raw[split] = K.data.copy()
+        say(f"budget {budget}: waves {plan.waves}")
+        np.testing.assert_array_equal(raw[1], raw[0])
+        np.testing.assert_array_equal(raw[2], raw[0])
 
 
 def test_vector_p2_elasticity_matches_oracle():

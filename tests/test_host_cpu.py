@@ -44,7 +44,7 @@ def test_header_constants_match_host():
     assert int(consts["SKFA_FSAI_FRAG"]) == _native.FSAI_FRAG == 8 * 64 * 4 + 2 * 8 * 4 * 64 * 8
     assert int(consts["SKFA_FLAG_SYMMETRIC_TENSOR"]) == _native.FLAG_SYMMETRIC_TENSOR
     from skfemntv._assembly import _TUNE_KEYS
-    for key in ("XCD_CHUNK", "LDS_ATOMIC", "PLACEMENT", "STREAMING", "ELEMENT_TABLE", "ENTRY_SPLIT", "MFMA"):
+    for key in ("XCD_CHUNK", "LDS_ATOMIC", "PLACEMENT", "STREAMING", "ELEMENT_TABLE", "ROW_SPLIT", "MFMA"):
         assert int(consts[f"SKFA_TUNE_{key}"]) == getattr(_native, f"TUNE_{key}") == _TUNE_KEYS[key.lower()]
 
 
