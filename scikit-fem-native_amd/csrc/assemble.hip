@@ -297,7 +297,12 @@ struct PayloadArgs {
     double* out;
 };
 
-template <int C, bool SYM>
+// Storage of the symmetric scalar payload (C == 1, SYM): 8 doubles (D upper
+// triangle, |det| m, pad), 6 doubles (D only: forms without a mass term) or
+// 8 floats (D, |det| m, pad; the FP32-products gather rounds D to FP32 anyway)
+constexpr int SYMOUT_F64 = 0, SYMOUT_F64_NOMASS = 1, SYMOUT_F32 = 2;
+
+template <int C, bool SYM, int SYMOUT = SYMOUT_F64>
 __global__ void __launch_bounds__(256) k_payload_h1(PayloadArgs A) {
     constexpr int P = C == 1 ? (SYM ? 8 : 10) : 14;
     for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < A.E; e += int64_t(gridDim.x) * blockDim.x) {
@@ -334,9 +339,22 @@ __global__ void __launch_bounds__(256) k_payload_h1(PayloadArgs A) {
                     Dm[i * 3 + j] = inv * (T[i * 3 + 0] * a[j * 3 + 0] + T[i * 3 + 1] * a[j * 3 + 1] + T[i * 3 + 2] * a[j * 3 + 2]);
             if (SYM) {
                 // X symmetric => D symmetric; store the upper triangle
-                o[0] = Dm[0]; o[1] = Dm[1]; o[2] = Dm[2]; o[3] = Dm[4]; o[4] = Dm[5]; o[5] = Dm[8];
-                o[6] = ad * m;
-                o[7] = 0.;
+                if constexpr (SYMOUT == SYMOUT_F32) {
+                    float4* of = reinterpret_cast<float4*>(A.out) + e * 2;
+                    of[0] = make_float4(float(Dm[0]), float(Dm[1]), float(Dm[2]), float(Dm[4]));
+                    of[1] = make_float4(float(Dm[5]), float(Dm[8]), float(ad * m), 0.f);
+                } else if constexpr (SYMOUT == SYMOUT_F64_NOMASS) {
+                    double2* od = reinterpret_cast<double2*>(A.out) + e * 3;
+                    od[0] = make_double2(Dm[0], Dm[1]);
+                    od[1] = make_double2(Dm[2], Dm[4]);
+                    od[2] = make_double2(Dm[5], Dm[8]);
+                } else {
+                    double2* od = reinterpret_cast<double2*>(o);
+                    od[0] = make_double2(Dm[0], Dm[1]);
+                    od[1] = make_double2(Dm[2], Dm[4]);
+                    od[2] = make_double2(Dm[5], Dm[8]);
+                    od[3] = make_double2(ad * m, 0.);
+                }
             } else {
 #pragma unroll
                 for (int t = 0; t < 9; ++t) o[t] = Dm[t];
@@ -421,13 +439,18 @@ __device__ inline void grad_row(int q, int a, const T* D, const T* __restrict__ 
     }
 }
 
-// Payload layouts of the gather kernel (doubles per element).
-//   PL_FULL  (C==1): D (9, row-major), |det| m
-//   PL_SYM   (C==1, symmetric D): D00 D01 D02 D11 D12 D22, |det| m, pad
-//   PL_VEC   (C==3): B (9), g + s/2, s/2, lam, |det| m, pad
-constexpr int PL_FULL = 0, PL_SYM = 1, PL_VEC = 3;
+// Payload layouts of the gather kernel (values per element).
+//   PL_FULL    (C==1): D (9, row-major), |det| m                 10 doubles
+//   PL_SYM     (C==1, symmetric D): D00 D01 D02 D11 D12 D22, |det| m, pad
+//   PL_SYM6    (C==1, symmetric D, no mass term): D00 .. D22      6 doubles
+//   PL_SYM32   (C==1, symmetric D, FP32 products): as PL_SYM      8 floats
+//   PL_VEC     (C==3): B (9), g + s/2, s/2, lam, |det| m, pad    14 doubles
+constexpr int PL_FULL = 0, PL_SYM = 1, PL_SYM6 = 2, PL_VEC = 3, PL_SYM32 = 4;
 template <int PL>
-struct PayloadLayout { static constexpr int stride = PL == PL_FULL ? 10 : PL == PL_SYM ? 8 : 14; };
+struct PayloadLayout {
+    static constexpr int stride = PL == PL_FULL ? 10 : PL == PL_SYM ? 8 : PL == PL_SYM6 ? 6 : PL == PL_SYM32 ? 8 : 14;
+    using type = typename std::conditional<PL == PL_SYM32, float, double>::type;
+};
 
 template <int K, int RB>
 __device__ inline void load_records(const GatherArgs& A, const Lane& L, int64_t st, Record<RB>* rec) {
@@ -440,17 +463,30 @@ __device__ inline void load_records(const GatherArgs& A, const Lane& L, int64_t 
     }
 }
 
-template <int K, int RB, int P>
-__device__ inline void load_payloads(const GatherArgs& A, const Record<RB>* rec, double (*pl)[P]) {
+template <int K, int RB, int P, typename PT>
+__device__ inline void load_payloads(const GatherArgs& A, const Record<RB>* rec, PT (*pl)[P]) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (rec[k].packed >= 0) {
-            const double2* src = reinterpret_cast<const double2*>(A.payload + int64_t(rec[k].packed >> 4) * P);
+            if constexpr (sizeof(PT) == 4) {
+                static_assert(P % 4 == 0, "float payloads are read as float4");
+                const float4* src = reinterpret_cast<const float4*>(A.payload) + int64_t(rec[k].packed >> 4) * (P / 4);
 #pragma unroll
-            for (int t = 0; t < P / 2; ++t) {
-                const double2 v = src[t];
-                pl[k][2 * t] = v.x;
-                pl[k][2 * t + 1] = v.y;
+                for (int t = 0; t < P / 4; ++t) {
+                    const float4 v = src[t];
+                    pl[k][4 * t] = v.x;
+                    pl[k][4 * t + 1] = v.y;
+                    pl[k][4 * t + 2] = v.z;
+                    pl[k][4 * t + 3] = v.w;
+                }
+            } else {
+                const double2* src = reinterpret_cast<const double2*>(A.payload + int64_t(rec[k].packed >> 4) * P);
+#pragma unroll
+                for (int t = 0; t < P / 2; ++t) {
+                    const double2 v = src[t];
+                    pl[k][2 * t] = v.x;
+                    pl[k][2 * t + 1] = v.y;
+                }
             }
         }
     }
@@ -467,7 +503,8 @@ __device__ inline void load_payloads(const GatherArgs& A, const Record<RB>* rec,
 constexpr int SP_INCIDENCE = 0, SP_ENTRY = 1, SP_COLUMN = 2;
 
 template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, int SP>
-__device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Record<RB>& rec, const double* pl,
+__device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Record<RB>& rec,
+                                    const typename PayloadLayout<PL>::type* pl,
                                     double* span_lds, int Qg, int Qm, bool mass,
                                     const typename std::conditional<F32, float, double>::type* dg,
                                     const typename std::conditional<F32, float, double>::type* pm,
@@ -483,11 +520,11 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
 #pragma unroll
         for (int t = 0; t < 9; ++t) D[t] = T(pl[t]);
         mf = T(pl[9]);
-    } else if constexpr (PL == PL_SYM) {
+    } else if constexpr (PL == PL_SYM || PL == PL_SYM6 || PL == PL_SYM32) {
         D[0] = T(pl[0]); D[1] = T(pl[1]); D[2] = T(pl[2]);
         D[3] = T(pl[1]); D[4] = T(pl[3]); D[5] = T(pl[4]);
         D[6] = T(pl[2]); D[7] = T(pl[4]); D[8] = T(pl[5]);
-        mf = T(pl[6]);
+        mf = PL == PL_SYM6 ? T(0) : T(pl[PL == PL_SYM6 ? 0 : 6]);
     } else {
         const double* B = pl;
         const double iso = (L.c == L.d) ? pl[9] : 0.;
@@ -608,8 +645,9 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
         L.ladd = L.p;
         nsteps = (L.steps + L.S - 1) / L.S;
     }
+    using PT = typename PayloadLayout<PL>::type;
     Record<RB> r0[K], r1[K], r2[K];
-    double p0[K][P], p1[K][P];
+    PT p0[K][P], p1[K][P];
     load_records<K, RB>(A, L, 0, r0);
     zero_span(span_lds, L.span, lane);
     // stage the lane-varying tables (rows a of dg and pm) behind the span
@@ -1375,24 +1413,33 @@ static int ensure_payload(skfa_plan* P, int stride) {
     return P->payload.alloc(need ? need : 1);
 }
 
+// Record layouts a plan produces (4 + n * offset bytes, rounded up to 8, 16
+// or 32): 4-node elements (8, 1) and (16, 2), 10-node elements (16, 1) and
+// (32, 2).  Only those are instantiated.
+template <int NL, typename F>
+static int with_record_layout(const skfa_plan* P, F&& launch) {
+    using std::integral_constant;
+    if constexpr (NL == 4) {
+        if (P->rec_bytes == 8 && P->off_bytes == 1) return launch(integral_constant<int, 8>{}, integral_constant<int, 1>{});
+        if (P->rec_bytes == 16 && P->off_bytes == 2) return launch(integral_constant<int, 16>{}, integral_constant<int, 2>{});
+    } else {
+        if (P->rec_bytes == 16 && P->off_bytes == 1) return launch(integral_constant<int, 16>{}, integral_constant<int, 1>{});
+        if (P->rec_bytes == 32 && P->off_bytes == 2) return launch(integral_constant<int, 32>{}, integral_constant<int, 2>{});
+    }
+    set_error("unsupported gather record layout");
+    return kUnsupported;
+}
+
 template <int NL, int C, int PL, bool F32, int QG, int SP>
 static int launch_gather_k(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
     const dim3 grid = wave_grid(P), block(64);
     if (SP == SP_ENTRY) lds += (size_t(A.Qg) * NL * 3 + size_t(A.Qm) * NL) * (F32 ? 4 : 8);
-    if (P->rec_bytes == 8 && P->off_bytes == 1)
-        k_gather_h1<NL, C, PL, F32, 8, 1, QG, SP><<<grid, block, lds, s>>>(A);
-    else if (P->rec_bytes == 16 && P->off_bytes == 1)
-        k_gather_h1<NL, C, PL, F32, 16, 1, QG, SP><<<grid, block, lds, s>>>(A);
-    else if (P->rec_bytes == 16 && P->off_bytes == 2)
-        k_gather_h1<NL, C, PL, F32, 16, 2, QG, SP><<<grid, block, lds, s>>>(A);
-    else if (P->rec_bytes == 32 && P->off_bytes == 2)
-        k_gather_h1<NL, C, PL, F32, 32, 2, QG, SP><<<grid, block, lds, s>>>(A);
-    else {
-        set_error("unsupported gather record layout");
-        return kUnsupported;
-    }
-    SKFA_LAUNCH_CHECK();
-    return kOk;
+    return with_record_layout<NL>(P, [&](auto rb, auto ow) {
+        constexpr int RB = decltype(rb)::value, OFFW = decltype(ow)::value;
+        k_gather_h1<NL, C, PL, F32, RB, OFFW, QG, SP><<<grid, block, lds, s>>>(A);
+        SKFA_LAUNCH_CHECK();
+        return kOk;
+    });
 }
 
 // how the S lanes of a row share its work, from the plan's tuning
@@ -1422,20 +1469,12 @@ static int launch_table(const skfa_plan* P, const GatherArgs& A, hipStream_t s) 
     const size_t lds = tab_bytes + size_t(W) * span_stride * 8;
     SKFA_REQUIRE(lds <= 160 * 1024, "LDS request exceeds 160 KiB");
     const dim3 grid(unsigned(std::max(nblocks, 1))), block(64 * W);
-    if (P->rec_bytes == 8 && P->off_bytes == 1)
-        k_gather_table<NL, F32, 8, 1, MASS, W><<<grid, block, lds, s>>>(A, span_stride, nblocks);
-    else if (P->rec_bytes == 16 && P->off_bytes == 1)
-        k_gather_table<NL, F32, 16, 1, MASS, W><<<grid, block, lds, s>>>(A, span_stride, nblocks);
-    else if (P->rec_bytes == 16 && P->off_bytes == 2)
-        k_gather_table<NL, F32, 16, 2, MASS, W><<<grid, block, lds, s>>>(A, span_stride, nblocks);
-    else if (P->rec_bytes == 32 && P->off_bytes == 2)
-        k_gather_table<NL, F32, 32, 2, MASS, W><<<grid, block, lds, s>>>(A, span_stride, nblocks);
-    else {
-        set_error("unsupported gather record layout");
-        return kUnsupported;
-    }
-    SKFA_LAUNCH_CHECK();
-    return kOk;
+    return with_record_layout<NL>(P, [&](auto rb, auto ow) {
+        constexpr int RB = decltype(rb)::value, OFFW = decltype(ow)::value;
+        k_gather_table<NL, F32, RB, OFFW, MASS, W><<<grid, block, lds, s>>>(A, span_stride, nblocks);
+        SKFA_LAUNCH_CHECK();
+        return kOk;
+    });
 }
 
 template <int NL, int PL, bool F32>
@@ -1444,20 +1483,28 @@ static int launch_mfma(const skfa_plan* P, const GatherArgs& A, hipStream_t s) {
     const size_t span = size_t((P->lds_bytes / 8 + 1) & ~1) * 8;
     const size_t lds = span + (size_t(A.Qg) * NL * 3 + size_t(A.Qm) * NL) * (F32 ? 4 : 8);
     SKFA_REQUIRE(lds <= 160 * 1024, "LDS request exceeds 160 KiB");
-    if (P->rec_bytes == 8 && P->off_bytes == 1)
-        k_gather_mfma<NL, PL, F32, 8, 1><<<grid, block, lds, s>>>(A);
-    else if (P->rec_bytes == 16 && P->off_bytes == 1)
-        k_gather_mfma<NL, PL, F32, 16, 1><<<grid, block, lds, s>>>(A);
-    else if (P->rec_bytes == 16 && P->off_bytes == 2)
-        k_gather_mfma<NL, PL, F32, 16, 2><<<grid, block, lds, s>>>(A);
-    else if (P->rec_bytes == 32 && P->off_bytes == 2)
-        k_gather_mfma<NL, PL, F32, 32, 2><<<grid, block, lds, s>>>(A);
-    else {
-        set_error("unsupported gather record layout");
-        return kUnsupported;
-    }
-    SKFA_LAUNCH_CHECK();
-    return kOk;
+    return with_record_layout<NL>(P, [&](auto rb, auto ow) {
+        constexpr int RB = decltype(rb)::value, OFFW = decltype(ow)::value;
+        k_gather_mfma<NL, PL, F32, RB, OFFW><<<grid, block, lds, s>>>(A);
+        SKFA_LAUNCH_CHECK();
+        return kOk;
+    });
+}
+
+static bool mfma_gather(const skfa_plan* P, int nqg, int nqm) {
+    return P->tune_mfma > 0 && nqg <= 4 && nqm <= 4;
+}
+
+static bool table_gather(const skfa_plan* P) {
+    return P->tune_table < 0 ? P->n == 4 : P->tune_table != 0;
+}
+
+// Payload layout of a symmetric scalar form: the MFMA and element-table
+// gathers read PL_SYM; the quadrature gather reads the compact layouts
+static int sym_layout(const skfa_plan* P, int nqg, int nqm, bool f32, bool mass) {
+    if (mfma_gather(P, nqg, nqm) || table_gather(P)) return PL_SYM;
+    if (f32) return PL_SYM32;
+    return mass ? PL_SYM : PL_SYM6;
 }
 
 template <int NL, int C, bool F32>
@@ -1467,7 +1514,7 @@ static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, size_t lds
     // MFMA row group, the long-row waves where column-split lanes would
     // recompute whole element rows), the rest by the column-split gather
     const int mode = P->tune_mfma < 0 ? 0 : P->tune_mfma;
-    if (mode != 0 && A.Qg <= 4 && A.Qm <= 4) {
+    if (mfma_gather(P, A.Qg, A.Qm)) {
         GatherArgs M = A;
         if (mode == 2) M.rmax = 16;
         if (int rc = sym ? launch_mfma<NL, PL_SYM, F32>(P, M, s) : launch_mfma<NL, PL_FULL, F32>(P, M, s)) return rc;
@@ -1479,8 +1526,7 @@ static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, size_t lds
     }
     // element tables win for 4-node elements (few table rows, long column
     // split); the quadrature gather with uniform table reads wins for P2
-    const bool table = P->tune_table < 0 ? NL == 4 : P->tune_table != 0;
-    if (sym && table) {
+    if (sym && table_gather(P)) {
         // element tables from the reduced quadrature tables, then the
         // table-driven gather
         const size_t nt = size_t(NL) * NL * kNK;
@@ -1495,27 +1541,27 @@ static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, size_t lds
         T.tab32 = P->ntab32.p;
         return A.has_mass ? launch_table<NL, F32, true>(P, T, s) : launch_table<NL, F32, false>(P, T, s);
     }
-    if (sym) return launch_gather_pl<NL, 1, PL_SYM, F32>(P, A, lds, s);
+    if (sym) {
+        const int lay = sym_layout(P, A.Qg, A.Qm, F32, A.has_mass != 0);
+        if constexpr (F32) {
+            if (lay == PL_SYM32) return launch_gather_pl<NL, 1, PL_SYM32, true>(P, A, lds, s);
+        } else {
+            if (lay == PL_SYM6) return launch_gather_pl<NL, 1, PL_SYM6, false>(P, A, lds, s);
+        }
+        return launch_gather_pl<NL, 1, PL_SYM, F32>(P, A, lds, s);
+    }
     return launch_gather_pl<NL, 1, PL_FULL, F32>(P, A, lds, s);
 }
 
 template <int NL, int C, bool ISO>
 static int launch_general(const skfa_plan* P, const H1Args& A, size_t lds, hipStream_t s) {
     const dim3 grid = wave_grid(P), block(64);
-    if (P->rec_bytes == 8 && P->off_bytes == 1)
-        k_assemble_h1_general<NL, C, ISO, 8, 1><<<grid, block, lds, s>>>(A);
-    else if (P->rec_bytes == 16 && P->off_bytes == 1)
-        k_assemble_h1_general<NL, C, ISO, 16, 1><<<grid, block, lds, s>>>(A);
-    else if (P->rec_bytes == 16 && P->off_bytes == 2)
-        k_assemble_h1_general<NL, C, ISO, 16, 2><<<grid, block, lds, s>>>(A);
-    else if (P->rec_bytes == 32 && P->off_bytes == 2)
-        k_assemble_h1_general<NL, C, ISO, 32, 2><<<grid, block, lds, s>>>(A);
-    else {
-        set_error("unsupported gather record layout");
-        return kUnsupported;
-    }
-    SKFA_LAUNCH_CHECK();
-    return kOk;
+    return with_record_layout<NL>(P, [&](auto rb, auto ow) {
+        constexpr int RB = decltype(rb)::value, OFFW = decltype(ow)::value;
+        k_assemble_h1_general<NL, C, ISO, RB, OFFW><<<grid, block, lds, s>>>(A);
+        SKFA_LAUNCH_CHECK();
+        return kOk;
+    });
 }
 
 template <int NL, int C>
@@ -1579,7 +1625,12 @@ int skfa_assemble_bilinear(skfa_plan* P, const double* xyz, int64_t npoints, con
         pa.E = P->E;
         pa.cm = c_value; pa.cg = c_gradient; pa.cs = c_symgrad; pa.cl = c_divergence; pa.cA = c_tensor;
         pa.out = P->payload.p;
-        if (P->C == 1 && sym)
+        const int lay = sym ? sym_layout(P, nqg, nqm, precision == 1, c_value != nullptr) : PL_FULL;
+        if (P->C == 1 && lay == PL_SYM32)
+            k_payload_h1<1, true, SYMOUT_F32><<<elem_grid(P->E), 256, 0, s>>>(pa);
+        else if (P->C == 1 && lay == PL_SYM6)
+            k_payload_h1<1, true, SYMOUT_F64_NOMASS><<<elem_grid(P->E), 256, 0, s>>>(pa);
+        else if (P->C == 1 && sym)
             k_payload_h1<1, true><<<elem_grid(P->E), 256, 0, s>>>(pa);
         else if (P->C == 1)
             k_payload_h1<1, false><<<elem_grid(P->E), 256, 0, s>>>(pa);
