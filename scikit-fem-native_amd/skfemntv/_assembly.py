@@ -101,9 +101,9 @@ def basis_plan(basis):
         d = basis.device_data()
         C = basis.elem._dim
         # vector fields carry 9-entry node blocks: a larger span per wave keeps
-        # more rows per wave (measured: 48^3 vector P2 2.8 ms at 32 KiB vs
-        # 5.3 ms at 12 KiB; scalar P2 is fastest at 12-16 KiB)
-        budget = 32 * 1024 if C == 3 else 12 * 1024
+        # more rows per wave (measured on 48^3 vector P2: 2.50 ms at 16 KiB,
+        # 2.72 ms at 32 KiB, 2.97 ms at 12 KiB; scalar P2 is fastest at 12 KiB)
+        budget = 16 * 1024 if C == 3 else 12 * 1024
         plan = Plan(d["node_conn"], basis.mesh.nelements, d["node_conn"].shape[1], d["nnodes"], C,
                     lds_budget=budget)
         basis._plan = plan
