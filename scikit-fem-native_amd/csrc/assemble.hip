@@ -699,6 +699,175 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
     write_span(span_lds, L.span, lane, A.values + L.lo * C * C, A.plan.streaming);
 }
 
+// ------------------------------------------------------------------ vector P2 gather
+// 10-node elements, 3 components, element-constant coefficients (payload
+// PL_VEC: B = adj / sqrt|det|, iso = g + s/2, hs = s/2, lam, |det| m).
+// Every (c, d) block entry of the node pair (a, b) derives from one 3x3
+// matrix of the pair:
+//   G_ab = sum_q w_q (B^T dphi_a(q)) (B^T dphi_b(q))^T
+//   K_(a,c),(b,d) = lam G_cd + hs G_dc + [c == d] (iso tr G_ab + |det| m M_ab)
+// (the form of the lane-per-(c,d) gather's D_cd, contracted once per pair).
+// Lane roles: node row i of the wave, column node b of the element row,
+// incidence part p (lanes per row = 10 S); the lane's b is fixed for the
+// wave, so its reference gradients and mass values live in registers.
+// In-row offset of a fixed local node b: the record word holding it is
+// picked with per-lane masks (all-ones for that word) rather than an index,
+// which keeps the record in registers.
+template <int RB, int OFFW>
+struct OffsetPick {
+    static constexpr int NW = (RB - 4) / 4;
+    uint32_t mask[NW];
+    int shift;
+    __device__ explicit OffsetPick(int b) {
+        constexpr int per = 4 / OFFW;
+#pragma unroll
+        for (int t = 0; t < NW; ++t) mask[t] = (b / per == t) ? 0xffffffffu : 0u;
+        shift = (b % per) * 8 * OFFW;
+    }
+    __device__ int operator()(const Record<RB>& r) const {
+        uint32_t word = 0;
+#pragma unroll
+        for (int t = 0; t < NW; ++t) {
+            uint32_t wt;
+            __builtin_memcpy(&wt, r.off + 4 * t, 4);
+            word |= wt & mask[t];
+        }
+        return int((word >> shift) & (OFFW == 1 ? 0xffu : 0xffffu));
+    }
+};
+
+template <bool F32, int RB, int OFFW, int QG>
+__global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
+    extern __shared__ double span_lds[];
+    // one step in flight: a PL_VEC payload is 14 doubles per lane
+    constexpr int NL = 10, C = 3, P = 14, K = 1;
+    using T = typename std::conditional<F32, float, double>::type;
+    const int lane = threadIdx.x;
+    const int wv = placed_wave(A.plan.xcd_seg, int(A.plan.nwaves), A.plan.placement, A.plan.xcd_chunk);
+    if (wv < 0) return;
+    const PlanArgs& PA = A.plan;
+    const int64_t r0w = PA.wave_r0[wv], r1w = PA.wave_r0[wv + 1];
+    Lane L;  // record addressing of load_records
+    L.R = int(r1w - r0w);
+    L.S = PA.wave_split[wv];
+    const int lpr = NL * L.S;
+    L.i = lane / lpr;
+    const int rem = lane - L.i * lpr;
+    const int b = rem / L.S;
+    L.p = rem - b * L.S;
+    L.c = L.d = 0;
+    L.lmul = L.S;
+    L.ladd = L.p;
+    L.active = L.i < L.R;
+    L.rec0 = PA.wave_rec[wv];
+    L.steps = (PA.wave_rec[wv + 1] - L.rec0) / L.R;
+    L.lo = PA.indptr[r0w];
+    L.span = (PA.indptr[r1w] - L.lo) * C * C;
+    L.rb = 0;
+    int row_len = 0;
+    if (L.active) {
+        const int64_t rn = r0w + L.i;
+        row_len = int(PA.indptr[rn + 1] - PA.indptr[rn]);
+        L.rb = int64_t(C) * C * (PA.indptr[rn] - L.lo);
+    }
+    const int64_t nsteps = (L.steps + L.S - 1) / L.S;
+    Record<RB> r0[K], r1[K], r2[K];
+    double p0[K][P], p1[K][P];
+    load_records<K, RB>(A, L, 0, r0);
+    zero_span(span_lds, L.span, lane);
+    // weighted rows w_q dphi_a(q) and w_q phi_a(q) (lane-varying a) behind the span
+    const int Qm = A.Qm;
+    const int lg = QG * NL * 3;
+    const T* src = F32 ? reinterpret_cast<const T*>(A.tab32) : reinterpret_cast<const T*>(A.tab);
+    T* wdg = reinterpret_cast<T*>(span_lds + L.span);
+    T* wpm = wdg + lg;
+    for (int k = lane; k < lg + Qm * NL; k += 64) {
+        if (k < lg)
+            wdg[k] = src[QG + k] * src[k / (NL * 3)];
+        else
+            wpm[k - lg] = src[QG + lg + Qm + (k - lg)] * src[QG + lg + (k - lg) / NL];
+    }
+    // the lane's column node b: unweighted gradients and values
+    T gb[QG][3];
+#pragma unroll
+    for (int q = 0; q < QG; ++q)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) gb[q][j] = src[QG + (q * NL + b) * 3 + j];
+    load_payloads<K, RB, P>(A, r0, p0);
+    load_records<K, RB>(A, L, K, r1);
+    __syncthreads();
+    const bool mass = A.has_mass != 0;
+    const T* phb = src + QG + lg + Qm + b;   // phi_b(q) at stride NL
+    const OffsetPick<RB, OFFW> pick(b);
+    for (int64_t st = 0; st < nsteps; st += K) {
+        load_payloads<K, RB, P>(A, r1, p1);
+        load_records<K, RB>(A, L, st + 2 * K, r2);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (r0[k].packed < 0) continue;
+            const double* pl = p0[k];
+            const int a = r0[k].packed & 15;
+            T B[9];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) B[t] = T(pl[t]);
+            T G[3][3];
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+#pragma unroll
+                for (int y = 0; y < 3; ++y) G[x][y] = T(0);
+#pragma unroll
+            for (int q = 0; q < QG; ++q) {
+                const T* ga = wdg + (q * NL + a) * 3;
+                const T g0 = ga[0], g1 = ga[1], g2 = ga[2];
+                T va[3], vb[3];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    va[x] = fma(B[0 * 3 + x], g0, fma(B[1 * 3 + x], g1, B[2 * 3 + x] * g2));
+                    vb[x] = fma(B[0 * 3 + x], gb[q][0], fma(B[1 * 3 + x], gb[q][1], B[2 * 3 + x] * gb[q][2]));
+                }
+#pragma unroll
+                for (int x = 0; x < 3; ++x)
+#pragma unroll
+                    for (int y = 0; y < 3; ++y) G[x][y] = fma(va[x], vb[y], G[x][y]);
+            }
+            const T iso = T(pl[9]), hs = T(pl[10]), lam = T(pl[11]);
+            T diag = iso * ((G[0][0] + G[1][1]) + G[2][2]);
+            if (mass) {
+                T mab = T(0);
+                for (int q = 0; q < Qm; ++q) mab = fma(wpm[q * NL + a], phb[q * NL], mab);
+                diag = fma(T(pl[12]), mab, diag);
+            }
+            double v[3][3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    T e = fma(lam, G[c][d], hs * G[d][c]);
+                    if (c == d) e += diag;
+                    v[c][d] = double(e);
+                }
+            double* col = span_lds + int(L.rb) + C * pick(r0[k]);
+            // the S parts of a row add in turn (one writer per address)
+            for (int turn = 0; turn < L.S; ++turn)
+                if (L.p == turn) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) lds_add(col + c * C * row_len + d, v[c][d], A.plan.lds_atomic);
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            r0[k] = r1[k];
+            r1[k] = r2[k];
+#pragma unroll
+            for (int t = 0; t < P; ++t) p0[k][t] = p1[k][t];
+        }
+    }
+    __syncthreads();
+    write_span(span_lds, L.span, lane, A.values + L.lo * C * C, A.plan.streaming);
+}
+
 // ------------------------------------------------------------------ MFMA gather
 // Scalar fields (C == 1), element-constant coefficients, up to 4 points in the
 // reduced gradient and mass rules.  A wave's rows are taken 16 at a time:
@@ -1491,6 +1660,17 @@ static int launch_mfma(const skfa_plan* P, const GatherArgs& A, hipStream_t s) {
     });
 }
 
+template <bool F32>
+static int launch_vec10(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
+    const dim3 grid = wave_grid(P), block(64);
+    return with_record_layout<10>(P, [&](auto rb, auto ow) {
+        constexpr int RB = decltype(rb)::value, OFFW = decltype(ow)::value;
+        k_gather_vec10<F32, RB, OFFW, 4><<<grid, block, lds, s>>>(A);
+        SKFA_LAUNCH_CHECK();
+        return kOk;
+    });
+}
+
 static bool mfma_gather(const skfa_plan* P, int nqg, int nqm) {
     return P->tune_mfma > 0 && nqg <= 4 && nqm <= 4;
 }
@@ -1509,7 +1689,12 @@ static int sym_layout(const skfa_plan* P, int nqg, int nqm, bool f32, bool mass)
 
 template <int NL, int C, bool F32>
 static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, size_t lds, hipStream_t s) {
-    if constexpr (C == 3) return launch_gather_pl<NL, 3, PL_VEC, F32>(P, A, lds, s);
+    if constexpr (C == 3) {
+        if constexpr (NL == 10) {
+            if (A.Qg == 4) return launch_vec10<F32>(P, A, lds, s);
+        }
+        return launch_gather_pl<NL, 3, PL_VEC, F32>(P, A, lds, s);
+    }
     // MFMA row contraction: 1 = every wave; 2 = waves of at most 16 rows (one
     // MFMA row group, the long-row waves where column-split lanes would
     // recompute whole element rows), the rest by the column-split gather

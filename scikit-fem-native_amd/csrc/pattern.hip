@@ -301,6 +301,10 @@ static int plan_build(skfa_plan* P, const int32_t* conn, hipStream_t s) {
     SKFA_HIP(hipMemcpyAsync(hinc.data(), P->inc_ptr.p, sizeof(int64_t) * (nn + 1), hipMemcpyDeviceToHost, s));
     SKFA_HIP(hipStreamSynchronize(s));
     const int cc = P->C * P->C;
+    // lanes per node row and part: C*C (one lane per component pair), or 10
+    // for 10-node 3-component fields, whose gather gives each column node
+    // of an element row its own lane
+    const int unit = (P->C == 3 && P->n == 10) ? 10 : cc;
     std::vector<int64_t> w_r0, w_rec;
     std::vector<int32_t> w_split;
     std::vector<int64_t> r_rec(static_cast<size_t>(nn));
@@ -309,10 +313,9 @@ static int plan_build(skfa_plan* P, const int32_t* conn, hipStream_t s) {
     int64_t r = 0;
     while (r < nn) {
         int chosen_R = 0, chosen_S = 0;
-        for (int S = 1; S <= 64 / cc; S <<= 1) {
-            // power-of-two splits: a lane owns the in-row offsets o with
-            // (o & (S-1)) == part
-            int R = 64 / (cc * S);
+        for (int S = 1; S <= 64 / unit; S <<= 1) {
+            // power-of-two splits: the S parts of a row share its work
+            int R = 64 / (unit * S);
             if (R < 1) break;
             const int64_t r1 = std::min<int64_t>(r + R, nn);
             const int64_t bytes = (hptr[size_t(r1)] - hptr[size_t(r)]) * cc * 8;

@@ -233,7 +233,7 @@ def test_row_splits_agree_at_small_lds_budgets(components):
 
 def test_vector_p2_elasticity_matches_oracle():
     import skfemntv as sk
-    from skfemntv.helpers import ddot, div, sym_grad
+    from skfemntv.helpers import ddot, div, dot, grad, sym_grad
     say("vector p2 elasticity")
     p, t = kuhn(3, 0.15, seed=2)
     p2, t2 = host.p2_mesh(p, t)
@@ -261,6 +261,24 @@ def test_vector_p2_elasticity_matches_oracle():
         r = np.zeros(N)
         r[c::3] = 1.
         assert np.abs(K @ r).max() <= 1e-11 * np.abs(K.data).max()
+    # with a mass term (rho u . v), a gradient term and FP32 products
+    rho = np.exp(rng.standard_normal(E))
+    kap = np.exp(rng.standard_normal(E))
+
+    @sk.BilinearForm
+    def dynamic(u, v, w):
+        return (2. * w.mu * ddot(sym_grad(u), sym_grad(v)) + w.lam * (div(u) * div(v))
+                + w.kap * ddot(grad(u), grad(v)) + w.rho * dot(u, v))
+
+    say("vector p2 elasticity + gradient + mass")
+    K2 = sk.asm(dynamic, basis, mu=mu[:, None], lam=lam[:, None], kap=kap[:, None], rho=rho[:, None])
+    ip, ix, vals2, N = pipeline.bilinear(p2, t2, X, W, True, components=3,
+                                         value=np.repeat(rho[:, None], Q, 1),
+                                         gradient=np.repeat(kap[:, None], Q, 1),
+                                         symgrad=np.repeat(2 * mu[:, None], Q, 1),
+                                         divergence=np.repeat(lam[:, None], Q, 1))
+    err2 = check_csr(K2, ip, ix, vals2)
+    say(f"elasticity + mass rel err {err2:.3e}")
 
 
 def test_p2_mass_and_perqp_tensor_match_oracle():
