@@ -452,8 +452,8 @@ struct PayloadLayout {
     using type = typename std::conditional<PL == PL_SYM32, float, double>::type;
 };
 
-template <int K, int RB>
-__device__ inline void load_records(const GatherArgs& A, const Lane& L, int64_t st, Record<RB>* rec) {
+template <int K, int RB, class Args>
+__device__ inline void load_records(const Args& A, const Lane& L, int64_t st, Record<RB>* rec) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         rec[k].packed = -1;
@@ -463,8 +463,8 @@ __device__ inline void load_records(const GatherArgs& A, const Lane& L, int64_t 
     }
 }
 
-template <int K, int RB, int P, typename PT>
-__device__ inline void load_payloads(const GatherArgs& A, const Record<RB>* rec, PT (*pl)[P]) {
+template <int K, int RB, int P, typename PT, class Args>
+__device__ inline void load_payloads(const Args& A, const Record<RB>* rec, PT (*pl)[P]) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (rec[k].packed >= 0) {
@@ -1160,95 +1160,109 @@ struct GatherN1Args {
     double* values;
 };
 
-template <int RB, int OFFW>
+// Element row a of one incidence (the lane's record and payload) into the
+// wave's span.  QN = 4: the 4-point rule of element-constant coefficients at
+// compile time; 0: A.Q points (per-point coefficients).
+template <int RB, int OFFW, int QN>
+__device__ inline void n1_step(const GatherN1Args& A, const Lane& L, const Record<RB>& rec, const double* pl,
+                               double* span_lds, const double* phi_l, const double* curl_l) {
+    const int Q = QN > 0 ? QN : A.Q;
+    const double* W = A.tab;
+    const double* phi = W + Q;
+    const double* curl = phi + Q * 18;
+    const int a = rec.packed & 15;
+    const double* m6 = pl;
+    const double* c6 = pl + 6;
+    const int64_t bits = __double_as_longlong(pl[12]);
+    double Kr[6];
+#pragma unroll
+    for (int b = 0; b < 6; ++b) Kr[b] = 0.;
+    if (A.include_mass) {
+        const int64_t e = rec.packed >> 4;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double* pa = phi_l + (q * 6 + a) * 3;
+            const double f = A.kappa_qp ? W[q] * A.kappa_qp[e * Q + q] : W[q];
+            const double u0 = f * (pa[0] * m6[0] + pa[1] * m6[1] + pa[2] * m6[2]);
+            const double u1 = f * (pa[0] * m6[1] + pa[1] * m6[3] + pa[2] * m6[4]);
+            const double u2 = f * (pa[0] * m6[2] + pa[1] * m6[4] + pa[2] * m6[5]);
+#pragma unroll
+            for (int b = 0; b < 6; ++b) {
+                const double* pb = phi + (q * 6 + b) * 3;
+                Kr[b] += u0 * pb[0] + u1 * pb[1] + u2 * pb[2];
+            }
+        }
+    }
+    if (A.include_curl) {
+        const double* ca = curl_l + a * 3;
+        double cw = 1.;
+        if (A.mu_qp) {
+            const int64_t e = rec.packed >> 4;
+            cw = 0.;
+            for (int q = 0; q < Q; ++q) cw += W[q] * A.mu_qp[e * Q + q];
+        }
+        const double u0 = cw * (ca[0] * c6[0] + ca[1] * c6[1] + ca[2] * c6[2]);
+        const double u1 = cw * (ca[0] * c6[1] + ca[1] * c6[3] + ca[2] * c6[4]);
+        const double u2 = cw * (ca[0] * c6[2] + ca[1] * c6[4] + ca[2] * c6[5]);
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            const double* cb = curl + b * 3;
+            Kr[b] += u0 * cb[0] + u1 * cb[1] + u2 * cb[2];
+        }
+    }
+    const double sa = ((bits >> a) & 1) ? -1. : 1.;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) Kr[b] *= sa * (((bits >> b) & 1) ? -1. : 1.);
+    // one writer per address: the S parts of a row add in turn
+    double* row = span_lds + int(L.rb);
+    for (int turn = 0; turn < L.S; ++turn)
+        if (L.p == turn) {
+#pragma unroll
+            for (int b = 0; b < 6; ++b) lds_add(row + rec_offset<RB, OFFW>(rec, b), Kr[b], A.plan.lds_atomic);
+        }
+}
+
+// Software-pipelined like k_gather_h1 (payloads one step ahead, records
+// two), with the incidence split: part p of a row takes its incidences
+// p, p+S, ... and the parts add in turn, so every CSR entry receives its
+// element contributions in incidence order.
+template <int RB, int OFFW, int QN>
 __global__ void __launch_bounds__(64) k_gather_n1(GatherN1Args A) {
     extern __shared__ double span_lds[];
-    constexpr int K = 4;
+    constexpr int K = 1, P = 14;   // a 14-double payload per lane and step
     const int lane = threadIdx.x;
     Lane L;
     const int wv = placed_wave(A.plan.xcd_seg, int(A.plan.nwaves), A.plan.placement, A.plan.xcd_chunk);
     if (wv < 0) return;
     lane_setup<1>(A.plan, wv, lane, L);
+    L.lmul = L.S;
+    L.ladd = L.p;
+    const int64_t nsteps = (L.steps + L.S - 1) / L.S;
+    Record<RB> r0[K], r1[K], r2[K];
+    double p0[K][P], p1[K][P];
+    load_records<K, RB>(A, L, 0, r0);
     zero_span(span_lds, L.span, lane);
-    const int Q = A.Q;
+    const int Q = QN > 0 ? QN : A.Q;
     const int tab_len = Q + Q * 18 + 18;
     double* tab_lds = span_lds + L.span;
     for (int k = lane; k < tab_len; k += 64) tab_lds[k] = A.tab[k];
+    load_payloads<K, RB, P>(A, r0, p0);
+    load_records<K, RB>(A, L, K, r1);
     __syncthreads();
-    const double* W = A.tab;
-    const double* phi = W + Q;
-    const double* curl = phi + Q * 18;
     const double* phi_l = tab_lds + Q;
     const double* curl_l = phi_l + Q * 18;
-    for (int64_t st = 0; st < L.steps; st += K) {
-        Record<RB> rec[K];
+    for (int64_t st = 0; st < nsteps; st += K) {
+        load_payloads<K, RB, P>(A, r1, p1);
+        load_records<K, RB>(A, L, st + 2 * K, r2);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (r0[k].packed >= 0) n1_step<RB, OFFW, QN>(A, L, r0[k], p0[k], span_lds, phi_l, curl_l);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            rec[k].packed = -1;
-            if (L.active && st + k < L.steps) load_record<RB>(A.plan.records + (L.rec0 + (st + k) * L.R + L.i) * RB, rec[k]);
-        }
-        double pl[K][14];
+            r0[k] = r1[k];
+            r1[k] = r2[k];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (rec[k].packed >= 0) {
-                const double2* src = reinterpret_cast<const double2*>(A.payload + int64_t(rec[k].packed >> 4) * 14);
-#pragma unroll
-                for (int t = 0; t < 7; ++t) {
-                    const double2 v = src[t];
-                    pl[k][2 * t] = v.x;
-                    pl[k][2 * t + 1] = v.y;
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (rec[k].packed < 0) continue;
-            const int a = rec[k].packed & 15;
-            const double* m6 = pl[k];
-            const double* c6 = pl[k] + 6;
-            const int64_t bits = __double_as_longlong(pl[k][12]);
-            double Kr[6];
-#pragma unroll
-            for (int b = 0; b < 6; ++b) Kr[b] = 0.;
-            if (A.include_mass) {
-                const int64_t e = rec[k].packed >> 4;
-                for (int q = 0; q < Q; ++q) {
-                    const double* pa = phi_l + (q * 6 + a) * 3;
-                    const double f = A.kappa_qp ? W[q] * A.kappa_qp[e * Q + q] : W[q];
-                    const double u0 = f * (pa[0] * m6[0] + pa[1] * m6[1] + pa[2] * m6[2]);
-                    const double u1 = f * (pa[0] * m6[1] + pa[1] * m6[3] + pa[2] * m6[4]);
-                    const double u2 = f * (pa[0] * m6[2] + pa[1] * m6[4] + pa[2] * m6[5]);
-#pragma unroll
-                    for (int b = 0; b < 6; ++b) {
-                        const double* pb = phi + (q * 6 + b) * 3;
-                        Kr[b] += u0 * pb[0] + u1 * pb[1] + u2 * pb[2];
-                    }
-                }
-            }
-            if (A.include_curl) {
-                const double* ca = curl_l + a * 3;
-                double cw = 1.;
-                if (A.mu_qp) {
-                    const int64_t e = rec[k].packed >> 4;
-                    cw = 0.;
-                    for (int q = 0; q < Q; ++q) cw += W[q] * A.mu_qp[e * Q + q];
-                }
-                const double u0 = cw * (ca[0] * c6[0] + ca[1] * c6[1] + ca[2] * c6[2]);
-                const double u1 = cw * (ca[0] * c6[1] + ca[1] * c6[3] + ca[2] * c6[4]);
-                const double u2 = cw * (ca[0] * c6[2] + ca[1] * c6[4] + ca[2] * c6[5]);
-#pragma unroll
-                for (int b = 0; b < 6; ++b) {
-                    const double* cb = curl + b * 3;
-                    Kr[b] += u0 * cb[0] + u1 * cb[1] + u2 * cb[2];
-                }
-            }
-            const double sa = ((bits >> a) & 1) ? -1. : 1.;
-#pragma unroll
-            for (int b = 0; b < 6; ++b) {
-                const double sb = ((bits >> b) & 1) ? -1. : 1.;
-                const int o = rec_offset<RB, OFFW>(rec[k], b);
-                if ((o & (L.S - 1)) == L.p) lds_add(span_lds + L.rb + o, sa * sb * Kr[b], A.plan.lds_atomic);
-            }
+            for (int t = 0; t < P; ++t) p0[k][t] = p1[k][t];
         }
     }
     __syncthreads();
@@ -1893,11 +1907,18 @@ int skfa_assemble_tet_n1(skfa_plan* P, const double* xyz, const int32_t* vconn, 
     A.values = values;
     const size_t lds = size_t(P->lds_bytes) + size_t(nq + nq * 18 + 18) * 8;
     const dim3 grid = wave_grid(P), block(64);
-    if (P->rec_bytes == 16 && P->off_bytes == 1)
-        k_gather_n1<16, 1><<<grid, block, lds, s>>>(A);
-    else if (P->rec_bytes == 16 && P->off_bytes == 2)
-        k_gather_n1<16, 2><<<grid, block, lds, s>>>(A);
-    else {
+    // 6 edges: 4 + 6 offset bytes (16 B records) with 1- or 2-byte offsets
+    if (P->rec_bytes == 16 && P->off_bytes == 1) {
+        if (nq == 4)
+            k_gather_n1<16, 1, 4><<<grid, block, lds, s>>>(A);
+        else
+            k_gather_n1<16, 1, 0><<<grid, block, lds, s>>>(A);
+    } else if (P->rec_bytes == 16 && P->off_bytes == 2) {
+        if (nq == 4)
+            k_gather_n1<16, 2, 4><<<grid, block, lds, s>>>(A);
+        else
+            k_gather_n1<16, 2, 0><<<grid, block, lds, s>>>(A);
+    } else {
         set_error("unsupported gather record layout");
         return kUnsupported;
     }

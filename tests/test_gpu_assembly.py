@@ -336,6 +336,13 @@ def test_tet_n1_maxwell_matches_oracle():
     assert K2 is K
     ip, ix, data, *_ = pipeline.n1_maxwell(p, t, np.full((E, Q), 2.3), np.ones((E, Q)), include_curl=False)
     check_csr(K2, ip, ix, data)
+    # a 2 KiB span budget puts 8 lanes on each edge row (incidence split):
+    # the same bits as the default plan
+    from skfemntv._assembly import Plan
+    ref = asm_.assemble_maxwell(mass_coefficient=kappa[:, None], curl_coefficient=1.).data.copy()
+    asm_.plan = Plan(basis._edges_dev, E, 6, basis.N, 1, lds_budget=2048)
+    K3 = asm_.assemble_maxwell(mass_coefficient=kappa[:, None], curl_coefficient=1.)
+    np.testing.assert_array_equal(K3.data, ref)
 
 
 def test_assembly_is_bitwise_reproducible():
