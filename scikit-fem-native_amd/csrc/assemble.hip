@@ -302,108 +302,139 @@ struct PayloadArgs {
 // 8 floats (D, |det| m, pad; the FP32-products gather rounds D to FP32 anyway)
 constexpr int SYMOUT_F64 = 0, SYMOUT_F64_NOMASS = 1, SYMOUT_F32 = 2;
 
+// Payloads are written in blocks of kPayloadBlock consecutive elements: each
+// thread puts its element's values into an LDS block buffer, and the block's
+// contiguous output range then goes out as 16-byte stores.  (A lane-per-
+// element store of a 48-112 B payload touches a different cache line per
+// lane and value.)  Every layout is a whole number of 16-byte units and the
+// blocks start at multiples of 256 elements, so the ranges are 16-byte aligned.
+constexpr int kPayloadBlock = 256;
+
+template <typename V, int PO>
+__device__ inline void flush_payload_block(const V* buf, V* out, int64_t e0, int n) {
+    static_assert((PO * sizeof(V)) % 16 == 0, "payload rows are whole 16-byte units");
+    const int nvec = n * PO * int(sizeof(V)) / 16;
+    const uint4* src = reinterpret_cast<const uint4*>(buf);
+    uint4* dst = reinterpret_cast<uint4*>(out + e0 * PO);
+    for (int k = threadIdx.x; k < nvec; k += blockDim.x) dst[k] = src[k];
+}
+
+template <int C, bool SYM, int SYMOUT>
+struct PayloadOut {
+    using V = typename std::conditional<C == 1 && SYM && SYMOUT == SYMOUT_F32, float, double>::type;
+    static constexpr int PO = C == 1 ? (SYM ? (SYMOUT == SYMOUT_F64_NOMASS ? 6 : 8) : 10) : 14;
+};
+
 template <int C, bool SYM, int SYMOUT = SYMOUT_F64>
-__global__ void __launch_bounds__(256) k_payload_h1(PayloadArgs A) {
-    constexpr int P = C == 1 ? (SYM ? 8 : 10) : 14;
-    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < A.E; e += int64_t(gridDim.x) * blockDim.x) {
-        double J[9];
-        Geo G;
-        affine_jacobian(A.xyz, A.vconn + e * 4, J);
-        adjugate(J, G);
-        const double ad = fabs(G.det);
-        const double m = A.cm ? A.cm[e] : 0.;
-        const double g = A.cg ? A.cg[e] : 0.;
-        const double s = A.cs ? A.cs[e] : 0.;
-        const double lam = A.cl ? A.cl[e] : 0.;
-        double* o = A.out + e * P;
-        const double* a = G.adj;
-        if (C == 1) {
-            double X[9];
+__global__ void __launch_bounds__(kPayloadBlock) k_payload_h1(PayloadArgs A) {
+    using V = typename PayloadOut<C, SYM, SYMOUT>::V;
+    constexpr int PO = PayloadOut<C, SYM, SYMOUT>::PO;
+    __shared__ V buf[kPayloadBlock * PO];
+    for (int64_t e0 = int64_t(blockIdx.x) * kPayloadBlock; e0 < A.E; e0 += int64_t(gridDim.x) * kPayloadBlock) {
+        const int n = int(A.E - e0 < kPayloadBlock ? A.E - e0 : kPayloadBlock);
+        if (int(threadIdx.x) < n) {
+            const int64_t e = e0 + threadIdx.x;
+            V* o = buf + threadIdx.x * PO;
+            double J[9];
+            Geo G;
+            affine_jacobian(A.xyz, A.vconn + e * 4, J);
+            adjugate(J, G);
+            const double ad = fabs(G.det);
+            const double m = A.cm ? A.cm[e] : 0.;
+            const double g = A.cg ? A.cg[e] : 0.;
+            const double s = A.cs ? A.cs[e] : 0.;
+            const double lam = A.cl ? A.cl[e] : 0.;
+            const double* a = G.adj;
+            if constexpr (C == 1) {
+                double X[9];
 #pragma unroll
-            for (int t = 0; t < 9; ++t) X[t] = A.cA ? A.cA[e * 9 + t] : 0.;
-            X[0] += g + 0.5 * s + 0.5 * s + lam;
-            X[4] += g + 0.5 * s;
-            X[8] += g + 0.5 * s;
-            double T[9];
+                for (int t = 0; t < 9; ++t) X[t] = A.cA ? A.cA[e * 9 + t] : 0.;
+                X[0] += g + 0.5 * s + 0.5 * s + lam;
+                X[4] += g + 0.5 * s;
+                X[8] += g + 0.5 * s;
+                double T[9];
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+                for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    T[i * 3 + j] = a[i * 3 + 0] * X[0 * 3 + j] + a[i * 3 + 1] * X[1 * 3 + j] + a[i * 3 + 2] * X[2 * 3 + j];
-            const double inv = 1. / ad;
-            double Dm[9];
+                    for (int j = 0; j < 3; ++j)
+                        T[i * 3 + j] = a[i * 3 + 0] * X[0 * 3 + j] + a[i * 3 + 1] * X[1 * 3 + j] + a[i * 3 + 2] * X[2 * 3 + j];
+                const double inv = 1. / ad;
+                double Dm[9];
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+                for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    Dm[i * 3 + j] = inv * (T[i * 3 + 0] * a[j * 3 + 0] + T[i * 3 + 1] * a[j * 3 + 1] + T[i * 3 + 2] * a[j * 3 + 2]);
-            if (SYM) {
-                // X symmetric => D symmetric; store the upper triangle
-                if constexpr (SYMOUT == SYMOUT_F32) {
-                    float4* of = reinterpret_cast<float4*>(A.out) + e * 2;
-                    of[0] = make_float4(float(Dm[0]), float(Dm[1]), float(Dm[2]), float(Dm[4]));
-                    of[1] = make_float4(float(Dm[5]), float(Dm[8]), float(ad * m), 0.f);
-                } else if constexpr (SYMOUT == SYMOUT_F64_NOMASS) {
-                    double2* od = reinterpret_cast<double2*>(A.out) + e * 3;
-                    od[0] = make_double2(Dm[0], Dm[1]);
-                    od[1] = make_double2(Dm[2], Dm[4]);
-                    od[2] = make_double2(Dm[5], Dm[8]);
+                    for (int j = 0; j < 3; ++j)
+                        Dm[i * 3 + j] = inv * (T[i * 3 + 0] * a[j * 3 + 0] + T[i * 3 + 1] * a[j * 3 + 1] + T[i * 3 + 2] * a[j * 3 + 2]);
+                if constexpr (SYM) {
+                    // X symmetric => D symmetric; store the upper triangle
+                    o[0] = V(Dm[0]); o[1] = V(Dm[1]); o[2] = V(Dm[2]);
+                    o[3] = V(Dm[4]); o[4] = V(Dm[5]); o[5] = V(Dm[8]);
+                    if constexpr (PO == 8) {
+                        o[6] = V(ad * m);
+                        o[7] = V(0);
+                    }
                 } else {
-                    double2* od = reinterpret_cast<double2*>(o);
-                    od[0] = make_double2(Dm[0], Dm[1]);
-                    od[1] = make_double2(Dm[2], Dm[4]);
-                    od[2] = make_double2(Dm[5], Dm[8]);
-                    od[3] = make_double2(ad * m, 0.);
+#pragma unroll
+                    for (int t = 0; t < 9; ++t) o[t] = Dm[t];
+                    o[9] = ad * m;
                 }
             } else {
+                const double sc = 1. / sqrt(ad);
 #pragma unroll
-                for (int t = 0; t < 9; ++t) o[t] = Dm[t];
-                o[9] = ad * m;
+                for (int t = 0; t < 9; ++t) o[t] = a[t] * sc;
+                o[9] = g + 0.5 * s;
+                o[10] = 0.5 * s;
+                o[11] = lam;
+                o[12] = ad * m;
+                o[13] = 0.;
             }
-        } else {
-            const double sc = 1. / sqrt(ad);
-#pragma unroll
-            for (int t = 0; t < 9; ++t) o[t] = a[t] * sc;
-            o[9] = g + 0.5 * s;
-            o[10] = 0.5 * s;
-            o[11] = lam;
-            o[12] = ad * m;
-            o[13] = 0.;
         }
+        __syncthreads();
+        flush_payload_block<V, PO>(buf, reinterpret_cast<V*>(A.out), e0, n);
+        __syncthreads();
     }
 }
 
 __constant__ int kN1Edge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
 
-__global__ void __launch_bounds__(256) k_payload_n1(PayloadArgs A) {
+__global__ void __launch_bounds__(kPayloadBlock) k_payload_n1(PayloadArgs A) {
+    constexpr int PO = 14;
+    __shared__ double buf[kPayloadBlock * PO];
     double wsum = 0.;
     for (int q = 0; q < A.Q; ++q) wsum += A.wtab[q];
-    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < A.E; e += int64_t(gridDim.x) * blockDim.x) {
-        const int32_t* c4 = A.vconn + e * 4;
-        double J[9];
-        Geo G;
-        affine_jacobian(A.xyz, c4, J);
-        adjugate(J, G);
-        const double inv = 1. / fabs(G.det);
-        // element-constant coefficients are folded in; per-point ones
-        // (cs/cl flags set) are applied in the gather kernel
-        const double k = A.cs ? 1. : (A.cm ? A.cm[e] : 0.);
-        const double mu = A.cl ? 1. : (A.cg ? A.cg[e] * wsum : 0.);
-        const double* a = G.adj;
-        double* o = A.out + e * 14;
-        const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (int64_t e0 = int64_t(blockIdx.x) * kPayloadBlock; e0 < A.E; e0 += int64_t(gridDim.x) * kPayloadBlock) {
+        const int n = int(A.E - e0 < kPayloadBlock ? A.E - e0 : kPayloadBlock);
+        if (int(threadIdx.x) < n) {
+            const int64_t e = e0 + threadIdx.x;
+            const int32_t* c4 = A.vconn + e * 4;
+            double J[9];
+            Geo G;
+            affine_jacobian(A.xyz, c4, J);
+            adjugate(J, G);
+            const double inv = 1. / fabs(G.det);
+            // element-constant coefficients are folded in; per-point ones
+            // (cs/cl flags set) are applied in the gather kernel
+            const double k = A.cs ? 1. : (A.cm ? A.cm[e] : 0.);
+            const double mu = A.cl ? 1. : (A.cg ? A.cg[e] * wsum : 0.);
+            const double* a = G.adj;
+            double* o = buf + threadIdx.x * PO;
+            const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
 #pragma unroll
-        for (int t = 0; t < 6; ++t) {
-            const int i = ij[t][0], j = ij[t][1];
-            o[t] = k * inv * (a[i * 3 + 0] * a[j * 3 + 0] + a[i * 3 + 1] * a[j * 3 + 1] + a[i * 3 + 2] * a[j * 3 + 2]);
-            o[6 + t] = mu * inv * (J[0 * 3 + i] * J[0 * 3 + j] + J[1 * 3 + i] * J[1 * 3 + j] + J[2 * 3 + i] * J[2 * 3 + j]);
+            for (int t = 0; t < 6; ++t) {
+                const int i = ij[t][0], j = ij[t][1];
+                o[t] = k * inv * (a[i * 3 + 0] * a[j * 3 + 0] + a[i * 3 + 1] * a[j * 3 + 1] + a[i * 3 + 2] * a[j * 3 + 2]);
+                o[6 + t] = mu * inv * (J[0 * 3 + i] * J[0 * 3 + j] + J[1 * 3 + i] * J[1 * 3 + j] + J[2 * 3 + i] * J[2 * 3 + j]);
+            }
+            int32_t v[4] = {c4[0], c4[1], c4[2], c4[3]};
+            int64_t bits = 0;
+#pragma unroll
+            for (int l = 0; l < 6; ++l) bits |= int64_t(v[kN1Edge[l][0]] < v[kN1Edge[l][1]] ? 0 : 1) << l;
+            o[12] = __longlong_as_double(bits);
+            o[13] = 0.;
         }
-        int32_t v[4] = {c4[0], c4[1], c4[2], c4[3]};
-        int64_t bits = 0;
-#pragma unroll
-        for (int l = 0; l < 6; ++l) bits |= int64_t(v[kN1Edge[l][0]] < v[kN1Edge[l][1]] ? 0 : 1) << l;
-        o[12] = __longlong_as_double(bits);
-        o[13] = 0.;
+        __syncthreads();
+        flush_payload_block<double, PO>(buf, A.out, e0, n);
+        __syncthreads();
     }
 }
 
