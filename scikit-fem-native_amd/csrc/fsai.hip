@@ -72,33 +72,58 @@ __device__ __forceinline__ v4s as_v4s(uint2 u) { return __builtin_bit_cast(v4s, 
 
 // ------------------------------------------------------------------ features
 // Per row: [0] sum_{k!=i} |At_ik|, [1] (sum_k A_ik)/A_ii, [2] row length/16,
-// [3] mean distance to off-diagonal neighbours (1 if none).
-__global__ void k_row_stats(const int64_t* __restrict__ ip, const int32_t* __restrict__ ix,
-                            const double* __restrict__ av, const double* __restrict__ rs,
-                            const double* __restrict__ xyz, int64_t n, double* __restrict__ stats) {
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-        double abs_sum = 0., sum = 0., dist = 0.;
-        int cnt = 0;
-        double aii = 0.;
-        for (int64_t k = ip[i]; k < ip[i + 1]; ++k) {
+// [3] mean distance to off-diagonal neighbours (1 if none).  16 lanes per
+// row: lane l takes the row's nonzeros l, l+16, ... (coalesced), then the 16
+// partial sums are combined by shuffles.
+__device__ __forceinline__ double sum16(double v) {
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 8);
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_row_stats(const int64_t* __restrict__ ip, const int32_t* __restrict__ ix,
+                                                   const double* __restrict__ av, const double* __restrict__ rs,
+                                                   const double* __restrict__ xyz, int64_t n, double* __restrict__ stats) {
+    const int sub = threadIdx.x & 15;
+    const int64_t groups = (int64_t(gridDim.x) * blockDim.x) >> 4;
+    // the loop bound is uniform over each 16-lane group, which the shuffles need
+    for (int64_t i = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4; i < n; i += groups) {
+        double abs_sum = 0., sum = 0., dist = 0., aii = 0., cnt = 0.;
+        const double rsi = rs[i];
+        double xi = 0., yi = 0., zi = 0.;
+        if (xyz) {
+            xi = xyz[3 * i];
+            yi = xyz[3 * i + 1];
+            zi = xyz[3 * i + 2];
+        }
+        for (int64_t k = ip[i] + sub; k < ip[i + 1]; k += 16) {
             const int32_t j = ix[k];
-            sum += av[k];
+            const double a = av[k];
+            sum += a;
             if (j == i) {
-                aii = av[k];
+                aii = a;
                 continue;
             }
-            abs_sum += fabs(av[k] * rs[i] * rs[j]);
+            abs_sum += fabs(a * rsi * rs[j]);
             if (xyz) {
-                const double dx = xyz[3 * j] - xyz[3 * i], dy = xyz[3 * j + 1] - xyz[3 * i + 1],
-                             dz = xyz[3 * j + 2] - xyz[3 * i + 2];
+                const double dx = xyz[3 * j] - xi, dy = xyz[3 * j + 1] - yi, dz = xyz[3 * j + 2] - zi;
                 dist += sqrt(dx * dx + dy * dy + dz * dz);
             }
-            ++cnt;
+            cnt += 1.;
         }
-        stats[4 * i + 0] = abs_sum;
-        stats[4 * i + 1] = sum / aii;
-        stats[4 * i + 2] = double(ip[i + 1] - ip[i]) / 16.;
-        stats[4 * i + 3] = (xyz && cnt) ? dist / cnt : 1.;
+        abs_sum = sum16(abs_sum);
+        sum = sum16(sum);
+        dist = sum16(dist);
+        cnt = sum16(cnt);
+        aii = sum16(aii);   // one lane holds the diagonal, the others 0
+        if (sub == 0) {
+            stats[4 * i + 0] = abs_sum;
+            stats[4 * i + 1] = sum / aii;
+            stats[4 * i + 2] = double(ip[i + 1] - ip[i]) / 16.;
+            stats[4 * i + 3] = (xyz && cnt > 0.) ? dist / cnt : 1.;
+        }
     }
 }
 
@@ -128,13 +153,35 @@ __device__ __forceinline__ int64_t row_of(const int64_t* __restrict__ lptr, int6
     return lo;
 }
 
-__global__ void __launch_bounds__(256) k_features(const int64_t* __restrict__ ip, const int32_t* __restrict__ ix,
-                                                  const double* __restrict__ av, const double* __restrict__ rs,
-                                                  const double* __restrict__ stats, const double* __restrict__ xyz,
-                                                  const int64_t* __restrict__ lptr, const int32_t* __restrict__ lidx,
-                                                  int64_t n, int64_t nnz_l, float* __restrict__ feat) {
-    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < nnz_l; q += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t i = row_of(lptr, n, q);
+// Blocks take 256 consecutive lower nonzeros.  Every row holds at least its
+// diagonal, so they span at most 256 rows: one search finds the first, and
+// each thread then searches the block's window of row pointers in LDS.
+constexpr int kFeatBlock = 256;
+
+__global__ void __launch_bounds__(kFeatBlock) k_features(const int64_t* __restrict__ ip, const int32_t* __restrict__ ix,
+                                                         const double* __restrict__ av, const double* __restrict__ rs,
+                                                         const double* __restrict__ stats, const double* __restrict__ xyz,
+                                                         const int64_t* __restrict__ lptr, const int32_t* __restrict__ lidx,
+                                                         int64_t n, int64_t nnz_l, float* __restrict__ feat) {
+    __shared__ int64_t win[kFeatBlock + 1];
+    __shared__ int64_t first_row;
+    for (int64_t q0 = int64_t(blockIdx.x) * kFeatBlock; q0 < nnz_l; q0 += int64_t(gridDim.x) * kFeatBlock) {
+        if (threadIdx.x == 0) first_row = row_of(lptr, n, q0);
+        __syncthreads();
+        const int64_t i0 = first_row;
+        // win[k] = lptr[i0 + k]; rows past n read as nnz_l
+        for (int k = threadIdx.x; k <= kFeatBlock; k += blockDim.x) win[k] = i0 + k <= n ? lptr[i0 + k] : nnz_l;
+        __syncthreads();
+        const int64_t q = q0 + threadIdx.x;
+        int lo = 0, hi = kFeatBlock;   // largest k with win[k] <= q
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (win[mid] <= q) lo = mid;
+            else hi = mid;
+        }
+        __syncthreads();
+        if (q >= nnz_l) continue;
+        const int64_t i = i0 + lo;
         const int32_t j = lidx[q];
         const int64_t bi = ip[i], ei = ip[i + 1];
         const int64_t bj = ip[j], ej = ip[j + 1];
@@ -302,19 +349,17 @@ struct MlpArgs {
 
 // Load one 16-row tile of features for this lane: rows tile0 + lane%16,
 // features 4*(lane/16) .. +3, rounded to BF16.
-__device__ __forceinline__ v4s load_x(const float* __restrict__ feat, int64_t row, int64_t nnz, int quad) {
-    uint16_t h[4] = {0, 0, 0, 0};
-    if (row < nnz) {
-        const float4 v = *reinterpret_cast<const float4*>(feat + row * kF + quad * 4);
-        h[0] = f2bf(v.x);
-        h[1] = f2bf(v.y);
-        h[2] = f2bf(v.z);
-        h[3] = f2bf(v.w);
-    }
+__device__ __forceinline__ float4 load_x_raw(const float* __restrict__ feat, int64_t row, int64_t nnz, int quad) {
+    return row < nnz ? *reinterpret_cast<const float4*>(feat + row * kF + quad * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ v4s x_bf16(float4 v) {
     uint2 u;
-    u.x = uint32_t(h[0]) | (uint32_t(h[1]) << 16);
-    u.y = uint32_t(h[2]) | (uint32_t(h[3]) << 16);
+    u.x = uint32_t(f2bf(v.x)) | (uint32_t(f2bf(v.y)) << 16);
+    u.y = uint32_t(f2bf(v.z)) | (uint32_t(f2bf(v.w)) << 16);
     return as_v4s(u);
+}
+__device__ __forceinline__ v4s load_x(const float* __restrict__ feat, int64_t row, int64_t nnz, int quad) {
+    return x_bf16(load_x_raw(feat, row, nnz, quad));
 }
 
 // Forward of one tile.  Outputs per lane (units ob*16 + quad*4 + r, column
@@ -384,31 +429,78 @@ __device__ __forceinline__ void stage_frags(const uint16_t* __restrict__ frag, i
     __syncthreads();
 }
 
+// Inference form of tile_forward: the output unit's dot product is taken
+// right after each 16-unit block of layer 2 (same summation order as
+// summing the full z2 afterwards), so only one accumulator tile is live.
+// b1, b2 and w3 come from LDS (float4 per lane: units ob*16 + quad*4 + 0..3).
+__device__ __forceinline__ float tile_forward_out(const uint2* __restrict__ w1f, const uint4* __restrict__ w2f,
+                                                  const float4* __restrict__ b1s, const float4* __restrict__ b2s,
+                                                  const float4* __restrict__ w3s, v4s x, int lane) {
+    const int quad = lane >> 4;
+    uint4 h1b[4];
+#pragma unroll
+    for (int kb2 = 0; kb2 < 4; ++kb2) {
+        v4f z[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int ob = 2 * kb2 + h;
+            v4f c = {0.f, 0.f, 0.f, 0.f};
+            c = mfma16(as_v4s(w1f[ob * 64 + lane]), x, c);
+            const float4 b = b1s[ob * 4 + quad];
+            c[0] += b.x;
+            c[1] += b.y;
+            c[2] += b.z;
+            c[3] += b.w;
+            z[h] = c;
+        }
+        h1b[kb2] = pack8(z[0], z[1], true);
+    }
+    float o = 0.f;
+#pragma unroll
+    for (int ob = 0; ob < 8; ++ob) {
+        v4f c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb2 = 0; kb2 < 4; ++kb2) c = mfma32(as_v8bf(w2f[(ob * 4 + kb2) * 64 + lane]), as_v8bf(h1b[kb2]), c);
+        const float4 b = b2s[ob * 4 + quad];
+        const float4 w = w3s[ob * 4 + quad];
+        c[0] += b.x;
+        c[1] += b.y;
+        c[2] += b.z;
+        c[3] += b.w;
+        o = fmaf(w.x, fmaxf(c[0], 0.f), o);
+        o = fmaf(w.y, fmaxf(c[1], 0.f), o);
+        o = fmaf(w.z, fmaxf(c[2], 0.f), o);
+        o = fmaf(w.w, fmaxf(c[3], 0.f), o);
+        // keep one block's fragments and accumulator live at a time
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    return o;
+}
+
 __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp_forward(MlpArgs a, const float* __restrict__ feat,
                                                                 int64_t nnz, float* __restrict__ out) {
     __shared__ uint2 w1s[kFragW1 / 4];
     __shared__ uint4 w2s[kFragW2 / 8];
+    __shared__ float4 b1s[kH / 4], b2s[kH / 4], w3s[kH / 4];
+    for (int k = threadIdx.x; k < kH; k += blockDim.x) {
+        reinterpret_cast<float*>(b1s)[k] = a.params[kOffB1 + k];
+        reinterpret_cast<float*>(b2s)[k] = a.params[kOffB2 + k];
+        reinterpret_cast<float*>(w3s)[k] = a.params[kOffW3 + k];
+    }
     stage_frags(a.frag, kFragW1, w1s, w2s);
     const int lane = threadIdx.x & 63;
     const int quad = lane >> 4;
     const int64_t tiles = (nnz + 15) / 16;
     const int64_t wave = int64_t(blockIdx.x) * kMlpWaves + (threadIdx.x >> 6);
     const int64_t nwaves = int64_t(gridDim.x) * kMlpWaves;
-    float w3[8][4];
-#pragma unroll
-    for (int ob = 0; ob < 8; ++ob)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) w3[ob][r] = a.params[kOffW3 + ob * 16 + quad * 4 + r];
     const float b3 = a.params[kOffB3];
+    // the next tile's features are in flight while this tile is computed
+    float4 xn = load_x_raw(feat, wave * 16 + (lane & 15), nnz, quad);
     for (int64_t tile = wave; tile < tiles; tile += nwaves) {
         const int64_t row = tile * 16 + (lane & 15);
-        TileFwd t;
-        tile_forward(w1s, w2s, a.params, load_x(feat, row, nnz, quad), lane, t);
-        float o = 0.f;
-#pragma unroll
-        for (int ob = 0; ob < 8; ++ob)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) o = fmaf(w3[ob][r], fmaxf(t.z2[ob][r], 0.f), o);
+        const float4 x = xn;
+        xn = load_x_raw(feat, (tile + nwaves) * 16 + (lane & 15), nnz, quad);
+        float o = tile_forward_out(w1s, w2s, b1s, b2s, w3s, x_bf16(x), lane);
         o = col_sum(o) + b3;
         if (quad == 0 && row < nnz) out[row] = o;
     }
@@ -646,7 +738,7 @@ int skfa_fsai_features(const int64_t* indptr, const int32_t* indices, const doub
     hipStream_t s = static_cast<hipStream_t>(stream);
     const unsigned g = grid_of(n, 256);
     k_rsqrt_diag<<<g, 256, 0, s>>>(indptr, indices, vals, n, rsqrt_diag);
-    k_row_stats<<<g, 256, 0, s>>>(indptr, indices, vals, rsqrt_diag, xyz, n, row_stats);
+    k_row_stats<<<grid_of(n * 16, 256), 256, 0, s>>>(indptr, indices, vals, rsqrt_diag, xyz, n, row_stats);
     int64_t nnz_l = 0;
     SKFA_HIP(hipMemcpyAsync(&nnz_l, lptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     SKFA_HIP(hipStreamSynchronize(s));
