@@ -117,6 +117,7 @@ struct PlanArgs {
     int64_t nnodes;
     int64_t nwaves;
     const int64_t* xcd_seg;  // 9 wave offsets (placement mode 1)
+    const WaveMeta* meta;    // per-wave constants
     int32_t placement;
     int32_t xcd_chunk;
     int32_t lds_atomic;
@@ -160,9 +161,10 @@ struct Lane {
 
 template <int C>
 __device__ inline void lane_setup(const PlanArgs& P, int w, int lane, Lane& L) {
-    const int64_t r0 = P.wave_r0[w], r1 = P.wave_r0[w + 1];
-    L.R = int(r1 - r0);
-    L.S = P.wave_split[w];
+    const WaveMeta m = P.meta[w];
+    const int64_t r0 = m.r0;
+    L.R = m.R;
+    L.S = m.S;
     const int lpr = C * C * L.S;
     L.i = lane / lpr;
     const int rem = lane - L.i * lpr;
@@ -173,15 +175,16 @@ __device__ inline void lane_setup(const PlanArgs& P, int w, int lane, Lane& L) {
     L.active = L.i < L.R;
     L.lmul = 1;
     L.ladd = 0;
-    L.rec0 = P.wave_rec[w];
-    L.steps = (P.wave_rec[w + 1] - L.rec0) / L.R;
-    L.lo = P.indptr[r0];
-    L.span = (P.indptr[r1] - L.lo) * C * C;
+    L.rec0 = m.rec0;
+    L.steps = m.steps;
+    L.lo = m.lo;
+    L.span = int64_t(m.span) * C * C;
     L.rb = 0;
     if (L.active) {
         const int64_t rn = r0 + L.i;
-        const int64_t len = P.indptr[rn + 1] - P.indptr[rn];
-        L.rb = int64_t(C) * C * (P.indptr[rn] - L.lo) + int64_t(L.c) * C * len + L.d;
+        const int64_t b = P.indptr[rn];
+        const int64_t len = P.indptr[rn + 1] - b;
+        L.rb = int64_t(C) * C * (b - L.lo) + int64_t(L.c) * C * len + L.d;
     }
 }
 
@@ -665,11 +668,8 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
     Lane L;
     const int wv = placed_wave(A.plan.xcd_seg, int(A.plan.nwaves), A.plan.placement, A.plan.xcd_chunk);
     if (wv < 0) return;
-    {
-        const int rows = int(A.plan.wave_r0[wv + 1] - A.plan.wave_r0[wv]);
-        if (rows < A.rmin || rows > A.rmax) return;
-    }
     lane_setup<C>(A.plan, wv, lane, L);
+    if (L.R < A.rmin || L.R > A.rmax) return;
     int64_t nsteps = L.steps;
     if constexpr (SP == SP_INCIDENCE) {
         L.lmul = L.S;
@@ -777,10 +777,11 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
     const int wv = placed_wave(A.plan.xcd_seg, int(A.plan.nwaves), A.plan.placement, A.plan.xcd_chunk);
     if (wv < 0) return;
     const PlanArgs& PA = A.plan;
-    const int64_t r0w = PA.wave_r0[wv], r1w = PA.wave_r0[wv + 1];
+    const WaveMeta wm = PA.meta[wv];
+    const int64_t r0w = wm.r0;
     Lane L;  // record addressing of load_records
-    L.R = int(r1w - r0w);
-    L.S = PA.wave_split[wv];
+    L.R = wm.R;
+    L.S = wm.S;
     const int lpr = NL * L.S;
     L.i = lane / lpr;
     const int rem = lane - L.i * lpr;
@@ -790,16 +791,17 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
     L.lmul = L.S;
     L.ladd = L.p;
     L.active = L.i < L.R;
-    L.rec0 = PA.wave_rec[wv];
-    L.steps = (PA.wave_rec[wv + 1] - L.rec0) / L.R;
-    L.lo = PA.indptr[r0w];
-    L.span = (PA.indptr[r1w] - L.lo) * C * C;
+    L.rec0 = wm.rec0;
+    L.steps = wm.steps;
+    L.lo = wm.lo;
+    L.span = int64_t(wm.span) * C * C;
     L.rb = 0;
     int row_len = 0;
     if (L.active) {
         const int64_t rn = r0w + L.i;
-        row_len = int(PA.indptr[rn + 1] - PA.indptr[rn]);
-        L.rb = int64_t(C) * C * (PA.indptr[rn] - L.lo);
+        const int64_t rbeg = PA.indptr[rn];
+        row_len = int(PA.indptr[rn + 1] - rbeg);
+        L.rb = int64_t(C) * C * (rbeg - L.lo);
     }
     const int64_t nsteps = (L.steps + L.S - 1) / L.S;
     Record<RB> r0[K], r1[K], r2[K];
@@ -1603,6 +1605,7 @@ static PlanArgs plan_args(const skfa_plan* P) {
     A.nnodes = P->nnodes;
     A.nwaves = P->nwaves;
     A.xcd_seg = P->xcd_seg.p;
+    A.meta = P->wave_meta.p;
     A.placement = P->tune_placement;
     A.xcd_chunk = P->tune_xcd_chunk;
     A.lds_atomic = P->tune_lds_atomic;

@@ -372,6 +372,27 @@ static int plan_build(skfa_plan* P, const int32_t* conn, hipStream_t s) {
     if (int rc = P->wave_r0.alloc(w_r0.size())) return rc;
     if (int rc = P->wave_rec.alloc(w_rec.size())) return rc;
     if (int rc = P->wave_split.alloc(std::max<size_t>(w_split.size(), 1))) return rc;
+    std::vector<WaveMeta> w_meta(w_split.size());
+    for (size_t w = 0; w < w_split.size(); ++w) {
+        WaveMeta& m = w_meta[w];
+        const int64_t R = w_r0[w + 1] - w_r0[w];
+        const int64_t steps = (w_rec[w + 1] - w_rec[w]) / R;
+        if (R < 1 || R > 64 || steps > 65535) {
+            set_error("gather wave layout out of range");
+            return kCapacity;
+        }
+        m.r0 = w_r0[w];
+        m.rec0 = w_rec[w];
+        m.lo = hptr[size_t(w_r0[w])];
+        m.span = int32_t(hptr[size_t(w_r0[w + 1])] - m.lo);
+        m.steps = uint16_t(steps);
+        m.R = uint8_t(R);
+        m.S = uint8_t(w_split[w]);
+    }
+    if (int rc = P->wave_meta.alloc(std::max<size_t>(w_meta.size(), 1))) return rc;
+    if (!w_meta.empty())
+        SKFA_HIP(hipMemcpyAsync(P->wave_meta.p, w_meta.data(), sizeof(WaveMeta) * w_meta.size(),
+                                hipMemcpyHostToDevice, s));
     if (int rc = P->row_rec.alloc(std::max<int64_t>(nn, 1))) return rc;
     if (int rc = P->row_stride.alloc(std::max<int64_t>(nn, 1))) return rc;
     SKFA_HIP(hipMemcpyAsync(P->wave_r0.p, w_r0.data(), sizeof(int64_t) * w_r0.size(), hipMemcpyHostToDevice, s));

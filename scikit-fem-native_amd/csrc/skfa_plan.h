@@ -9,6 +9,19 @@
 
 #include "skfa_internal.h"
 
+namespace skfa {
+// Per-wave constants of the gather, one 32-byte scalar load per wave.
+struct WaveMeta {
+    int64_t r0;       // first node row
+    int64_t rec0;     // first gather record
+    int64_t lo;       // node-level CSR offset of the first row
+    int32_t span;     // node-level CSR entries of the wave's rows
+    uint16_t steps;   // incidence layers (a multiple of S)
+    uint8_t R, S;     // rows, lanes (parts) per row
+};
+static_assert(sizeof(WaveMeta) == 32, "one s_load_dwordx8 per wave");
+}  // namespace skfa
+
 struct skfa_plan {
     int64_t E = 0;        // elements
     int32_t n = 0;        // nodes per element (local rows of the node-level map)
@@ -37,6 +50,7 @@ struct skfa_plan {
     skfa::DevBuf<int64_t> wave_r0;    // nwaves+1
     skfa::DevBuf<int64_t> wave_rec;   // nwaves+1
     skfa::DevBuf<int32_t> wave_split; // nwaves
+    skfa::DevBuf<skfa::WaveMeta> wave_meta;  // nwaves: the above, gathered per wave
     skfa::DevBuf<int64_t> row_rec;    // nnodes: record of the row's first incidence
     skfa::DevBuf<int32_t> row_stride; // nnodes: R of the row's wave
     int32_t rec_bytes = 0;
