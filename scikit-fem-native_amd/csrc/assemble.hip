@@ -1723,8 +1723,11 @@ static bool mfma_gather(const skfa_plan* P, int nqg, int nqm) {
     return P->tune_mfma > 0 && nqg <= 4 && nqm <= 4;
 }
 
+// element tables are automatic for 4-node elements at one row per lane;
+// with the rows split over lanes (small meshes) the quadrature gather's
+// incidence split avoids the table gather's repeated element rows
 static bool table_gather(const skfa_plan* P) {
-    return P->tune_table < 0 ? P->n == 4 : P->tune_table != 0;
+    return P->tune_table < 0 ? (P->n == 4 && P->min_split == 1) : P->tune_table != 0;
 }
 
 // Payload layout of a symmetric scalar form: the MFMA and element-table

@@ -310,10 +310,17 @@ static int plan_build(skfa_plan* P, const int32_t* conn, hipStream_t s) {
     std::vector<int64_t> r_rec(static_cast<size_t>(nn));
     std::vector<int32_t> r_stride(static_cast<size_t>(nn));
     int64_t rec_total = 0, steps_total = 0, lds_max = 0;
+    // Small meshes: at one row per lane the launch would hold fewer than
+    // kMinWaves waves (8 per CU on 256 CUs); start from a larger split so
+    // that the S lanes of a row share its incidences instead.
+    constexpr int64_t kMinWaves = 2048;
+    int s_min = 1;
+    while (s_min * 2 <= 64 / unit && (nn * unit * s_min + 63) / 64 < kMinWaves) s_min <<= 1;
+    P->min_split = s_min;
     int64_t r = 0;
     while (r < nn) {
         int chosen_R = 0, chosen_S = 0;
-        for (int S = 1; S <= 64 / unit; S <<= 1) {
+        for (int S = s_min; S <= 64 / unit; S <<= 1) {
             // power-of-two splits: the S parts of a row share its work
             int R = 64 / (unit * S);
             if (R < 1) break;

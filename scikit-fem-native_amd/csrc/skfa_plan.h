@@ -59,6 +59,7 @@ struct skfa_plan {
     int64_t total_steps = 0;          // sum over waves of steps (for reporting)
     int32_t lds_budget = 16 * 1024;   // bytes of LDS per wave requested at plan time
     int32_t lds_bytes = 0;            // largest span of any wave, bytes
+    int32_t min_split = 1;            // smallest S of the layout (> 1 for small meshes)
     skfa::DevBuf<uint8_t> records;
 
     // per-element payload of the element-constant fast path (allocated on
