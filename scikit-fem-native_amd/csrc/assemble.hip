@@ -1513,14 +1513,22 @@ __global__ void k_linear_const(const int64_t* __restrict__ inc_ptr, const int32_
         wphi[threadIdx.x] = s;
     }
     __syncthreads();
-    for (int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; node < nnodes;
-         node += int64_t(gridDim.x) * blockDim.x) {
+    // 16 lanes per node: lane l sums incidences l, l+16, ..., then a fixed
+    // shuffle tree adds the 16 partial sums (deterministic order); the loop
+    // bound is uniform over each 16-lane group
+    const int sub = threadIdx.x & 15;
+    const int64_t groups = (int64_t(gridDim.x) * blockDim.x) >> 4;
+    for (int64_t node = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4; node < nnodes; node += groups) {
         double acc = 0.;
-        for (int64_t k = inc_ptr[node]; k < inc_ptr[node + 1]; ++k) {
+        for (int64_t k = inc_ptr[node] + sub; k < inc_ptr[node + 1]; k += 16) {
             const int32_t packed = inc[k];
             acc += det[packed >> 4] * wphi[packed & 15];
         }
-        for (int c = 0; c < C; ++c) out[node * C + c] = acc * f[c];
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        acc += __shfl_xor(acc, 4);
+        acc += __shfl_xor(acc, 8);
+        if (sub < C) out[node * C + sub] = acc * f[sub];
     }
 }
 
@@ -1995,7 +2003,7 @@ int skfa_assemble_linear(skfa_plan* P, const double* xyz, const int32_t* vconn, 
             if (int rc = P->elem_det.alloc(size_t(std::max<int64_t>(P->E, 1)))) return rc;
         k_elem_absdet<<<unsigned(std::min<int64_t>((P->E + 255) / 256, 65536)), 256, 0, s>>>(A.xyz, vconn, P->E,
                                                                                               P->elem_det.p);
-        k_linear_const<<<unsigned(std::min<int64_t>((P->nnodes + 255) / 256, 65536)), 256, 0, s>>>(
+        k_linear_const<<<unsigned(std::min<int64_t>((P->nnodes * 16 + 255) / 256, 65536)), 256, 0, s>>>(
             A.inc_ptr, A.inc, P->nnodes, A.C, P->elem_det.p, tables, nq, A.n, A.gn, f, out);
         SKFA_LAUNCH_CHECK();
         return kOk;
