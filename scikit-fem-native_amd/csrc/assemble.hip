@@ -125,7 +125,7 @@ struct PlanArgs {
 };
 
 // Lane role inside a wave: node row r0+i, components (c,d) of the CxC node
-// block, column part p of S (the lane owns in-row offsets o with o%S == p).
+// block, part p of the row's S parts (how parts share a row: SP_* below).
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
 // XCD b % 8).  Wave placement modes (plan tuning):
 //   0: plain order (wave = block);

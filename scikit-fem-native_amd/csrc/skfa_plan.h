@@ -41,11 +41,12 @@ struct skfa_plan {
 
     // Wave-lockstep gather records.  Wave w owns node rows
     // [wave_r0[w], wave_r0[w+1]) (R_w = their count) and split factor S_w:
-    // C*C*S_w lanes cooperate on one node row, lane part p accumulating the
-    // in-row offsets o with o % S_w == p.  R_w and S_w are chosen so the
-    // wave's CSR span fits the LDS budget.  Record wave_rec[w] + s*R_w + i
-    // holds the s-th incidence of row wave_r0[w]+i and the in-row offsets of
-    // the element's n columns; padding records carry packed = -1.
+    // S_w parts (lane groups) share one node row; with the default incidence
+    // split part p takes the row's incidences p, p+S_w, ...  R_w and S_w are
+    // chosen so the wave's CSR span fits the LDS budget.  Record
+    // wave_rec[w] + s*R_w + i holds the s-th incidence of row wave_r0[w]+i
+    // and the in-row offsets of the element's n columns; the layer count is
+    // a multiple of S_w and padding records carry packed = -1.
     int64_t nwaves = 0;
     skfa::DevBuf<int64_t> wave_r0;    // nwaves+1
     skfa::DevBuf<int64_t> wave_rec;   // nwaves+1

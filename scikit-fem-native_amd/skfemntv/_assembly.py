@@ -140,6 +140,18 @@ def upload(host):
     return torch.from_numpy(arr).to(_native.device())
 
 
+def to_host(dev, out=None):
+    """Device tensor -> numpy array through pinned host memory (the copy runs
+    at full PCIe rate instead of through pageable staging).  ``out`` is a
+    pinned torch tensor to fill; by default a new one (torch's pinned
+    allocator reuses freed blocks) backs the returned array."""
+    import torch
+    host = torch.empty(dev.shape, dtype=dev.dtype, pin_memory=True) if out is None else out
+    host.copy_(dev, non_blocking=True)
+    torch.cuda.current_stream().synchronize()
+    return host.numpy()
+
+
 def to_per_qp(dev_tensor, Q):
     """(E,)+t -> (E,Q)+t on the device."""
     shape = dev_tensor.shape
@@ -198,9 +210,14 @@ class NativeBilinearForm:
         import torch
         self._values = torch.zeros(max(self.plan.nnz, 1), dtype=torch.float64, device=_native.device())
         ip, ix = self.plan.host_csr_arrays()
-        self._matrix = csr_matrix((np.zeros(self.plan.nnz), ix, ip), shape=(self.plan.nrows, self.plan.nrows),
-                                  copy=False)
+        # the returned CSR object's values live in pinned host memory, which
+        # the device fills directly on every call
+        self._pinned = torch.zeros(max(self.plan.nnz, 1), dtype=torch.float64, pin_memory=True)
+        self._matrix = csr_matrix((self._pinned[:self.plan.nnz].numpy(), ix, ip),
+                                  shape=(self.plan.nrows, self.plan.nrows), copy=False)
         self._matrix.resize((basis.N, basis.N))
+        if not np.shares_memory(self._matrix.data, self._pinned.numpy()):
+            self._pinned = None
         self._coefficient_shape = basis.coefficient_shape
         self.last_kernel_seconds = 0.
 
@@ -212,7 +229,10 @@ class NativeBilinearForm:
             raise ValueError("at least one coefficient is required")
         dev, per_qp = _prepare_scalar_kinds(self.basis, coefs)
         self.assemble_device(per_qp=per_qp, precision=precision, **dev)
-        np.copyto(self._matrix.data, self._values[:self.plan.nnz].cpu().numpy())
+        if self._pinned is not None:
+            to_host(self._values[:self.plan.nnz], self._pinned[:self.plan.nnz])
+        else:
+            np.copyto(self._matrix.data, to_host(self._values[:self.plan.nnz]))
         return self._matrix
 
     def assemble_device(self, *, value=None, gradient=None, symmetric_gradient=None, divergence=None,
@@ -258,10 +278,11 @@ class NativeCrossBilinearForm:
         except ValueError as error:
             raise ValueError(f"tensor coefficient must broadcast to {expected}") from error
         arr, per_qp = field_layout(coefficient[:, :, 0, :, 0, :], self.coefficient_shape, "tensor", (3, 3))
-        sym = bool(np.array_equal(arr, np.swapaxes(arr, -1, -2)))
+        # A == A^T per element: the three off-diagonal pairs agree
+        sym = all(np.array_equal(arr[..., i, j], arr[..., j, i]) for i, j in ((0, 1), (0, 2), (1, 2)))
         self.assemble_tensor_device(upload(arr), per_qp=per_qp, precision=precision, symmetric=sym)
         ip, ix = self.plan.host_csr_arrays()
-        data = self._values[:self.plan.nnz].cpu().numpy()
+        data = to_host(self._values[:self.plan.nnz])
         m = csr_matrix((data, ix.copy(), ip.copy()), shape=(self.plan.nrows, self.plan.nrows), copy=False)
         m.resize(self.shape)
         return m
@@ -322,7 +343,7 @@ class NativeLinearForm:
         start = time.perf_counter()
         out = self.assemble_device(upload(f) if f is not None else None, fq,
                                    upload(G) if G is not None else None, Gq)
-        result = out[:self.plan.nrows].cpu().numpy()
+        result = to_host(out[:self.plan.nrows])
         seconds = time.perf_counter() - start
         E, Q, _ = self._shape
         return result, LinearFormDiagnostics(entity_count=E, quadrature_point_count=Q, assembly_seconds=seconds)

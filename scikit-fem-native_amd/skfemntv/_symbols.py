@@ -68,7 +68,9 @@ def evaluate_coefficient(value, parameters, *, factor=1.0, squeeze=False):
     when asked and ndim > 2 (_coefficients.py:71-86)."""
     if value is None:
         return factor
-    evaluated = factor * np.asarray(resolve_coefficient(value, parameters), dtype=np.float64)
+    evaluated = np.asarray(resolve_coefficient(value, parameters), dtype=np.float64)
+    if factor != 1.0:   # x * 1.0 == x exactly; keep broadcast views as views
+        evaluated = factor * evaluated
     if squeeze and evaluated.ndim > 2:
         evaluated = np.squeeze(evaluated)
     return evaluated

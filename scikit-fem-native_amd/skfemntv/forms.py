@@ -153,7 +153,9 @@ def _native_bilinear_assemble(form, basis, kwargs, num_threads=0, *, memory_limi
                                                     memory_safety_factor=memory_safety_factor)
             by_trial[basis] = tensor_native
         for raw, factor in tensor_terms:
-            tensor = anisotropic_tensor_coefficient(raw, basis.coefficient_shape, 3) * factor
+            tensor = anisotropic_tensor_coefficient(raw, basis.coefficient_shape, 3)
+            if factor != 1.0:
+                tensor = tensor * factor
             matrix = tensor_native.assemble_tensor("gradient", "gradient", tensor, num_threads=num_threads,
                                                    precision=precision)
             result = matrix if result is None else result + matrix

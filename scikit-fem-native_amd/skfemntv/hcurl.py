@@ -18,7 +18,7 @@ import numpy as np
 from scipy.sparse import csr_matrix
 
 from . import _native
-from ._assembly import Plan, AssemblyMemoryEstimate, enforce_memory_budget, field_layout, upload, to_per_qp
+from ._assembly import to_host, Plan, AssemblyMemoryEstimate, enforce_memory_budget, field_layout, upload, to_per_qp
 from .basis import tet_rule_4
 
 TET_DIRECTED_EDGES = ((0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3))
@@ -270,7 +270,7 @@ class TetN1Assembler:
             if not cq:
                 cd = to_per_qp(cd, Q)
         self.assemble_device(md, cd, per_qp, include_mass, include_curl)
-        np.copyto(self._matrix.data, self._values[:self.plan.nnz].cpu().numpy())
+        np.copyto(self._matrix.data, to_host(self._values[:self.plan.nnz]))
         return self._matrix
 
     def assemble_device(self, kappa, mu, per_qp, include_mass=True, include_curl=True, out=None):
