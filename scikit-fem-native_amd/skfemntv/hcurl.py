@@ -244,7 +244,13 @@ class TetN1Assembler:
         self._xyz = torch.from_numpy(xyz).to(dev)
         self._values = torch.zeros(max(self.plan.nnz, 1), dtype=torch.float64, device=dev)
         ip, ix = self.plan.host_csr_arrays()
-        self._matrix = csr_matrix((np.zeros(self.plan.nnz), ix, ip), shape=(basis.N, basis.N), copy=False)
+        # the returned CSR object's values live in pinned host memory, which
+        # the device fills directly on every call
+        self._pinned = torch.zeros(max(self.plan.nnz, 1), dtype=torch.float64, pin_memory=True)
+        self._matrix = csr_matrix((self._pinned[:self.plan.nnz].numpy(), ix, ip), shape=(basis.N, basis.N),
+                                  copy=False)
+        if not np.shares_memory(self._matrix.data, self._pinned.numpy()):
+            self._pinned = None
         self._tables_basis = torch.from_numpy(_n1_tables(basis.X, basis.W)).to(dev)
         X4, W4 = tet_rule_4()
         m_basis, m_4 = _moments(basis.X, basis.W), _moments(X4, W4)
@@ -270,7 +276,10 @@ class TetN1Assembler:
             if not cq:
                 cd = to_per_qp(cd, Q)
         self.assemble_device(md, cd, per_qp, include_mass, include_curl)
-        np.copyto(self._matrix.data, to_host(self._values[:self.plan.nnz]))
+        if self._pinned is not None:
+            to_host(self._values[:self.plan.nnz], self._pinned[:self.plan.nnz])
+        else:
+            np.copyto(self._matrix.data, to_host(self._values[:self.plan.nnz]))
         return self._matrix
 
     def assemble_device(self, kappa, mu, per_qp, include_mass=True, include_curl=True, out=None):
