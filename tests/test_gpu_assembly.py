@@ -115,6 +115,12 @@ def test_p2_anisotropic_fp64_matches_oracle(layout):
     ip, ix, vals, N = pipeline.tensor(p2, t2, X, W, True, np.ascontiguousarray(A_eq))
     err = check_csr(K, ip, ix, vals)
     say(f"p2 anisotropic fp64 rel err {err:.3e}")
+    # the tensor path returns a fresh matrix: a second call leaves the first intact
+    first = K.data.copy()
+    K2 = sk.asm(aniso, basis, diffusion=2. * raw)
+    np.testing.assert_array_equal(K.data, first)
+    assert K2.data is not K.data
+    np.testing.assert_allclose(K2.data, 2. * first, rtol=1e-14, atol=0.)
 
 
 def test_p2_anisotropic_fp32_products_within_1e6():
