@@ -87,6 +87,66 @@ int skfa_plan_export_csr(const skfa_plan* plan, int64_t* indptr, int32_t* indice
 int skfa_plan_export_map(const skfa_plan* plan, int64_t* inc_ptr, int32_t* inc, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Rectangular plan between two nodal fields on one mesh (row = test field,
+ * column = trial field).  Replaces the constructor half of
+ * CrossBilinearAssembler for different row and column spaces: pattern,
+ * int64 scatter map and colouring
+ * (cpp/src/assembly/cross_bilinear_assembler.cpp:19-54,318-341).
+ *   row_conn (E, row_nodes_per_element) int32 row node ids in [0, row_nodes),
+ *   col_conn (E, col_nodes_per_element) int32 column node ids in [0, col_nodes);
+ *   row DOF = row node * row_components + c, column DOF likewise.
+ * The pattern holds every (row DOF, column DOF) pair of every element,
+ * sorted and unique per row, as the reference's build_pattern does.
+ * Limits: nodes per element <= 16, E < 2^27, <= 4096 element columns
+ * around one row node, nnz < 2^31. */
+typedef struct skfa_cross_plan skfa_cross_plan;
+int skfa_cross_plan_create(const int32_t* row_conn, int32_t row_nodes_per_element, int64_t row_nodes,
+                           int32_t row_components, const int32_t* col_conn, int32_t col_nodes_per_element,
+                           int64_t col_nodes, int32_t col_components, int64_t E, void* stream,
+                           skfa_cross_plan** out);
+int skfa_cross_plan_destroy(skfa_cross_plan* plan);
+/* host info[6]: rows, columns, nnz, nnz_node, max_row_len (column nodes),
+ * max incidences of one row node. */
+int skfa_cross_plan_info(const skfa_cross_plan* plan, int64_t* info);
+/* CSR pattern: indptr (rows+1) int64, indices (nnz) int32 (the reference's
+ * CrossBilinearAssembler.indptr / .indices, cross_bilinear_assembler.cpp:203). */
+int skfa_cross_plan_export_csr(const skfa_cross_plan* plan, int64_t* indptr, int32_t* indices, void* stream);
+
+/* Cross assembly into CSR values (nnz FP64, overwritten).  Replaces
+ * CrossBilinearAssembler::assemble (cpp/src/assembly/cross_bilinear_assembler.cpp:55-107,207-247,
+ * contraction cpp/include/native_fem/cross_contraction.hpp:12-84) for two
+ * tetrahedral nodal fields on one mesh, with the geometry computed on the
+ * GPU (as skfa_assemble_bilinear does).
+ *   tables   FP64 w[Q] | row phi[Q][nr] | row dphi[Q][nr][3] | col phi[Q][nc] |
+ *            col dphi[Q][nc][3] | gdphi[Q][gnodes][3] (the last block is read
+ *            only for isoparametric geometry, gconn != NULL)
+ *   row_kind, col_kind: SKFA_CROSS_VALUE or SKFA_CROSS_GRADIENT
+ *   mode     SKFA_CROSS_SCALAR: entry (a,r),(b,c) = [r == c] coef * (phi_a phi_b
+ *            or grad phi_a . grad phi_b); needs equal component counts and
+ *            row_kind == col_kind (the reference's scalar-coefficient branch);
+ *            SKFA_CROSS_ROW_DIVERGENCE: dphi_a/dx_r * phi_b * coef (vector rows
+ *            with 3 components, scalar columns: div(v) u);
+ *            SKFA_CROSS_COLUMN_DIVERGENCE: phi_a * dphi_b/dx_c * coef (scalar rows,
+ *            vector columns: div(u) v);
+ *            SKFA_CROSS_TENSOR: sum_ij row_i(a) T[r,(i),c,(j)] col_j(b) with T the
+ *            (E,Q,Cr,[3],Cc,[3]) tensor of cross_contraction.hpp:42-63.
+ *   coefficient FP64 scalar (modes 0-2) or tensor block (mode 3) of element e,
+ *            point q at coefficient + e*coef_stride_e + q*coef_stride_q
+ *            (strides in doubles; 0 broadcasts); NULL = 1 (modes 0-2 only).
+ * Each row-node group of one wave owns its CSR span in LDS and adds its
+ * incidences in element order, so results are deterministic. */
+#define SKFA_CROSS_VALUE 0
+#define SKFA_CROSS_GRADIENT 1
+#define SKFA_CROSS_SCALAR 0
+#define SKFA_CROSS_ROW_DIVERGENCE 1
+#define SKFA_CROSS_COLUMN_DIVERGENCE 2
+#define SKFA_CROSS_TENSOR 3
+int skfa_assemble_cross(const skfa_cross_plan* plan, const double* xyz, const int32_t* vconn,
+                        const int32_t* gconn, int32_t gnodes, const double* tables, int32_t nq, int32_t row_kind,
+                        int32_t col_kind, int32_t mode, const double* coefficient, int64_t coef_stride_e,
+                        int64_t coef_stride_q, double* values, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Bilinear assembly into CSR values (nnz FP64, overwritten).
  * Replaces BilinearFormAssembler::assemble (value / gradient /
  * symmetric_gradient / divergence coefficients,

@@ -43,6 +43,7 @@ PARITY = {
     "c.pattern_build": "reference C++ (_ref) on the same inputs, bit-exact",
     "c.assemble_bilinear": "reference C++ (_ref) on the same inputs + closed forms",
     "c.assemble_tensor": "reference C++ (_ref) on the same inputs + closed forms",
+    "c.assemble_cross": "reference C++ (_ref) CrossBilinearAssembler on the same inputs + closed forms",
     "c.assemble_linear": "reference C++ (_ref) on the same inputs + closed forms",
     "c.edge_topology": "reference C++ (_ref) on the same inputs, bit-exact",
     "c.integrate_tet_n1": "reference C++ (_ref) on the same inputs",
@@ -194,6 +195,26 @@ def assemble_tensor(grads, dx, tensor, scatter, nnz, threads=1, colour=None):
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]
     _check(L.oracle_assemble_tensor(_p(grads), _p(dx), _p(tensor), _p(scatter), E, n, Q, d, _p(out), nnz,
                                     _p(colour), ncol, int(threads)))
+    return out
+
+
+def assemble_cross(rshape, rgrads, cshape, cgrads, dx, scatter, nnz, Cr, Cc, row_kind, col_kind,
+                   coefficient=None, tensor=False):
+    """General cross kernel (serial).  kinds 'value'/'gradient'; coefficient
+    None, scalar (E,Q), or tensor (E,Q,Cr,[d],Cc,[d]) when tensor=True."""
+    rshape, rgrads, cshape, cgrads, dx = map(_f64, (rshape, rgrads, cshape, cgrads, dx))
+    coefficient = _f64(coefficient)
+    scatter = _i64(scatter)
+    E, Q, nr = rshape.shape
+    nc = cshape.shape[2]
+    d = rgrads.shape[3]
+    out = np.zeros(nnz)
+    L = lib()
+    L.oracle_assemble_cross.argtypes = [ctypes.c_void_p] * 6 + [ctypes.c_int] * 3 + [ctypes.c_void_p] + [
+        ctypes.c_int64] * 7 + [ctypes.c_void_p, ctypes.c_int64]
+    _check(L.oracle_assemble_cross(_p(rshape), _p(rgrads), _p(cshape), _p(cgrads), _p(dx), _p(coefficient),
+                                   int(bool(tensor)), int(row_kind == "gradient"), int(col_kind == "gradient"),
+                                   _p(scatter), E, nr, Cr, nc, Cc, Q, d, _p(out), nnz))
     return out
 
 

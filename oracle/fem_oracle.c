@@ -358,6 +358,66 @@ int oracle_assemble_tensor(const double* grads, const double* dx, const double* 
 }
 
 /* ------------------------------------------------------------------ */
+/* General cross kernel between two fields, serial element loop:       */
+/* CrossBilinearAssembler::assemble_entity (cross_bilinear_assembler.  */
+/* cpp:207-247) with contract_cross_basis / contract_scalar_cross_basis */
+/* (cross_contraction.hpp:42-97).  Row = test field (nr nodes, Cr      */
+/* components), column = trial field (nc, Cc).  row_kind / col_kind    */
+/* 0 value, 1 gradient.  coefficient: NULL (= 1) or scalar (E,Q) when  */
+/* tensor == 0; (E,Q,Cr,[d],Cc,[d]) when tensor == 1.  scatter is the  */
+/* rectangular map of oracle_pattern_build on (E,nr*Cr) x (E,nc*Cc).   */
+/* ------------------------------------------------------------------ */
+int oracle_assemble_cross(const double* rshape, const double* rgrads, const double* cshape,
+                          const double* cgrads, const double* dx, const double* coefficient, int tensor,
+                          int row_kind, int col_kind, const int64_t* scatter, int64_t E, int64_t nr, int64_t Cr,
+                          int64_t nc, int64_t Cc, int64_t Q, int64_t d, double* values, int64_t nnz) {
+    if (!tensor && Cr != Cc) return fail("scalar cross coefficient requires equal component counts");
+    if (!tensor && row_kind != col_kind) return fail("value-gradient coupling requires a tensor coefficient");
+    if (tensor && !coefficient) return fail("tensor coefficient is null");
+    memset(values, 0, sizeof(double) * (size_t)nnz);
+    const int64_t Lr = nr * Cr, Lc = nc * Cc;
+    const int64_t Dr = row_kind ? d : 1, Dc = col_kind ? d : 1;
+    for (int64_t e = 0; e < E; ++e)
+        for (int64_t q = 0; q < Q; ++q) {
+            const int64_t eq = e * Q + q;
+            const double scale = dx[eq];
+            for (int64_t a = 0; a < nr; ++a)
+                for (int64_t b = 0; b < nc; ++b) {
+                    if (!tensor) {
+                        const double material = coefficient ? coefficient[eq] : 1.;
+                        double contract = 0.;
+                        if (!row_kind) {
+                            contract = rshape[eq * nr + a] * cshape[eq * nc + b];
+                        } else {
+                            for (int64_t i = 0; i < d; ++i)
+                                contract += rgrads[(eq * nr + a) * d + i] * cgrads[(eq * nc + b) * d + i];
+                        }
+                        const double entry = scale * material * contract;
+                        for (int64_t r = 0; r < Cr; ++r)
+                            values[scatter[(e * Lr + a * Cr + r) * Lc + b * Cc + r]] += entry;
+                    } else {
+                        for (int64_t r = 0; r < Cr; ++r)
+                            for (int64_t c = 0; c < Cc; ++c) {
+                                double res = 0.;
+                                for (int64_t i = 0; i < Dr; ++i)
+                                    for (int64_t j = 0; j < Dc; ++j) {
+                                        const double rv = row_kind ? rgrads[(eq * nr + a) * d + i]
+                                                                   : rshape[eq * nr + a];
+                                        const double cv = col_kind ? cgrads[(eq * nc + b) * d + j]
+                                                                   : cshape[eq * nc + b];
+                                        const double t =
+                                            coefficient[(((eq * Cr + r) * Dr + i) * Cc + c) * Dc + j];
+                                        res += rv * t * cv;
+                                    }
+                                values[scatter[(e * Lr + a * Cr + r) * Lc + b * Cc + c]] += scale * res;
+                            }
+                    }
+                }
+        }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* Linear form, linear_form_assembler.cpp:103-137 (serial path).       */
 /* value: (C) constant or (E,Q,C); gradient: (C,d) or (E,Q,C,d).       */
 /* ------------------------------------------------------------------ */

@@ -58,6 +58,23 @@ def tensor(p, t, X, W, quadratic, A_eq, threads=1):
     return ip, ix, vals, N
 
 
+def cross(p, t, X, W, row, col, row_kind, col_kind, coefficient=None, tensor=False):
+    """Cross block between two fields on one mesh: row = test, col = trial,
+    each (quadratic, components).  Returns (indptr, indices, values, rows,
+    columns) in the reference's rectangular layout."""
+    (rq, Cr), (cq, Cc) = row, col
+    (rshape, rgrads, dx, x, det, tol, cond), _ = basis_geometry(p, t, X, W, rq)
+    (cshape, cgrads, _, _, _, _, _), _ = basis_geometry(p, t, X, W, cq)
+    nr, nc = (10 if rq else 4), (10 if cq else 4)
+    Nr, rdofs = dofs_of(t, p.shape[1], nr, Cr)
+    Nc, cdofs = dofs_of(t, p.shape[1], nc, Cc)
+    E = t.shape[1]
+    ip, ix, sc = _o.pattern(rdofs.reshape(E, -1), cdofs.reshape(E, -1), nrows=Nr)
+    vals = _o.assemble_cross(rshape, rgrads, cshape, cgrads, dx, sc, len(ix), Cr, Cc, row_kind, col_kind,
+                             coefficient=coefficient, tensor=tensor)
+    return ip, ix, vals, Nr, Nc
+
+
 def load_vector(p, t, X, W, quadratic, f_const, components=1):
     (shape, grads, dx, x, det, tol, cond), bad = basis_geometry(p, t, X, W, quadratic)
     nodes = 10 if quadratic else 4

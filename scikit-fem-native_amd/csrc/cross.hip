@@ -1,0 +1,281 @@
+// Rectangular (cross-basis) element assembly for gfx950 (MI355X).
+//
+// Replaces CrossBilinearAssembler::assemble for two nodal fields on one
+// tetrahedral mesh (cpp/src/assembly/cross_bilinear_assembler.cpp:55-107,
+// 207-247; contractions cpp/include/native_fem/cross_contraction.hpp:12-97):
+// Taylor-Hood divergence blocks, value and gradient couplings between P1 and
+// P2 fields, and caller-supplied contraction tensors.
+//
+// Execution model.  One 64-lane wave owns G consecutive row (test) nodes;
+// each node has a lane group of 64/G lanes, and lane j of a group owns the
+// local entries (row component r, column node b, column component c) with
+// index j, j + 64/G, ...  The CSR values of the wave's rows are one
+// contiguous span (row DOF node*Cr + r holds its node's column nodes times
+// Cc, node-major), accumulated in LDS.  A group walks its node's incidences
+// in element order; per incidence every lane rebuilds the element's
+// geometry (affine Jacobian once, or the isoparametric one per quadrature
+// point), integrates its entries over the quadrature points and adds them
+// at the record's in-row node offsets.  Entries of one incidence hit
+// distinct addresses and the groups own disjoint spans, so the sums follow
+// a fixed order and results are reproducible.  The span is then written to
+// HBM once, coalesced.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "skfa_internal.h"
+#include "skfa_plan.h"
+#include "../../include/skfem_amd.h"
+
+namespace skfa {
+namespace {
+
+struct CrossArgs {
+    const int64_t* inc_ptr;   // row-node incidences (records in the same order)
+    const int64_t* nptr;      // node-level rectangular indptr
+    const uint8_t* records;
+    int32_t rec_bytes, off_bytes;
+    int64_t row_nodes;
+    int32_t nr, nc, Cr, Cc;
+    int32_t rows_per_wave, group;  // G and 64 / G
+    const double4* xyz;
+    const int32_t* vconn;
+    const int32_t* gconn;
+    int32_t gn;
+    const double* tables;  // w | rphi | rdphi | cphi | cdphi | gdphi
+    int32_t Q;
+    int32_t row_grad, col_grad, mode;
+    const double* coef;
+    int64_t cse, csq;
+    double* values;
+};
+
+struct Jac {
+    double adj[9];  // adjugate (J^{-1} = adj / det), row-major
+    double det;
+};
+
+__device__ inline void jac_affine(const double4* __restrict__ xyz, const int32_t* __restrict__ c4, Jac& g) {
+    const double4 v0 = xyz[c4[0]], v1 = xyz[c4[1]], v2 = xyz[c4[2]], v3 = xyz[c4[3]];
+    const double J[9] = {v1.x - v0.x, v2.x - v0.x, v3.x - v0.x, v1.y - v0.y, v2.y - v0.y,
+                         v3.y - v0.y, v1.z - v0.z, v2.z - v0.z, v3.z - v0.z};
+    g.adj[0] = J[4] * J[8] - J[5] * J[7];
+    g.adj[1] = J[2] * J[7] - J[1] * J[8];
+    g.adj[2] = J[1] * J[5] - J[2] * J[4];
+    g.adj[3] = J[5] * J[6] - J[3] * J[8];
+    g.adj[4] = J[0] * J[8] - J[2] * J[6];
+    g.adj[5] = J[2] * J[3] - J[0] * J[5];
+    g.adj[6] = J[3] * J[7] - J[4] * J[6];
+    g.adj[7] = J[1] * J[6] - J[0] * J[7];
+    g.adj[8] = J[0] * J[4] - J[1] * J[3];
+    g.det = J[0] * g.adj[0] + J[1] * g.adj[3] + J[2] * g.adj[6];
+}
+
+// J_ij = sum_a x_{a,i} dN_a/dxi_j (basis_geometry.cpp:112-129)
+__device__ inline void jac_iso(const double4* __restrict__ xyz, const int32_t* __restrict__ cg, int gn,
+                               const double* __restrict__ gd, Jac& g) {
+    double J[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+    for (int a = 0; a < gn; ++a) {
+        const double4 x = xyz[cg[a]];
+        const double g0 = gd[a * 3 + 0], g1 = gd[a * 3 + 1], g2 = gd[a * 3 + 2];
+        J[0] += x.x * g0; J[1] += x.x * g1; J[2] += x.x * g2;
+        J[3] += x.y * g0; J[4] += x.y * g1; J[5] += x.y * g2;
+        J[6] += x.z * g0; J[7] += x.z * g1; J[8] += x.z * g2;
+    }
+    g.adj[0] = J[4] * J[8] - J[5] * J[7];
+    g.adj[1] = J[2] * J[7] - J[1] * J[8];
+    g.adj[2] = J[1] * J[5] - J[2] * J[4];
+    g.adj[3] = J[5] * J[6] - J[3] * J[8];
+    g.adj[4] = J[0] * J[8] - J[2] * J[6];
+    g.adj[5] = J[2] * J[3] - J[0] * J[5];
+    g.adj[6] = J[3] * J[7] - J[4] * J[6];
+    g.adj[7] = J[1] * J[6] - J[0] * J[7];
+    g.adj[8] = J[0] * J[4] - J[1] * J[3];
+    g.det = J[0] * g.adj[0] + J[1] * g.adj[3] + J[2] * g.adj[6];
+}
+
+// Physical gradient dphi/dx_i = sum_k dphi/dxi_k (J^{-1})_{ki}
+// (basis_geometry.cpp:161-173).
+__device__ inline void phys_grad(const double* __restrict__ rg, const Jac& g, double inv, double* out) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] = (rg[0] * g.adj[i] + rg[1] * g.adj[3 + i] + rg[2] * g.adj[6 + i]) * inv;
+}
+
+__global__ void __launch_bounds__(64) k_cross_rows(CrossArgs A) {
+    extern __shared__ double span[];
+    const int lane = threadIdx.x;
+    const int slot = lane / A.group;
+    const int j0 = lane - slot * A.group;
+    const int64_t r0 = int64_t(blockIdx.x) * A.rows_per_wave;
+    const int64_t r1 = min(r0 + int64_t(A.rows_per_wave), A.row_nodes);
+    const int64_t cc = int64_t(A.Cr) * A.Cc;
+    const int64_t lo = A.nptr[r0];
+    const int64_t span_len = (A.nptr[r1] - lo) * cc;
+    for (int64_t k = lane; k < span_len; k += 64) span[k] = 0.;
+    __syncthreads();
+    const int64_t r = r0 + slot;
+    if (r < r1) {
+        const int64_t len = A.nptr[r + 1] - A.nptr[r];
+        double* rs = span + (A.nptr[r] - lo) * cc;
+        const int ncc = A.nc * A.Cc;
+        const int nent = A.Cr * ncc;
+        const int Q = A.Q;
+        const double* W = A.tables;
+        const double* rphi = W + Q;
+        const double* rdphi = rphi + Q * A.nr;
+        const double* cphi = rdphi + Q * A.nr * 3;
+        const double* cdphi = cphi + Q * A.nc;
+        const double* gd = cdphi + Q * A.nc * 3;
+        const int Dr = A.row_grad ? 3 : 1, Dc = A.col_grad ? 3 : 1;
+        for (int64_t k = A.inc_ptr[r]; k < A.inc_ptr[r + 1]; ++k) {
+            const uint8_t* rec = A.records + k * A.rec_bytes;
+            const int32_t packed = *reinterpret_cast<const int32_t*>(rec);
+            const int64_t e = packed >> 4;
+            const int a = packed & 15;
+            Jac G;
+            if (!A.gconn) jac_affine(A.xyz, A.vconn + e * 4, G);
+            for (int j = j0; j < nent; j += A.group) {
+                const int cr = j / ncc;
+                const int rem = j - cr * ncc;
+                const int b = rem / A.Cc;
+                const int ccp = rem - b * A.Cc;
+                const int o = A.off_bytes == 1 ? int(rec[4 + b]) : int(reinterpret_cast<const uint16_t*>(rec + 4)[b]);
+                // the scalar mode couples equal components only; the entry
+                // stays in the pattern with value zero
+                if (A.mode == SKFA_CROSS_SCALAR && cr != ccp) continue;
+                double acc = 0.;
+                for (int q = 0; q < Q; ++q) {
+                    if (A.gconn) jac_iso(A.xyz, A.gconn + e * A.gn, A.gn, gd + q * A.gn * 3, G);
+                    const double inv = 1. / G.det;
+                    const double dx = fabs(G.det) * W[q];
+                    double ra[3], cb[3];
+                    if (A.row_grad) {
+                        phys_grad(rdphi + (q * A.nr + a) * 3, G, inv, ra);
+                    } else {
+                        ra[0] = rphi[q * A.nr + a];
+                        ra[1] = ra[2] = 0.;
+                    }
+                    if (A.col_grad) {
+                        phys_grad(cdphi + (q * A.nc + b) * 3, G, inv, cb);
+                    } else {
+                        cb[0] = cphi[q * A.nc + b];
+                        cb[1] = cb[2] = 0.;
+                    }
+                    double v;
+                    if (A.mode == SKFA_CROSS_TENSOR) {
+                        const double* T = A.coef + e * A.cse + q * A.csq;
+                        v = 0.;
+                        for (int i = 0; i < Dr; ++i)
+                            for (int jj = 0; jj < Dc; ++jj)
+                                v += ra[i] * T[((cr * Dr + i) * A.Cc + ccp) * Dc + jj] * cb[jj];
+                    } else {
+                        const double c = A.coef ? A.coef[e * A.cse + q * A.csq] : 1.;
+                        double contract;
+                        if (A.mode == SKFA_CROSS_ROW_DIVERGENCE)
+                            contract = ra[cr] * cb[0];
+                        else if (A.mode == SKFA_CROSS_COLUMN_DIVERGENCE)
+                            contract = ra[0] * cb[ccp];
+                        else if (A.row_grad)
+                            contract = ra[0] * cb[0] + ra[1] * cb[1] + ra[2] * cb[2];
+                        else
+                            contract = ra[0] * cb[0];
+                        v = c * contract;
+                    }
+                    acc += dx * v;
+                }
+                rs[(int64_t(cr) * len + o) * A.Cc + ccp] += acc;
+            }
+        }
+    }
+    __syncthreads();
+    for (int64_t k = lane; k < span_len; k += 64) A.values[lo * cc + k] = span[k];
+}
+
+}  // namespace
+}  // namespace skfa
+
+using namespace skfa;
+
+extern "C" {
+
+int skfa_assemble_cross(const skfa_cross_plan* P, const double* xyz, const int32_t* vconn, const int32_t* gconn,
+                        int32_t gnodes, const double* tables, int32_t nq, int32_t row_kind, int32_t col_kind,
+                        int32_t mode, const double* coefficient, int64_t coef_stride_e, int64_t coef_stride_q,
+                        double* values, void* stream) {
+    SKFA_REQUIRE(P != nullptr, "plan is null");
+    SKFA_REQUIRE(values != nullptr || P->nnz() == 0, "values pointer is null");
+    SKFA_REQUIRE(P->n_row <= 10 && P->n_col <= 10, "cross assembly supports 4- and 10-node tetrahedral fields");
+    SKFA_REQUIRE(row_kind == SKFA_CROSS_VALUE || row_kind == SKFA_CROSS_GRADIENT, "row kind must be value or gradient");
+    SKFA_REQUIRE(col_kind == SKFA_CROSS_VALUE || col_kind == SKFA_CROSS_GRADIENT,
+                 "column kind must be value or gradient");
+    SKFA_REQUIRE(mode >= SKFA_CROSS_SCALAR && mode <= SKFA_CROSS_TENSOR, "unknown cross mode");
+    SKFA_REQUIRE(coef_stride_e >= 0 && coef_stride_q >= 0, "negative coefficient stride");
+    if (mode == SKFA_CROSS_SCALAR) {
+        SKFA_REQUIRE(P->C_row == P->C_col, "scalar cross coefficient requires equal component counts");
+        SKFA_REQUIRE(row_kind == col_kind, "value-gradient coupling requires a tensor coefficient");
+    } else if (mode == SKFA_CROSS_ROW_DIVERGENCE) {
+        SKFA_REQUIRE(P->C_row == 3 && P->C_col == 1, "row divergence requires vector test and scalar trial");
+        SKFA_REQUIRE(row_kind == SKFA_CROSS_GRADIENT && col_kind == SKFA_CROSS_VALUE,
+                     "row divergence couples row gradients with column values");
+    } else if (mode == SKFA_CROSS_COLUMN_DIVERGENCE) {
+        SKFA_REQUIRE(P->C_row == 1 && P->C_col == 3, "column divergence requires scalar test and vector trial");
+        SKFA_REQUIRE(row_kind == SKFA_CROSS_VALUE && col_kind == SKFA_CROSS_GRADIENT,
+                     "column divergence couples row values with column gradients");
+    } else {
+        SKFA_REQUIRE(coefficient != nullptr, "tensor coefficient is null");
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (P->nnz() == 0 || P->E == 0 || P->row_nodes == 0) {
+        if (P->nnz()) SKFA_HIP(hipMemsetAsync(values, 0, sizeof(double) * P->nnz(), s));
+        return kOk;
+    }
+    SKFA_REQUIRE(xyz != nullptr && vconn != nullptr && tables != nullptr, "geometry pointer is null");
+    SKFA_REQUIRE(nq >= 1 && nq <= 4096, "quadrature size out of range");
+    SKFA_REQUIRE(gconn == nullptr || (gnodes >= 4 && gnodes <= 64), "geometry node count out of range");
+    // rows per wave: lane groups of the next power of two above the entry
+    // count of one element row, fewer rows when their spans outgrow LDS
+    const int nent = P->C_row * P->n_col * P->C_col;
+    int group = 1;
+    while (group < nent && group < 64) group <<= 1;
+    int rows = 64 / group;
+    const int64_t row_bytes = int64_t(P->max_row_len) * P->C_row * P->C_col * 8;
+    while (rows > 1 && rows * row_bytes > kMaxWaveLds) rows >>= 1;
+    if (row_bytes > kMaxWaveLds) {
+        set_error("the CSR span of one row node exceeds 64 KiB of LDS");
+        return kCapacity;
+    }
+    CrossArgs A{};
+    A.inc_ptr = P->inc_ptr.p;
+    A.nptr = P->indptr.p;
+    A.records = P->records.p;
+    A.rec_bytes = P->rec_bytes;
+    A.off_bytes = P->off_bytes;
+    A.row_nodes = P->row_nodes;
+    A.nr = P->n_row;
+    A.nc = P->n_col;
+    A.Cr = P->C_row;
+    A.Cc = P->C_col;
+    A.rows_per_wave = rows;
+    A.group = 64 / rows;
+    A.xyz = reinterpret_cast<const double4*>(xyz);
+    A.vconn = vconn;
+    A.gconn = gconn;
+    A.gn = gconn ? gnodes : 4;
+    A.tables = tables;
+    A.Q = nq;
+    A.row_grad = row_kind == SKFA_CROSS_GRADIENT;
+    A.col_grad = col_kind == SKFA_CROSS_GRADIENT;
+    A.mode = mode;
+    A.coef = coefficient;
+    A.cse = coef_stride_e;
+    A.csq = coef_stride_q;
+    A.values = values;
+    const int64_t waves = (P->row_nodes + rows - 1) / rows;
+    SKFA_REQUIRE(waves <= int64_t(INT32_MAX), "too many row nodes");
+    const size_t lds = size_t(rows * row_bytes);
+    k_cross_rows<<<dim3(unsigned(waves)), dim3(64), lds, s>>>(A);
+    SKFA_LAUNCH_CHECK();
+    return kOk;
+}
+
+}  // extern "C"

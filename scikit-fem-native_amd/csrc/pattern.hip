@@ -156,8 +156,10 @@ __global__ void k_inc_sort(const int64_t* __restrict__ inc_ptr, int32_t* __restr
 }
 
 // ------------------------------------------------------------------ rows
-// One 64-lane wave per node row.  Candidates = columns of every element
-// touching the row; a candidate is "first" if no earlier candidate has the
+// One 64-lane wave per node row.  Candidates = columns (nodes of `conn`, n
+// per element) of every element touching the row; for a rectangular plan
+// `conn` is the column field's connectivity while the incidences come from
+// the row field's. a candidate is "first" if no earlier candidate has the
 // same column; its rank is the number of first candidates with a smaller
 // column, which is both its CSR position within the row (sorted unique
 // order, as std::sort+std::unique gives) and the element->CSR offset.
@@ -204,14 +206,14 @@ __global__ void __launch_bounds__(64) k_row_analyze(const int32_t* __restrict__ 
         } else {
             const int64_t base = indptr[r];
             const int64_t rrec = row_rec[r];
-            const int64_t rstride = row_stride[r];
+            const int64_t rstride = row_stride ? row_stride[r] : 1;
             for (int i = lane; i < m; i += 64) {
                 const int32_t c = cand[i];
                 int rank = 0;
                 for (int j = 0; j < m; ++j) rank += (first[j] && cand[j] < c);
                 if (first[i]) indices[base + rank] = c;
                 const int k = i / n, b = i - (i / n) * n;
-                uint8_t* rec = records + (rrec + k * rstride) * rec_bytes;
+                uint8_t* rec = records + (rrec + int64_t(k) * rstride) * rec_bytes;
                 if (off_bytes == 1)
                     rec[4 + b] = uint8_t(rank);
                 else
@@ -429,22 +431,107 @@ static int plan_build(skfa_plan* P, const int32_t* conn, hipStream_t s) {
     return kOk;
 }
 
+// ------------------------------------------------------------------ cross plans
+__global__ void k_check_range(const int32_t* __restrict__ conn, int64_t total, int64_t nnodes,
+                              int32_t* __restrict__ status) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x) {
+        const int32_t v = conn[i];
+        if (v < 0 || v >= nnodes) atomicOr(status, 1);
+    }
+}
+
+// Rectangular pattern: incidences of the row field's nodes, candidates from
+// the column field's connectivity, one record per incidence.
+static int cross_plan_build(skfa_cross_plan* P, const int32_t* rconn, const int32_t* cconn, hipStream_t s) {
+    const int64_t E = P->E, nn = P->row_nodes;
+    const int nr = P->n_row, nc = P->n_col;
+    DevBuf<int32_t> cnt, status;
+    if (int rc = cnt.alloc(nn + 1)) return rc;
+    if (int rc = status.alloc(2)) return rc;
+    SKFA_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (nn + 1), s));
+    SKFA_HIP(hipMemsetAsync(status.p, 0, sizeof(int32_t) * 2, s));
+    k_inc_count<<<grid_for(E * nr, 256), 256, 0, s>>>(rconn, E * nr, nn, cnt.p, status.p);
+    k_check_range<<<grid_for(E * nc, 256), 256, 0, s>>>(cconn, E * nc, P->col_nodes, status.p);
+    SKFA_LAUNCH_CHECK();
+    if (int rc = P->inc_ptr.alloc(nn + 1)) return rc;
+    if (int rc = exclusive_scan_i32_to_i64(cnt.p, P->inc_ptr.p, nn, s)) return rc;
+    int32_t st = 0;
+    SKFA_HIP(hipMemcpyAsync(&st, status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    if (st & 1) {
+        set_error("element node index is out of bounds");
+        return kInvalid;
+    }
+    if (int rc = P->inc.alloc(size_t(E * nr))) return rc;
+    SKFA_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (nn + 1), s));
+    k_inc_fill<<<grid_for(E * nr, 256), 256, 0, s>>>(rconn, E, nr, nn, P->inc_ptr.p, cnt.p, P->inc.p);
+    SKFA_LAUNCH_CHECK();
+    k_inc_sort<<<grid_for(nn, 256), 256, 0, s>>>(P->inc_ptr.p, P->inc.p, nn);
+    SKFA_LAUNCH_CHECK();
+
+    DevBuf<int32_t> rowlen;
+    if (int rc = rowlen.alloc(nn + 1)) return rc;
+    const unsigned rows_grid = unsigned(std::min<int64_t>(std::max<int64_t>(nn, 1), 1 << 20));
+    k_row_analyze<<<rows_grid, 64, 0, s>>>(cconn, nc, P->inc_ptr.p, P->inc.p, nn, 0, rowlen.p, nullptr, nullptr,
+                                           nullptr, nullptr, nullptr, 0, 1, status.p);
+    SKFA_LAUNCH_CHECK();
+    SKFA_HIP(hipMemcpyAsync(&st, status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    if (st & 2) {
+        set_error("a node row touches more than 4096 element columns");
+        return kCapacity;
+    }
+    if (int rc = P->indptr.alloc(nn + 1)) return rc;
+    if (int rc = exclusive_scan_i32_to_i64(rowlen.p, P->indptr.p, nn, s)) return rc;
+    SKFA_HIP(hipMemcpyAsync(&P->nnz_node, P->indptr.p + nn, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipMemsetAsync(status.p, 0, sizeof(int32_t) * 2, s));
+    k_max_diff_i64<<<grid_for(nn, 256, 1024), 256, 0, s>>>(P->indptr.p, nn, status.p);
+    k_max_diff_i64<<<grid_for(nn, 256, 1024), 256, 0, s>>>(P->inc_ptr.p, nn, status.p + 1);
+    SKFA_LAUNCH_CHECK();
+    int32_t mx[2];
+    SKFA_HIP(hipMemcpyAsync(mx, status.p, sizeof(mx), hipMemcpyDeviceToHost, s));
+    SKFA_HIP(hipStreamSynchronize(s));
+    P->max_row_len = mx[0];
+    P->max_inc = mx[1];
+    if (P->nnz() > INT32_MAX) {
+        set_error("rectangular pattern exceeds 2^31 nonzeros");
+        return kCapacity;
+    }
+    if (P->max_row_len > 65536) {
+        set_error("a node row has more than 65536 columns");
+        return kCapacity;
+    }
+    if (int rc = P->indices.alloc(size_t(P->nnz_node))) return rc;
+    P->off_bytes = P->max_row_len <= 256 ? 1 : 2;
+    const int raw = 4 + nc * P->off_bytes;
+    P->rec_bytes = raw <= 8 ? 8 : raw <= 16 ? 16 : raw <= 32 ? 32 : 64;
+    const size_t rec_total = size_t(E * nr) * P->rec_bytes;
+    if (int rc = P->records.alloc(rec_total ? rec_total : 1)) return rc;
+    k_row_analyze<<<rows_grid, 64, 0, s>>>(cconn, nc, P->inc_ptr.p, P->inc.p, nn, 1, nullptr, P->indptr.p,
+                                           P->indices.p, P->inc_ptr.p, nullptr, P->records.p, P->rec_bytes,
+                                           P->off_bytes, status.p);
+    SKFA_LAUNCH_CHECK();
+    SKFA_HIP(hipStreamSynchronize(s));
+    return kOk;
+}
+
 // ------------------------------------------------------------------ CSR export
 __global__ void k_export_csr(const int64_t* __restrict__ nptr, const int32_t* __restrict__ nidx, int64_t nnodes, int C,
-                             int64_t* __restrict__ indptr, int32_t* __restrict__ indices) {
-    // row 3n+c has columns 3m+d for each m in node row n, m-major d-minor
+                             int CC, int64_t* __restrict__ indptr, int32_t* __restrict__ indices) {
+    // row C*n+c has columns CC*m+d for each m in node row n, m-major d-minor
+    // (C row components, CC column components)
     for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nnodes * C;
          r += int64_t(gridDim.x) * blockDim.x) {
         const int64_t node = r / C;
         const int c = int(r - node * C);
         const int64_t len = nptr[node + 1] - nptr[node];
-        const int64_t start = int64_t(C) * C * nptr[node] + int64_t(c) * C * len;
+        const int64_t start = int64_t(C) * CC * nptr[node] + int64_t(c) * CC * len;
         indptr[r] = start;
         for (int64_t j = 0; j < len; ++j) {
             const int32_t m = nidx[nptr[node] + j];
-            for (int d = 0; d < C; ++d) indices[start + j * C + d] = m * C + d;
+            for (int d = 0; d < CC; ++d) indices[start + j * CC + d] = m * CC + d;
         }
-        if (r == nnodes * C - 1) indptr[r + 1] = start + len * C;
+        if (r == nnodes * C - 1) indptr[r + 1] = start + len * CC;
     }
 }
 
@@ -650,8 +737,8 @@ int skfa_plan_export_csr(const skfa_plan* P, int64_t* indptr, int32_t* indices, 
         SKFA_HIP(hipMemsetAsync(indptr, 0, sizeof(int64_t), s));
         return kOk;
     }
-    k_export_csr<<<grid_for(P->nnodes * P->C, 256), 256, 0, s>>>(P->indptr.p, P->indices.p, P->nnodes, P->C, indptr,
-                                                                  indices);
+    k_export_csr<<<grid_for(P->nnodes * P->C, 256), 256, 0, s>>>(P->indptr.p, P->indices.p, P->nnodes, P->C, P->C,
+                                                                  indptr, indices);
     SKFA_LAUNCH_CHECK();
     return kOk;
 }
@@ -662,6 +749,67 @@ int skfa_plan_export_map(const skfa_plan* P, int64_t* inc_ptr, int32_t* inc, voi
     SKFA_HIP(hipMemcpyAsync(inc_ptr, P->inc_ptr.p, sizeof(int64_t) * (P->nnodes + 1), hipMemcpyDeviceToDevice, s));
     if (P->E * P->n)
         SKFA_HIP(hipMemcpyAsync(inc, P->inc.p, sizeof(int32_t) * P->E * P->n, hipMemcpyDeviceToDevice, s));
+    return kOk;
+}
+
+int skfa_cross_plan_create(const int32_t* row_conn, int32_t row_nodes_per_element, int64_t row_nodes,
+                           int32_t row_components, const int32_t* col_conn, int32_t col_nodes_per_element,
+                           int64_t col_nodes, int32_t col_components, int64_t E, void* stream,
+                           skfa_cross_plan** out) {
+    SKFA_REQUIRE(out != nullptr, "plan output pointer is null");
+    *out = nullptr;
+    SKFA_REQUIRE(E >= 0 && row_nodes >= 0 && col_nodes >= 0, "negative sizes");
+    SKFA_REQUIRE(row_nodes_per_element >= 1 && row_nodes_per_element <= kMaxLocal &&
+                     col_nodes_per_element >= 1 && col_nodes_per_element <= kMaxLocal,
+                 "nodes per element must be in [1, 16]");
+    SKFA_REQUIRE(E < kMaxElements, "element count exceeds 2^27");
+    SKFA_REQUIRE(row_components >= 1 && row_components <= 3 && col_components >= 1 && col_components <= 3,
+                 "components must be 1, 2 or 3");
+    SKFA_REQUIRE(E == 0 || (row_conn != nullptr && col_conn != nullptr), "connectivity pointer is null");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    skfa_cross_plan* P = new skfa_cross_plan();
+    P->E = E;
+    P->n_row = row_nodes_per_element;
+    P->n_col = col_nodes_per_element;
+    P->row_nodes = row_nodes;
+    P->col_nodes = col_nodes;
+    P->C_row = row_components;
+    P->C_col = col_components;
+    int rc = cross_plan_build(P, row_conn, col_conn, s);
+    if (rc != kOk) {
+        delete P;
+        return rc;
+    }
+    *out = P;
+    return kOk;
+}
+
+int skfa_cross_plan_destroy(skfa_cross_plan* plan) {
+    delete plan;
+    return kOk;
+}
+
+int skfa_cross_plan_info(const skfa_cross_plan* P, int64_t* info) {
+    SKFA_REQUIRE(P != nullptr && info != nullptr, "null argument");
+    info[0] = P->nrows();
+    info[1] = P->ncols();
+    info[2] = P->nnz();
+    info[3] = P->nnz_node;
+    info[4] = P->max_row_len;
+    info[5] = P->max_inc;
+    return kOk;
+}
+
+int skfa_cross_plan_export_csr(const skfa_cross_plan* P, int64_t* indptr, int32_t* indices, void* stream) {
+    SKFA_REQUIRE(P != nullptr && indptr != nullptr, "null argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (P->row_nodes == 0) {
+        SKFA_HIP(hipMemsetAsync(indptr, 0, sizeof(int64_t), s));
+        return kOk;
+    }
+    k_export_csr<<<grid_for(P->row_nodes * P->C_row, 256), 256, 0, s>>>(P->indptr.p, P->indices.p, P->row_nodes,
+                                                                         P->C_row, P->C_col, indptr, indices);
+    SKFA_LAUNCH_CHECK();
     return kOk;
 }
 

@@ -89,6 +89,34 @@ struct skfa_plan {
     int64_t nnz() const { return nnz_node * C * C; }
 };
 
+// Rectangular (row field x column field) pattern on one mesh: the device
+// replacement for CrossBilinearAssembler::build_pattern
+// (cpp/src/assembly/cross_bilinear_assembler.cpp:318-326).  Rows are the
+// nodes of the row (test) field, columns the nodes of the column (trial)
+// field; DOF = node * components + component on either side.
+struct skfa_cross_plan {
+    int64_t E = 0;
+    int32_t n_row = 0, n_col = 0;        // nodes per element of each field
+    int64_t row_nodes = 0, col_nodes = 0;
+    int32_t C_row = 1, C_col = 1;
+    int32_t max_row_len = 0, max_inc = 0;
+    // row-node -> (element, local row node) incidences, sorted by element
+    skfa::DevBuf<int64_t> inc_ptr;
+    skfa::DevBuf<int32_t> inc;
+    // node-level rectangular pattern
+    skfa::DevBuf<int64_t> indptr;
+    skfa::DevBuf<int32_t> indices;
+    int64_t nnz_node = 0;
+    // one record per incidence (record k = incidence k): packed (e << 4) | a,
+    // then the in-row node offsets of the element's n_col column nodes
+    skfa::DevBuf<uint8_t> records;
+    int32_t rec_bytes = 0, off_bytes = 1;
+
+    int64_t nrows() const { return row_nodes * C_row; }
+    int64_t ncols() const { return col_nodes * C_col; }
+    int64_t nnz() const { return nnz_node * C_row * C_col; }
+};
+
 namespace skfa {
 constexpr int kMaxCandidates = 4096;   // per node row: incidences * n
 constexpr int kMaxLocal = 16;          // local nodes per element (packed in 4 bits)

@@ -246,53 +246,267 @@ class NativeBilinearForm:
         return vals
 
 
+_CROSS_KIND = {"value": 0, "gradient": 1}
+_CROSS_MODE = {"scalar": 0, "row_divergence": 1, "column_divergence": 2, "tensor": 3}
+
+
+class CrossPlan:
+    """Device rectangular pattern between a test (row) and a trial (column)
+    basis on one mesh (CrossBilinearAssembler's constructor,
+    cross_bilinear_assembler.cpp:19-54,318-341)."""
+
+    def __init__(self, test_basis, trial_basis):
+        import torch
+        L = _native.lib()
+        dr, dc = test_basis.device_data(), trial_basis.device_data()
+        h = ctypes.c_void_p()
+        _native.check(L.skfa_cross_plan_create(
+            _native.ptr(dr["node_conn"]), dr["node_conn"].shape[1], dr["nnodes"], test_basis.elem._dim,
+            _native.ptr(dc["node_conn"]), dc["node_conn"].shape[1], dc["nnodes"], trial_basis.elem._dim,
+            test_basis.mesh.nelements, _native.stream(), ctypes.byref(h)))
+        self._h = h
+        info = (ctypes.c_int64 * 6)()
+        _native.check(L.skfa_cross_plan_info(h, info))
+        self.nrows, self.ncols, self.nnz, self.nnz_node, self.max_row_len, self.max_inc = (int(v) for v in info)
+        dev = _native.device()
+        self.indptr = torch.empty(self.nrows + 1, dtype=torch.int64, device=dev)
+        self.indices = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
+        _native.check(L.skfa_cross_plan_export_csr(h, _native.ptr(self.indptr), _native.ptr(self.indices),
+                                                   _native.stream()))
+        self._host = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    host_csr_arrays = Plan.host_csr_arrays
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _native.load_library().skfa_cross_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def cross_tables(test_basis, trial_basis):
+    """FP64 tables of the cross kernel: w | test phi | test dphi | trial phi |
+    trial dphi | geometry dphi (isoparametric meshes)."""
+    from .basis import tabulate
+    X, W = test_basis.X, test_basis.W
+    rs, rg = tabulate(X, test_basis._quadratic)
+    cs, cg = tabulate(X, trial_basis._quadratic)
+    _, gg = tabulate(X, test_basis.mesh.t.shape[0] == 10)
+    return np.concatenate([np.ravel(W), rs.ravel(), rg.ravel(), cs.ravel(), cg.ravel(), gg.ravel()]).astype(
+        np.float64)
+
+
+def _same_mesh(a, b):
+    return a is b or (a.p.shape == b.p.shape and a.t.shape == b.t.shape and np.array_equal(a.p, b.p)
+                      and np.array_equal(a.t, b.t))
+
+
 class NativeCrossBilinearForm:
-    """Gradient x gradient tensor assembly on one scalar basis
-    (bilinear_form.py:102-178,336-367); returns a fresh CSR copy."""
+    """Assembly between a trial and a test basis (bilinear_form.py:100-362);
+    every call returns a fresh CSR copy of shape (test N, trial N).
+
+    Two volume bases on one tetrahedral mesh with one quadrature rule; rows
+    follow the test basis, columns the trial basis.  The gradient x gradient
+    tensor of one scalar basis with itself runs on the square row-gather
+    kernels; every other pairing runs the rectangular cross kernel."""
 
     def __init__(self, test_basis, trial_basis, *, memory_limit_bytes=None, memory_safety_factor=1.25):
         if test_basis is not trial_basis:
-            raise UnsupportedNativeForm("cross-basis assembly between different bases is outside the MI355X build")
+            if len(test_basis.W) != len(trial_basis.W) or test_basis.mesh.nelements != trial_basis.mesh.nelements:
+                raise ValueError("trial and test quadrature shapes must match")
+            if not np.allclose(test_basis.W, trial_basis.W):
+                raise ValueError("trial and test quadrature weights must match")
+            if not (_same_mesh(test_basis.mesh, trial_basis.mesh) and np.allclose(test_basis.X, trial_basis.X)):
+                raise ValueError("trial and test quadrature coordinates must match")
+        self.test_basis, self.trial_basis = test_basis, trial_basis
         self.basis = test_basis
-        self.memory_estimate = estimate_memory_upper(test_basis)
+        self.memory_estimate = (estimate_memory_upper(test_basis) if test_basis is trial_basis
+                                else estimate_cross_memory(test_basis, trial_basis))
         enforce_memory_budget(self.memory_estimate, memory_limit_bytes, memory_safety_factor)
-        self.plan = basis_plan(test_basis)
-        import torch
-        self._values = torch.zeros(max(self.plan.nnz, 1), dtype=torch.float64, device=_native.device())
         self.shape = (test_basis.N, trial_basis.N)
         self.coefficient_shape = test_basis.coefficient_shape
         self.test_components = test_basis.elem._dim
         self.trial_components = trial_basis.elem._dim
         self.dimension = 3
+        self._square = None
+        self._cross = None
+
+    # ---------------------------------------------------------- plans
+    @property
+    def plan(self):
+        """Square gather plan (gradient x gradient tensor of one scalar basis)."""
+        if self._square is None:
+            import torch
+            plan = basis_plan(self.test_basis)
+            self._square = (plan, torch.zeros(max(plan.nnz, 1), dtype=torch.float64, device=_native.device()))
+        return self._square[0]
+
+    @plan.setter
+    def plan(self, plan):
+        """Use another square plan of the same basis (e.g. another LDS budget)."""
+        import torch
+        self._square = (plan, torch.zeros(max(plan.nnz, 1), dtype=torch.float64, device=_native.device()))
+
+    @property
+    def _values(self):
+        self.plan
+        return self._square[1]
+
+    def cross_plan(self):
+        if self._cross is None:
+            import torch
+            plan = CrossPlan(self.test_basis, self.trial_basis)
+            values = torch.zeros(max(plan.nnz, 1), dtype=torch.float64, device=_native.device())
+            tables = upload(cross_tables(self.test_basis, self.trial_basis))
+            self._cross = (plan, values, tables)
+        return self._cross[0]
+
+    # ---------------------------------------------------------- launches
+    def assemble_cross_device(self, row_kind, column_kind, mode, coefficient=None, strides=(0, 0), out=None):
+        """Launch the cross kernel with a device coefficient (scalar per
+        (e,q) for modes scalar/row_divergence/column_divergence, tensor block
+        for mode tensor) addressed as coefficient[e*strides[0] + q*strides[1]]."""
+        plan = self.cross_plan()
+        _, values, tables = self._cross
+        vals = values if out is None else out
+        b = self.test_basis
+        d = b.device_data()
+        iso = not b._affine
+        L = _native.lib()
+        _native.check(L.skfa_assemble_cross(
+            plan.handle, _native.ptr(d["xyz"]), _native.ptr(d["vconn"]), _native.ptr(d["gconn"]) if iso else None,
+            d["gnodes"] if iso else 4, _native.ptr(tables), len(b.W), _CROSS_KIND[row_kind],
+            _CROSS_KIND[column_kind], _CROSS_MODE[mode], _native.ptr(coefficient), int(strides[0]),
+            int(strides[1]), _native.ptr(vals), _native.stream()))
+        return vals
+
+    def _cross_matrix(self):
+        plan, values, _ = self._cross
+        ip, ix = plan.host_csr_arrays()
+        data = to_host(values[:plan.nnz])
+        m = csr_matrix((data, ix.copy(), ip.copy()), shape=(plan.nrows, plan.ncols), copy=False)
+        m.resize(self.shape)
+        return m
+
+    def _scalar_field(self, coefficient):
+        """(device array, strides) of a scalar field broadcast to (E,Q)."""
+        arr = np.asarray(coefficient, dtype=np.float64)
+        E, Q = self.coefficient_shape
+        if arr.ndim == 0:
+            return upload(arr.reshape(1)), (0, 0)
+        host, per_qp = field_layout(arr, (E, Q), "cross")
+        return upload(host), ((Q, 1) if per_qp else (1, 0))
+
+    def _tensor_field(self, coefficient, expected):
+        """(device array, strides) of a tensor broadcast to expected =
+        (E,Q)+block; element- or point-constant tensors upload one block per
+        element or a single block."""
+        full = np.broadcast_to(np.asarray(coefficient, dtype=np.float64), expected)
+        block = int(np.prod(expected[2:]))
+        if full.strides[0] == 0 and full.strides[1] == 0:
+            return upload(np.ascontiguousarray(full[0, 0]).ravel()), (0, 0)
+        if full.strides[1] == 0 or expected[1] == 1:
+            return upload(np.ascontiguousarray(full[:, 0]).reshape(-1)), (block, 0)
+        return upload(np.ascontiguousarray(full).reshape(-1)), (expected[1] * block, block)
+
+    # ---------------------------------------------------------- reference API
+    def assemble(self, kind, coefficient):
+        """One term of a cross-basis form (bilinear_form.py:180-241): kind
+        'value', 'gradient', 'row_divergence' or 'column_divergence' with a
+        scalar coefficient broadcastable to (E,Q)."""
+        if kind in ("row_divergence", "column_divergence"):
+            if kind == "row_divergence" and (self.test_components != self.dimension or self.trial_components != 1):
+                raise ValueError("row divergence requires vector test and scalar trial")
+            if kind == "column_divergence" and (self.trial_components != self.dimension
+                                                or self.test_components != 1):
+                raise ValueError("column divergence requires scalar test and vector trial")
+            row_kind, column_kind = (("gradient", "value") if kind == "row_divergence" else ("value", "gradient"))
+            mode = kind
+        elif kind in ("value", "gradient"):
+            if self.test_components != self.trial_components:
+                raise ValueError("value and gradient contractions require matching component counts")
+            row_kind = column_kind = kind
+            mode = "scalar"
+        else:
+            raise UnsupportedNativeForm(f"cross-basis assembly does not support the {kind!r} kind")
+        coef, strides = self._scalar_field(coefficient)
+        self.assemble_cross_device(row_kind, column_kind, mode, coef, strides)
+        return self._cross_matrix()
+
+    def assemble_kinds(self, row_kind, column_kind, coefficient=1., *, num_threads=0):
+        """Value/gradient contractions with a scalar coefficient
+        (bilinear_form.py:243-325, volume bases: no normal gradients)."""
+        valid = {"value", "gradient", "normal_gradient"}
+        if row_kind not in valid or column_kind not in valid:
+            raise ValueError("cross kind must be value, gradient, or normal_gradient")
+        if self.test_components != self.trial_components:
+            raise ValueError("cross contractions require matching components")
+        if row_kind == "normal_gradient":
+            raise ValueError("row normal_gradient requires test-basis normals")
+        if column_kind == "normal_gradient":
+            raise ValueError("column normal_gradient requires trial-basis normals")
+        if row_kind == column_kind:
+            coef, strides = self._scalar_field(coefficient)
+            self.assemble_cross_device(row_kind, column_kind, "scalar", coef, strides)
+            return self._cross_matrix()
+        # value x gradient with a scalar: the reference's tensor has
+        # T[c,i,c] = s (or T[c,c,j] = s) for every direction
+        C = self.test_components
+        shape = (C, 3, C) if row_kind == "gradient" else (C, C, 3)
+        unit = np.zeros(shape)
+        for c in range(C):
+            if row_kind == "gradient":
+                unit[c, :, c] = 1.
+            else:
+                unit[c, c, :] = 1.
+        scalar = np.broadcast_to(np.asarray(coefficient, dtype=np.float64), self.coefficient_shape)
+        return self.assemble_tensor(row_kind, column_kind, scalar[(...,) + (None,) * 3] * unit)
 
     def assemble_tensor(self, row_kind, column_kind, coefficient, *, num_threads=0, precision="fp64"):
+        """Caller-supplied contraction tensor (bilinear_form.py:327-362),
+        coefficient broadcastable to (E,Q,Cr,[3],Cc,[3])."""
         valid = {"value", "gradient"}
         if row_kind not in valid or column_kind not in valid:
             raise ValueError("tensor kind must be value or gradient")
-        if (row_kind, column_kind) != ("gradient", "gradient") or self.test_components != 1:
-            raise UnsupportedNativeForm("the MI355X build assembles gradient x gradient tensors of scalar fields")
-        expected = self.coefficient_shape + (1, 3, 1, 3)
+        row_axes = (self.test_components,) + ((self.dimension,) if row_kind == "gradient" else ())
+        column_axes = (self.trial_components,) + ((self.dimension,) if column_kind == "gradient" else ())
+        expected = tuple(self.coefficient_shape) + row_axes + column_axes
         coefficient = np.asarray(coefficient, dtype=np.float64)
         try:
             coefficient = np.broadcast_to(coefficient, expected)
         except ValueError as error:
             raise ValueError(f"tensor coefficient must broadcast to {expected}") from error
-        arr, per_qp = field_layout(coefficient[:, :, 0, :, 0, :], self.coefficient_shape, "tensor", (3, 3))
-        # A == A^T per element: the three off-diagonal pairs agree
-        sym = all(np.array_equal(arr[..., i, j], arr[..., j, i]) for i, j in ((0, 1), (0, 2), (1, 2)))
-        self.assemble_tensor_device(upload(arr), per_qp=per_qp, precision=precision, symmetric=sym)
-        ip, ix = self.plan.host_csr_arrays()
-        data = to_host(self._values[:self.plan.nnz])
-        m = csr_matrix((data, ix.copy(), ip.copy()), shape=(self.plan.nrows, self.plan.nrows), copy=False)
-        m.resize(self.shape)
-        return m
+        square = (self.test_basis is self.trial_basis and (row_kind, column_kind) == ("gradient", "gradient")
+                  and self.test_components == 1)
+        if square:
+            arr, per_qp = field_layout(coefficient[:, :, 0, :, 0, :], self.coefficient_shape, "tensor", (3, 3))
+            # A == A^T per element: the three off-diagonal pairs agree
+            sym = all(np.array_equal(arr[..., i, j], arr[..., j, i]) for i, j in ((0, 1), (0, 2), (1, 2)))
+            self.assemble_tensor_device(upload(arr), per_qp=per_qp, precision=precision, symmetric=sym)
+            ip, ix = self.plan.host_csr_arrays()
+            data = to_host(self._values[:self.plan.nnz])
+            m = csr_matrix((data, ix.copy(), ip.copy()), shape=(self.plan.nrows, self.plan.nrows), copy=False)
+            m.resize(self.shape)
+            return m
+        if precision != "fp64":
+            raise UnsupportedNativeForm("FP32 products are available for the square gradient tensor path")
+        coef, strides = self._tensor_field(coefficient, expected)
+        self.assemble_cross_device(row_kind, column_kind, "tensor", coef, strides)
+        return self._cross_matrix()
 
     def assemble_tensor_device(self, tensor, *, per_qp=False, precision="fp64", out=None, symmetric=False):
         """Tensor (E,3,3) [(E,Q,3,3) when per_qp] already on the device;
         ``symmetric`` declares A == A^T per element (smaller payload)."""
         vals = self._values if out is None else out
-        assemble_bilinear_device(self.basis, self.plan, vals, tensor=tensor, per_qp=per_qp, precision=precision,
-                                 symmetric_tensor=symmetric)
+        assemble_bilinear_device(self.test_basis, self.plan, vals, tensor=tensor, per_qp=per_qp,
+                                 precision=precision, symmetric_tensor=symmetric)
         return vals
 
 
@@ -387,6 +601,19 @@ def estimate_memory_upper(basis):
     nnz = min(N * N, E * L * L)
     b = 8 * (N + 1) + 12 * nnz + 2 * 32 * E * n + 4 * E * n + 8 * (N + 1)
     return AssemblyMemoryEstimate("bilinear", N, N, nnz, int(b))
+
+
+def estimate_cross_memory(test_basis, trial_basis):
+    """Upper bound of the device bytes of one cross assembler: rectangular
+    pattern + values (nnz <= min(Nr Nc, E Lr Lc)), one record per row
+    incidence and the incidence lists."""
+    E = test_basis.mesh.nelements
+    nr, nc = test_basis._node_local.shape[0], trial_basis._node_local.shape[0]
+    Lr, Lc = nr * test_basis.elem._dim, nc * trial_basis.elem._dim
+    Nr, Nc = test_basis.N, trial_basis.N
+    nnz = min(Nr * Nc, E * Lr * Lc)
+    b = 8 * (Nr + 1) + 12 * nnz + 32 * E * nr + 4 * E * nr + 8 * (Nr + 1)
+    return AssemblyMemoryEstimate("cross", Nr, Nc, nnz, int(b))
 
 
 def enforce_memory_budget(estimate, limit, safety_factor=1.25):

@@ -164,6 +164,34 @@ def _native_bilinear_assemble(form, basis, kwargs, num_threads=0, *, memory_limi
     return result
 
 
+def _native_cross_bilinear_assemble(form, trial_basis, test_basis, kwargs, *, memory_limit_bytes=None,
+                                    memory_safety_factor=1.25):
+    """forms.py:407-468: ``asm(form, trial_basis, test_basis)``; rows follow
+    the test basis, columns the trial basis, one fresh CSR per call."""
+    expression = trace_expression(form.function, (TrialValue(), TestValue()), _geometry(test_basis), kwargs,
+                                  context="cross-basis BilinearForm")
+    terms = extract_terms(expression, BilinearTerm, BilinearSum,
+                          message="cross-basis BilinearForm must reduce to value, gradient, or divergence "
+                                  "contractions")
+    by_trial = form._cross_cache.setdefault(trial_basis, WeakKeyDictionary())
+    native = by_trial.get(test_basis)
+    if native is None:
+        native = NativeCrossBilinearForm(test_basis, trial_basis, memory_limit_bytes=memory_limit_bytes,
+                                         memory_safety_factor=memory_safety_factor)
+        by_trial[test_basis] = native
+    elif memory_limit_bytes is not None:
+        from ._assembly import enforce_memory_budget
+        enforce_memory_budget(native.memory_estimate, memory_limit_bytes, memory_safety_factor)
+    result = None
+    for term in terms:
+        if term.kind not in ("value", "gradient", "row_divergence", "column_divergence"):
+            raise UnsupportedNativeForm(f"cross-basis BilinearForm does not support {term.kind!r} terms")
+        coefficient = evaluate_coefficient(term.coefficient, kwargs, factor=term.factor, squeeze=True)
+        matrix = native.assemble(term.kind, coefficient)
+        result = matrix if result is None else result + matrix
+    return result
+
+
 def asm(form, *bases, num_threads=None, memory_limit_bytes=None, memory_safety_factor=1.25, **kwargs):
     """Assemble strictly with the native (MI355X) backend.
 
@@ -186,6 +214,14 @@ def asm(form, *bases, num_threads=None, memory_limit_bytes=None, memory_safety_f
     if isinstance(form, _BilinearForm):
         if kwargs.pop("integration", None) is not None:
             raise UnsupportedNativeForm("interface assembly is outside the MI355X build")
+        if len(bases) == 2:
+            if any(isinstance(b, (list, tuple)) for b in bases):
+                raise UnsupportedNativeForm("interior facet assembly is outside the MI355X build")
+            if num_threads is not None:
+                raise UnsupportedNativeForm("per-call threads are not yet supported for cross-basis BilinearForm")
+            return _native_cross_bilinear_assemble(form, bases[0], bases[1], kwargs,
+                                                   memory_limit_bytes=memory_limit_bytes,
+                                                   memory_safety_factor=memory_safety_factor)
         if len(bases) != 1:
             raise UnsupportedNativeForm("native BilinearForm currently requires one shared basis")
         return _native_bilinear_assemble(form, bases[0], kwargs, requested, memory_limit_bytes=memory_limit_bytes,

@@ -376,3 +376,116 @@ def test_n1_reference_moments_and_curl_kernel():
     v = p[:, t]
     vol = np.abs(np.linalg.det(np.stack([v[:, k] - v[:, 0] for k in (1, 2, 3)], axis=1).transpose(2, 0, 1))) / 6
     assert abs(u @ (Km @ u) - np.sum(kappa * vol) * cvec @ cvec) <= 1e-12
+
+
+# ------------------------------------------------------------------ cross blocks
+# (row field, column field, row kind, column kind, coefficient form); fields
+# are (quadratic, components); row = test, column = trial
+_CROSS_CASES = {
+    "column_divergence_p1_vp2": ((False, 1), (True, 3), "value", "gradient", "coldiv"),
+    "row_divergence_vp2_p1": ((True, 3), (False, 1), "gradient", "value", "rowdiv"),
+    "mass_p1_p2": ((False, 1), (True, 1), "value", "value", "scalar"),
+    "gradient_vp2_vp1": ((True, 3), (False, 3), "gradient", "gradient", "scalar"),
+    "tensor_vp1_p2": ((False, 3), (True, 1), "gradient", "value", "tensor"),
+}
+
+
+def _cross_inputs(case, p2, t2, X, W, seed):
+    (rq, Cr), (cq, Cc), rk, ck, form = _CROSS_CASES[case]
+    E, Q = t2.shape[1], len(W)
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.5, 2.0, (E, Q))
+    if form == "scalar":
+        return c, False
+    if form == "coldiv":
+        T = np.zeros((E, Q, 1, 3, 3))
+        for k in range(3):
+            T[:, :, 0, k, k] = c
+        return T, True
+    if form == "rowdiv":
+        T = np.zeros((E, Q, 3, 3, 1))
+        for k in range(3):
+            T[:, :, k, k, 0] = c
+        return T, True
+    Dr, Dc = (3 if rk == "gradient" else 1), (3 if ck == "gradient" else 1)
+    shape = (E, Q, Cr) + ((Dr,) if rk == "gradient" else ()) + (Cc,) + ((Dc,) if ck == "gradient" else ())
+    return rng.standard_normal(shape), True
+
+
+@pytest.mark.parametrize("case", sorted(_CROSS_CASES))
+def test_c_cross_kernel_matches_reference_cpp(ref, case):
+    """General cross kernel and rectangular pattern against the reference's
+    compiled CrossBilinearAssembler on identical inputs (bit-exact)."""
+    p2, t2 = _small_p2(seed=7)
+    X, W = rule4()
+    (rq, Cr), (cq, Cc), rk, ck, _ = _CROSS_CASES[case]
+    coef, tensor = _cross_inputs(case, p2, t2, X, W, seed=3)
+    ip, ix, mine, Nr, Nc = pipeline.cross(p2, t2, X, W, (rq, Cr), (cq, Cc), rk, ck, coefficient=coef,
+                                          tensor=tensor)
+    (rs, rg, dx, *_), _ = pipeline.basis_geometry(p2, t2, X, W, rq)
+    (cs, cg, *_), _ = pipeline.basis_geometry(p2, t2, X, W, cq)
+    _, rd = pipeline.dofs_of(t2, p2.shape[1], 10 if rq else 4, Cr)
+    _, cd = pipeline.dofs_of(t2, p2.shape[1], 10 if cq else 4, Cc)
+    Cb = ref.CrossBilinearAssembler(rd, cd, rs, cs, dx, rg, cg)
+    theirs, _ = Cb.assemble(np.ascontiguousarray(coef), rk, ck, 1)
+    np.testing.assert_array_equal(ip, Cb.indptr)
+    np.testing.assert_array_equal(ix, Cb.indices)
+    assert (Cb.rows, Cb.columns) == (Nr, Nc)
+    np.testing.assert_array_equal(mine, theirs)
+
+
+def _linear_interpolant(p, t, quadratic, components, field):
+    """Nodal interpolant of a vector-valued function (DOF order node*C + c)."""
+    nodes = 10 if quadratic else 4
+    used = np.unique(t[:nodes])
+    vals = field(p[:, used])  # (C, nused)
+    return np.ascontiguousarray(vals.T).ravel()
+
+
+def test_cross_oracle_closed_forms():
+    """Closed forms that pin orientation, components and directions of the
+    cross kernel without the reference build:
+    * Taylor-Hood B (test P1, trial vector P2, div(u) q): B u = int q div u,
+      so u = (x,0,0) or (0,y,0) gives the P1 load vector sum_{e in a} |T_e|/4
+      and u = (y,0,0) gives zero (a swapped direction/component fails);
+    * the row-divergence block (test vector P2, trial P1) is B transposed;
+    * P1 x P2 mass times the constant 1 is the same load vector;
+    * P1 x P2 stiffness annihilates constants and x_P1^T K x_P2 = |Omega|."""
+    p2, t2 = _small_p2(seed=11, n=2)
+    X, W = rule4()
+    E, Q = t2.shape[1], len(W)
+    (rs, rg, dx, *_), _ = pipeline.basis_geometry(p2, t2, X, W, False)
+    Nr, rd = pipeline.dofs_of(t2, p2.shape[1], 4, 1)
+    load = np.zeros(Nr)
+    np.add.at(load, rd[:, :, 0].ravel(), np.repeat(dx.sum(axis=1) / 4., 4))
+    assert abs(load.sum() - 1.) < 1e-13
+    T = np.zeros((E, Q, 1, 3, 3))
+    for k in range(3):
+        T[:, :, 0, k, k] = 1.
+    ip, ix, vals, Nr, Nc = pipeline.cross(p2, t2, X, W, (False, 1), (True, 3), "value", "gradient",
+                                          coefficient=T, tensor=True)
+    from scipy.sparse import csr_matrix
+    B = csr_matrix((vals, ix, ip), shape=(Nr, Nc))
+    ux = _linear_interpolant(p2, t2, True, 3, lambda x: np.vstack([x[0], 0 * x[0], 0 * x[0]]))
+    uy = _linear_interpolant(p2, t2, True, 3, lambda x: np.vstack([0 * x[0], x[1], 0 * x[0]]))
+    uyx = _linear_interpolant(p2, t2, True, 3, lambda x: np.vstack([x[1], 0 * x[0], 0 * x[0]]))
+    np.testing.assert_allclose(B @ ux, load, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(B @ uy, load, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(B @ uyx, 0., rtol=0, atol=1e-14)
+    T2 = np.zeros((E, Q, 3, 3, 1))
+    for k in range(3):
+        T2[:, :, k, k, 0] = 1.
+    ip2, ix2, vals2, Nr2, Nc2 = pipeline.cross(p2, t2, X, W, (True, 3), (False, 1), "gradient", "value",
+                                               coefficient=T2, tensor=True)
+    Bt = csr_matrix((vals2, ix2, ip2), shape=(Nr2, Nc2))
+    assert (Nr2, Nc2) == (Nc, Nr)
+    np.testing.assert_allclose(Bt.toarray(), B.T.toarray(), rtol=0, atol=1e-15)
+    ip3, ix3, vals3, Nr3, Nc3 = pipeline.cross(p2, t2, X, W, (False, 1), (True, 1), "value", "value")
+    M = csr_matrix((vals3, ix3, ip3), shape=(Nr3, Nc3))
+    np.testing.assert_allclose(M @ np.ones(Nc3), load, rtol=0, atol=1e-15)
+    ip4, ix4, vals4, Nr4, Nc4 = pipeline.cross(p2, t2, X, W, (False, 1), (True, 1), "gradient", "gradient")
+    K = csr_matrix((vals4, ix4, ip4), shape=(Nr4, Nc4))
+    np.testing.assert_allclose(K @ np.ones(Nc4), 0., rtol=0, atol=1e-13)
+    x1 = _linear_interpolant(p2, t2, False, 1, lambda x: x[:1])
+    x2 = _linear_interpolant(p2, t2, True, 1, lambda x: x[:1])
+    assert abs(x1 @ (K @ x2) - 1.) < 1e-13
