@@ -209,7 +209,10 @@ def timed(fn, steps, warmup, L, ev0, ev1, stream):
 def other_workload(name, steps, warmup, L, ev0, ev1, stream):
     """The other assembly configurations at full size (one JSON line each):
     c1 P1 Laplace + load 32^3, c3 vector-P2 elasticity 48^3, c4 N1 curl-curl +
-    kappa mass 96^3.  Algorithmic bytes per element: 106.7, 3255 and 356."""
+    kappa mass 96^3, th the Taylor-Hood divergence block (vector P2 x P1) on
+    the 64^3 jittered cube.  Algorithmic bytes per element: 106.7, 3255, 356,
+    and for th 24 V/E + 56 + 480 + 8 nnz/E (vertices, both connectivities,
+    int32 element->CSR map, values)."""
     import torch
     import skfemntv as sk
     from skfemntv._assembly import NativeBilinearForm, NativeLinearForm, upload
@@ -265,6 +268,27 @@ def other_workload(name, steps, warmup, L, ev0, ev1, stream):
             asm_.assemble_device(kappa, one, False)
         bpe, nnz = 356.0, asm_.plan.nnz
         desc = f"N1 curl-curl + kappa mass, 96^3 Kuhn cube, {E} tets, FP64"
+    elif name == "th":
+        # Taylor-Hood divergence block B = int q div u: trial vector P2,
+        # test P1, on the 64^3 jittered P2 cube of the headline workload
+        mesh = jittered_p2_cube(64, 0.2, 0)
+        t2 = mesh.t
+        X, W = sk.basis.tet_rule_4()
+        vp2 = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP2(), dim=3), quadrature=(X, W))
+        p1 = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP1(), dim=1), quadrature=(X, W))
+        E = mesh.nelements
+        form = sk.NativeCrossBilinearForm(p1, vp2)
+        plan = form.cross_plan()
+        one = upload(np.ones(1))
+
+        def step(precision="fp64"):
+            form.assemble_cross_device("value", "gradient", "column_divergence", one, (0, 0))
+        nnz = plan.nnz
+        V = int(np.unique(t2[:4]).size)
+        # vertices (24 B each) + int32 connectivity of both fields (4 + 10
+        # nodes) + int32 element->CSR map (4 x 30 entries) + FP64 values
+        bpe = (24. * V + 4. * 14 * E + 4. * 4 * 30 * E + 8. * nnz) / E
+        desc = f"Taylor-Hood divergence block div(u) q, vector P2 x P1, 64^3 Kuhn cube jittered 0.2h, {E} tets, FP64"
     else:
         raise SystemExit(f"unknown workload {name}")
     torch.cuda.synchronize()
@@ -303,7 +327,7 @@ def main():
     ap.add_argument("--cells", type=int, nargs=2, default=(16, 48), help="batched workload: cells per axis range")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (counter calibration)")
-    ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c3", "c4"),
+    ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c3", "c4", "th"),
                     help="c2 (default): the headline line; c1/c3/c4: the other assembly configurations")
     args = ap.parse_args()
     if args.gpus != 1:

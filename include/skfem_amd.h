@@ -133,8 +133,15 @@ int skfa_cross_plan_export_csr(const skfa_cross_plan* plan, int64_t* indptr, int
  *   coefficient FP64 scalar (modes 0-2) or tensor block (mode 3) of element e,
  *            point q at coefficient + e*coef_stride_e + q*coef_stride_q
  *            (strides in doubles; 0 broadcasts); NULL = 1 (modes 0-2 only).
- * Each row-node group of one wave owns its CSR span in LDS and adds its
- * incidences in element order, so results are deterministic. */
+ *   rtab     optional reduced table rt[a][b][k][l] = sum_q w_q h_ak(q) h_bl(q)
+ *            (nr x nc x [3] x [3]; h = phi for a value side, dphi/dxi for a
+ *            gradient side).  With rtab, affine geometry (gconn NULL), modes
+ *            0-2 and an element-constant coefficient (coef_stride_q 0) a
+ *            reduced-table kernel runs for the field pairs (nodes, components)
+ *            P1 x vector P2, vector P2 x P1, P1 x P2, P2 x P1, P1 x P1, P2 x P2;
+ *            otherwise, or with rtab NULL, the quadrature kernel.
+ * The CSR span of the rows of one wave is accumulated in LDS and every entry
+ * adds its incidences in element order, so results are deterministic. */
 #define SKFA_CROSS_VALUE 0
 #define SKFA_CROSS_GRADIENT 1
 #define SKFA_CROSS_SCALAR 0
@@ -142,9 +149,9 @@ int skfa_cross_plan_export_csr(const skfa_cross_plan* plan, int64_t* indptr, int
 #define SKFA_CROSS_COLUMN_DIVERGENCE 2
 #define SKFA_CROSS_TENSOR 3
 int skfa_assemble_cross(const skfa_cross_plan* plan, const double* xyz, const int32_t* vconn,
-                        const int32_t* gconn, int32_t gnodes, const double* tables, int32_t nq, int32_t row_kind,
-                        int32_t col_kind, int32_t mode, const double* coefficient, int64_t coef_stride_e,
-                        int64_t coef_stride_q, double* values, void* stream);
+                        const int32_t* gconn, int32_t gnodes, const double* tables, int32_t nq, const double* rtab,
+                        int32_t row_kind, int32_t col_kind, int32_t mode, const double* coefficient,
+                        int64_t coef_stride_e, int64_t coef_stride_q, double* values, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Bilinear assembly into CSR values (nnz FP64, overwritten).

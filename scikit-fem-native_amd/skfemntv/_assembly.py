@@ -303,6 +303,20 @@ def cross_tables(test_basis, trial_basis):
         np.float64)
 
 
+def cross_reduced_table(test_basis, trial_basis, row_kind, column_kind):
+    """rt[a][b][k][l] = sum_q w_q h_ak(q) h_bl(q) with h = phi (value side) or
+    dphi/dxi (gradient side), FP64 (nr, nc, [3], [3]) flattened: the element
+    matrix of affine elements with element-constant coefficients depends on
+    the rule only through these sums."""
+    from .basis import tabulate
+    X, W = test_basis.X, test_basis.W
+    rs, rg = tabulate(X, test_basis._quadratic)
+    cs, cg = tabulate(X, trial_basis._quadratic)
+    hr = rg if row_kind == "gradient" else rs[:, :, None]
+    hc = cg if column_kind == "gradient" else cs[:, :, None]
+    return np.einsum("q,qak,qbl->abkl", W, hr, hc).astype(np.float64).ravel()
+
+
 def _same_mesh(a, b):
     return a is b or (a.p.shape == b.p.shape and a.t.shape == b.t.shape and np.array_equal(a.p, b.p)
                       and np.array_equal(a.t, b.t))
@@ -337,6 +351,7 @@ class NativeCrossBilinearForm:
         self.dimension = 3
         self._square = None
         self._cross = None
+        self._rtabs = {}
 
     # ---------------------------------------------------------- plans
     @property
@@ -379,10 +394,17 @@ class NativeCrossBilinearForm:
         b = self.test_basis
         d = b.device_data()
         iso = not b._affine
+        rtab = None
+        if not iso and mode != "tensor":
+            key = (row_kind, column_kind)
+            rtab = self._rtabs.get(key)
+            if rtab is None:
+                rtab = upload(cross_reduced_table(self.test_basis, self.trial_basis, row_kind, column_kind))
+                self._rtabs[key] = rtab
         L = _native.lib()
         _native.check(L.skfa_assemble_cross(
             plan.handle, _native.ptr(d["xyz"]), _native.ptr(d["vconn"]), _native.ptr(d["gconn"]) if iso else None,
-            d["gnodes"] if iso else 4, _native.ptr(tables), len(b.W), _CROSS_KIND[row_kind],
+            d["gnodes"] if iso else 4, _native.ptr(tables), len(b.W), _native.ptr(rtab), _CROSS_KIND[row_kind],
             _CROSS_KIND[column_kind], _CROSS_MODE[mode], _native.ptr(coefficient), int(strides[0]),
             int(strides[1]), _native.ptr(vals), _native.stream()))
         return vals

@@ -75,7 +75,7 @@ def _declare(lib):
         "skfa_cross_plan_destroy": (ctypes.c_int, [P]),
         "skfa_cross_plan_info": (ctypes.c_int, [P, ctypes.POINTER(I64)]),
         "skfa_cross_plan_export_csr": (ctypes.c_int, [P, P, P, P]),
-        "skfa_assemble_cross": (ctypes.c_int, [P, P, P, P, I32, P, I32, I32, I32, I32, P, I64, I64, P, P]),
+        "skfa_assemble_cross": (ctypes.c_int, [P, P, P, P, I32, P, I32, P, I32, I32, I32, P, I64, I64, P, P]),
         "skfa_assemble_bilinear": (ctypes.c_int, [P, P, I64, P, P, I32, P, I32, P, P, I32, I32,
                                                   P, P, P, P, P, I32, I32, I32, P, P]),
         "skfa_assemble_linear": (ctypes.c_int, [P, P, P, P, I32, P, I32, P, I32, P, I32, P, P]),

@@ -80,6 +80,13 @@ def test_cross_forms_through_asm_match_oracle(n, jitter):
                                           coefficient=div_tensor(cq, False), tensor=True)
     check_csr(K, ip, ix, vals, (B["p1"].N, B["vp2"].N))
 
+    say(f"cross asm column divergence, per-point coefficient n={n}")
+    cp = rng.uniform(0.5, 2., (E, Q))
+    K = sk.asm(coldiv, B["vp2"], B["p1"], c=cp)
+    _, _, vals, _, _ = pipeline.cross(p2, t2, X, W, (False, 1), (True, 3), "value", "gradient",
+                                      coefficient=div_tensor(cp, False), tensor=True)
+    check_csr(K, ip, ix, vals, (B["p1"].N, B["vp2"].N))
+
     say(f"cross asm row divergence n={n}")
     rowdiv = sk.BilinearForm(lambda u, v, w: div(v) * u)
     K = sk.asm(rowdiv, B["p1"], B["vp2"])
