@@ -196,3 +196,23 @@ def n1_basis(p, t, X, W, signs):
     s = signs.T[:, :, None, None].astype(np.float64)
     dx = np.abs(det)[:, None] * W[None, :]
     return dx, np.ascontiguousarray(values * s), np.ascontiguousarray(curls * s), det
+
+
+def composite_dofs(npoints, conns, components):
+    """Composite-basis numbering (basis.py:2102-2127), restated as the
+    reference's loop: nodes in increasing id, at each node the fields that use
+    it in order, each with its consecutive components.  conns: per field the
+    (n_f, E) node connectivity.  Returns (N, [element dofs (E, n_f, C_f)])."""
+    active = [set(np.unique(c).tolist()) for c in conns]
+    field_node = [dict() for _ in conns]
+    nxt = 0
+    for node in range(npoints):
+        for f, (act, C) in enumerate(zip(active, components)):
+            if node in act:
+                field_node[f][node] = nxt
+                nxt += C
+    out = []
+    for f, (c, C) in enumerate(zip(conns, components)):
+        first = np.vectorize(lambda v: field_node[f][int(v)])(c.T)  # (E, n_f)
+        out.append(first[:, :, None] + np.arange(C)[None, None, :])
+    return nxt, out

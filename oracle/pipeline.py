@@ -75,6 +75,40 @@ def cross(p, t, X, W, row, col, row_kind, col_kind, coefficient=None, tensor=Fal
     return ip, ix, vals, Nr, Nc
 
 
+def composite_block(p, t, X, W, fields, row_field, col_field, kind, coefficient=None):
+    """One block of a composite basis as the reference assembles it
+    (bilinear_form.py:370-455): the cross kernel on the composite DOFs of the
+    two fields, divergence kinds through the (Cr,[d],Cc,[d]) unit tensor
+    times the scalar.  fields: [(quadratic, components), ...].  Returns
+    (indptr, indices, values, N) with N x N shape."""
+    conns = [t[:(10 if q else 4)] for q, _ in fields]
+    N, dofs = host.composite_dofs(p.shape[1], conns, [C for _, C in fields])
+    (rq, Cr), (cq, Cc) = fields[row_field], fields[col_field]
+    (rs, rg, dx, *_), _ = basis_geometry(p, t, X, W, rq)
+    (cs, cg, *_), _ = basis_geometry(p, t, X, W, cq)
+    E, Q = dx.shape
+    scalar = np.broadcast_to(np.asarray(1. if coefficient is None else coefficient, dtype=float), (E, Q))
+    if kind in ("row_divergence", "column_divergence"):
+        if kind == "row_divergence":
+            unit = np.zeros((Cr, 3, Cc))
+            for c in range(Cr):
+                unit[c, c, 0] = 1.
+            rk, ck = "gradient", "value"
+        else:
+            unit = np.zeros((Cr, Cc, 3))
+            for c in range(Cc):
+                unit[0, c, c] = 1.
+            rk, ck = "value", "gradient"
+        coef, tensor = scalar[(...,) + (None,) * 3] * unit, True
+    else:
+        coef, tensor, rk, ck = np.ascontiguousarray(scalar), False, kind, kind
+    rows = dofs[row_field].reshape(E, -1)
+    cols = dofs[col_field].reshape(E, -1)
+    ip, ix, sc = _o.pattern(rows, cols, nrows=N)
+    vals = _o.assemble_cross(rs, rg, cs, cg, dx, sc, len(ix), Cr, Cc, rk, ck, coefficient=coef, tensor=tensor)
+    return ip, ix, vals, N
+
+
 def load_vector(p, t, X, W, quadratic, f_const, components=1):
     (shape, grads, dx, x, det, tol, cond), bad = basis_geometry(p, t, X, W, quadratic)
     nodes = 10 if quadratic else 4

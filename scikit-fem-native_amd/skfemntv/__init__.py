@@ -9,10 +9,11 @@ PyTorch-ROCm only provides device buffers and the stream.
 
 from ._symbols import UnsupportedNativeForm
 from .basis import Basis, GeometryDiagnostics
-from .element import ElementTetP1, ElementTetP2, ElementVector
+from .element import ElementComposite, ElementTetP1, ElementTetP2, ElementVector
 from .forms import BilinearForm, LinearForm, asm
 from ._assembly import (AssemblyMemoryBudgetError, AssemblyMemoryEstimate, LinearFormDiagnostics,
-                        NativeBilinearForm, NativeCrossBilinearForm, NativeLinearForm)
+                        NativeBilinearForm, NativeCompositeBilinearForm, NativeCompositeLinearForm,
+                        NativeCrossBilinearForm, NativeLinearForm)
 from .hcurl import (AffineTetN1Basis, HcurlTetGeometryDiagnostics, TetN1Assembler,
                     estimate_tet_n1_assembly_memory)
 from .mesh import MeshTet, MeshTet2
@@ -27,6 +28,7 @@ __all__ = [
     "set_num_threads", "thread_limit", "AssemblyMemoryBudgetError", "AssemblyMemoryEstimate",
     "AffineTetN1Basis", "HcurlTetGeometryDiagnostics", "TetN1Assembler", "estimate_tet_n1_assembly_memory",
     "LinearFormDiagnostics", "NativeLinearForm", "NativeBilinearForm", "NativeCrossBilinearForm",
+    "NativeCompositeBilinearForm", "NativeCompositeLinearForm", "ElementComposite",
     "LinearForm", "BilinearForm", "Basis", "GeometryDiagnostics", "ElementTetP1", "ElementTetP2",
     "ElementVector", "MeshTet", "MeshTet2", "asm", "UnsupportedNativeForm", "NativeUnavailable", "NativeError",
 ]

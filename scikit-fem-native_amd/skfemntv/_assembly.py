@@ -532,6 +532,82 @@ class NativeCrossBilinearForm:
         return vals
 
 
+def _composite_csr(ip, ix, data, row_map, col_map, N):
+    """Field-local CSR -> composite numbering.  Both maps are strictly
+    increasing (composite DOFs follow node order within a field), so rows
+    keep their order and columns stay sorted: only indptr is expanded and the
+    column ids relabelled."""
+    counts = np.zeros(N, dtype=np.int64)
+    counts[row_map] = np.diff(np.asarray(ip, dtype=np.int64))
+    indptr = np.zeros(N + 1, dtype=np.int64)
+    np.cumsum(counts, out=indptr[1:])
+    indices = col_map[np.asarray(ix, dtype=np.int64)]
+    if indptr[-1] < 2 ** 31 - 1:
+        indptr, indices = indptr.astype(np.int32), indices.astype(np.int32)
+    return csr_matrix((data, indices, indptr), shape=(N, N), copy=False)
+
+
+class NativeCompositeBilinearForm:
+    """Blocks between the subfields of a composite basis
+    (bilinear_form.py:370-455); each call returns a fresh (N, N) CSR in the
+    composite numbering.  Value and gradient couplings of a field with itself
+    run on the square row-gather kernels, every other block on the cross
+    kernels."""
+
+    def __init__(self, basis):
+        self.basis = basis
+        self._assemblers = {}
+
+    def _square(self, field):
+        key = ("square", field)
+        native = self._assemblers.get(key)
+        if native is None:
+            native = NativeBilinearForm(self.basis.subbases[field])
+            self._assemblers[key] = native
+        return native
+
+    def _cross(self, row_field, column_field):
+        key = ("cross", row_field, column_field)
+        native = self._assemblers.get(key)
+        if native is None:
+            native = NativeCrossBilinearForm(self.basis.subbases[row_field], self.basis.subbases[column_field])
+            self._assemblers[key] = native
+        return native
+
+    def assemble(self, row_field, column_field, *, kind, coefficient):
+        row = self.basis.subbases[row_field]
+        column = self.basis.subbases[column_field]
+        rc, cc = row.elem._dim, column.elem._dim
+        divergence = kind in ("row_divergence", "column_divergence")
+        if kind not in ("value", "gradient") and not divergence:
+            raise UnsupportedNativeForm(f"composite blocks do not support the {kind!r} kind")
+        if rc != cc and not divergence:
+            raise ValueError("component-mismatched composite coupling requires an explicit tensor contraction")
+        if kind == "row_divergence" and (rc != 3 or cc != 1):
+            raise ValueError("row divergence requires vector test and scalar trial")
+        if kind == "column_divergence" and (cc != 3 or rc != 1):
+            raise ValueError("column divergence requires scalar test and vector trial")
+        if row_field == column_field and not divergence:
+            native = self._square(row_field)
+            dev, per_qp = _prepare_scalar_kinds(row, {kind: coefficient})
+            native.assemble_device(per_qp=per_qp, **dev)
+            plan, values = native.plan, native._values
+        else:
+            native = self._cross(row_field, column_field)
+            if divergence:
+                row_kind, column_kind = ("gradient", "value") if kind == "row_divergence" else ("value", "gradient")
+                mode = kind
+            else:
+                row_kind = column_kind = kind
+                mode = "scalar"
+            coef, strides = native._scalar_field(coefficient)
+            native.assemble_cross_device(row_kind, column_kind, mode, coef, strides)
+            plan, values = native.cross_plan(), native._cross[1]
+        ip, ix = plan.host_csr_arrays()
+        data = to_host(values[:plan.nnz])
+        return _composite_csr(ip, ix, data, row._composite_dofs, column._composite_dofs, self.basis.N)
+
+
 @dataclass(frozen=True)
 class LinearFormDiagnostics:
     entity_count: int
@@ -645,3 +721,27 @@ def enforce_memory_budget(estimate, limit, safety_factor=1.25):
     if need > int(limit):
         raise AssemblyMemoryBudgetError(
             f"assembly needs about {need} bytes (safety factor {safety_factor}) but the limit is {int(limit)} bytes")
+
+
+class NativeCompositeLinearForm:
+    """Linear assemblers per composite subfield (linear_form.py:110-132);
+    ``assemble(field, value=, gradient=)`` returns the field's load vector in
+    the composite numbering (length N, zeros at the other fields' DOFs)."""
+
+    def __init__(self, basis):
+        self.basis = basis
+        self._assemblers = {}
+
+    def assembler(self, field):
+        native = self._assemblers.get(field)
+        if native is None:
+            native = NativeLinearForm(self.basis.subbases[field])
+            self._assemblers[field] = native
+        return native
+
+    def assemble(self, field, *, value=None, gradient=None):
+        vector, _ = self.assembler(field).assemble(value=value, gradient=gradient)
+        result = np.zeros(self.basis.N, dtype=np.float64)
+        sub = self.basis.subbases[field]
+        result[sub._composite_dofs] = vector[:sub._field_N]
+        return result

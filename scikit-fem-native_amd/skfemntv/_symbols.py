@@ -276,14 +276,16 @@ class QuadratureValue:
         return QuadratureValue(getattr(ufunc, method)(*values, **kwargs))
 
     def __mul__(self, other):
-        if isinstance(other, BilinearTerm):
+        if isinstance(other, (BilinearTerm, CompositeBilinearTerm)):
             return other * np.asarray(self.value)
+        if isinstance(other, CompositeField):
+            return CompositeWeightedField(other, np.asarray(self.value))
         if isinstance(other, Divergence):
             return Divergence(other.role, other.factor, np.asarray(self.value))
         return QuadratureValue(self.value * np.asarray(other))
 
     def __rmul__(self, other):
-        if isinstance(other, BilinearTerm):
+        if isinstance(other, (BilinearTerm, CompositeBilinearTerm)):
             return other * np.asarray(self.value)
         return QuadratureValue(np.asarray(other) * self.value)
 
@@ -354,3 +356,209 @@ def extract_terms(expression, term_type, sum_type, *, message):
     if isinstance(expression, term_type):
         return (expression,)
     raise UnsupportedNativeForm(message)
+
+
+# ------------------------------------------------------------------ composite fields
+# Subfields of a composite basis (python/skfemntv/_composite_fields.py,
+# _form_terms.py:117-196): a field is (role, index, kind); products of a
+# trial and a test field give a block term (row = test field, column = trial
+# field) of kind value / gradient / row_divergence / column_divergence.
+def _block(left, right, kind):
+    if not isinstance(left, CompositeField) or not isinstance(right, CompositeField):
+        raise UnsupportedNativeForm("composite contraction requires two subfields")
+    if left.role == right.role:
+        raise UnsupportedNativeForm("composite contraction requires trial and test subfields")
+    test, trial = (left, right) if left.role == "test" else (right, left)
+    return CompositeBilinearTerm(test.field, trial.field, kind)
+
+
+def _divergence_block(value, divergence):
+    field = divergence.field
+    if value.role == field.role:
+        raise UnsupportedNativeForm("divergence coupling requires trial and test subfields")
+    if field.role == "test":
+        return CompositeBilinearTerm(field.field, value.field, "row_divergence")
+    return CompositeBilinearTerm(value.field, field.field, "column_divergence")
+
+
+@dataclass(frozen=True)
+class CompositeField:
+    role: str
+    field: int
+    kind: str = "value"
+
+    def __mul__(self, other):
+        if isinstance(other, CompositeDivergence):
+            return _divergence_block(self, other)
+        if isinstance(other, CompositeField):
+            if self.kind != "value" or other.kind != "value":
+                return NotImplemented
+            return _block(self, other, "value")
+        if is_symbolic(other):
+            return CompositeWeightedField(self, other)
+        if np.isscalar(other) or hasattr(other, "__array__") or (hasattr(other, "value") and hasattr(other, "grad")):
+            return CompositeWeightedField(self, np.asarray(other))
+        return NotImplemented
+
+    __rmul__ = __mul__
+
+    def __neg__(self):
+        return CompositeWeightedField(self, -1.0)
+
+
+@dataclass(frozen=True)
+class CompositeWeightedField:
+    field: CompositeField
+    coefficient: Any
+
+    def __mul__(self, other):
+        if isinstance(other, CompositeDivergence):
+            term = _divergence_block(self.field, other)
+        elif isinstance(other, CompositeField):
+            term = _block(self.field, other, "value")
+        else:
+            return NotImplemented
+        return CompositeBilinearTerm(term.row_field, term.column_field, term.kind, self.coefficient, term.factor)
+
+    __rmul__ = __mul__
+
+    def _linear_term(self):
+        if self.field.role != "test":
+            raise UnsupportedNativeForm("composite LinearForm requires test subfields")
+        return CompositeLinearTerm(self.field.field, self.field.kind, self.coefficient)
+
+    def __add__(self, other):
+        return self._linear_term() + other
+
+    __radd__ = __add__
+
+    def __neg__(self):
+        if isinstance(self.coefficient, str) or is_symbolic(self.coefficient):
+            return CompositeLinearTerm(self.field.field, self.field.kind, self.coefficient, -1.0)
+        return CompositeWeightedField(self.field, -np.asarray(self.coefficient))
+
+    def __sub__(self, other):
+        return self + (-other)
+
+
+@dataclass(frozen=True)
+class CompositeDivergence:
+    field: CompositeField
+
+    def __mul__(self, other):
+        if isinstance(other, (CompositeField, CompositeWeightedField)):
+            return other * self
+        return NotImplemented
+
+    __rmul__ = __mul__
+
+
+@dataclass(frozen=True)
+class CompositeBilinearTerm:
+    row_field: int
+    column_field: int
+    kind: str
+    coefficient: Any = None
+    factor: float = 1.0
+
+    def __mul__(self, other):
+        if np.isscalar(other):
+            return CompositeBilinearTerm(self.row_field, self.column_field, self.kind, self.coefficient,
+                                         self.factor * other)
+        if is_symbolic(other):
+            return CompositeBilinearTerm(self.row_field, self.column_field, self.kind, other, self.factor)
+        if isinstance(other, np.ndarray) or (hasattr(other, "value") and hasattr(other, "grad")):
+            if self.coefficient is not None:
+                raise UnsupportedNativeForm("multiple composite coefficients are not supported")
+            return CompositeBilinearTerm(self.row_field, self.column_field, self.kind, np.asarray(other),
+                                         self.factor)
+        return NotImplemented
+
+    __rmul__ = __mul__
+
+    def __neg__(self):
+        return CompositeBilinearTerm(self.row_field, self.column_field, self.kind, self.coefficient, -self.factor)
+
+    def __add__(self, other):
+        if isinstance(other, CompositeBilinearTerm):
+            return CompositeBilinearSum((self, other))
+        if isinstance(other, CompositeBilinearSum):
+            return CompositeBilinearSum((self,) + other.terms)
+        return NotImplemented
+
+    def __sub__(self, other):
+        return self + (-other)
+
+
+@dataclass(frozen=True)
+class CompositeBilinearSum:
+    terms: tuple
+
+    def __add__(self, other):
+        if isinstance(other, CompositeBilinearTerm):
+            return CompositeBilinearSum(self.terms + (other,))
+        if isinstance(other, CompositeBilinearSum):
+            return CompositeBilinearSum(self.terms + other.terms)
+        return NotImplemented
+
+    __radd__ = __add__
+
+    def __neg__(self):
+        return CompositeBilinearSum(tuple(-t for t in self.terms))
+
+    def __sub__(self, other):
+        return self + (-other)
+
+
+@dataclass(frozen=True)
+class CompositeLinearTerm:
+    field: int
+    kind: str
+    coefficient: Any
+    factor: float = 1.0
+
+    def __mul__(self, other):
+        if np.isscalar(other):
+            return CompositeLinearTerm(self.field, self.kind, self.coefficient, self.factor * other)
+        return NotImplemented
+
+    __rmul__ = __mul__
+
+    def __neg__(self):
+        return CompositeLinearTerm(self.field, self.kind, self.coefficient, -self.factor)
+
+    def __add__(self, other):
+        if isinstance(other, CompositeWeightedField):
+            other = other._linear_term()
+        if isinstance(other, CompositeLinearTerm):
+            return CompositeLinearSum((self, other))
+        if isinstance(other, CompositeLinearSum):
+            return CompositeLinearSum((self,) + other.terms)
+        return NotImplemented
+
+    __radd__ = __add__
+
+    def __sub__(self, other):
+        return self + (-other)
+
+
+@dataclass(frozen=True)
+class CompositeLinearSum:
+    terms: tuple
+
+    def __add__(self, other):
+        if isinstance(other, CompositeWeightedField):
+            other = other._linear_term()
+        if isinstance(other, CompositeLinearTerm):
+            return CompositeLinearSum(self.terms + (other,))
+        if isinstance(other, CompositeLinearSum):
+            return CompositeLinearSum(self.terms + other.terms)
+        return NotImplemented
+
+    __radd__ = __add__
+
+    def __neg__(self):
+        return CompositeLinearSum(tuple(-t for t in self.terms))
+
+    def __sub__(self, other):
+        return self + (-other)

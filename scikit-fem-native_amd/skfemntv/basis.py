@@ -18,7 +18,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native
-from .element import ElementTetP1, ElementTetP2, ElementVector
+from .element import ElementComposite, ElementTetP1, ElementTetP2, ElementVector
 from .mesh import MeshTet, MeshTet2
 
 _EPS = np.finfo(np.float64).eps
@@ -175,6 +175,11 @@ class Basis:
     def __init__(self, mesh: MeshTet, element: ElementVector, intorder=2, quadrature=None, elements=None):
         intorder = 2 if intorder is None else int(intorder)
         self.intorder = intorder
+        if isinstance(element, ElementComposite):
+            if elements is not None:
+                raise NotImplementedError("element restriction is outside the MI355X assembly build")
+            self._init_composite(mesh, element, intorder, quadrature)
+            return
         scalar = element.elem if isinstance(element, ElementVector) else None
         if not isinstance(element, ElementVector) or not isinstance(scalar, (ElementTetP1, ElementTetP2)):
             raise NotImplementedError(
@@ -226,6 +231,61 @@ class Basis:
         self._device = None
         self._lazy = {}
         self._check_geometry()
+
+    # ---------------------------------------------------------- composite
+    def _init_composite(self, mesh, element, intorder, quadrature=None):
+        """Composite basis (basis.py:2076-2158): one sub-basis per field and a
+        node-interleaved numbering -- nodes in increasing id, at each node the
+        fields in order, each with its consecutive components.  Sub-bases keep
+        their field-local node numbering for the kernels; their
+        ``element_dofs`` / ``nodal_dofs`` / ``N`` follow the composite numbering
+        as in the reference."""
+        fields = tuple(f if isinstance(f, ElementVector) else ElementVector(f, dim=1) for f in element.elems)
+        subbases = [Basis(mesh, f, intorder=intorder, quadrature=quadrature) for f in fields]
+        shapes = {(b.X.shape, b.W.shape) for b in subbases}
+        if len(shapes) != 1 or any(not np.array_equal(subbases[0].X, b.X) for b in subbases[1:]):
+            raise ValueError("composite fields require a common quadrature rule; "
+                             "increase intorder for mixed-order elements")
+        components = tuple(f._dim for f in fields)
+        npoints = mesh.p.shape[1]
+        active = np.zeros((len(subbases), npoints), dtype=bool)
+        for k, b in enumerate(subbases):
+            active[k, b.active_nodes] = True
+        per_field = active * np.asarray(components)[:, None]          # DOFs of field k at each node
+        node_start = np.concatenate([[0], np.cumsum(per_field.sum(axis=0))])[:-1]
+        within = np.cumsum(per_field, axis=0) - per_field              # DOFs of earlier fields at the node
+        N = int(per_field.sum())
+        for k, (b, C) in enumerate(zip(subbases, components)):
+            first = node_start[b.active_nodes] + within[k, b.active_nodes]
+            nodal = first[None, :] + np.arange(C)[:, None]             # (C, active nodes)
+            b._field_N = b.N
+            b._composite_dofs = np.ascontiguousarray(nodal.T).ravel()  # field-local DOF -> composite DOF
+            b._composite_parent = self
+            b.nodal_dofs = nodal
+            b.N = N
+            b.element_dofs = nodal[:, b._node_local].transpose(2, 1, 0).reshape(
+                mesh.nelements, b._node_local.shape[0] * C).T
+        common = np.flatnonzero(active.all(axis=0))
+        self.mesh, self.elem = mesh, element
+        self.subbases = tuple(subbases)
+        self.field_components = components
+        self.N = N
+        self.nodal_dofs = np.concatenate(
+            [b.nodal_dofs[:, np.searchsorted(b.active_nodes, common)] for b in subbases], axis=0)
+        self.element_dofs = np.concatenate([b.element_dofs for b in subbases], axis=0)
+        doflocs = np.empty((mesh.p.shape[0], N))
+        for b in subbases:
+            for c in range(b.nodal_dofs.shape[0]):
+                doflocs[:, b.nodal_dofs[c]] = mesh.p[:, b.active_nodes]
+        self.doflocs = doflocs
+        first = subbases[0]
+        self.X, self.W = first.X, first.W
+        self.quadrature = (self.X.copy(), self.W.copy())
+        self.normals = None
+        self.basis = tuple()
+        self.tind = first.tind
+        self.geometry_diagnostics = first.geometry_diagnostics
+        self._composite_first = first
 
     # ---------------------------------------------------------- device data
     def device_data(self):
@@ -381,12 +441,16 @@ class Basis:
 
     @property
     def dx(self):
+        if "_composite_first" in self.__dict__:
+            return self._composite_first.dx
         if "dx" not in self._lazy:
             self._coordinates_only()
         return self._lazy["dx"]
 
     @property
     def global_coordinates(self):
+        if "_composite_first" in self.__dict__:
+            return self._composite_first.global_coordinates
         return self._coordinates_only()
 
     @property

@@ -7,7 +7,10 @@ import numpy as np
 
 class _ComposableElement:
     def __mul__(self, other):
-        raise NotImplementedError("composite (mixed) elements are outside the MI355X assembly build")
+        """``e1 * e2`` builds a composite element (basis.py:926-930)."""
+        if isinstance(other, ElementComposite):
+            return ElementComposite(self, *other.elems)
+        return ElementComposite(self, other)
 
 
 class ElementTetP1(_ComposableElement):
@@ -29,4 +32,19 @@ class ElementVector(_ComposableElement):
         self._dim = element.doflocs.shape[1] if dim is None else dim
 
 
-__all__ = ["ElementTetP1", "ElementTetP2", "ElementVector"]
+class ElementComposite(_ComposableElement):
+    """Ordered collection of nodal H1 subfields (basis.py:1064-1076)."""
+
+    def __init__(self, *elements):
+        if len(elements) < 2:
+            raise ValueError("ElementComposite requires at least two fields")
+        flattened = []
+        for element in elements:
+            if isinstance(element, ElementComposite):
+                flattened.extend(element.elems)
+            else:
+                flattened.append(element)
+        self.elems = tuple(flattened)
+
+
+__all__ = ["ElementTetP1", "ElementTetP2", "ElementVector", "ElementComposite"]

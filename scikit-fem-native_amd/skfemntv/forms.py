@@ -14,11 +14,14 @@ from weakref import WeakKeyDictionary
 
 import numpy as np
 
-from ._assembly import NativeBilinearForm, NativeCrossBilinearForm, NativeLinearForm
-from ._symbols import (BilinearSum, BilinearTerm, Coefficient, CoefficientComponent, Divergence, LinearSum,
-                       LinearTerm, QuadratureValue, SymmetricGradient, TensorGradient, TestGradient, TestValue,
-                       TrialGradient, TrialValue, UnsupportedNativeForm, evaluate_coefficient, extract_terms,
-                       is_symbolic, trace_expression)
+from ._assembly import (NativeBilinearForm, NativeCompositeBilinearForm, NativeCompositeLinearForm,
+                        NativeCrossBilinearForm, NativeLinearForm)
+from ._symbols import (BilinearSum, BilinearTerm, Coefficient, CoefficientComponent, CompositeBilinearSum,
+                       CompositeBilinearTerm, CompositeDivergence, CompositeField, CompositeLinearSum,
+                       CompositeLinearTerm, CompositeWeightedField, Divergence, LinearSum, LinearTerm,
+                       QuadratureValue, SymmetricGradient, TensorGradient, TestGradient, TestValue, TrialGradient,
+                       TrialValue, UnsupportedNativeForm, evaluate_coefficient, extract_terms, is_symbolic,
+                       trace_expression)
 
 
 class _LinearForm:
@@ -192,6 +195,62 @@ def _native_cross_bilinear_assemble(form, trial_basis, test_basis, kwargs, *, me
     return result
 
 
+def _native_composite_bilinear_assemble(form, basis, kwargs):
+    """forms.py:489-539: the form takes the trial subfields, then the test
+    subfields; every term is one (test field, trial field) block."""
+    trial = tuple(CompositeField("trial", k) for k in range(len(basis.subbases)))
+    test = tuple(CompositeField("test", k) for k in range(len(basis.subbases)))
+    expression = trace_expression(form.function, trial + test, _geometry(basis), kwargs,
+                                  context="composite BilinearForm")
+    terms = extract_terms(expression, CompositeBilinearTerm, CompositeBilinearSum,
+                          message="composite BilinearForm must reduce to subfield value or gradient contractions")
+    native = form._native_cache.get(basis)
+    if native is None:
+        native = NativeCompositeBilinearForm(basis)
+        form._native_cache[basis] = native
+    result = None
+    for term in terms:
+        coefficient = evaluate_coefficient(term.coefficient, kwargs, factor=term.factor, squeeze=True)
+        block = native.assemble(term.row_field, term.column_field, kind=term.kind, coefficient=coefficient)
+        result = block if result is None else result + block
+    return result
+
+
+def _native_composite_linear_assemble(form, basis, kwargs):
+    """forms.py:206-273: value / gradient terms per test subfield."""
+    test = tuple(CompositeField("test", k) for k in range(len(basis.subbases)))
+    expression = trace_expression(form.function, test, _geometry(basis), kwargs, context="composite LinearForm")
+    if isinstance(expression, CompositeWeightedField):
+        expression = expression._linear_term()
+    terms = extract_terms(expression, CompositeLinearTerm, CompositeLinearSum,
+                          message="composite LinearForm must reduce to subfield value or gradient contractions")
+    native = form._native_cache.get(basis)
+    if native is None:
+        native = NativeCompositeLinearForm(basis)
+        form._native_cache[basis] = native
+    grouped = {}
+    for term in terms:
+        coefficient = evaluate_coefficient(term.coefficient, kwargs, factor=term.factor)
+        field_native = native.assembler(term.field)
+        if term.kind == "value":
+            if field_native._shape[-1] == 1 and np.shape(coefficient) == field_native._shape[:2]:
+                coefficient = np.asarray(coefficient)[..., None]
+            elif np.ndim(coefficient) and np.shape(coefficient)[0] == field_native._shape[-1]:
+                coefficient = np.moveaxis(coefficient, 0, -1)
+        elif term.kind == "gradient":
+            if np.ndim(coefficient) >= 2 and np.shape(coefficient)[:2] == field_native._gradient_shape[-2:]:
+                coefficient = np.moveaxis(coefficient, (0, 1), (-2, -1))
+        else:
+            raise UnsupportedNativeForm(f"unsupported composite linear kind {term.kind!r}")
+        values = grouped.setdefault(term.field, {"value": None, "gradient": None})
+        previous = values[term.kind]
+        values[term.kind] = coefficient if previous is None else previous + coefficient
+    result = np.zeros(basis.N, dtype=np.float64)
+    for field, values in grouped.items():
+        result += native.assemble(field, **values)
+    return result
+
+
 def asm(form, *bases, num_threads=None, memory_limit_bytes=None, memory_safety_factor=1.25, **kwargs):
     """Assemble strictly with the native (MI355X) backend.
 
@@ -208,6 +267,11 @@ def asm(form, *bases, num_threads=None, memory_limit_bytes=None, memory_safety_f
         if kwargs.pop("integration", None) is not None:
             raise UnsupportedNativeForm("interface assembly is outside the MI355X build")
         if len(bases) == 1:
+            if hasattr(bases[0], "subbases"):
+                if num_threads is not None:
+                    raise UnsupportedNativeForm("per-call threads are not yet supported for composite LinearForm")
+                return _native_composite_linear_assemble(form, bases[0], kwargs)
+            _reject_composite_member(bases[0])
             return _native_linear_assemble(form, bases[0], kwargs, requested)
         raise UnsupportedNativeForm("native LinearForm requires one basis, or two bases with an interface "
                                     "integration")
@@ -219,14 +283,28 @@ def asm(form, *bases, num_threads=None, memory_limit_bytes=None, memory_safety_f
                 raise UnsupportedNativeForm("interior facet assembly is outside the MI355X build")
             if num_threads is not None:
                 raise UnsupportedNativeForm("per-call threads are not yet supported for cross-basis BilinearForm")
+            for b in bases:
+                _reject_composite_member(b)
             return _native_cross_bilinear_assemble(form, bases[0], bases[1], kwargs,
                                                    memory_limit_bytes=memory_limit_bytes,
                                                    memory_safety_factor=memory_safety_factor)
         if len(bases) != 1:
             raise UnsupportedNativeForm("native BilinearForm currently requires one shared basis")
+        if hasattr(bases[0], "subbases"):
+            if num_threads is not None:
+                raise UnsupportedNativeForm("per-call threads are not yet supported for composite BilinearForm")
+            return _native_composite_bilinear_assemble(form, bases[0], kwargs)
+        _reject_composite_member(bases[0])
         return _native_bilinear_assemble(form, bases[0], kwargs, requested, memory_limit_bytes=memory_limit_bytes,
                                          memory_safety_factor=memory_safety_factor)
     raise TypeError("skfemntv.asm accepts forms created by skfemntv.LinearForm or skfemntv.BilinearForm")
+
+
+def _reject_composite_member(basis):
+    # a sub-basis of a composite basis carries composite DOF numbers; its
+    # blocks are assembled through the composite basis
+    if getattr(basis, "_composite_parent", None) is not None:
+        raise UnsupportedNativeForm("assemble composite subfields through their composite basis")
 
 
 # ------------------------------------------------------------------ operators
@@ -244,6 +322,18 @@ def dot(left, right):
         if right.gradient.__class__ is left.__class__:
             raise UnsupportedNativeForm("anisotropic contraction requires trial and test gradients")
         return BilinearTerm("gradient_tensor", right.coefficient, right.gradient.factor * left.factor)
+    if isinstance(left, CompositeField) and isinstance(right, CompositeField):
+        if left.kind == "gradient" and right.kind == "gradient":
+            return _composite_block(left, right, "gradient")
+        if left.kind != "value" or right.kind != "value":
+            raise UnsupportedNativeForm("composite dot requires two values or two gradients")
+        return _composite_block(left, right, "value")
+    for a, b in ((left, right), (right, left)):
+        if isinstance(b, CompositeField) and b.role == "test" and b.kind == "value":
+            if is_symbolic(a):
+                return CompositeLinearTerm(b.field, "value", a)
+            if _is_value_like(a):
+                return CompositeLinearTerm(b.field, "value", np.asarray(a))
     if is_symbolic(left) and isinstance(right, TestValue):
         return LinearTerm("value", left)
     if is_symbolic(right) and isinstance(left, TestValue):
@@ -277,6 +367,16 @@ def mul(left, right):
 def ddot(left, right):
     if isinstance(left, SymmetricGradient) and isinstance(right, SymmetricGradient) and left.role != right.role:
         return BilinearTerm("symmetric_gradient", factor=left.factor * right.factor)
+    if isinstance(left, CompositeField) and isinstance(right, CompositeField):
+        if left.kind != "gradient" or right.kind != "gradient":
+            raise UnsupportedNativeForm("composite ddot requires two gradients")
+        return _composite_block(left, right, "gradient")
+    for a, b in ((left, right), (right, left)):
+        if isinstance(b, CompositeField) and b.role == "test" and b.kind == "gradient":
+            if is_symbolic(a):
+                return CompositeLinearTerm(b.field, "gradient", a)
+            if isinstance(a, (np.ndarray, QuadratureValue)):
+                return CompositeLinearTerm(b.field, "gradient", np.asarray(a))
     if is_symbolic(left) and isinstance(right, TestGradient):
         return LinearTerm("gradient", left)
     if is_symbolic(right) and isinstance(left, TestGradient):
@@ -291,7 +391,16 @@ def ddot(left, right):
     return np.einsum("ij...,ij...->...", left, right)
 
 
+def _composite_block(left, right, kind):
+    if left.role == right.role:
+        raise UnsupportedNativeForm("composite contraction requires trial and test subfields")
+    test, trial = (left, right) if left.role == "test" else (right, left)
+    return CompositeBilinearTerm(test.field, trial.field, kind)
+
+
 def grad(value):
+    if isinstance(value, CompositeField):
+        return CompositeField(value.role, value.field, "gradient")
     if isinstance(value, TestValue):
         return TestGradient(value.factor)
     if isinstance(value, TrialValue):
@@ -303,6 +412,10 @@ def grad(value):
 
 
 def div(value):
+    if isinstance(value, CompositeField):
+        if value.kind != "value":
+            raise UnsupportedNativeForm("div() expects a composite value")
+        return CompositeDivergence(value)
     if isinstance(value, TrialValue):
         return Divergence("trial", value.factor)
     if isinstance(value, TestValue):

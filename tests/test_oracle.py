@@ -489,3 +489,23 @@ def test_cross_oracle_closed_forms():
     x1 = _linear_interpolant(p2, t2, False, 1, lambda x: x[:1])
     x2 = _linear_interpolant(p2, t2, True, 1, lambda x: x[:1])
     assert abs(x1 @ (K @ x2) - 1.) < 1e-13
+
+
+def test_composite_numbering_restatement():
+    """The composite numbering (a literal restatement of the reference's node
+    loop) equals its characterisation: DOFs ordered by (node, field,
+    component), each field's components consecutive at a node."""
+    p2, t2 = _small_p2(seed=2, n=2)
+    conns = [t2[:10], t2[:4], t2[:4]]
+    comps = [3, 1, 2]
+    N, dofs = host.composite_dofs(p2.shape[1], conns, comps)
+    keys = []
+    for f, (c, C) in enumerate(zip(conns, comps)):
+        for node in np.unique(c):
+            for k in range(C):
+                keys.append((int(node), f, k))
+    assert N == len(keys)
+    order = {key: i for i, key in enumerate(sorted(keys))}
+    for f, (c, C) in enumerate(zip(conns, comps)):
+        expect = np.array([[[order[(int(v), f, k)] for k in range(C)] for v in col] for col in c.T])
+        np.testing.assert_array_equal(dofs[f], expect)
