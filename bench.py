@@ -210,9 +210,10 @@ def other_workload(name, steps, warmup, L, ev0, ev1, stream):
     """The other assembly configurations at full size (one JSON line each):
     c1 P1 Laplace + load 32^3, c3 vector-P2 elasticity 48^3, c4 N1 curl-curl +
     kappa mass 96^3, th the Taylor-Hood divergence block (vector P2 x P1) on
-    the 64^3 jittered cube.  Algorithmic bytes per element: 106.7, 3255, 356,
-    and for th 24 V/E + 56 + 480 + 8 nnz/E (vertices, both connectivities,
-    int32 element->CSR map, values)."""
+    the 64^3 jittered cube, stokes the Taylor-Hood Stokes operator on a
+    composite basis at 48^3.  Algorithmic bytes per element: 106.7, 3255, 356,
+    for th 24 V/E + 56 + 480 + 8 nnz/E (vertices, both connectivities, int32
+    element->CSR map, values), for stokes the sum of its three blocks."""
     import torch
     import skfemntv as sk
     from skfemntv._assembly import NativeBilinearForm, NativeLinearForm, upload
@@ -289,6 +290,40 @@ def other_workload(name, steps, warmup, L, ev0, ev1, stream):
         # nodes) + int32 element->CSR map (4 x 30 entries) + FP64 values
         bpe = (24. * V + 4. * 14 * E + 4. * 4 * 30 * E + 8. * nnz) / E
         desc = f"Taylor-Hood divergence block div(u) q, vector P2 x P1, 64^3 Kuhn cube jittered 0.2h, {E} tets, FP64"
+    elif name == "stokes":
+        # Taylor-Hood Stokes operator on a composite basis (vector P2 x P1),
+        # 48^3 Kuhn cube jittered 0.15h: nu grad u : grad v on the velocity
+        # block, -div(v) p and -div(u) q on the coupling blocks
+        n = 48
+        ax = np.linspace(0., 1., n + 1)
+        m = sk.MeshTet.init_tensor(ax, ax, ax)
+        p = m.p.copy()
+        interior = np.all((p > 1e-12) & (p < 1 - 1e-12), axis=0)
+        rng = np.random.default_rng(0)
+        p[:, interior] += rng.uniform(-0.15 / n, 0.15 / n, size=(3, int(interior.sum())))
+        mesh = sk.MeshTet2.from_mesh(sk.MeshTet(p, m.t))
+        X, W = sk.basis.tet_rule_4()
+        basis = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP2(), dim=3) * sk.ElementTetP1(), quadrature=(X, W))
+        E = mesh.nelements
+        comp = sk.NativeCompositeBilinearForm(basis)
+        velocity = comp._square(0)
+        bt, b = comp._cross(0, 1), comp._cross(1, 0)
+        nu = upload(np.random.default_rng(1).uniform(0.5, 2., E))
+        one = upload(np.ones(1))
+
+        def step(precision="fp64"):
+            velocity.assemble_device(gradient=nu)
+            bt.assemble_cross_device("gradient", "value", "row_divergence", one, (0, 0))
+            b.assemble_cross_device("value", "gradient", "column_divergence", one, (0, 0))
+        nnz_a, nnz_b = velocity.plan.nnz, b.cross_plan().nnz
+        nnz = nnz_a + 2 * nnz_b
+        V = int(np.unique(mesh.t[:4]).size)
+        # per block: vertices + int32 connectivity + coefficient + int32
+        # element->CSR map + FP64 values (velocity 30 x 30, coupling 30 x 4)
+        bpe = (3 * 24. * V + (40. + 8. + 4. * 900) * E + 8. * nnz_a
+               + 2 * ((56. + 4. * 120) * E + 8. * nnz_b)) / E
+        desc = (f"Taylor-Hood Stokes (vector P2 x P1 composite): nu grad:grad + both divergence blocks, 48^3 "
+                f"Kuhn cube jittered 0.15h, {E} tets, FP64")
     else:
         raise SystemExit(f"unknown workload {name}")
     torch.cuda.synchronize()
@@ -327,7 +362,7 @@ def main():
     ap.add_argument("--cells", type=int, nargs=2, default=(16, 48), help="batched workload: cells per axis range")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (counter calibration)")
-    ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c3", "c4", "th"),
+    ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c3", "c4", "th", "stokes"),
                     help="c2 (default): the headline line; c1/c3/c4: the other assembly configurations")
     args = ap.parse_args()
     if args.gpus != 1:
