@@ -190,6 +190,15 @@ int skfa_assemble_bilinear(skfa_plan* plan, const double* xyz, int64_t npoints, 
                            const double* c_divergence, const double* c_tensor, int32_t per_qp, int32_t precision,
                            int32_t flags, double* values, void* stream);
 #define SKFA_FLAG_SYMMETRIC_TENSOR 1
+/* SKFA_FLAG_VECTOR3_DIAGONAL: `plan` is the scalar (C = 1) plan of a
+ * 3-component field's node connectivity and the form has only value /
+ * gradient coefficients (element-constant, affine elements).  Its element
+ * matrix is [c == d] times the scalar one (bilinear_form_assembler.cpp:
+ * 83-89 with no symmetric-gradient or divergence term), so the scalar gather
+ * runs and writes each entry as the 3x3 diagonal node block of the
+ * 3-component DOF-level CSR (the C = 3 plan's pattern).  `values` then holds
+ * 9 x the scalar plan's nnz. */
+#define SKFA_FLAG_VECTOR3_DIAGONAL 2
 
 /* Linear form f.v + G:grad(v) into out (nrows FP64, overwritten).
  * Replaces LinearFormAssembler::assemble

@@ -122,6 +122,7 @@ struct PlanArgs {
     int32_t xcd_chunk;
     int32_t lds_atomic;
     int32_t streaming;       // non-temporal record loads and value stores
+    int32_t diag3 = 0;       // scalar plan written as the 3x3 diagonal blocks of a 3-component field
 };
 
 // Lane role inside a wave: node row r0+i, components (c,d) of the CxC node
@@ -208,6 +209,32 @@ __device__ inline void write_span(const double* lds, int64_t span, int lane, dou
         for (int64_t k = lane; k < span; k += 64) __builtin_nontemporal_store(lds[k], out + k);
     else
         for (int64_t k = lane; k < span; k += 64) out[k] = lds[k];
+}
+
+// Write a scalar span as the 3x3 node blocks K_(a,c),(b,d) = [c == d] s_ab
+// of a 3-component field (value / gradient kinds of vector fields): node row
+// r with len columns becomes DOF rows 3r+c, whose entries (column node j,
+// component d) sit at 9 nptr[r] + 3 c len + 3 j + d.
+__device__ inline void write_span_diag3(const double* lds, const WaveMeta& m, const int64_t* __restrict__ nptr,
+                                        int lane, double* values, bool streaming) {
+    for (int i = 0; i < m.R; ++i) {
+        const int64_t b = nptr[m.r0 + i];
+        const int len = int(nptr[m.r0 + i + 1] - b);
+        const double* sr = lds + (b - m.lo);
+        double* out = values + 9 * b;
+        const int n = 9 * len, len3 = 3 * len;
+        for (int k = lane; k < n; k += 64) {
+            const int c = k / len3;
+            const int rem = k - c * len3;
+            const int j = rem / 3;
+            const int d = rem - 3 * j;
+            const double v = c == d ? sr[j] : 0.;
+            if (streaming)
+                __builtin_nontemporal_store(v, out + k);
+            else
+                out[k] = v;
+        }
+    }
 }
 
 template <int RB>
@@ -727,7 +754,10 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
         }
     }
     __syncthreads();
-    write_span(span_lds, L.span, lane, A.values + L.lo * C * C, A.plan.streaming);
+    if (C == 1 && A.plan.diag3)
+        write_span_diag3(span_lds, A.plan.meta[wv], A.plan.indptr, lane, A.values, A.plan.streaming);
+    else
+        write_span(span_lds, L.span, lane, A.values + L.lo * C * C, A.plan.streaming);
 }
 
 // ------------------------------------------------------------------ vector P2 gather
@@ -1174,7 +1204,10 @@ __global__ void __launch_bounds__(64 * WPB) k_gather_table(GatherArgs A, int32_t
         }
     }
     // the span is this wave's own: its LDS operations complete in order
-    write_span(span_lds, L.span, lane, A.values + L.lo, A.plan.streaming);
+    if (A.plan.diag3)
+        write_span_diag3(span_lds, A.plan.meta[wv], A.plan.indptr, lane, A.values, A.plan.streaming);
+    else
+        write_span(span_lds, L.span, lane, A.values + L.lo, A.plan.streaming);
 }
 
 // N1 rows are global edges; incidence (e, local edge a).
@@ -1758,7 +1791,7 @@ static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, size_t lds
     // MFMA row group, the long-row waves where column-split lanes would
     // recompute whole element rows), the rest by the column-split gather
     const int mode = P->tune_mfma < 0 ? 0 : P->tune_mfma;
-    if (mfma_gather(P, A.Qg, A.Qm)) {
+    if (!A.plan.diag3 && mfma_gather(P, A.Qg, A.Qm)) {
         GatherArgs M = A;
         if (mode == 2) M.rmax = 16;
         if (int rc = sym ? launch_mfma<NL, PL_SYM, F32>(P, M, s) : launch_mfma<NL, PL_FULL, F32>(P, M, s)) return rc;
@@ -1848,9 +1881,13 @@ int skfa_assemble_bilinear(skfa_plan* P, const double* xyz, int64_t npoints, con
     SKFA_REQUIRE(fast || (tables != nullptr && nq >= 1 && nq <= 4096), "tables / quadrature size invalid");
     SKFA_REQUIRE(!iso || (gnodes >= 4 && gnodes <= 64), "geometry node count out of range");
     (void)npoints;
+    const bool diag3 = (flags & SKFA_FLAG_VECTOR3_DIAGONAL) != 0;
+    SKFA_REQUIRE(!diag3 || (P->C == 1 && fast && !c_symgrad && !c_divergence && !c_tensor),
+                 "diagonal 3-component output needs a scalar plan, affine elements, element-constant value / "
+                 "gradient coefficients only");
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (P->nnz() == 0 || P->E == 0) {
-        if (P->nnz()) SKFA_HIP(hipMemsetAsync(values, 0, sizeof(double) * P->nnz(), s));
+        if (P->nnz()) SKFA_HIP(hipMemsetAsync(values, 0, sizeof(double) * P->nnz() * (diag3 ? 9 : 1), s));
         return kOk;
     }
     GatherArgs G{};
@@ -1882,6 +1919,7 @@ int skfa_assemble_bilinear(skfa_plan* P, const double* xyz, int64_t npoints, con
             k_payload_h1<3, false><<<elem_grid(P->E), 256, 0, s>>>(pa);
         SKFA_LAUNCH_CHECK();
         G.plan = plan_args(P);
+        G.plan.diag3 = diag3 ? 1 : 0;
         G.payload = P->payload.p;
         G.tab = rtab;
         G.tab32 = rtab32;

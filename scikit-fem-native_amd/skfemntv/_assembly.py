@@ -158,12 +158,30 @@ def to_per_qp(dev_tensor, Q):
     return dev_tensor.unsqueeze(1).expand((shape[0], Q) + tuple(shape[1:])).contiguous()
 
 
+def basis_scalar_plan(basis):
+    """Scalar (C = 1) plan over a vector basis's node connectivity, for
+    value / gradient forms written as 3x3 diagonal node blocks."""
+    plan = getattr(basis, "_plan_scalar", None)
+    if plan is None:
+        d = basis.device_data()
+        plan = Plan(d["node_conn"], basis.mesh.nelements, d["node_conn"].shape[1], d["nnodes"], 1)
+        basis._plan_scalar = plan
+    return plan
+
+
 def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=None, symmetric_gradient=None,
                              divergence=None, tensor=None, per_qp=False, precision="fp64", symmetric_tensor=False):
     """Launch skfa_assemble_bilinear with device coefficient tensors."""
     d = basis.device_data()
     L = _native.lib()
     iso = not basis._affine
+    flags = _native.FLAG_SYMMETRIC_TENSOR if symmetric_tensor else 0
+    if (plan.components == 3 and basis.elem._dim == 3 and not iso and not per_qp and symmetric_gradient is None
+            and divergence is None and tensor is None):
+        # value / gradient kinds of a 3-component field: K = [c == d] K_scalar,
+        # assembled by the scalar gather and written as diagonal node blocks
+        plan = basis_scalar_plan(basis)
+        flags |= _native.FLAG_VECTOR3_DIAGONAL
     tables = d["tables_iso"] if iso else d["tables"]
     prec = {"fp64": 0, "fp32": 1}[precision]
     if prec == 1 and (iso or per_qp):
@@ -174,8 +192,7 @@ def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=No
         _native.ptr(d["gconn"]) if iso else None, d["gnodes"] if iso else 4,
         _native.ptr(tables), len(basis.W), _native.ptr(d["rtab"]), _native.ptr(d["rtab32"]), d["nqg"], d["nqm"],
         _native.ptr(value), _native.ptr(gradient), _native.ptr(symmetric_gradient), _native.ptr(divergence),
-        _native.ptr(tensor), int(per_qp), prec, _native.FLAG_SYMMETRIC_TENSOR if symmetric_tensor else 0,
-        _native.ptr(values_dev), _native.stream()))
+        _native.ptr(tensor), int(per_qp), prec, flags, _native.ptr(values_dev), _native.stream()))
 
 
 def _prepare_scalar_kinds(basis, coefficients):

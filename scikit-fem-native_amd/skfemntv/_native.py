@@ -35,6 +35,7 @@ FSAI_FRAG = 34816
 
 
 FLAG_SYMMETRIC_TENSOR = 1
+FLAG_VECTOR3_DIAGONAL = 2
 TUNE_XCD_CHUNK = 1
 TUNE_LDS_ATOMIC = 2
 TUNE_PLACEMENT = 3
