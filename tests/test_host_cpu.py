@@ -43,6 +43,12 @@ def test_header_constants_match_host():
     assert int(consts["SKFA_FSAI_FEATURES"]) == _native.FSAI_FEATURES == of.NF
     assert int(consts["SKFA_FSAI_FRAG"]) == _native.FSAI_FRAG == 8 * 64 * 4 + 2 * 8 * 4 * 64 * 8
     assert int(consts["SKFA_FLAG_SYMMETRIC_TENSOR"]) == _native.FLAG_SYMMETRIC_TENSOR
+    assert int(consts["SKFA_FLAG_VECTOR3_DIAGONAL"]) == _native.FLAG_VECTOR3_DIAGONAL
+    from skfemntv._assembly import _CROSS_KIND, _CROSS_MODE
+    assert int(consts["SKFA_CROSS_VALUE"]) == _CROSS_KIND["value"]
+    assert int(consts["SKFA_CROSS_GRADIENT"]) == _CROSS_KIND["gradient"]
+    for name in ("scalar", "row_divergence", "column_divergence", "tensor"):
+        assert int(consts[f"SKFA_CROSS_{name.upper()}"]) == _CROSS_MODE[name]
     from skfemntv._assembly import _TUNE_KEYS
     for key in ("XCD_CHUNK", "LDS_ATOMIC", "PLACEMENT", "STREAMING", "ELEMENT_TABLE", "ROW_SPLIT", "MFMA"):
         assert int(consts[f"SKFA_TUNE_{key}"]) == getattr(_native, f"TUNE_{key}") == _TUNE_KEYS[key.lower()]
@@ -132,3 +138,37 @@ def test_batch_generator_is_seeded_and_bounded():
     assert interior.sum() == 2 * 3 * 1
     w = np.linalg.eigvalsh(A)
     assert np.all(w > 0) and w.min() >= 1e-2 * (1 - 1e-9) and w.max() <= 1e2 * (1 + 1e-9)
+
+
+def test_composite_relabelling_keeps_csr_order():
+    """A field-local block relabelled by increasing row/column maps equals
+    the dense block scattered into the composite numbering."""
+    from scipy.sparse import random as sprandom
+    from skfemntv._assembly import _composite_csr
+    rng = np.random.default_rng(0)
+    local = sprandom(7, 5, density=0.5, random_state=1, format="csr")
+    local.sort_indices()
+    N = 20
+    row_map = np.sort(rng.choice(N, 7, replace=False))
+    col_map = np.sort(rng.choice(N, 5, replace=False))
+    K = _composite_csr(local.indptr, local.indices, local.data, row_map, col_map, N)
+    dense = np.zeros((N, N))
+    dense[np.ix_(row_map, col_map)] = local.toarray()
+    np.testing.assert_array_equal(K.toarray(), dense)
+    assert K.has_sorted_indices and K.shape == (N, N)
+
+
+def test_composite_symbols_trace_stokes_blocks():
+    """u, p, v, q trace to (test, trial) blocks with the reference's kinds."""
+    from skfemntv._symbols import CompositeBilinearSum, CompositeField, CompositeLinearSum
+    from skfemntv.forms import ddot, div, dot, grad
+    u, p, v, q = (CompositeField("trial", 0), CompositeField("trial", 1), CompositeField("test", 0),
+                  CompositeField("test", 1))
+    form = ddot(grad(u), grad(v)) - div(v) * p - div(u) * q + 2. * (p * q)
+    assert isinstance(form, CompositeBilinearSum)
+    got = [(t.row_field, t.column_field, t.kind, t.factor) for t in form.terms]
+    assert got == [(0, 0, "gradient", 1.), (0, 1, "row_divergence", -1.), (1, 0, "column_divergence", -1.),
+                   (1, 1, "value", 2.)]
+    lin = dot(np.array([1., 0., 0.]), v) + q * np.array([3.])
+    assert isinstance(lin, CompositeLinearSum)
+    assert [(t.field, t.kind) for t in lin.terms] == [(0, "value"), (1, "value")]
