@@ -210,6 +210,22 @@ int skfa_assemble_linear(skfa_plan* plan, const double* xyz, const int32_t* vcon
                          int32_t gnodes, const double* tables, int32_t nq, const double* f, int32_t f_per_qp,
                          const double* G, int32_t G_per_qp, double* out, void* stream);
 
+/* Facet (boundary) load vector into out (nnodes * components FP64,
+ * overwritten).  Replaces LinearFormAssembler::assemble on the facet DOFs of
+ * a FacetBasis (cpp/src/assembly/linear_form_assembler.cpp:41-137 driven by
+ * python/skfemntv/linear_form.py:17-107 over basis.py:2442-2685).
+ *   plan        skfa_plan_create on the (F, n) node connectivity of the
+ *               facets' parent elements, one component (its incidence lists
+ *               are used)
+ *   face        (F) int32 local face (0..3) of each facet in its parent
+ *   face_shape  (4, Q, n) FP64 parent shape functions at the facet points of
+ *               each local face
+ *   weights     (F, Q, components) FP64 dx * f
+ * out[node * components + c] sums its (facet, local node) incidences in facet
+ * order. */
+int skfa_assemble_facet_linear(const skfa_plan* plan, const int32_t* face, const double* face_shape, int32_t nq,
+                               const double* weights, double* out, int32_t components, void* stream);
+
 /* Per (e,q) geometry: replaces tabulate_basis_geometry
  * (cpp/src/assembly/basis_geometry.cpp:49-176).  Outputs (E,Q[,...]) FP64,
  * each optional (NULL = not written): det, tol (64 eps max|J|^3), cond,

@@ -216,3 +216,77 @@ def composite_dofs(npoints, conns, components):
         first = np.vectorize(lambda v: field_node[f][int(v)])(c.T)  # (E, n_f)
         out.append(first[:, :, None] + np.arange(C)[None, None, :])
     return nxt, out
+
+
+TET_FACES = ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3))
+
+
+def facet_topology(t):
+    """Facets, t2f, f2t restated as the reference builds them
+    (basis.py:96-148): face-major indexing, unique sorted triples, f2t filled
+    by a loop over (local face, element)."""
+    t4 = np.asarray(t[:4], dtype=np.int64)
+    E = t4.shape[1]
+    indexing = np.hstack(tuple(t4[np.asarray(face)] for face in TET_FACES))
+    canonical, first, inverse = np.unique(np.sort(indexing, axis=0), axis=1, return_index=True,
+                                          return_inverse=True)
+    t2f = np.asarray(inverse).reshape(4, E)
+    f2t = np.full((2, canonical.shape[1]), -1, dtype=np.int64)
+    for facet, element in zip(t2f.ravel(order="C"), np.tile(np.arange(E), 4)):
+        row = 0 if f2t[0, facet] == -1 else 1
+        f2t[row, facet] = element
+    return canonical, t2f, f2t
+
+
+def facet_basis(p, t, quadratic, components=1, intorder=2):
+    """Tetrahedral FacetBasis on the boundary facets, restated per facet and
+    point as the reference computes it (basis.py:2575-2685).  Returns a dict
+    with facets, parents, local faces, shape (F,Q,n), dx (F,Q), x (F,Q,3),
+    normals (F,Q,3), dofs (F,n,C) and N."""
+    facets, t2f, f2t = facet_topology(t)
+    boundary = np.flatnonzero(f2t[1] == -1)
+    if intorder >= 4:
+        a, b = .445948490915965, .091576213509771
+        bary = np.array([[a, a, 1 - 2 * a], [a, 1 - 2 * a, a], [1 - 2 * a, a, a],
+                         [b, b, 1 - 2 * b], [b, 1 - 2 * b, b], [1 - 2 * b, b, b]])
+        wts = np.array([.1116907948390055] * 3 + [.054975871827661] * 3)
+    else:
+        bary = np.array([[2 / 3, 1 / 6, 1 / 6], [1 / 6, 2 / 3, 1 / 6], [1 / 6, 1 / 6, 2 / 3]])
+        wts = np.full(3, 1 / 6)
+    pts = bary[:, 1:]
+    V = np.array([[0., 0., 0.], [1., 0., 0.], [0., 1., 0.], [0., 0., 1.]])
+    nodes = 10 if quadratic else 4
+    N, edofs = element_dofs(t[:nodes], p.shape[1], components)
+    E = t.shape[1]
+    dofs_e = edofs.T.reshape(E, nodes, components)
+    F, Q = boundary.size, len(wts)
+    out = {"shape": np.zeros((F, Q, nodes)), "dx": np.zeros((F, Q)), "x": np.zeros((F, Q, 3)),
+           "normals": np.zeros((F, Q, 3)), "parents": np.zeros(F, dtype=np.int64), "local": np.zeros(F, dtype=int)}
+    for f, facet in enumerate(boundary):
+        e = int(f2t[0, facet])
+        li = int(np.flatnonzero(t2f[:, e] == facet)[0])
+        corner = TET_FACES[li]
+        rc = V[list(corner)]
+        x = p[:, t[:, e]]
+        for q, (r, s) in enumerate(pts):
+            ref = (1 - r - s) * rc[0] + r * rc[1] + s * rc[2]
+            der = np.stack((rc[1] - rc[0], rc[2] - rc[0]), axis=1)
+            sh, _ = tabulate_p2(ref[:, None]) if quadratic else tabulate_p1(ref[:, None])
+            gsh, ggr = tabulate_p2(ref[:, None]) if t.shape[0] == 10 else tabulate_p1(ref[:, None])
+            jac = x @ ggr[0]
+            tang = jac @ der
+            point = gsh[0] @ x.T
+            nrm = np.cross(tang[:, 0], tang[:, 1])
+            nrm /= np.linalg.norm(nrm)
+            if np.dot(nrm, point - x.mean(axis=1)) < 0.:
+                nrm = -nrm
+            out["shape"][f, q] = sh[0]
+            out["x"][f, q] = point
+            out["normals"][f, q] = nrm
+            out["dx"][f, q] = np.linalg.norm(np.cross(tang[:, 0], tang[:, 1])) * wts[q]
+        out["parents"][f] = e
+        out["local"][f] = li
+    out["dofs"] = dofs_e[out["parents"]]
+    out["N"] = N
+    out["facets"] = boundary
+    return out

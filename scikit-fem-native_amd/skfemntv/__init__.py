@@ -17,6 +17,7 @@ from ._assembly import (AssemblyMemoryBudgetError, AssemblyMemoryEstimate, Linea
 from .hcurl import (AffineTetN1Basis, HcurlTetGeometryDiagnostics, TetN1Assembler,
                     estimate_tet_n1_assembly_memory)
 from .mesh import MeshTet, MeshTet2
+from .facet import FacetBasis, NativeFacetLinearForm
 from .runtime import (BACKEND, CAPABILITIES, available_num_threads, get_num_threads, has_capability,
                       set_num_threads, thread_limit)
 from ._native import NativeUnavailable, NativeError
@@ -28,7 +29,8 @@ __all__ = [
     "set_num_threads", "thread_limit", "AssemblyMemoryBudgetError", "AssemblyMemoryEstimate",
     "AffineTetN1Basis", "HcurlTetGeometryDiagnostics", "TetN1Assembler", "estimate_tet_n1_assembly_memory",
     "LinearFormDiagnostics", "NativeLinearForm", "NativeBilinearForm", "NativeCrossBilinearForm",
-    "NativeCompositeBilinearForm", "NativeCompositeLinearForm", "ElementComposite",
+    "NativeCompositeBilinearForm", "NativeCompositeLinearForm", "ElementComposite", "FacetBasis",
+    "NativeFacetLinearForm",
     "LinearForm", "BilinearForm", "Basis", "GeometryDiagnostics", "ElementTetP1", "ElementTetP2",
     "ElementVector", "MeshTet", "MeshTet2", "asm", "UnsupportedNativeForm", "NativeUnavailable", "NativeError",
 ]

@@ -1,0 +1,199 @@
+"""Boundary (facet) bases of tetrahedral meshes and their linear forms.
+
+``FacetBasis(mesh, element, facets=None, intorder=2, quadrature=None)``
+mirrors the tetrahedral branch of python/skfemntv/basis.py:2442-2685: the
+same facet selection (boundary facets by default, in facet-id order), parent
+elements, triangle quadrature rules, quadrature-point placement on the
+parent's local face, outward normals, ``dx = |t1 x t2| w`` and
+``global_coordinates``.  The per-facet geometry is computed vectorised on
+the host (facets are O(n^2) against O(n^3) elements); the facet load vector
+``asm(LinearForm, facet_basis)`` is reduced on the GPU per DOF node in facet
+order (skfa_assemble_facet_linear).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _native
+from .basis import Basis, tabulate, validate_quadrature
+from .element import ElementTetP1, ElementTetP2, ElementVector
+from .mesh import TET_FACES
+
+_REF_VERTICES = np.array([[0., 0., 0.], [1., 0., 0.], [0., 1., 0.], [0., 0., 1.]])
+
+
+def triangle_rule(intorder=2, quadrature=None):
+    """Face rule (basis.py:2518-2528, 2557-2566): 3 points (intorder < 4),
+    the 6-point rule (intorder 4), collapsed Gauss above, or a given
+    (points (2,Q), weights (Q,)).  Returns (r, s) points (Q, 2), weights."""
+    if quadrature is not None:
+        points, weights = validate_quadrature(quadrature, 2)
+        return points.T, weights
+    if intorder > 4:
+        count = max(1, (int(intorder) + 3) // 2)
+        g, gw = np.polynomial.legendre.leggauss(count)
+        g, gw = (g + 1.) / 2., gw / 2.
+        pts, wts = [], []
+        for i in range(count):
+            for j in range(count):
+                a, b = g[i], g[j]
+                pts.append((a, (1. - a) * b))
+                wts.append((1. - a) * gw[i] * gw[j])
+        return np.asarray(pts), np.asarray(wts)
+    if intorder >= 4:
+        a, b = .445948490915965, .091576213509771
+        bary = np.array([[a, a, 1 - 2 * a], [a, 1 - 2 * a, a], [1 - 2 * a, a, a],
+                         [b, b, 1 - 2 * b], [b, 1 - 2 * b, b], [1 - 2 * b, b, b]])
+        return bary[:, 1:], np.array([.1116907948390055] * 3 + [.054975871827661] * 3)
+    bary = np.array([[2 / 3, 1 / 6, 1 / 6], [1 / 6, 2 / 3, 1 / 6], [1 / 6, 1 / 6, 2 / 3]])
+    return bary[:, 1:], np.full(3, 1 / 6)
+
+
+class FacetBasis:
+    """Tetrahedral facet basis (basis.py:2442-2685, tetrahedral branch)."""
+
+    def __init__(self, mesh, element, facets=None, intorder=2, quadrature=None):
+        intorder = 2 if intorder is None else int(intorder)
+        if mesh.t.shape[0] not in (4, 10):
+            raise NotImplementedError("the MI355X build supports tetrahedral facet bases")
+        scalar = element.elem if isinstance(element, ElementVector) else None
+        if not isinstance(scalar, (ElementTetP1, ElementTetP2)):
+            raise NotImplementedError("FacetBasis supports ElementVector(ElementTetP1|ElementTetP2)")
+        if isinstance(facets, str):
+            try:
+                facets = mesh.boundaries[facets]
+            except KeyError as error:
+                raise KeyError(f"unknown boundary {facets!r}") from error
+        volume = Basis(mesh, element, intorder=intorder)
+        facet_ids = mesh.boundary_facets() if facets is None else np.asarray(facets, dtype=np.int64)
+        if facet_ids.ndim != 1:
+            raise NotImplementedError("the MI355X FacetBasis takes facet ids")
+        F = facet_ids.size
+        parents = mesh.f2t[0, facet_ids]
+        if np.any(parents < 0):
+            raise ValueError("facet does not have side 0")
+        # local face of the parent that is this facet (first match)
+        match = mesh.t2f[:, parents] == facet_ids[None, :]
+        local = np.argmax(match, axis=0)
+        face_points, face_weights = triangle_rule(intorder, quadrature)
+        Q = len(face_weights)
+        quadratic = isinstance(scalar, ElementTetP2)
+        # reference points (1-r-s) V[c0] + r V[c1] + s V[c2] per local face
+        corners = np.asarray(TET_FACES)                                  # (4, 3)
+        Vc = _REF_VERTICES[corners]                                      # (4, 3, 3)
+        r, s_ = face_points[:, 0], face_points[:, 1]
+        ref = ((1 - r - s_)[None, :, None] * Vc[:, None, 0] + r[None, :, None] * Vc[:, None, 1]
+               + s_[None, :, None] * Vc[:, None, 2])                     # (4, Q, 3)
+        derivatives = np.stack((Vc[:, 1] - Vc[:, 0], Vc[:, 2] - Vc[:, 0]), axis=2)  # (4, 3, 2)
+        shape, refgrad = tabulate(ref.reshape(-1, 3).T, quadratic)
+        n = shape.shape[1]
+        shape = shape.reshape(4, Q, n)
+        refgrad = refgrad.reshape(4, Q, n, 3)
+        gn = mesh.t.shape[0]
+        gshape, ggrad = tabulate(ref.reshape(-1, 3).T, gn == 10)
+        gshape, ggrad = gshape.reshape(4, Q, gn), ggrad.reshape(4, Q, gn, 3)
+        x = mesh.p[:, mesh.t[:, parents]].transpose(2, 0, 1)            # (F, 3, gn)
+        J = np.einsum("fig,fqgj->fqij", x, ggrad[local])                 # (F, Q, 3, 3)
+        tangents = np.einsum("fqij,fjk->fqik", J, derivatives[local])    # (F, Q, 3, 2)
+        points = np.einsum("fqg,fig->fqi", gshape[local], x)             # (F, Q, 3)
+        cross = np.cross(tangents[..., 0], tangents[..., 1])
+        area = np.linalg.norm(cross, axis=-1)
+        normal = cross / area[..., None]
+        centre = x.mean(axis=2)
+        outward = np.einsum("fqi,fqi->fq", normal, points - centre[:, None, :])
+        normal = np.where((outward < 0.)[..., None], -normal, normal)
+        self.mesh, self.elem = mesh, element
+        self.X, self.W = face_points.T.copy(), np.asarray(face_weights, dtype=np.float64)
+        self.quadrature = (self.X.copy(), self.W.copy())
+        self.N = volume.N
+        self.doflocs = volume.doflocs
+        self.element_dofs = volume.element_dofs[:, parents]
+        self.tabulated_shape = shape[local]                              # (F, Q, n)
+        self.tabulated_gradients = np.einsum("fqak,fqkj->fqaj", refgrad[local], np.linalg.inv(J))
+        self.dx = area * self.W[None, :]
+        self.global_coordinates = points
+        self.normals = normal
+        self.parent_elements = parents
+        self.facet_ids = facet_ids.copy()
+        self.facet_sides = np.zeros(F, dtype=np.int8)
+        self.normal_signs = np.ones(F, dtype=np.int8)
+        self.local_faces = [tuple(TET_FACES[k]) for k in local]
+        self.volume_basis = volume
+        self._local_face = local.astype(np.int32)
+        self._face_shape = shape                                         # (4, Q, n)
+        self._device = None
+
+    @property
+    def coefficient_shape(self):
+        return self.dx.shape
+
+    @property
+    def nelems(self):
+        return self.dx.shape[0]
+
+    def device_data(self):
+        if self._device is None:
+            import torch
+            _native.lib()
+            dev = _native.device()
+            local = self.volume_basis._node_local[:, self.parent_elements]   # (n, F) node ids
+            self._device = {
+                "node_conn": torch.from_numpy(np.ascontiguousarray(local.T, dtype=np.int32)).to(dev),
+                "nnodes": len(self.volume_basis.active_nodes),
+                "face": torch.from_numpy(self._local_face).to(dev),
+                "face_shape": torch.from_numpy(np.ascontiguousarray(self._face_shape)).to(dev),
+            }
+        return self._device
+
+
+class NativeFacetLinearForm:
+    """f . v on a FacetBasis (linear_form.py:17-107 over facet DOFs); returns
+    (vector of length N, LinearFormDiagnostics) like NativeLinearForm."""
+
+    def __init__(self, basis):
+        from ._assembly import Plan
+        import torch
+        self.basis = basis
+        d = basis.device_data()
+        F, Q = basis.dx.shape
+        C = basis.elem._dim
+        self._shape = (F, Q, C)
+        self._gradient_shape = (F, Q, C, 3)
+        self.plan = Plan(d["node_conn"], F, d["node_conn"].shape[1], d["nnodes"], 1)
+        self._out = torch.zeros(max(d["nnodes"] * C, 1), dtype=torch.float64, device=_native.device())
+
+    @property
+    def ndofs(self):
+        return self.basis.N
+
+    def assemble(self, *, value=None, gradient=None, num_threads=0):
+        import time
+        from ._assembly import LinearFormDiagnostics, to_host, upload
+        from ._symbols import UnsupportedNativeForm
+        if gradient is not None:
+            raise UnsupportedNativeForm("facet linear forms take value coefficients in the MI355X build")
+        if value is None:
+            raise ValueError("at least one coefficient is required")
+        F, Q, C = self._shape
+        f = np.asarray(value, dtype=np.float64)
+        try:
+            f = np.broadcast_to(f, (F, Q, C))
+        except ValueError as error:
+            raise ValueError(f"value coefficient must broadcast to {(F, Q, C)}") from error
+        start = time.perf_counter()
+        weights = upload(np.ascontiguousarray(self.basis.dx[..., None] * f))
+        d = self.basis.device_data()
+        L = _native.lib()
+        _native.check(L.skfa_assemble_facet_linear(
+            self.plan.handle, _native.ptr(d["face"]), _native.ptr(d["face_shape"]), Q, _native.ptr(weights),
+            _native.ptr(self._out), C, _native.stream()))
+        nodal = to_host(self._out[:d["nnodes"] * C])
+        # node-level rows are the volume basis's DOF order (node * C + c)
+        result = np.zeros(self.basis.N, dtype=np.float64)
+        result[:nodal.shape[0]] = nodal
+        seconds = time.perf_counter() - start
+        return result, LinearFormDiagnostics(entity_count=F, quadrature_point_count=Q, assembly_seconds=seconds)
+
+
+__all__ = ["FacetBasis", "NativeFacetLinearForm", "triangle_rule"]

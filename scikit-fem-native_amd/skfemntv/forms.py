@@ -57,8 +57,12 @@ def BilinearForm(function=None, **kwargs):
 
 def _geometry(basis):
     # Physical quadrature coordinates, component first; computed on the GPU
-    # only when the form reads w.x.
-    return {"x": QuadratureValue(lambda: np.moveaxis(basis.global_coordinates, -1, 0))}
+    # only when the form reads w.x.  Facet bases add their outward normals
+    # as w.n (forms.py:418-421).
+    geometry = {"x": QuadratureValue(lambda: np.moveaxis(basis.global_coordinates, -1, 0))}
+    if getattr(basis, "normals", None) is not None:
+        geometry["n"] = QuadratureValue(np.moveaxis(basis.normals, -1, 0))
+    return geometry
 
 
 def _compile_linear(form, basis, kwargs):
@@ -73,7 +77,8 @@ def _native_linear_assemble(form, basis, kwargs, num_threads=0):
     terms = _compile_linear(form, basis, kwargs)
     native = form._native_cache.get(basis)
     if native is None:
-        native = NativeLinearForm(basis)
+        from .facet import FacetBasis, NativeFacetLinearForm
+        native = NativeFacetLinearForm(basis) if isinstance(basis, FacetBasis) else NativeLinearForm(basis)
         form._native_cache[basis] = native
     value = None
     gradient = None
@@ -294,6 +299,8 @@ def asm(form, *bases, num_threads=None, memory_limit_bytes=None, memory_safety_f
             if num_threads is not None:
                 raise UnsupportedNativeForm("per-call threads are not yet supported for composite BilinearForm")
             return _native_composite_bilinear_assemble(form, bases[0], kwargs)
+        if hasattr(bases[0], "parent_elements"):
+            raise UnsupportedNativeForm("facet bilinear forms are outside the MI355X build")
         _reject_composite_member(bases[0])
         return _native_bilinear_assemble(form, bases[0], kwargs, requested, memory_limit_bytes=memory_limit_bytes,
                                          memory_safety_factor=memory_safety_factor)

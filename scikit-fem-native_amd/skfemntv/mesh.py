@@ -12,6 +12,8 @@ import numpy as np
 # Kuhn split of one cube, corners numbered x + 2y + 4z (basis.py:607-608).
 KUHN_TETS = np.array([(0, 1, 3, 7), (0, 3, 2, 7), (0, 2, 6, 7), (0, 6, 4, 7), (0, 4, 5, 7), (0, 5, 1, 7)],
                      dtype=np.int64)
+# Local faces of a tetrahedron (basis.py:45-46).
+TET_FACES = ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3))
 # Local edges of a tetrahedron (basis.py:652, edge_topology.cpp:40-42).
 TET_EDGES = np.array([(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)], dtype=np.int64)
 
@@ -62,6 +64,56 @@ class MeshTet:
     @property
     def boundaries(self):
         return dict(getattr(self, "_boundaries", {}))
+
+    # ---------------------------------------------------------- facets
+    def _build_topology(self):
+        """Facets as the reference numbers them (basis.py:96-148): the local
+        faces (0,1,2),(0,1,3),(0,2,3),(1,2,3) of every element, face-major,
+        sorted per column; facets are the unique sorted triples in
+        lexicographic order; f2t holds the first and second element met in
+        that face-major order (-1 on the boundary)."""
+        if hasattr(self, "_facets"):
+            return
+        t4 = np.asarray(self.t[:4], dtype=np.int64)
+        E = t4.shape[1]
+        indexing = np.hstack(tuple(t4[np.asarray(face)] for face in TET_FACES))  # (3, 4E), face-major
+        canonical, inverse = np.unique(np.sort(indexing, axis=0), axis=1, return_inverse=True)
+        inverse = np.asarray(inverse).ravel()
+        F = canonical.shape[1]
+        elements = np.tile(np.arange(E, dtype=np.int64), 4)
+        order = np.argsort(inverse, kind="stable")          # occurrences of each facet in face-major order
+        counts = np.bincount(inverse, minlength=F)
+        if counts.max(initial=0) > 2:
+            raise ValueError("non-manifold facet has more than two cells")
+        start = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        f2t = np.full((2, F), -1, dtype=np.int64)
+        f2t[0] = elements[order[start]]
+        two = counts == 2
+        f2t[1, two] = elements[order[start[two] + 1]]
+        self._facets = np.ascontiguousarray(canonical, dtype=np.int64)
+        self._t2f = np.ascontiguousarray(inverse.reshape(4, E), dtype=np.int64)
+        self._f2t = f2t
+
+    @property
+    def facets(self):
+        self._build_topology()
+        return self._facets
+
+    @property
+    def t2f(self):
+        self._build_topology()
+        return self._t2f
+
+    @property
+    def f2t(self):
+        self._build_topology()
+        return self._f2t
+
+    def boundary_facets(self):
+        return np.flatnonzero(self.f2t[1] == -1)
+
+    def interior_facets(self):
+        return np.flatnonzero(self.f2t[1] != -1)
 
 
 def first_seen_edges(t4):
@@ -114,4 +166,4 @@ class MeshTet2(MeshTet):
         self._boundaries = {}
 
 
-__all__ = ["MeshTet", "MeshTet2", "first_seen_edges", "KUHN_TETS", "TET_EDGES"]
+__all__ = ["MeshTet", "MeshTet2", "first_seen_edges", "KUHN_TETS", "TET_EDGES", "TET_FACES"]

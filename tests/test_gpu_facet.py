@@ -1,0 +1,79 @@
+"""GPU parity of the facet (boundary) path: FacetBasis geometry against the
+oracle's per-facet restatement of the reference, and the facet load vector
+from skfa_assemble_facet_linear against the oracle's LinearFormAssembler
+restatement on the same facet tabulation.  Each case prints its name first.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import host
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def p2_kuhn(n, jitter, seed):
+    ax = np.linspace(0., 1., n + 1)
+    p, t = host.kuhn_mesh(ax, ax, ax)
+    if jitter:
+        p = host.jitter_interior(p, 1. / n, jitter, seed)
+    return p, t, host.p2_mesh(p, t)
+
+
+@pytest.mark.parametrize("quadratic,components", [(False, 1), (True, 1), (True, 3)])
+def test_facet_basis_and_load_match_oracle(quadratic, components):
+    import skfemntv as sk
+    from skfemntv.helpers import dot
+    say(f"facet basis quadratic={quadratic} C={components}")
+    p, t, (p2, t2) = p2_kuhn(3, 0.2, seed=7)
+    mesh = sk.MeshTet2(p2, t2) if quadratic else sk.MeshTet(p, t)
+    elem = sk.ElementVector(sk.ElementTetP2() if quadratic else sk.ElementTetP1(), dim=components)
+    fb = sk.FacetBasis(mesh, elem)
+    ref = host.facet_basis(p2 if quadratic else p, t2 if quadratic else t, quadratic, components)
+    np.testing.assert_array_equal(fb.facet_ids, ref["facets"])
+    np.testing.assert_array_equal(fb.parent_elements, ref["parents"])
+    F = fb.facet_ids.size
+    np.testing.assert_array_equal(fb.element_dofs.T.reshape(F, -1, components), ref["dofs"])
+    for name, mine, theirs in (("dx", fb.dx, ref["dx"]), ("x", fb.global_coordinates, ref["x"]),
+                               ("normals", fb.normals, ref["normals"]), ("shape", fb.tabulated_shape, ref["shape"])):
+        np.testing.assert_allclose(mine, theirs, rtol=0, atol=1e-14, err_msg=name)
+    Q = fb.dx.shape[1]
+    n = ref["shape"].shape[2]
+    grads = np.zeros((F, Q, n, 3))
+    say(f"facet load constant quadratic={quadratic} C={components}")
+    f = np.arange(1., components + 1.)
+    b = sk.asm(sk.LinearForm(lambda v, w: dot(w.f, v)), fb, f=f)
+    expect = oracle.assemble_linear(ref["shape"], grads, ref["dx"], ref["dofs"], ref["N"], components, value=f)
+    assert b.shape == (fb.N,)
+    err = np.linalg.norm(b - expect) / np.linalg.norm(expect)
+    assert err <= FP64_TOL, err
+    if components == 1:
+        say(f"facet load x.n quadratic={quadratic}")
+        # g = x . n on the boundary: sum of the load = int x.n ds = 3 |Omega|
+        g = sk.LinearForm(lambda v, w: dot(w.x[0] * w.n[0] + w.x[1] * w.n[1] + w.x[2] * w.n[2], v))
+        b = sk.asm(g, fb)
+        gv = np.einsum("fqi,fqi->fq", ref["x"], ref["normals"])[..., None]
+        expect = oracle.assemble_linear(ref["shape"], grads, ref["dx"], ref["dofs"], ref["N"], 1, value=gv)
+        err = np.linalg.norm(b - expect) / np.linalg.norm(expect)
+        assert err <= FP64_TOL, err
+        assert abs(b.sum() - 3.) <= 1e-12
+
+
+def test_facet_bilinear_forms_are_rejected():
+    import skfemntv as sk
+    from skfemntv.helpers import dot
+    say("facet bilinear rejected")
+    p, t, _ = p2_kuhn(1, 0.0, seed=0)
+    fb = sk.FacetBasis(sk.MeshTet(p, t), sk.ElementVector(sk.ElementTetP1(), dim=1))
+    with pytest.raises(sk.UnsupportedNativeForm):
+        sk.asm(sk.BilinearForm(lambda u, v, w: dot(u, v)), fb)

@@ -509,3 +509,57 @@ def test_composite_numbering_restatement():
     for f, (c, C) in enumerate(zip(conns, comps)):
         expect = np.array([[[order[(int(v), f, k)] for k in range(C)] for v in col] for col in c.T])
         np.testing.assert_array_equal(dofs[f], expect)
+
+
+# ------------------------------------------------------------------ facets
+def test_facet_topology_matches_restatement_and_invariants():
+    """The vectorised mesh facet topology equals the oracle's loop
+    restatement, facets are sorted unique triples in lexicographic order, and
+    every element face maps to its facet."""
+    from skfemntv.mesh import MeshTet
+    for n in (1, 2, 3):
+        ax = np.linspace(0, 1, n + 1)
+        m = MeshTet.init_tensor(ax, ax, ax)
+        facets, t2f, f2t = host.facet_topology(m.t)
+        np.testing.assert_array_equal(m.facets, facets)
+        np.testing.assert_array_equal(m.t2f, t2f)
+        np.testing.assert_array_equal(m.f2t, f2t)
+        assert np.all(np.diff(facets, axis=0) > 0)
+        keys = [tuple(c) for c in facets.T]
+        assert keys == sorted(keys) and len(set(keys)) == len(keys)
+        for k, face in enumerate(host.TET_FACES):
+            np.testing.assert_array_equal(np.sort(m.t[list(face)], axis=0), facets[:, t2f[k]])
+        assert m.boundary_facets().size == 12 * n * n
+
+
+@pytest.mark.parametrize("quadratic", [False, True])
+def test_facet_oracle_closed_forms(quadratic):
+    """Boundary of the jittered unit cube: sum dx = 6, int x.n ds = 3 |Omega|
+    (divergence theorem), normals unit and outward; the load f = 1 gives
+    sum_{facets of a} area/3 per P1 vertex, and on P2 zero at vertices and
+    area/3 per facet at edge nodes."""
+    p2, t2 = _small_p2(seed=3, n=2)
+    p, t = (p2, t2) if quadratic else (p2[:, :27], t2[:4])
+    fb = host.facet_basis(p, t, quadratic)
+    dx, x, nrm = fb["dx"], fb["x"], fb["normals"]
+    assert abs(dx.sum() - 6.) < 1e-13
+    assert abs((dx * np.einsum("fqi,fqi->fq", x, nrm)).sum() - 3.) < 1e-13
+    np.testing.assert_allclose(np.linalg.norm(nrm, axis=-1), 1., rtol=0, atol=1e-15)
+    F, Q, n = fb["shape"].shape
+    grads = np.zeros((F, Q, n, 3))
+    b = oracle.assemble_linear(fb["shape"], grads, dx, fb["dofs"], fb["N"], 1, value=np.array([1.]))
+    area = dx.sum(axis=1)
+    expect = np.zeros(fb["N"])
+    par = fb["parents"]
+    for f in range(F):
+        corner = [t[c, par[f]] for c in host.TET_FACES[fb["local"][f]]]
+        if quadratic:
+            from oracle.host import TET_FACES  # noqa: F401
+            edges = {(0, 1): 4, (1, 2): 5, (0, 2): 6, (0, 3): 7, (1, 3): 8, (2, 3): 9}
+            lf = host.TET_FACES[fb["local"][f]]
+            for i, j in ((lf[0], lf[1]), (lf[1], lf[2]), (lf[0], lf[2])):
+                expect[fb["dofs"][f, edges[(min(i, j), max(i, j))], 0]] += area[f] / 3.
+        else:
+            for v in corner:
+                expect[v] += area[f] / 3.
+    np.testing.assert_allclose(b, expect, rtol=0, atol=1e-15)
