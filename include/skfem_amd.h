@@ -226,6 +226,22 @@ int skfa_assemble_linear(skfa_plan* plan, const double* xyz, const int32_t* vcon
 int skfa_assemble_facet_linear(const skfa_plan* plan, const int32_t* face, const double* face_shape, int32_t nq,
                                const double* weights, double* out, int32_t components, void* stream);
 
+/* Facet mass-type bilinear form (value kind: sum_q w phi_a phi_b on the
+ * diagonal component blocks) into the CSR values of `plan` (nnz FP64,
+ * overwritten).  Replaces BilinearFormAssembler::assemble with a value
+ * coefficient on the facet DOFs of a FacetBasis
+ * (cpp/src/assembly/bilinear_form_assembler.cpp:39-105 driven by
+ * python/skfemntv/bilinear_form.py:13-99).
+ *   plan        skfa_plan_create on the (F, n) node connectivity of the
+ *               facets' parent elements, with the field's components (its
+ *               pattern is the reference's facet pattern)
+ *   conn        that (F, n) int32 connectivity
+ *   face, face_shape as for skfa_assemble_facet_linear; weights (F, Q) dx * coefficient
+ * Each node row adds its (facet, local node) incidences in facet order. */
+int skfa_assemble_facet_bilinear(const skfa_plan* plan, const int32_t* conn, const int32_t* face,
+                                 const double* face_shape, int32_t nq, const double* weights, double* values,
+                                 void* stream);
+
 /* Per (e,q) geometry: replaces tabulate_basis_geometry
  * (cpp/src/assembly/basis_geometry.cpp:49-176).  Outputs (E,Q[,...]) FP64,
  * each optional (NULL = not written): det, tol (64 eps max|J|^3), cond,

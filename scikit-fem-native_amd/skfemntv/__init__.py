@@ -17,7 +17,7 @@ from ._assembly import (AssemblyMemoryBudgetError, AssemblyMemoryEstimate, Linea
 from .hcurl import (AffineTetN1Basis, HcurlTetGeometryDiagnostics, TetN1Assembler,
                     estimate_tet_n1_assembly_memory)
 from .mesh import MeshTet, MeshTet2
-from .facet import FacetBasis, NativeFacetLinearForm
+from .facet import FacetBasis, NativeFacetBilinearForm, NativeFacetLinearForm
 from .runtime import (BACKEND, CAPABILITIES, available_num_threads, get_num_threads, has_capability,
                       set_num_threads, thread_limit)
 from ._native import NativeUnavailable, NativeError
@@ -30,7 +30,7 @@ __all__ = [
     "AffineTetN1Basis", "HcurlTetGeometryDiagnostics", "TetN1Assembler", "estimate_tet_n1_assembly_memory",
     "LinearFormDiagnostics", "NativeLinearForm", "NativeBilinearForm", "NativeCrossBilinearForm",
     "NativeCompositeBilinearForm", "NativeCompositeLinearForm", "ElementComposite", "FacetBasis",
-    "NativeFacetLinearForm",
+    "NativeFacetLinearForm", "NativeFacetBilinearForm",
     "LinearForm", "BilinearForm", "Basis", "GeometryDiagnostics", "ElementTetP1", "ElementTetP2",
     "ElementVector", "MeshTet", "MeshTet2", "asm", "UnsupportedNativeForm", "NativeUnavailable", "NativeError",
 ]

@@ -196,4 +196,51 @@ class NativeFacetLinearForm:
         return result, LinearFormDiagnostics(entity_count=F, quadrature_point_count=Q, assembly_seconds=seconds)
 
 
-__all__ = ["FacetBasis", "NativeFacetLinearForm", "triangle_rule"]
+class NativeFacetBilinearForm:
+    """Value-kind bilinear forms on a FacetBasis (bilinear_form.py:13-99 over
+    facet DOFs): boundary mass / Robin terms.  ``assemble`` returns the same
+    CSR object on every call, shape (N, N), with the reference's facet
+    pattern (every pair of DOFs of each facet's parent element)."""
+
+    def __init__(self, basis):
+        from scipy.sparse import csr_matrix
+        from ._assembly import Plan
+        import torch
+        self.basis = basis
+        d = basis.device_data()
+        F, Q = basis.dx.shape
+        C = basis.elem._dim
+        self.plan = Plan(d["node_conn"], F, d["node_conn"].shape[1], d["nnodes"], C)
+        self._values = torch.zeros(max(self.plan.nnz, 1), dtype=torch.float64, device=_native.device())
+        ip, ix = self.plan.host_csr_arrays()
+        self._matrix = csr_matrix((np.zeros(self.plan.nnz), ix, ip), shape=(self.plan.nrows, self.plan.nrows),
+                                  copy=False)
+        self._matrix.resize((basis.N, basis.N))
+
+    def assemble(self, *, value=None, gradient=None, symmetric_gradient=None, divergence=None, num_threads=0,
+                 precision="fp64"):
+        from ._assembly import to_host, upload
+        from ._symbols import UnsupportedNativeForm
+        if gradient is not None or symmetric_gradient is not None or divergence is not None:
+            raise UnsupportedNativeForm("facet bilinear forms take the value kind in the MI355X build")
+        if precision != "fp64":
+            raise UnsupportedNativeForm("facet bilinear forms are assembled in FP64")
+        if value is None:
+            raise ValueError("at least one coefficient is required")
+        F, Q = self.basis.dx.shape
+        c = np.asarray(value, dtype=np.float64)
+        try:
+            c = np.broadcast_to(c, (F, Q))
+        except ValueError as error:
+            raise ValueError(f"value coefficient must broadcast to {(F, Q)}") from error
+        weights = upload(np.ascontiguousarray(self.basis.dx * c))
+        d = self.basis.device_data()
+        L = _native.lib()
+        _native.check(L.skfa_assemble_facet_bilinear(
+            self.plan.handle, _native.ptr(d["node_conn"]), _native.ptr(d["face"]), _native.ptr(d["face_shape"]), Q,
+            _native.ptr(weights), _native.ptr(self._values), _native.stream()))
+        np.copyto(self._matrix.data, to_host(self._values[:self.plan.nnz]))
+        return self._matrix
+
+
+__all__ = ["FacetBasis", "NativeFacetBilinearForm", "NativeFacetLinearForm", "triangle_rule"]

@@ -144,11 +144,18 @@ def _native_bilinear_assemble(form, basis, kwargs, num_threads=0, *, memory_limi
         else:
             raise UnsupportedNativeForm(f"unsupported bilinear term kind {term.kind!r}")
     result = None
+    facet = hasattr(basis, "parent_elements")
+    if facet and tensor_terms:
+        raise UnsupportedNativeForm("facet bilinear forms take the value kind in the MI355X build")
     if any(v is not None for v in kinds.values()):
         native = form._native_cache.get(basis)
         if native is None:
-            native = NativeBilinearForm(basis, memory_limit_bytes=memory_limit_bytes,
-                                        memory_safety_factor=memory_safety_factor)
+            if facet:
+                from .facet import NativeFacetBilinearForm
+                native = NativeFacetBilinearForm(basis)
+            else:
+                native = NativeBilinearForm(basis, memory_limit_bytes=memory_limit_bytes,
+                                            memory_safety_factor=memory_safety_factor)
             form._native_cache[basis] = native
         result = native.assemble(num_threads=num_threads, precision=precision, **kinds)
     if tensor_terms:
@@ -299,8 +306,6 @@ def asm(form, *bases, num_threads=None, memory_limit_bytes=None, memory_safety_f
             if num_threads is not None:
                 raise UnsupportedNativeForm("per-call threads are not yet supported for composite BilinearForm")
             return _native_composite_bilinear_assemble(form, bases[0], kwargs)
-        if hasattr(bases[0], "parent_elements"):
-            raise UnsupportedNativeForm("facet bilinear forms are outside the MI355X build")
         _reject_composite_member(bases[0])
         return _native_bilinear_assemble(form, bases[0], kwargs, requested, memory_limit_bytes=memory_limit_bytes,
                                          memory_safety_factor=memory_safety_factor)

@@ -69,11 +69,35 @@ def test_facet_basis_and_load_match_oracle(quadratic, components):
         assert abs(b.sum() - 3.) <= 1e-12
 
 
-def test_facet_bilinear_forms_are_rejected():
+@pytest.mark.parametrize("quadratic,components", [(False, 1), (True, 1), (True, 3)])
+def test_facet_mass_matches_oracle(quadratic, components):
+    """Boundary mass k u.v on the facet DOFs: the reference's facet pattern
+    (every DOF pair of each facet's parent element) bit-exact, values
+    against the oracle's BilinearFormAssembler restatement; 1^T M 1 = C x
+    boundary area for k = 1."""
     import skfemntv as sk
-    from skfemntv.helpers import dot
-    say("facet bilinear rejected")
-    p, t, _ = p2_kuhn(1, 0.0, seed=0)
-    fb = sk.FacetBasis(sk.MeshTet(p, t), sk.ElementVector(sk.ElementTetP1(), dim=1))
+    from skfemntv.helpers import ddot, dot, grad
+    say(f"facet mass quadratic={quadratic} C={components}")
+    p, t, (p2, t2) = p2_kuhn(3, 0.2, seed=5)
+    mesh = sk.MeshTet2(p2, t2) if quadratic else sk.MeshTet(p, t)
+    elem = sk.ElementVector(sk.ElementTetP2() if quadratic else sk.ElementTetP1(), dim=components)
+    fb = sk.FacetBasis(mesh, elem)
+    ref = host.facet_basis(p2 if quadratic else p, t2 if quadratic else t, quadratic, components)
+    F, Q, n = ref["shape"].shape
+    k = np.random.default_rng(components).uniform(0.5, 2., (F, 1))
+    M = sk.asm(sk.BilinearForm(lambda u, v, w: w.k * dot(u, v)), fb, k=k)
+    rows = ref["dofs"].reshape(F, -1)
+    ip, ix, sc = oracle.pattern(rows, nrows=ref["N"])
+    vals = oracle.assemble_bilinear(ref["shape"], np.zeros((F, Q, n, 3)), ref["dx"], sc, len(ix), components,
+                                    value=np.ascontiguousarray(np.broadcast_to(k, (F, Q))))
+    M = M.tocsr()
+    assert M.shape == (fb.N, fb.N)
+    np.testing.assert_array_equal(np.asarray(M.indptr[:len(ip)], dtype=np.int64), ip)
+    np.testing.assert_array_equal(np.asarray(M.indices, dtype=np.int64), ix)
+    err = np.linalg.norm(M.data - vals) / np.linalg.norm(vals)
+    assert err <= FP64_TOL, err
+    M1 = sk.asm(sk.BilinearForm(lambda u, v, w: dot(u, v)), fb)
+    one = np.ones(fb.N)
+    assert abs(one @ (M1 @ one) - 6. * components) <= 1e-12
     with pytest.raises(sk.UnsupportedNativeForm):
-        sk.asm(sk.BilinearForm(lambda u, v, w: dot(u, v)), fb)
+        sk.asm(sk.BilinearForm(lambda u, v, w: ddot(grad(u), grad(v))), fb)
