@@ -800,8 +800,10 @@ struct OffsetPick {
 template <bool F32, int RB, int OFFW, int QG>
 __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
     extern __shared__ double span_lds[];
-    // one step in flight: a PL_VEC payload is 14 doubles per lane
-    constexpr int NL = 10, C = 3, P = 14, K = 1;
+    // two steps in flight: the PL_VEC payloads take 2 x 14 doubles per lane
+    // (~250 VGPRs, 2 waves/SIMD, below the ~2.75 the LDS span allows);
+    // measured on 48^3 elasticity: K = 2 0.976 ms, K = 1 1.045 ms
+    constexpr int NL = 10, C = 3, P = 14, K = 2;
     using T = typename std::conditional<F32, float, double>::type;
     const int lane = threadIdx.x;
     const int wv = placed_wave(A.plan.xcd_seg, int(A.plan.nwaves), A.plan.placement, A.plan.xcd_chunk);
