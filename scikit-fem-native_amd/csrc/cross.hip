@@ -255,9 +255,37 @@ __global__ void __launch_bounds__(64) k_cross_fast(CrossFastArgs A) {
     // lanes of the wave run the same number of steps (uniform loop bound)
     int64_t steps = active ? (k1 - k0 + A.S - 1) / A.S : 0;
     for (int off = 32; off > 0; off >>= 1) steps = max(steps, (int64_t)__shfl_xor(steps, off, 64));
+    // records of the next step are loaded while this step computes
+    uint32_t cw[8], nw[8];
+    auto load_rec = [&](int64_t kk, uint32_t* w) {
+        // records are 8, 16 or 32 bytes (4 + NC offset bytes, rounded up)
+        const uint8_t* base = A.records + kk * A.rec_bytes;
+        if (A.rec_bytes == 8) {
+            const uint2 v = *reinterpret_cast<const uint2*>(base);
+            w[0] = v.x; w[1] = v.y;
+            w[2] = w[3] = w[4] = w[5] = w[6] = w[7] = 0u;
+        } else {
+            const uint4* src = reinterpret_cast<const uint4*>(base);
+            const uint4 v0 = src[0];
+            w[0] = v0.x; w[1] = v0.y; w[2] = v0.z; w[3] = v0.w;
+            if (A.rec_bytes > 16) {
+                const uint4 v1 = src[1];
+                w[4] = v1.x; w[5] = v1.y; w[6] = v1.z; w[7] = v1.w;
+            } else {
+                w[4] = w[5] = w[6] = w[7] = 0u;
+            }
+        }
+    };
+#pragma unroll
+    for (int t = 0; t < 8; ++t) cw[t] = nw[t] = 0u;
+    int64_t kcur = k0 + p;
+    bool hcur = active && kcur < k1;
+    if (hcur) load_rec(kcur, cw);
     for (int64_t st = 0; st < steps; ++st) {
-        const int64_t k = k0 + p + st * A.S;
-        const bool have = active && k < k1;
+        const int64_t knext = kcur + A.S;
+        const bool hnext = active && knext < k1;
+        if (hnext) load_rec(knext, nw);
+        const bool have = hcur;
         double acc[NENT];
         int off_b[NC];
 #pragma unroll
@@ -265,13 +293,16 @@ __global__ void __launch_bounds__(64) k_cross_fast(CrossFastArgs A) {
 #pragma unroll
         for (int b = 0; b < NC; ++b) off_b[b] = 0;
         if (have) {
-            const uint8_t* rec = A.records + k * A.rec_bytes;
-            const int32_t packed = *reinterpret_cast<const int32_t*>(rec);
+            const int32_t packed = int32_t(cw[0]);
             const int64_t e = packed >> 4;
             const int a = packed & 15;
+            if (A.off_bytes == 1) {
 #pragma unroll
-            for (int b = 0; b < NC; ++b)
-                off_b[b] = A.off_bytes == 1 ? int(rec[4 + b]) : int(reinterpret_cast<const uint16_t*>(rec + 4)[b]);
+                for (int b = 0; b < NC; ++b) off_b[b] = int((cw[1 + b / 4] >> (8 * (b % 4))) & 0xffu);
+            } else {
+#pragma unroll
+                for (int b = 0; b < NC; ++b) off_b[b] = int((cw[1 + b / 2] >> (16 * (b % 2))) & 0xffffu);
+            }
             Jac G;
             jac_affine(A.xyz, A.vconn + e * 4, G);
             const double inv = 1. / G.det;
@@ -336,6 +367,10 @@ __global__ void __launch_bounds__(64) k_cross_fast(CrossFastArgs A) {
                             (void)atomicAdd(&rs[(int64_t(c) * len + off_b[b]) * CC + d], acc[(c * NC + b) * CC + d]);
             }
         }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) cw[t] = nw[t];
+        hcur = hnext;
+        kcur = knext;
     }
     __syncthreads();
     for (int64_t k = lane; k < span_len; k += 64) A.values[lo * cc + k] = span[k];
