@@ -199,7 +199,49 @@ __global__ void __launch_bounds__(64) k_cross_rows(CrossArgs A) {
 // or dphi/dxi) and the element's J^{-1}, |det J| and coefficient, kept in
 // registers, and added to the span by the S parts in turn (one writer per
 // address per ds_add_f64): every entry sums its incidences in element order,
-// as in the general kernel.
+// as in the general kernel.  Loads are software-pipelined: a lane's records
+// run two steps ahead and its elements' corner ids one step ahead, so a
+// step waits only for the four vertex coordinates.
+// Gather record words (8, 16 or 32 bytes: 4 + NC offset bytes, rounded up).
+struct RecWords {
+    uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+};
+
+__device__ inline RecWords load_rec(const uint8_t* __restrict__ records, int64_t k, int rec_bytes) {
+    RecWords r;
+    const uint8_t* base = records + k * rec_bytes;
+    if (rec_bytes == 8) {
+        const uint2 v = *reinterpret_cast<const uint2*>(base);
+        r.w[0] = v.x;
+        r.w[1] = v.y;
+    } else {
+        const uint4* src = reinterpret_cast<const uint4*>(base);
+        const uint4 v0 = src[0];
+        r.w[0] = v0.x; r.w[1] = v0.y; r.w[2] = v0.z; r.w[3] = v0.w;
+        if (rec_bytes > 16) {
+            const uint4 v1 = src[1];
+            r.w[4] = v1.x; r.w[5] = v1.y; r.w[6] = v1.z; r.w[7] = v1.w;
+        }
+    }
+    return r;
+}
+
+__device__ inline void jac_affine4(const double4* __restrict__ xyz, int4 c, Jac& g) {
+    const double4 v0 = xyz[c.x], v1 = xyz[c.y], v2 = xyz[c.z], v3 = xyz[c.w];
+    const double J[9] = {v1.x - v0.x, v2.x - v0.x, v3.x - v0.x, v1.y - v0.y, v2.y - v0.y,
+                         v3.y - v0.y, v1.z - v0.z, v2.z - v0.z, v3.z - v0.z};
+    g.adj[0] = J[4] * J[8] - J[5] * J[7];
+    g.adj[1] = J[2] * J[7] - J[1] * J[8];
+    g.adj[2] = J[1] * J[5] - J[2] * J[4];
+    g.adj[3] = J[5] * J[6] - J[3] * J[8];
+    g.adj[4] = J[0] * J[8] - J[2] * J[6];
+    g.adj[5] = J[2] * J[3] - J[0] * J[5];
+    g.adj[6] = J[3] * J[7] - J[4] * J[6];
+    g.adj[7] = J[1] * J[6] - J[0] * J[7];
+    g.adj[8] = J[0] * J[4] - J[1] * J[3];
+    g.det = J[0] * g.adj[0] + J[1] * g.adj[3] + J[2] * g.adj[6];
+}
+
 // LDS span target of one fast-path wave: more rows per wave means fewer
 // turns per row; measured on the 64^3 Taylor-Hood divergence block: 8 KiB
 // 0.69 ms, 16 KiB 0.39 ms, 32 KiB 0.375 ms
@@ -256,35 +298,21 @@ __global__ void __launch_bounds__(64) k_cross_fast(CrossFastArgs A) {
     int64_t steps = active ? (k1 - k0 + A.S - 1) / A.S : 0;
     for (int off = 32; off > 0; off >>= 1) steps = max(steps, (int64_t)__shfl_xor(steps, off, 64));
     // records of the next step are loaded while this step computes
-    uint32_t cw[8], nw[8];
-    auto load_rec = [&](int64_t kk, uint32_t* w) {
-        // records are 8, 16 or 32 bytes (4 + NC offset bytes, rounded up)
-        const uint8_t* base = A.records + kk * A.rec_bytes;
-        if (A.rec_bytes == 8) {
-            const uint2 v = *reinterpret_cast<const uint2*>(base);
-            w[0] = v.x; w[1] = v.y;
-            w[2] = w[3] = w[4] = w[5] = w[6] = w[7] = 0u;
-        } else {
-            const uint4* src = reinterpret_cast<const uint4*>(base);
-            const uint4 v0 = src[0];
-            w[0] = v0.x; w[1] = v0.y; w[2] = v0.z; w[3] = v0.w;
-            if (A.rec_bytes > 16) {
-                const uint4 v1 = src[1];
-                w[4] = v1.x; w[5] = v1.y; w[6] = v1.z; w[7] = v1.w;
-            } else {
-                w[4] = w[5] = w[6] = w[7] = 0u;
-            }
-        }
-    };
-#pragma unroll
-    for (int t = 0; t < 8; ++t) cw[t] = nw[t] = 0u;
+    RecWords cw, nw;
+    // records run two steps ahead, element corners one step ahead
+    RecWords nnw;
+    int4 cv = make_int4(0, 0, 0, 0), nv = make_int4(0, 0, 0, 0);
     int64_t kcur = k0 + p;
     bool hcur = active && kcur < k1;
-    if (hcur) load_rec(kcur, cw);
+    bool hnext = active && kcur + A.S < k1;
+    if (hcur) cw = load_rec(A.records, kcur, A.rec_bytes);
+    if (hnext) nw = load_rec(A.records, kcur + A.S, A.rec_bytes);
+    if (hcur) cv = *reinterpret_cast<const int4*>(A.vconn + (int64_t(int32_t(cw.w[0])) >> 4) * 4);
     for (int64_t st = 0; st < steps; ++st) {
         const int64_t knext = kcur + A.S;
-        const bool hnext = active && knext < k1;
-        if (hnext) load_rec(knext, nw);
+        const bool hnn = active && knext + A.S < k1;
+        if (hnn) nnw = load_rec(A.records, knext + A.S, A.rec_bytes);
+        if (hnext) nv = *reinterpret_cast<const int4*>(A.vconn + (int64_t(int32_t(nw.w[0])) >> 4) * 4);
         const bool have = hcur;
         double acc[NENT];
         int off_b[NC];
@@ -293,18 +321,18 @@ __global__ void __launch_bounds__(64) k_cross_fast(CrossFastArgs A) {
 #pragma unroll
         for (int b = 0; b < NC; ++b) off_b[b] = 0;
         if (have) {
-            const int32_t packed = int32_t(cw[0]);
+            const int32_t packed = int32_t(cw.w[0]);
             const int64_t e = packed >> 4;
             const int a = packed & 15;
             if (A.off_bytes == 1) {
 #pragma unroll
-                for (int b = 0; b < NC; ++b) off_b[b] = int((cw[1 + b / 4] >> (8 * (b % 4))) & 0xffu);
+                for (int b = 0; b < NC; ++b) off_b[b] = int((cw.w[1 + b / 4] >> (8 * (b % 4))) & 0xffu);
             } else {
 #pragma unroll
-                for (int b = 0; b < NC; ++b) off_b[b] = int((cw[1 + b / 2] >> (16 * (b % 2))) & 0xffffu);
+                for (int b = 0; b < NC; ++b) off_b[b] = int((cw.w[1 + b / 2] >> (16 * (b % 2))) & 0xffffu);
             }
             Jac G;
-            jac_affine(A.xyz, A.vconn + e * 4, G);
+            jac_affine4(A.xyz, cv, G);
             const double inv = 1. / G.det;
             const double s = fabs(G.det) * (A.coef ? A.coef[e * A.cse] : 1.);
             double Ji[9];  // J^{-1}
@@ -367,9 +395,11 @@ __global__ void __launch_bounds__(64) k_cross_fast(CrossFastArgs A) {
                             (void)atomicAdd(&rs[(int64_t(c) * len + off_b[b]) * CC + d], acc[(c * NC + b) * CC + d]);
             }
         }
-#pragma unroll
-        for (int t = 0; t < 8; ++t) cw[t] = nw[t];
+        cw = nw;
+        nw = nnw;
+        cv = nv;
         hcur = hnext;
+        hnext = hnn;
         kcur = knext;
     }
     __syncthreads();
