@@ -210,3 +210,41 @@ def test_taylor_hood_divergence_closed_form_at_scale():
     assert Bt.shape == (Bm.shape[1], Bm.shape[0])
     diff = (Bt - Bm.T.tocsr())
     assert diff.nnz == 0 or np.abs(diff.data).max() <= 1e-14 * np.abs(Bm.data).max()
+
+
+def fan_mesh(m=120):
+    """One hub vertex at the origin with m ring points and two apexes: 2m
+    tetrahedra share the hub, whose P2 row then spans > 256 column nodes."""
+    ang = 2 * np.pi * np.arange(m) / m
+    ring = np.vstack([np.cos(ang), np.sin(ang), 0.05 * np.sin(3 * ang)])
+    p = np.hstack([np.zeros((3, 1)), ring, [[0.], [0.], [1.]], [[0.], [0.], [-1.]]])
+    top, bot = m + 1, m + 2
+    i = np.arange(m)
+    j = (i + 1) % m
+    t = np.hstack([np.vstack([np.zeros(m, int), 1 + i, 1 + j, np.full(m, top)]),
+                   np.vstack([np.zeros(m, int), 1 + j, 1 + i, np.full(m, bot)])])
+    return p, t
+
+
+def test_cross_wide_rows_use_two_byte_offsets():
+    """Rows longer than 256 column nodes take 2-byte record offsets (and
+    32-byte records) in both cross kernels; pattern and values against the
+    oracle."""
+    import skfemntv as sk
+    from skfemntv.helpers import div
+    say("cross wide rows (fan mesh)")
+    p, t = fan_mesh(120)
+    p2, t2 = host.p2_mesh(p, t)
+    X, W = host.tet_rule_4()
+    B = bases(p2, t2, X, W)
+    native = sk.NativeCrossBilinearForm(B["p1"], B["vp2"])
+    assert native.cross_plan().max_row_len > 256
+    E, Q = t2.shape[1], 4
+    for coef, name in ((np.ones((E, 1)), "element-constant"), (np.random.default_rng(2).uniform(.5, 2., (E, Q)),
+                                                              "per-point")):
+        say(f"cross wide rows {name}")
+        K = sk.asm(sk.BilinearForm(lambda u, v, w: w.c * (div(u) * v)), B["vp2"], B["p1"], c=coef)
+        ip, ix, vals, _, _ = pipeline.cross(p2, t2, X, W, (False, 1), (True, 3), "value", "gradient",
+                                            coefficient=div_tensor(np.broadcast_to(coef, (E, Q)), False),
+                                            tensor=True)
+        check_csr(K, ip, ix, vals, (B["p1"].N, B["vp2"].N))
