@@ -1368,12 +1368,24 @@ __device__ inline void build_D(const Geo& G, const Coef& k, int c, int d, bool h
     const double inv = 1. / fabs(G.det);
     const double* a = G.adj;
     const double iso = (c == d) ? (k.g + 0.5 * k.s) : 0.;
+    // columns c and d of the adjugate: compile-time for one component,
+    // else exact 0/1 weights (a runtime register index would go to scratch)
+    double ac[3], ad[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if constexpr (C == 1) {
+            ac[i] = ad[i] = a[i * 3];
+        } else {
+            ac[i] = a[i * 3] * double(c == 0) + a[i * 3 + 1] * double(c == 1) + a[i * 3 + 2] * double(c == 2);
+            ad[i] = a[i * 3] * double(d == 0) + a[i * 3 + 1] * double(d == 1) + a[i * 3 + 2] * double(d == 2);
+        }
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const double aat = a[i * 3 + 0] * a[j * 3 + 0] + a[i * 3 + 1] * a[j * 3 + 1] + a[i * 3 + 2] * a[j * 3 + 2];
-            D[i * 3 + j] = iso * aat + 0.5 * k.s * a[i * 3 + d] * a[j * 3 + c] + k.lam * a[i * 3 + c] * a[j * 3 + d];
+            D[i * 3 + j] = iso * aat + 0.5 * k.s * ad[i] * ac[j] + k.lam * ac[i] * ad[j];
         }
     if (C == 1 && has_tensor) {
         double T[9];
@@ -1406,11 +1418,20 @@ __global__ void __launch_bounds__(64) k_assemble_h1_general(H1Args A) {
     const Tables T = tables_at(A.tables, Q, NL, A.gn);
     const bool has_tensor = A.cA != nullptr;
     const bool has_mass = A.cm != nullptr;
-    for (int64_t st = 0; st < L.steps; ++st) {
-        if (!L.active) continue;
+    // incidence split: part p of a row takes its incidence layers p, p+S, ...
+    // (the plan pads the layers to a multiple of S) and the S parts add their
+    // element rows in turn, so every entry sums in the row's element order
+    const int64_t nsteps = (L.steps + L.S - 1) / L.S;
+    for (int64_t st = 0; st < nsteps; ++st) {
+        const int64_t layer = st * L.S + L.p;
         Record<RB> rec;
-        load_record<RB>(A.plan.records + (L.rec0 + st * L.R + L.i) * RB, rec);
-        if (rec.packed < 0) continue;
+        rec.packed = -1;
+        if (L.active && layer < L.steps) load_record<RB>(A.plan.records + (L.rec0 + layer * L.R + L.i) * RB, rec);
+        const bool valid = rec.packed >= 0;
+        double Kr[NL];
+#pragma unroll
+        for (int b = 0; b < NL; ++b) Kr[b] = 0.;
+        if (valid) {
         const int64_t e = rec.packed >> 4;
         const int a = rec.packed & 15;
         Geo G;
@@ -1422,9 +1443,6 @@ __global__ void __launch_bounds__(64) k_assemble_h1_general(H1Args A) {
             affine_jacobian(A.xyz, A.vconn + e * 4, Jm);
             adjugate(Jm, G);
         }
-        double Kr[NL];
-#pragma unroll
-        for (int b = 0; b < NL; ++b) Kr[b] = 0.;
         for (int q = 0; q < Q; ++q) {
             if (ISO) {
                 iso_jacobian(A.xyz, A.gconn + e * A.gn, A.gn, T.gdphi + q * A.gn * 3, Jm);
@@ -1449,10 +1467,15 @@ __global__ void __launch_bounds__(64) k_assemble_h1_general(H1Args A) {
                 for (int b = 0; b < NL; ++b) Kr[b] += v * T.phi[q * NL + b];
             }
         }
+        }
+        for (int turn = 0; turn < L.S; ++turn) {
+            if (valid && L.p == turn) {
 #pragma unroll
-        for (int b = 0; b < NL; ++b) {
-            const int o = rec_offset<RB, OFFW>(rec, b);
-            if ((o & (L.S - 1)) == L.p) lds_add(span_lds + L.rb + int64_t(C) * o, Kr[b], A.plan.lds_atomic);
+                for (int b = 0; b < NL; ++b) {
+                    const int o = rec_offset<RB, OFFW>(rec, b);
+                    lds_add(span_lds + L.rb + int64_t(C) * o, Kr[b], A.plan.lds_atomic);
+                }
+            }
         }
     }
     __syncthreads();
