@@ -1364,9 +1364,15 @@ __device__ inline void load_coef(const H1Args& A, int64_t idx, Coef& k) {
 }
 
 template <int C>
-__device__ inline void build_D(const Geo& G, const Coef& k, int c, int d, bool has_tensor, double* D) {
+__device__ inline void build_D(const Geo& G, const Coef& k, int c, int d, bool has_tensor, bool has_scalar,
+                               double* D) {
     const double inv = 1. / fabs(G.det);
     const double* a = G.adj;
+    if (!has_scalar) {
+        // tensor-only forms: no gradient, symmetric-gradient or divergence terms
+#pragma unroll
+        for (int t = 0; t < 9; ++t) D[t] = 0.;
+    } else {
     const double iso = (c == d) ? (k.g + 0.5 * k.s) : 0.;
     // columns c and d of the adjugate: compile-time for one component,
     // else exact 0/1 weights (a runtime register index would go to scratch)
@@ -1387,6 +1393,7 @@ __device__ inline void build_D(const Geo& G, const Coef& k, int c, int d, bool h
             const double aat = a[i * 3 + 0] * a[j * 3 + 0] + a[i * 3 + 1] * a[j * 3 + 1] + a[i * 3 + 2] * a[j * 3 + 2];
             D[i * 3 + j] = iso * aat + 0.5 * k.s * ad[i] * ac[j] + k.lam * ac[i] * ad[j];
         }
+    }
     if (C == 1 && has_tensor) {
         double T[9];
 #pragma unroll
@@ -1418,6 +1425,7 @@ __global__ void __launch_bounds__(64) k_assemble_h1_general(H1Args A) {
     const Tables T = tables_at(A.tables, Q, NL, A.gn);
     const bool has_tensor = A.cA != nullptr;
     const bool has_mass = A.cm != nullptr;
+    const bool has_scalar = A.cg != nullptr || A.cs != nullptr || A.cl != nullptr;
     // incidence split: part p of a row takes its incidence layers p, p+S, ...
     // (the plan pads the layers to a multiple of S) and the S parts add their
     // element rows in turn, so every entry sums in the row's element order
@@ -1449,7 +1457,7 @@ __global__ void __launch_bounds__(64) k_assemble_h1_general(H1Args A) {
                 adjugate(Jm, G);
             }
             load_coef(A, A.per_qp ? e * Q + q : e, k);
-            build_D<C>(G, k, L.c, L.d, has_tensor, D);
+            build_D<C>(G, k, L.c, L.d, has_tensor, has_scalar, D);
             mfac = (L.c == L.d) ? fabs(G.det) * k.m : 0.;
             const double wq = T.w[q];
             const double* ga = T.dphi + (q * NL + a) * 3;
