@@ -591,6 +591,7 @@ class NativeCompositeBilinearForm:
     def __init__(self, basis):
         self.basis = basis
         self._assemblers = {}
+        self._patterns = {}
 
     def _square(self, field):
         key = ("square", field)
@@ -637,9 +638,18 @@ class NativeCompositeBilinearForm:
             coef, strides = native._scalar_field(coefficient)
             native.assemble_cross_device(row_kind, column_kind, mode, coef, strides)
             plan, values = native.cross_plan(), native._cross[1]
-        ip, ix = plan.host_csr_arrays()
+        key = (row_field, column_field, row_field == column_field and not divergence)
+        pattern = self._patterns.get(key)
+        if pattern is None:
+            ip, ix = plan.host_csr_arrays()
+            m = _composite_csr(ip, ix, np.zeros(len(ix)), row._composite_dofs, column._composite_dofs,
+                               self.basis.N)
+            pattern = (m.indptr, m.indices)
+            self._patterns[key] = pattern
         data = to_host(values[:plan.nnz])
-        return _composite_csr(ip, ix, data, row._composite_dofs, column._composite_dofs, self.basis.N)
+        # the relabelled pattern is fixed per block; each result owns a copy
+        return csr_matrix((data, host_copy(pattern[1]), host_copy(pattern[0])), shape=(self.basis.N, self.basis.N),
+                          copy=False)
 
 
 @dataclass(frozen=True)
