@@ -37,6 +37,8 @@ import numpy as np  # noqa: E402
 METRIC = "assembled elements/s + % HBM roofline; learned-FSAI vs Jacobi PCG time to 1e-8"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_ELEMENT_C2 = 825.7  # algorithmic bytes per tet of one FP64 assembly
+# rocprofv3 FETCH_SIZE/WRITE_SIZE summaries of the other workloads at their default sizes
+TRAFFIC_FILES = {"th": "th_divergence_block_64_hbm_traffic.json"}
 
 
 def log(msg):
@@ -337,6 +339,12 @@ def other_workload(name, steps, warmup, L, ev0, ev1, stream):
            "roofline": {"bound": "hbm", "achieved": bpe * E / sec / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": bpe * E / sec / 1e9 / HBM_PEAK_GBS, "traffic": None},
            "nnz_per_s": nnz / sec, "setup_s": setup}
+    # measured HBM bytes of one launch (FETCH_SIZE + WRITE_SIZE, calibrated
+    # on a copy of known size), from the committed rocprofv3 summary
+    traffic_file = os.path.join(ROOT, "profiles", f"{TRAFFIC_FILES.get(name, '')}")
+    if name in TRAFFIC_FILES and os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            out["roofline"]["traffic"] = json.load(f)["per_assembly_bytes"]
     if name in ("c1", "c3"):
         sec32 = timed(lambda: step("fp32"), steps, warmup, L, ev0, ev1, stream)
         step("fp64")
