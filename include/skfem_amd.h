@@ -242,6 +242,50 @@ int skfa_assemble_facet_bilinear(const skfa_plan* plan, const int32_t* conn, con
                                  const double* face_shape, int32_t nq, const double* weights, double* values,
                                  void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Cut-cell (ragged quadrature) forms.  Cell e owns the points
+ * [offsets[e], offsets[e+1]) of flat per-point tables: weights (P), shape
+ * values (P, n) and physical gradients (P, n, dim).  The plan is
+ * skfa_cross_plan_create on the (cells, n) node connectivity of the cells
+ * that have points (DOF = node * components + component), row and column
+ * connectivity equal for the square forms; offsets are those cells' point
+ * ranges (cells + 1 int64).
+ *
+ * Bilinear forms into the CSR values of the plan (nnz FP64, overwritten).
+ * Replaces CutBilinearFormAssembler::assemble
+ * (cpp/src/assembly/cut_form_assembler.cpp:136-205) and
+ * CutCrossAssembler::assemble (cpp/src/assembly/cut_cross_assembler.cpp:70-137,
+ * contraction cpp/include/native_fem/cross_contraction.hpp:42-97).
+ *   terms   SKFA_CUT_VALUE and/or SKFA_CUT_GRADIENT: entry (a,c),(b,c) =
+ *           w_q (value_q phi_a psi_b + gradient_q grad phi_a . grad psi_b)
+ *           summed over the cell's points, other component pairs zero (value
+ *           or gradient NULL = 1); needs equal component counts;
+ *           SKFA_CUT_TENSOR: entry (a,r),(b,c) = w_q sum_ij h_ai T[q,r,(i),c,(j)]
+ *           h_bj with h = phi or grad phi by row_kind / col_kind and T the
+ *           (P, Cr, [row_dim], Cc, [col_dim]) tensor.
+ *   one_lane_per_row  1 when a cell repeats a column node (one lane then adds
+ *           all of a row's entries), else 0.
+ * Each entry adds its cells in cell order; results are deterministic. */
+#define SKFA_CUT_VALUE 1
+#define SKFA_CUT_GRADIENT 2
+#define SKFA_CUT_TENSOR 4
+int skfa_assemble_cut_bilinear(const skfa_cross_plan* plan, const int64_t* offsets, const double* weights,
+                               const double* row_shape, const double* row_grad, int32_t row_dim,
+                               const double* col_shape, const double* col_grad, int32_t col_dim, int32_t terms,
+                               const double* value, const double* gradient, int32_t row_kind, int32_t col_kind,
+                               const double* tensor, int32_t one_lane_per_row, double* values, void* stream);
+
+/* Cut-cell load vector out (row nodes * components FP64, overwritten):
+ * out[node*C + c] = sum over the node's cells of sum_q w_q (phi_a(q) f_c(q) +
+ * grad phi_a(q) . G_c(q)).  Replaces CutLinearFormAssembler::assemble
+ * (cpp/src/assembly/cut_form_assembler.cpp:67-134).
+ *   value     (C) constant or (P, C) per point (value_per_point = 1); NULL = no term
+ *   gradient  (C, dim) constant or (P, C, dim) per point; NULL = no term */
+int skfa_assemble_cut_linear(const skfa_cross_plan* plan, const int64_t* offsets, const double* weights,
+                             const double* shape, const double* grad, int32_t dim, const double* value,
+                             int32_t value_per_point, const double* gradient, int32_t gradient_per_point,
+                             double* out, void* stream);
+
 /* Per (e,q) geometry: replaces tabulate_basis_geometry
  * (cpp/src/assembly/basis_geometry.cpp:49-176).  Outputs (E,Q[,...]) FP64,
  * each optional (NULL = not written): det, tol (64 eps max|J|^3), cond,

@@ -48,6 +48,10 @@ PARITY = {
     "c.edge_topology": "reference C++ (_ref) on the same inputs, bit-exact",
     "c.integrate_tet_n1": "reference C++ (_ref) on the same inputs",
     "c.scatter_add": "numpy.add.at on the same inputs",
+    "cut.cut_pattern": "reference C++ (_ref) CutBilinearFormAssembler/CutCrossAssembler, bit-exact",
+    "cut.cut_bilinear": "reference C++ (_ref) on the same inputs + partition of unity, zero stiffness row sums",
+    "cut.cut_linear": "reference C++ (_ref) on the same inputs + partition of unity, K x = load of grad x",
+    "cut.cut_cross": "reference C++ (_ref) CutCrossAssembler on the same inputs (scalar and tensor)",
 }
 
 _lib = None

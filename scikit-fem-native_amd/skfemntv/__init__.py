@@ -18,6 +18,7 @@ from .hcurl import (AffineTetN1Basis, HcurlTetGeometryDiagnostics, TetN1Assemble
                     estimate_tet_n1_assembly_memory)
 from .mesh import MeshTet, MeshTet2
 from .facet import FacetBasis, NativeFacetBilinearForm, NativeFacetLinearForm
+from .cut import CutBilinearFormAssembler, CutCrossAssembler, CutLinearFormAssembler
 from .runtime import (BACKEND, CAPABILITIES, available_num_threads, get_num_threads, has_capability,
                       set_num_threads, thread_limit)
 from ._native import NativeUnavailable, NativeError
@@ -31,6 +32,7 @@ __all__ = [
     "LinearFormDiagnostics", "NativeLinearForm", "NativeBilinearForm", "NativeCrossBilinearForm",
     "NativeCompositeBilinearForm", "NativeCompositeLinearForm", "ElementComposite", "FacetBasis",
     "NativeFacetLinearForm", "NativeFacetBilinearForm",
+    "CutLinearFormAssembler", "CutBilinearFormAssembler", "CutCrossAssembler",
     "LinearForm", "BilinearForm", "Basis", "GeometryDiagnostics", "ElementTetP1", "ElementTetP2",
     "ElementVector", "MeshTet", "MeshTet2", "asm", "UnsupportedNativeForm", "NativeUnavailable", "NativeError",
 ]

@@ -19,6 +19,7 @@ SYMBOLS = (
     "skfa_plan_create", "skfa_plan_create_budget", "skfa_plan_set_tuning", "skfa_plan_destroy", "skfa_plan_info", "skfa_plan_export_csr", "skfa_plan_export_map",
     "skfa_cross_plan_create", "skfa_cross_plan_destroy", "skfa_cross_plan_info", "skfa_cross_plan_export_csr",
     "skfa_assemble_cross", "skfa_assemble_facet_linear", "skfa_assemble_facet_bilinear",
+    "skfa_assemble_cut_bilinear", "skfa_assemble_cut_linear",
     "skfa_assemble_bilinear", "skfa_assemble_linear", "skfa_tabulate_geometry",
     "skfa_edge_topology", "skfa_assemble_tet_n1",
     "skfa_event_create", "skfa_event_destroy", "skfa_event_record", "skfa_event_elapsed_ms",
@@ -79,6 +80,9 @@ def _declare(lib):
         "skfa_assemble_cross": (ctypes.c_int, [P, P, P, P, I32, P, I32, P, I32, I32, I32, P, I64, I64, P, P]),
         "skfa_assemble_facet_linear": (ctypes.c_int, [P, P, P, I32, P, P, I32, P]),
         "skfa_assemble_facet_bilinear": (ctypes.c_int, [P, P, P, P, I32, P, P, P]),
+        "skfa_assemble_cut_bilinear": (ctypes.c_int, [P, P, P, P, P, I32, P, P, I32, I32, P, P, I32, I32, P,
+                                                      I32, P, P]),
+        "skfa_assemble_cut_linear": (ctypes.c_int, [P, P, P, P, P, I32, P, I32, P, I32, P, P]),
         "skfa_assemble_bilinear": (ctypes.c_int, [P, P, I64, P, P, I32, P, I32, P, P, I32, I32,
                                                   P, P, P, P, P, I32, I32, I32, P, P]),
         "skfa_assemble_linear": (ctypes.c_int, [P, P, P, P, I32, P, I32, P, I32, P, I32, P, P]),
