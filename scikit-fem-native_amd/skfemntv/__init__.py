@@ -19,6 +19,7 @@ from .hcurl import (AffineTetN1Basis, HcurlTetGeometryDiagnostics, TetN1Assemble
 from .mesh import MeshTet, MeshTet2
 from .facet import FacetBasis, NativeFacetBilinearForm, NativeFacetLinearForm
 from .cut import CutBilinearFormAssembler, CutCrossAssembler, CutLinearFormAssembler
+from .cut_basis import CutCellBasis, CutCellQuadrature, DiscreteField
 from .runtime import (BACKEND, CAPABILITIES, available_num_threads, get_num_threads, has_capability,
                       set_num_threads, thread_limit)
 from ._native import NativeUnavailable, NativeError
@@ -32,7 +33,8 @@ __all__ = [
     "LinearFormDiagnostics", "NativeLinearForm", "NativeBilinearForm", "NativeCrossBilinearForm",
     "NativeCompositeBilinearForm", "NativeCompositeLinearForm", "ElementComposite", "FacetBasis",
     "NativeFacetLinearForm", "NativeFacetBilinearForm",
-    "CutLinearFormAssembler", "CutBilinearFormAssembler", "CutCrossAssembler",
+    "CutLinearFormAssembler", "CutBilinearFormAssembler", "CutCrossAssembler", "CutCellBasis",
+    "CutCellQuadrature", "DiscreteField",
     "LinearForm", "BilinearForm", "Basis", "GeometryDiagnostics", "ElementTetP1", "ElementTetP2",
     "ElementVector", "MeshTet", "MeshTet2", "asm", "UnsupportedNativeForm", "NativeUnavailable", "NativeError",
 ]

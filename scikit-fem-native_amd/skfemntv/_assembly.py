@@ -235,6 +235,10 @@ class NativeBilinearForm:
 
     def __init__(self, basis, *, memory_limit_bytes=None, memory_safety_factor=1.25):
         self.basis = basis
+        self._cut = None
+        if hasattr(basis, "cell_offsets"):
+            self._init_cut(basis, memory_limit_bytes, memory_safety_factor)
+            return
         self.memory_estimate = estimate_memory_upper(basis)
         enforce_memory_budget(self.memory_estimate, memory_limit_bytes, memory_safety_factor)
         self.plan = basis_plan(basis)
@@ -252,12 +256,49 @@ class NativeBilinearForm:
         self._coefficient_shape = basis.coefficient_shape
         self.last_kernel_seconds = 0.
 
+    def _init_cut(self, basis, memory_limit_bytes, memory_safety_factor):
+        """Cut-cell basis (bilinear_form.py:33-41): the ragged-quadrature
+        assembler over the cells that have points."""
+        from .cut import CutBilinearFormAssembler
+        self.memory_estimate = estimate_cut_memory(basis.active_cell_dofs, basis.active_cell_dofs, basis.N,
+                                                   basis.N, "bilinear")
+        enforce_memory_budget(self.memory_estimate, memory_limit_bytes, memory_safety_factor)
+        self._cut = CutBilinearFormAssembler(basis.active_cell_dofs, basis.active_cell_offsets, basis.shape,
+                                             basis.gradients, basis.weights)
+        c = self._cut
+        self._matrix = csr_matrix((c.values, c.indices, c.indptr), shape=(c.ndofs, c.ndofs), copy=False)
+        self._matrix.resize((basis.N, basis.N))
+        self._shares = np.shares_memory(self._matrix.data, c.values)
+        self._coefficient_shape = basis.coefficient_shape
+        self.last_kernel_seconds = 0.
+
+    def _assemble_cut(self, value, gradient, symmetric_gradient, divergence):
+        if symmetric_gradient is not None or divergence is not None:
+            raise ValueError("cut-basis symmetric-gradient and divergence assembly is not supported")
+        coefs = []
+        for name, c in (("value", value), ("gradient", gradient)):
+            if c is None:
+                coefs.append(None)
+                continue
+            c = np.asarray(c, dtype=np.float64)
+            try:
+                c = np.broadcast_to(c, self._coefficient_shape)
+            except ValueError as error:
+                raise ValueError(f"{name} coefficient must broadcast to {self._coefficient_shape}") from error
+            coefs.append(np.ascontiguousarray(c).reshape(-1))
+        self._cut.assemble(coefs[0], coefs[1])
+        if not self._shares:
+            np.copyto(self._matrix.data, self._cut.values)
+        return self._matrix
+
     def assemble(self, *, value=None, gradient=None, symmetric_gradient=None, divergence=None, num_threads=0,
                  precision="fp64"):
         coefs = {"value": value, "gradient": gradient, "symmetric_gradient": symmetric_gradient,
                  "divergence": divergence}
         if all(v is None for v in coefs.values()):
             raise ValueError("at least one coefficient is required")
+        if self._cut is not None:
+            return self._assemble_cut(value, gradient, symmetric_gradient, divergence)
         dev, per_qp = _prepare_scalar_kinds(self.basis, coefs)
         self.assemble_device(per_qp=per_qp, precision=precision, **dev)
         if self._pinned is not None:
@@ -363,6 +404,10 @@ class NativeCrossBilinearForm:
     kernels; every other pairing runs the rectangular cross kernel."""
 
     def __init__(self, test_basis, trial_basis, *, memory_limit_bytes=None, memory_safety_factor=1.25):
+        self._cutx = None
+        if hasattr(test_basis, "cell_offsets") and hasattr(trial_basis, "cell_offsets"):
+            self._init_cut(test_basis, trial_basis, memory_limit_bytes, memory_safety_factor)
+            return
         if test_basis is not trial_basis:
             if len(test_basis.W) != len(trial_basis.W) or test_basis.mesh.nelements != trial_basis.mesh.nelements:
                 raise ValueError("trial and test quadrature shapes must match")
@@ -383,6 +428,46 @@ class NativeCrossBilinearForm:
         self._square = None
         self._cross = None
         self._rtabs = {}
+
+    def _init_cut(self, test_basis, trial_basis, memory_limit_bytes, memory_safety_factor):
+        """Two cut-cell bases on the same points (bilinear_form.py:134-155)."""
+        from .cut import CutCrossAssembler
+        if test_basis.dx.shape != trial_basis.dx.shape:
+            raise ValueError("trial and test quadrature shapes must match")
+        if not np.allclose(test_basis.dx, trial_basis.dx):
+            raise ValueError("trial and test quadrature weights must match")
+        if not np.allclose(test_basis.global_coordinates, trial_basis.global_coordinates):
+            raise ValueError("trial and test quadrature coordinates must match")
+        if not np.array_equal(test_basis.active_cell_offsets, trial_basis.active_cell_offsets):
+            raise ValueError("cut cross bases have different cell offsets")
+        self.test_basis, self.trial_basis = test_basis, trial_basis
+        self.basis = test_basis
+        self.memory_estimate = estimate_cut_memory(test_basis.active_cell_dofs, trial_basis.active_cell_dofs,
+                                                   test_basis.N, trial_basis.N, "cross")
+        enforce_memory_budget(self.memory_estimate, memory_limit_bytes, memory_safety_factor)
+        self._cutx = CutCrossAssembler(test_basis.active_cell_dofs, trial_basis.active_cell_dofs,
+                                       test_basis.active_cell_offsets, test_basis.shape, trial_basis.shape,
+                                       test_basis.weights, test_basis.gradients, trial_basis.gradients)
+        self.shape = (test_basis.N, trial_basis.N)
+        self.coefficient_shape = test_basis.coefficient_shape
+        self.test_components = test_basis.elem._dim
+        self.trial_components = trial_basis.elem._dim
+        self.dimension = 3
+        self._square = self._cross = None
+        self._rtabs = {}
+
+    def _cut_assemble(self, row_kind, column_kind, coefficient):
+        """CutCrossAssembler.assemble with a (P,) scalar or a
+        (P, Cr, [3], Cc, [3]) tensor; a fresh CSR copy per call."""
+        c = self._cutx
+        c.assemble(coefficient, row_kind, column_kind)
+        m = csr_matrix((c.values.copy(), c.indices.copy(), c.indptr.copy()), shape=(c.rows, c.columns))
+        m.resize(self.shape)
+        return m
+
+    def _cut_scalar(self, coefficient):
+        arr = np.broadcast_to(np.asarray(coefficient, dtype=np.float64), self.coefficient_shape)
+        return np.ascontiguousarray(arr).reshape(-1)
 
     # ---------------------------------------------------------- plans
     @property
@@ -489,6 +574,16 @@ class NativeCrossBilinearForm:
             mode = "scalar"
         else:
             raise UnsupportedNativeForm(f"cross-basis assembly does not support the {kind!r} kind")
+        if self._cutx is not None:
+            scalar = self._cut_scalar(coefficient)
+            if mode == "scalar":
+                return self._cut_assemble(row_kind, column_kind, scalar)
+            # the reference's divergence tensor: T[c,c,0] (row) or T[0,c,c] (column)
+            C = self.dimension
+            unit = np.zeros((C, C, 1) if mode == "row_divergence" else (1, C, C))
+            for k in range(C):
+                unit[(k, k, 0) if mode == "row_divergence" else (0, k, k)] = 1.
+            return self._cut_assemble(row_kind, column_kind, scalar[:, None, None, None] * unit)
         coef, strides = self._scalar_field(coefficient)
         self.assemble_cross_device(row_kind, column_kind, mode, coef, strides)
         return self._cross_matrix()
@@ -506,6 +601,8 @@ class NativeCrossBilinearForm:
         if column_kind == "normal_gradient":
             raise ValueError("column normal_gradient requires trial-basis normals")
         if row_kind == column_kind:
+            if self._cutx is not None:
+                return self._cut_assemble(row_kind, column_kind, self._cut_scalar(coefficient))
             coef, strides = self._scalar_field(coefficient)
             self.assemble_cross_device(row_kind, column_kind, "scalar", coef, strides)
             return self._cross_matrix()
@@ -536,6 +633,8 @@ class NativeCrossBilinearForm:
             coefficient = np.broadcast_to(coefficient, expected)
         except ValueError as error:
             raise ValueError(f"tensor coefficient must broadcast to {expected}") from error
+        if self._cutx is not None:
+            return self._cut_assemble(row_kind, column_kind, np.ascontiguousarray(coefficient[:, 0]))
         square = (self.test_basis is self.trial_basis and (row_kind, column_kind) == ("gradient", "gradient")
                   and self.test_components == 1)
         if square:
@@ -664,6 +763,17 @@ class NativeLinearForm:
 
     def __init__(self, basis):
         self.basis = basis
+        self._cut = None
+        if hasattr(basis, "cell_offsets"):
+            # cut-cell basis (linear_form.py:33-41)
+            from .cut import CutLinearFormAssembler
+            self._cut = CutLinearFormAssembler(basis.active_cell_dofs, basis.active_cell_offsets, basis.shape,
+                                               basis.gradients, basis.weights)
+            P = len(basis.weights)
+            C = basis.elem._dim
+            self._shape = (P, 1, C)
+            self._gradient_shape = (P, 1, C, 3)
+            return
         self.plan = basis_plan(basis)
         E, Q = basis.coefficient_shape
         C = basis.elem._dim
@@ -674,7 +784,7 @@ class NativeLinearForm:
 
     @property
     def ndofs(self):
-        return self.plan.nrows
+        return self._cut.ndofs if self._cut is not None else self.plan.nrows
 
     @staticmethod
     def _coefficient(name, coefficient, shape):
@@ -696,6 +806,15 @@ class NativeLinearForm:
         G, Gq = self._coefficient("gradient", gradient, self._gradient_shape)
         if f is None and G is None:
             raise ValueError("at least one coefficient is required")
+        if self._cut is not None:
+            if f is not None and f.ndim == 3:
+                f = f[:, 0, :]
+            if G is not None and G.ndim == 4:
+                G = G[:, 0, :, :]
+            result, seconds = self._cut.assemble(f, G)
+            return result, LinearFormDiagnostics(entity_count=self._cut.cell_count,
+                                                 quadrature_point_count=self._cut.point_count,
+                                                 assembly_seconds=seconds)
         start = time.perf_counter()
         out = self.assemble_device(upload(f) if f is not None else None, fq,
                                    upload(G) if G is not None else None, Gq)
@@ -756,6 +875,17 @@ def estimate_cross_memory(test_basis, trial_basis):
     nnz = min(Nr * Nc, E * Lr * Lc)
     b = 8 * (Nr + 1) + 12 * nnz + 32 * E * nr + 4 * E * nr + 8 * (Nr + 1)
     return AssemblyMemoryEstimate("cross", Nr, Nc, nnz, int(b))
+
+
+def estimate_cut_memory(row_dofs, col_dofs, N_rows, N_cols, kind):
+    """Upper bound of the device bytes of one cut assembler: pattern +
+    values (nnz <= cells Lr Lc), one record per row incidence and the
+    incidence lists."""
+    cells, nr, Cr = row_dofs.shape
+    _, nc, Cc = col_dofs.shape
+    nnz = min(N_rows * N_cols, cells * nr * Cr * nc * Cc)
+    b = 8 * (N_rows + 1) + 12 * nnz + 36 * cells * nr + 8 * (N_rows + 1)
+    return AssemblyMemoryEstimate(kind, N_rows, N_cols, nnz, int(b))
 
 
 def enforce_memory_budget(estimate, limit, safety_factor=1.25):

@@ -128,3 +128,94 @@ def test_cut_edge_cases():
     Lf = sk.CutLinearFormAssembler(dofs, z, e, np.zeros((0, 10, 3)), np.zeros(0))
     out, _ = Lf.assemble(np.ones(1))
     assert out.shape == (int(dofs.max()) + 1,) and not out.any()
+
+
+def _full_domain_cut(basis, keep=None):
+    """Cut quadrature whose points in every kept cell are the basis's own
+    rule, mapped affinely (the reference's full-domain cut-basis test)."""
+    import skfemntv as sk
+    m = basis.mesh
+    E = m.nelements
+    X, W = basis.X, basis.W
+    Q = len(W)
+    keep = np.ones(E, dtype=bool) if keep is None else keep
+    counts = np.where(keep, Q, 0)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    corners = m.p[:, m.t[:4]]
+    det = np.abs(np.linalg.det(np.moveaxis(corners[:, 1:4] - corners[:, [0]], -1, 0)))
+    Xp = np.tile(X.T, (int(keep.sum()), 1))
+    w = (det[keep][:, None] * W[None, :]).ravel()
+    return sk.CutCellBasis(basis, sk.CutCellQuadrature.from_reference_points(m, off, Xp, w))
+
+
+def _mesh(quadratic, n=3, seed=1):
+    import skfemntv as sk
+    from oracle import host
+    ax = np.linspace(0., 1., n + 1)
+    p, t = host.kuhn_mesh(ax, ax, ax)
+    p = host.jitter_interior(p, 1. / n, 0.2, seed)
+    if quadratic:
+        p2, t2 = host.p2_mesh(p, t)
+        return sk.MeshTet2(p2, t2)
+    return sk.MeshTet(p, t)
+
+
+@pytest.mark.parametrize("quadratic,components", [(False, 1), (True, 3)])
+def test_cut_basis_full_domain_asm_matches_regular_basis(quadratic, components):
+    import skfemntv as sk
+    from skfemntv.helpers import ddot, dot, grad
+    say(f"cut basis through asm, full domain, quadratic={quadratic} C={components}")
+    mesh = _mesh(quadratic)
+    elem = sk.ElementVector(sk.ElementTetP2() if quadratic else sk.ElementTetP1(), dim=components)
+    regular = sk.Basis(mesh, elem)
+    cutb = _full_domain_cut(regular)
+    assert cutb.nelems == mesh.nelements and cutb.npoints == mesh.nelements * len(regular.W)
+    mass_stiffness = sk.BilinearForm(lambda u, v, w: (1. + w.x[0]) * dot(u, v) + ddot(grad(u), grad(v)))
+    load = sk.LinearForm(lambda v, w: dot(w.force, v))
+    force = np.arange(1., components + 1.)
+    A_cut = sk.asm(mass_stiffness, cutb).toarray()
+    A_reg = sk.asm(mass_stiffness, regular).toarray()
+    assert rel(A_cut, A_reg) <= FP64_TOL
+    f_cut = sk.asm(load, cutb, force=force)
+    f_reg = sk.asm(load, regular, force=force)
+    assert f_cut.shape == (regular.N,) and rel(f_cut, f_reg) <= FP64_TOL
+    # interpolation reproduces the field's own values and gradients
+    u = 2. + 3. * regular.doflocs[0] - regular.doflocs[1]
+    field = cutb.interpolate(u)
+    x = cutb.points
+    want = 2. + 3. * x[:, 0] - x[:, 1]
+    got = field.value if components == 1 else field.value[0]
+    assert np.abs(got - want).max() < 1e-12
+    with pytest.raises(ValueError, match="symmetric-gradient"):
+        sk.NativeBilinearForm(cutb).assemble(symmetric_gradient=1.)
+
+
+def test_cut_basis_partial_domain_and_cross_blocks():
+    import skfemntv as sk
+    from skfemntv.helpers import div, dot
+    say("cut basis: half the cells, Taylor-Hood divergence block and P1 mass")
+    mesh = _mesh(True, n=3, seed=3)
+    from skfemntv.basis import tet_rule_keast11
+    X, W = tet_rule_keast11()
+    vp2 = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP2(), dim=3), quadrature=(X, W))
+    p1 = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP1(), dim=1), quadrature=(X, W))
+    centroid = mesh.p[:, mesh.t[:4]].mean(axis=1)
+    keep = centroid[0] < 0.5
+    cut_v, cut_p = _full_domain_cut(vp2, keep), _full_domain_cut(p1, keep)
+    coldiv = sk.BilinearForm(lambda u, v, w: div(u) * v)
+    B = sk.asm(coldiv, cut_v, cut_p).toarray()
+    # the same block from regular bases, restricted to the kept cells by a
+    # per-element coefficient that is 1 on kept cells and 0 elsewhere
+    c = np.broadcast_to(keep[:, None].astype(float), (mesh.nelements, len(W)))
+    coldiv_c = sk.BilinearForm(lambda u, v, w: w.c * (div(u) * v))
+    B_reg = sk.asm(coldiv_c, vp2, p1, c=c).toarray()
+    assert B.shape == (p1.N, vp2.N)
+    assert rel(B, B_reg) <= FP64_TOL
+    mass = sk.BilinearForm(lambda u, v, w: dot(u, v))
+    mass_c = sk.BilinearForm(lambda u, v, w: w.c * dot(u, v))
+    M = sk.asm(mass, cut_p).toarray()
+    M_reg = sk.asm(mass_c, p1, c=c).toarray()
+    assert rel(M, M_reg) <= FP64_TOL
+    # the cut pattern holds only the kept cells' DOF pairs
+    kept_nodes = np.unique(p1.element_dofs[:, keep])
+    assert set(np.flatnonzero(np.abs(M).sum(axis=1) > 0)) <= set(kept_nodes.tolist())
