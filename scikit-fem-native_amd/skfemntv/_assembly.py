@@ -405,8 +405,9 @@ class NativeCrossBilinearForm:
 
     def __init__(self, test_basis, trial_basis, *, memory_limit_bytes=None, memory_safety_factor=1.25):
         self._cutx = None
-        if hasattr(test_basis, "cell_offsets") and hasattr(trial_basis, "cell_offsets"):
-            self._init_cut(test_basis, trial_basis, memory_limit_bytes, memory_safety_factor)
+        facet = hasattr(test_basis, "parent_elements") and hasattr(trial_basis, "parent_elements")
+        if facet or (hasattr(test_basis, "cell_offsets") and hasattr(trial_basis, "cell_offsets")):
+            self._init_cut(test_basis, trial_basis, memory_limit_bytes, memory_safety_factor, facet)
             return
         if test_basis is not trial_basis:
             if len(test_basis.W) != len(trial_basis.W) or test_basis.mesh.nelements != trial_basis.mesh.nelements:
@@ -429,8 +430,10 @@ class NativeCrossBilinearForm:
         self._cross = None
         self._rtabs = {}
 
-    def _init_cut(self, test_basis, trial_basis, memory_limit_bytes, memory_safety_factor):
-        """Two cut-cell bases on the same points (bilinear_form.py:134-155)."""
+    def _init_cut(self, test_basis, trial_basis, memory_limit_bytes, memory_safety_factor, facet=False):
+        """Two cut-cell bases on the same points (bilinear_form.py:134-155),
+        or two facet bases on the same facets, whose facets are cells of Q
+        points each: both run the ragged-quadrature cross kernel."""
         from .cut import CutCrossAssembler
         if test_basis.dx.shape != trial_basis.dx.shape:
             raise ValueError("trial and test quadrature shapes must match")
@@ -438,16 +441,26 @@ class NativeCrossBilinearForm:
             raise ValueError("trial and test quadrature weights must match")
         if not np.allclose(test_basis.global_coordinates, trial_basis.global_coordinates):
             raise ValueError("trial and test quadrature coordinates must match")
-        if not np.array_equal(test_basis.active_cell_offsets, trial_basis.active_cell_offsets):
-            raise ValueError("cut cross bases have different cell offsets")
+        if facet:
+            if not np.array_equal(test_basis.facet_ids, trial_basis.facet_ids):
+                raise ValueError("trial and test facet bases must cover the same facets")
+            rt, rc = test_basis.ragged(), trial_basis.ragged()
+            rows = (rt["dofs"], rt["offsets"], rt["shape"], rt["gradients"], rt["weights"], rt["normals"])
+            cols = (rc["dofs"], rc["shape"], rc["gradients"], rc["normals"])
+        else:
+            if not np.array_equal(test_basis.active_cell_offsets, trial_basis.active_cell_offsets):
+                raise ValueError("cut cross bases have different cell offsets")
+            rows = (test_basis.active_cell_dofs, test_basis.active_cell_offsets, test_basis.shape,
+                    test_basis.gradients, test_basis.weights, test_basis.normal_vectors)
+            cols = (trial_basis.active_cell_dofs, trial_basis.shape, trial_basis.gradients,
+                    trial_basis.normal_vectors)
         self.test_basis, self.trial_basis = test_basis, trial_basis
         self.basis = test_basis
-        self.memory_estimate = estimate_cut_memory(test_basis.active_cell_dofs, trial_basis.active_cell_dofs,
-                                                   test_basis.N, trial_basis.N, "cross")
+        self.memory_estimate = estimate_cut_memory(rows[0], cols[0], test_basis.N, trial_basis.N, "cross")
         enforce_memory_budget(self.memory_estimate, memory_limit_bytes, memory_safety_factor)
-        self._cutx = CutCrossAssembler(test_basis.active_cell_dofs, trial_basis.active_cell_dofs,
-                                       test_basis.active_cell_offsets, test_basis.shape, trial_basis.shape,
-                                       test_basis.weights, test_basis.gradients, trial_basis.gradients)
+        self._cutx = CutCrossAssembler(rows[0], cols[0], rows[1], rows[2], cols[1], rows[4], rows[3], cols[2])
+        self._cut_normals = (np.asarray(rows[5], dtype=np.float64).reshape(-1, 3),
+                             np.asarray(cols[3], dtype=np.float64).reshape(-1, 3))
         self.shape = (test_basis.N, trial_basis.N)
         self.coefficient_shape = test_basis.coefficient_shape
         self.test_components = test_basis.elem._dim
@@ -468,6 +481,34 @@ class NativeCrossBilinearForm:
     def _cut_scalar(self, coefficient):
         arr = np.broadcast_to(np.asarray(coefficient, dtype=np.float64), self.coefficient_shape)
         return np.ascontiguousarray(arr).reshape(-1)
+
+    def _cut_kinds(self, row_kind, column_kind, coefficient):
+        """Value / gradient / normal-gradient contractions on ragged points
+        (bilinear_form.py:243-325): the scalar times unit directions, or the
+        basis normals for a normal-gradient side, as a contraction tensor."""
+        s = self._cut_scalar(coefficient)
+        rg, cg = row_kind != "value", column_kind != "value"
+        rn = self._cut_normals[0] if row_kind == "normal_gradient" else None
+        cn = self._cut_normals[1] if column_kind == "normal_gradient" else None
+        if rg == cg and rn is None and cn is None:
+            return self._cut_assemble(row_kind, column_kind, s)
+        P, C, D = len(s), self.test_components, self.dimension
+        T = np.zeros((P, C) + ((D,) if rg else ()) + (C,) + ((D,) if cg else ()))
+        if rg and cg:
+            # factor[i, j] = (n_i or delta_ij) * (m_j or 1)
+            left = rn[:, :, None] * np.ones((1, 1, D)) if rn is not None else np.broadcast_to(np.eye(D), (P, D, D))
+            factor = left * (cn[:, None, :] if cn is not None else 1.)
+            for c in range(C):
+                T[:, c, :, c, :] = s[:, None, None] * factor
+        elif rg:
+            direction = rn if rn is not None else np.ones((P, D))
+            for c in range(C):
+                T[:, c, :, c] = s[:, None] * direction
+        else:
+            direction = cn if cn is not None else np.ones((P, D))
+            for c in range(C):
+                T[:, c, c, :] = s[:, None] * direction
+        return self._cut_assemble("gradient" if rg else "value", "gradient" if cg else "value", T)
 
     # ---------------------------------------------------------- plans
     @property
@@ -596,6 +637,8 @@ class NativeCrossBilinearForm:
             raise ValueError("cross kind must be value, gradient, or normal_gradient")
         if self.test_components != self.trial_components:
             raise ValueError("cross contractions require matching components")
+        if self._cutx is not None:
+            return self._cut_kinds(row_kind, column_kind, coefficient)
         if row_kind == "normal_gradient":
             raise ValueError("row normal_gradient requires test-basis normals")
         if column_kind == "normal_gradient":
@@ -634,7 +677,8 @@ class NativeCrossBilinearForm:
         except ValueError as error:
             raise ValueError(f"tensor coefficient must broadcast to {expected}") from error
         if self._cutx is not None:
-            return self._cut_assemble(row_kind, column_kind, np.ascontiguousarray(coefficient[:, 0]))
+            return self._cut_assemble(row_kind, column_kind,
+                                      np.ascontiguousarray(coefficient).reshape((-1,) + expected[2:]))
         square = (self.test_basis is self.trial_basis and (row_kind, column_kind) == ("gradient", "gradient")
                   and self.test_components == 1)
         if square:

@@ -128,6 +128,21 @@ class FacetBasis:
     def coefficient_shape(self):
         return self.dx.shape
 
+    def ragged(self):
+        """The facets as cut-style cells of Q points each: DOFs (F, n, C),
+        point offsets, per-point shape values, physical gradients, weights
+        and normals, facet-major (the arguments of the ragged-quadrature
+        assemblers in ``skfemntv.cut``)."""
+        F, Q = self.dx.shape
+        n = self.tabulated_shape.shape[2]
+        C = self.elem._dim
+        return {"dofs": np.ascontiguousarray(self.element_dofs.T.reshape(F, n, C), dtype=np.int64),
+                "offsets": np.arange(F + 1, dtype=np.int64) * Q,
+                "shape": np.ascontiguousarray(self.tabulated_shape.reshape(F * Q, n)),
+                "gradients": np.ascontiguousarray(self.tabulated_gradients.reshape(F * Q, n, 3)),
+                "weights": np.ascontiguousarray(self.dx.reshape(F * Q)),
+                "normals": np.ascontiguousarray(self.normals.reshape(F * Q, 3))}
+
     @property
     def nelems(self):
         return self.dx.shape[0]
@@ -167,12 +182,38 @@ class NativeFacetLinearForm:
     def ndofs(self):
         return self.basis.N
 
+    def _assemble_ragged(self, value, gradient):
+        """Gradient terms G:grad(v): the facets as cells of Q points through
+        the ragged-quadrature load-vector kernel."""
+        import time
+        from ._assembly import LinearFormDiagnostics
+        from .cut import CutLinearFormAssembler
+        F, Q, C = self._shape
+        if getattr(self, "_cut", None) is None:
+            r = self.basis.ragged()
+            self._cut = CutLinearFormAssembler(r["dofs"], r["offsets"], r["shape"], r["gradients"], r["weights"])
+        start = time.perf_counter()
+        f = None
+        if value is not None:
+            try:
+                f = np.broadcast_to(np.asarray(value, dtype=np.float64), (F, Q, C)).reshape(F * Q, C)
+            except ValueError as error:
+                raise ValueError(f"value coefficient must broadcast to {(F, Q, C)}") from error
+        try:
+            G = np.broadcast_to(np.asarray(gradient, dtype=np.float64), (F, Q, C, 3)).reshape(F * Q, C, 3)
+        except ValueError as error:
+            raise ValueError(f"gradient coefficient must broadcast to {(F, Q, C, 3)}") from error
+        out, _ = self._cut.assemble(f, G)
+        result = np.zeros(self.basis.N, dtype=np.float64)
+        result[:out.shape[0]] = out
+        seconds = time.perf_counter() - start
+        return result, LinearFormDiagnostics(entity_count=F, quadrature_point_count=Q, assembly_seconds=seconds)
+
     def assemble(self, *, value=None, gradient=None, num_threads=0):
         import time
         from ._assembly import LinearFormDiagnostics, to_host, upload
-        from ._symbols import UnsupportedNativeForm
         if gradient is not None:
-            raise UnsupportedNativeForm("facet linear forms take value coefficients in the MI355X build")
+            return self._assemble_ragged(value, gradient)
         if value is None:
             raise ValueError("at least one coefficient is required")
         F, Q, C = self._shape
@@ -221,13 +262,15 @@ class NativeFacetBilinearForm:
                  precision="fp64"):
         from ._assembly import to_host, upload
         from ._symbols import UnsupportedNativeForm
-        if gradient is not None or symmetric_gradient is not None or divergence is not None:
-            raise UnsupportedNativeForm("facet bilinear forms take the value kind in the MI355X build")
+        if symmetric_gradient is not None or divergence is not None:
+            raise UnsupportedNativeForm("facet bilinear forms take the value and gradient kinds in the MI355X build")
         if precision != "fp64":
             raise UnsupportedNativeForm("facet bilinear forms are assembled in FP64")
-        if value is None:
+        if value is None and gradient is None:
             raise ValueError("at least one coefficient is required")
         F, Q = self.basis.dx.shape
+        if gradient is not None:
+            return self._assemble_ragged(value, gradient)
         c = np.asarray(value, dtype=np.float64)
         try:
             c = np.broadcast_to(c, (F, Q))
@@ -240,6 +283,33 @@ class NativeFacetBilinearForm:
             self.plan.handle, _native.ptr(d["node_conn"]), _native.ptr(d["face"]), _native.ptr(d["face_shape"]), Q,
             _native.ptr(weights), _native.ptr(self._values), _native.stream()))
         np.copyto(self._matrix.data, to_host(self._values[:self.plan.nnz]))
+        return self._matrix
+
+    def _assemble_ragged(self, value, gradient):
+        """Gradient terms g grad(u).grad(v) (with an optional value term):
+        the facets as cells of Q points through the ragged-quadrature kernel,
+        whose pattern is this form's facet pattern."""
+        from .cut import CutBilinearFormAssembler
+        F, Q = self.basis.dx.shape
+        if getattr(self, "_cut", None) is None:
+            r = self.basis.ragged()
+            cut = CutBilinearFormAssembler(r["dofs"], r["offsets"], r["shape"], r["gradients"], r["weights"])
+            ip, ix = self.plan.host_csr_arrays()
+            if not (np.array_equal(cut.indices, ix) and np.array_equal(cut.indptr, ip[:len(cut.indptr)])):
+                raise RuntimeError("facet gradient pattern differs from the facet plan's pattern")
+            self._cut = cut
+        coefs = []
+        for name, c in (("value", value), ("gradient", gradient)):
+            if c is None:
+                coefs.append(None)
+                continue
+            try:
+                coefs.append(np.ascontiguousarray(np.broadcast_to(np.asarray(c, dtype=np.float64),
+                                                                  (F, Q))).reshape(F * Q))
+            except ValueError as error:
+                raise ValueError(f"{name} coefficient must broadcast to {(F, Q)}") from error
+        self._cut.assemble(coefs[0], coefs[1])
+        np.copyto(self._matrix.data, self._cut.values)
         return self._matrix
 
 

@@ -76,7 +76,7 @@ def test_facet_mass_matches_oracle(quadratic, components):
     against the oracle's BilinearFormAssembler restatement; 1^T M 1 = C x
     boundary area for k = 1."""
     import skfemntv as sk
-    from skfemntv.helpers import ddot, dot, grad
+    from skfemntv.helpers import dot
     say(f"facet mass quadratic={quadratic} C={components}")
     p, t, (p2, t2) = p2_kuhn(3, 0.2, seed=5)
     mesh = sk.MeshTet2(p2, t2) if quadratic else sk.MeshTet(p, t)
@@ -100,4 +100,64 @@ def test_facet_mass_matches_oracle(quadratic, components):
     one = np.ones(fb.N)
     assert abs(one @ (M1 @ one) - 6. * components) <= 1e-12
     with pytest.raises(sk.UnsupportedNativeForm):
-        sk.asm(sk.BilinearForm(lambda u, v, w: ddot(grad(u), grad(v))), fb)
+        sk.NativeFacetBilinearForm(fb).assemble(symmetric_gradient=1.)
+
+
+@pytest.mark.parametrize("quadratic,components", [(False, 1), (True, 1), (True, 3)])
+def test_facet_gradient_and_normal_gradient_terms(quadratic, components):
+    """Gradient terms on a FacetBasis run the ragged-quadrature kernels with
+    the facets as cells of Q points: the physical gradients reproduce an
+    affine field exactly; grad:grad + mass and G:grad(v) match the oracle's
+    restatement of the reference's cut assemblers on the same tables; the
+    normal-gradient kinds satisfy K^T x = int n_x psi and K x = int phi n_x
+    for the coordinate field x."""
+    import skfemntv as sk
+    from skfemntv.helpers import ddot, dot, grad
+    from oracle import cut
+    say(f"facet gradient terms quadratic={quadratic} C={components}")
+    p, t, (p2, t2) = p2_kuhn(3, 0.2, seed=11)
+    mesh = sk.MeshTet2(p2, t2) if quadratic else sk.MeshTet(p, t)
+    elem = sk.ElementVector(sk.ElementTetP2() if quadratic else sk.ElementTetP1(), dim=components)
+    fb = sk.FacetBasis(mesh, elem)
+    F, Q = fb.dx.shape
+    r = fb.ragged()
+    dofs = r["dofs"]
+    # affine field u = 2 + 3x - y + z/2 in component 0: its discrete gradient is exact
+    u = 2. + 3. * fb.doflocs[0] - fb.doflocs[1] + 0.5 * fb.doflocs[2]
+    gh = np.einsum("fqnk,fn->fqk", fb.tabulated_gradients, u[dofs[:, :, 0]])
+    assert np.abs(gh - np.array([3., -1., 0.5])).max() < 1e-12
+    K = sk.asm(sk.BilinearForm(lambda u, v, w: ddot(grad(u), grad(v)) + 2. * dot(u, v)), fb)
+    P = F * Q
+    ip, ix, ref = cut.cut_bilinear(dofs, r["offsets"], r["shape"], r["gradients"], r["weights"],
+                                   value=np.full(P, 2.), gradient=np.ones(P))
+    n0 = len(ip) - 1
+    dense = np.zeros((n0, n0))
+    for row in range(n0):
+        dense[row, ix[ip[row]:ip[row + 1]]] = ref[ip[row]:ip[row + 1]]
+    Kd = K.toarray()
+    assert K.shape == (fb.N, fb.N) and not Kd[n0:].any()
+    assert np.linalg.norm(Kd[:n0, :n0] - dense) <= FP64_TOL * np.linalg.norm(dense)
+    rng = np.random.default_rng(components)
+    G = rng.standard_normal((F, Q, components, 3))
+    out, diag = sk.NativeFacetLinearForm(fb).assemble(gradient=G)
+    want = cut.cut_linear(dofs, r["offsets"], r["shape"], r["gradients"], r["weights"],
+                          gradient=G.reshape(P, components, 3))
+    assert out.shape == (fb.N,) and diag.entity_count == F
+    assert np.linalg.norm(out[:len(want)] - want) <= FP64_TOL * np.linalg.norm(want)
+    if components == 1:
+        X = sk.NativeCrossBilinearForm(fb, fb)
+        x = fb.doflocs[0]
+        load_nx = cut.cut_linear(dofs, r["offsets"], r["shape"], r["gradients"], r["weights"],
+                                 value=r["normals"][:, :1])
+        Kr = X.assemble_kinds("normal_gradient", "value", 1.)
+        lhs = (Kr.T @ x)[:len(load_nx)]
+        assert np.linalg.norm(lhs - load_nx) <= 1e-12 * np.linalg.norm(load_nx)
+        Kc = X.assemble_kinds("value", "normal_gradient", 1.)
+        lhs = (Kc @ x)[:len(load_nx)]
+        assert np.linalg.norm(lhs - load_nx) <= 1e-12 * np.linalg.norm(load_nx)
+        # gradient x gradient with a normal on one side: (n . grad u)(sum_j d_j v)
+        Kg = X.assemble_kinds("normal_gradient", "normal_gradient", 1.)
+        nn = X.assemble_kinds("gradient", "gradient", 1.)
+        assert Kg.shape == nn.shape == (fb.N, fb.N)
+        assert abs((Kg @ x) @ x - cut.cut_linear(dofs, r["offsets"], r["shape"], r["gradients"], r["weights"],
+                                                 value=r["normals"][:, :1] ** 2).sum()) < 1e-12
