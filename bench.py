@@ -37,6 +37,8 @@ import numpy as np  # noqa: E402
 METRIC = "assembled elements/s + % HBM roofline; learned-FSAI vs Jacobi PCG time to 1e-8"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_ELEMENT_C2 = 825.7  # algorithmic bytes per tet of one FP64 assembly
+# the same without the int32 element->CSR map (10 x 10 entries per tet): the "floor"
+FLOOR_BYTES_PER_ELEMENT_C2 = BYTES_PER_ELEMENT_C2 - 4. * 10 * 10
 # rocprofv3 FETCH_SIZE/WRITE_SIZE summaries of the other workloads at their default sizes
 TRAFFIC_FILES = {"th": "th_divergence_block_64_hbm_traffic.json"}
 
@@ -480,7 +482,9 @@ def main():
                    "elements": E, "nnz": form.plan.nnz, "dofs": basis.N, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_payload_h1 + k_gather_h1 (one assembly)"},
+                     "kernel": "k_payload_h1 + k_gather_h1 (one assembly)",
+                     "floor_bytes": FLOOR_BYTES_PER_ELEMENT_C2 * E,
+                     "floor_frac": FLOOR_BYTES_PER_ELEMENT_C2 * E / kernel_s / 1e9 / HBM_PEAK_GBS},
         "fp32_products": {"ms_per_step": kernel32_s * 1e3, "speedup_vs_fp64": kernel_s / kernel32_s,
                           "rel_frobenius_vs_fp64": err32},
         "nnz_per_s": form.plan.nnz / kernel_s,
