@@ -262,13 +262,13 @@ class NativeFacetBilinearForm:
                  precision="fp64"):
         from ._assembly import to_host, upload
         from ._symbols import UnsupportedNativeForm
-        if symmetric_gradient is not None or divergence is not None:
-            raise UnsupportedNativeForm("facet bilinear forms take the value and gradient kinds in the MI355X build")
         if precision != "fp64":
             raise UnsupportedNativeForm("facet bilinear forms are assembled in FP64")
-        if value is None and gradient is None:
+        if value is None and gradient is None and symmetric_gradient is None and divergence is None:
             raise ValueError("at least one coefficient is required")
         F, Q = self.basis.dx.shape
+        if symmetric_gradient is not None or divergence is not None:
+            return self._assemble_tensor(value, gradient, symmetric_gradient, divergence)
         if gradient is not None:
             return self._assemble_ragged(value, gradient)
         c = np.asarray(value, dtype=np.float64)
@@ -310,6 +310,56 @@ class NativeFacetBilinearForm:
                 raise ValueError(f"{name} coefficient must broadcast to {(F, Q)}") from error
         self._cut.assemble(coefs[0], coefs[1])
         np.copyto(self._matrix.data, self._cut.values)
+        return self._matrix
+
+    def _per_point(self, name, c):
+        F, Q = self.basis.dx.shape
+        if c is None:
+            return None
+        try:
+            return np.ascontiguousarray(np.broadcast_to(np.asarray(c, dtype=np.float64), (F, Q))).reshape(F * Q)
+        except ValueError as error:
+            raise ValueError(f"{name} coefficient must broadcast to {(F, Q)}") from error
+
+    def _assemble_tensor(self, value, gradient, symmetric_gradient, divergence):
+        """Symmetric-gradient and divergence kinds
+        (bilinear_form_assembler.cpp:76-100): with T[c,i,d,j] =
+        g d_cd d_ij + s/2 (d_id d_jc + d_cd d_ij) + k d_ic d_jd the entry
+        (a,c),(b,d) is sum_ij dphi_a/dx_i T dphi_b/dx_j, run as a
+        gradient-gradient contraction on the ragged cross kernel; a value
+        term is a second, scalar launch on the same pattern."""
+        from .cut import CutCrossAssembler
+        C = self.basis.elem._dim
+        if C != 3:
+            raise ValueError("symmetric-gradient and divergence kinds need a 3-component field")
+        if getattr(self, "_cutx", None) is None:
+            r = self.basis.ragged()
+            cx = CutCrossAssembler(r["dofs"], r["dofs"], r["offsets"], r["shape"], r["shape"], r["weights"],
+                                   r["gradients"], r["gradients"])
+            ip, ix = self.plan.host_csr_arrays()
+            if not (np.array_equal(cx.indices, ix) and np.array_equal(cx.indptr, ip[:len(cx.indptr)])):
+                raise RuntimeError("facet tensor pattern differs from the facet plan's pattern")
+            self._cutx = cx
+        v, g = self._per_point("value", value), self._per_point("gradient", gradient)
+        sg, dv = self._per_point("symmetric_gradient", symmetric_gradient), self._per_point("divergence", divergence)
+        P = len(self.basis.dx.ravel())
+        I3 = np.eye(3)
+        T = np.zeros((P, 3, 3, 3, 3))
+        # axes (c, i, d, j)
+        unit_grad = np.einsum("cd,ij->cidj", I3, I3)
+        if g is not None:
+            T += g[:, None, None, None, None] * unit_grad
+        if sg is not None:
+            T += (0.5 * sg)[:, None, None, None, None] * (np.einsum("id,jc->cidj", I3, I3) + unit_grad)
+        if dv is not None:
+            T += dv[:, None, None, None, None] * np.einsum("ic,jd->cidj", I3, I3)
+        cx = self._cutx
+        cx.assemble(T, "gradient", "gradient")
+        data = cx.values.copy()
+        if v is not None:
+            cx.assemble(v, "value", "value")
+            data += cx.values
+        np.copyto(self._matrix.data, data)
         return self._matrix
 
 

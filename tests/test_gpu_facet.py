@@ -99,8 +99,21 @@ def test_facet_mass_matches_oracle(quadratic, components):
     M1 = sk.asm(sk.BilinearForm(lambda u, v, w: dot(u, v)), fb)
     one = np.ones(fb.N)
     assert abs(one @ (M1 @ one) - 6. * components) <= 1e-12
-    with pytest.raises(sk.UnsupportedNativeForm):
-        sk.NativeFacetBilinearForm(fb).assemble(symmetric_gradient=1.)
+    if components == 1:
+        with pytest.raises(ValueError, match="3-component"):
+            sk.NativeFacetBilinearForm(fb).assemble(symmetric_gradient=1.)
+    else:
+        say("facet symmetric-gradient + divergence + gradient + value kinds")
+        rng = np.random.default_rng(9)
+        coefs = {k: rng.uniform(0.5, 2., (F, Q)) for k in ("value", "gradient", "symgrad", "divergence")}
+        M = sk.NativeFacetBilinearForm(fb).assemble(value=coefs["value"], gradient=coefs["gradient"],
+                                                    symmetric_gradient=coefs["symgrad"],
+                                                    divergence=coefs["divergence"])
+        vals = oracle.assemble_bilinear(ref["shape"], np.ascontiguousarray(fb.tabulated_gradients), ref["dx"], sc,
+                                        len(ix), components, **coefs)
+        np.testing.assert_array_equal(np.asarray(M.indices, dtype=np.int64), ix)
+        err = np.linalg.norm(M.data - vals) / np.linalg.norm(vals)
+        assert err <= FP64_TOL, err
 
 
 @pytest.mark.parametrize("quadratic,components", [(False, 1), (True, 1), (True, 3)])
