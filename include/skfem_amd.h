@@ -217,9 +217,11 @@ int skfa_assemble_linear(skfa_plan* plan, const double* xyz, const int32_t* vcon
  *   plan        skfa_plan_create on the (F, n) node connectivity of the
  *               facets' parent elements, one component (its incidence lists
  *               are used)
- *   face        (F) int32 local face (0..3) of each facet in its parent
- *   face_shape  (4, Q, n) FP64 parent shape functions at the facet points of
- *               each local face
+ *   face        (F) int32 row of face_shape for each facet (its corner
+ *               orientation in the parent; 0 <= face < K)
+ *   face_shape  (K, Q, n) FP64 parent shape functions at the facet points,
+ *               one row per corner orientation (local face and the order
+ *               of its corners, K <= 24)
  *   weights     (F, Q, components) FP64 dx * f
  * out[node * components + c] sums its (facet, local node) incidences in facet
  * order. */

@@ -238,13 +238,16 @@ def facet_topology(t):
     return canonical, t2f, f2t
 
 
-def facet_basis(p, t, quadratic, components=1, intorder=2):
-    """Tetrahedral FacetBasis on the boundary facets, restated per facet and
-    point as the reference computes it (basis.py:2575-2685).  Returns a dict
-    with facets, parents, local faces, shape (F,Q,n), dx (F,Q), x (F,Q,3),
-    normals (F,Q,3), dofs (F,n,C) and N."""
+def facet_basis(p, t, quadratic, components=1, intorder=2, side=0, interior=False):
+    """Tetrahedral FacetBasis on the boundary facets (or, with interior=True,
+    on one side of the interior facets: InteriorFacetBasis), restated per
+    facet and point as the reference computes it (basis.py:2575-2685): the
+    rule is placed by the parent's local corners of the facet's corners in
+    the facet's own order, and side 1 of an interior facet flips its
+    normal.  Returns a dict with facets, parents, local faces, shape
+    (F,Q,n), dx (F,Q), x (F,Q,3), normals (F,Q,3), dofs (F,n,C) and N."""
     facets, t2f, f2t = facet_topology(t)
-    boundary = np.flatnonzero(f2t[1] == -1)
+    boundary = np.flatnonzero(f2t[1] >= 0) if interior else np.flatnonzero(f2t[1] == -1)
     if intorder >= 4:
         a, b = .445948490915965, .091576213509771
         bary = np.array([[a, a, 1 - 2 * a], [a, 1 - 2 * a, a], [1 - 2 * a, a, a],
@@ -263,10 +266,10 @@ def facet_basis(p, t, quadratic, components=1, intorder=2):
     out = {"shape": np.zeros((F, Q, nodes)), "dx": np.zeros((F, Q)), "x": np.zeros((F, Q, 3)),
            "normals": np.zeros((F, Q, 3)), "parents": np.zeros(F, dtype=np.int64), "local": np.zeros(F, dtype=int)}
     for f, facet in enumerate(boundary):
-        e = int(f2t[0, facet])
+        e = int(f2t[side, facet])
         li = int(np.flatnonzero(t2f[:, e] == facet)[0])
-        corner = TET_FACES[li]
-        rc = V[list(corner)]
+        corner = [int(np.flatnonzero(t[:4, e] == node)[0]) for node in facets[:, facet]]
+        rc = V[corner]
         x = p[:, t[:, e]]
         for q, (r, s) in enumerate(pts):
             ref = (1 - r - s) * rc[0] + r * rc[1] + s * rc[2]
@@ -279,6 +282,8 @@ def facet_basis(p, t, quadratic, components=1, intorder=2):
             nrm = np.cross(tang[:, 0], tang[:, 1])
             nrm /= np.linalg.norm(nrm)
             if np.dot(nrm, point - x.mean(axis=1)) < 0.:
+                nrm = -nrm
+            if interior and side == 1:
                 nrm = -nrm
             out["shape"][f, q] = sh[0]
             out["x"][f, q] = point

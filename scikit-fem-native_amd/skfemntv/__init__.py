@@ -17,7 +17,7 @@ from ._assembly import (AssemblyMemoryBudgetError, AssemblyMemoryEstimate, Linea
 from .hcurl import (AffineTetN1Basis, HcurlTetGeometryDiagnostics, TetN1Assembler,
                     estimate_tet_n1_assembly_memory)
 from .mesh import MeshTet, MeshTet2
-from .facet import FacetBasis, NativeFacetBilinearForm, NativeFacetLinearForm
+from .facet import FacetBasis, InteriorFacetBasis, NativeFacetBilinearForm, NativeFacetLinearForm
 from .cut import CutBilinearFormAssembler, CutCrossAssembler, CutLinearFormAssembler
 from .cut_basis import CutCellBasis, CutCellQuadrature, DiscreteField
 from .runtime import (BACKEND, CAPABILITIES, available_num_threads, get_num_threads, has_capability,
@@ -31,7 +31,7 @@ __all__ = [
     "set_num_threads", "thread_limit", "AssemblyMemoryBudgetError", "AssemblyMemoryEstimate",
     "AffineTetN1Basis", "HcurlTetGeometryDiagnostics", "TetN1Assembler", "estimate_tet_n1_assembly_memory",
     "LinearFormDiagnostics", "NativeLinearForm", "NativeBilinearForm", "NativeCrossBilinearForm",
-    "NativeCompositeBilinearForm", "NativeCompositeLinearForm", "ElementComposite", "FacetBasis",
+    "NativeCompositeBilinearForm", "NativeCompositeLinearForm", "ElementComposite", "FacetBasis", "InteriorFacetBasis",
     "NativeFacetLinearForm", "NativeFacetBilinearForm",
     "CutLinearFormAssembler", "CutBilinearFormAssembler", "CutCrossAssembler", "CutCellBasis",
     "CutCellQuadrature", "DiscreteField",

@@ -53,7 +53,7 @@ def triangle_rule(intorder=2, quadrature=None):
 class FacetBasis:
     """Tetrahedral facet basis (basis.py:2442-2685, tetrahedral branch)."""
 
-    def __init__(self, mesh, element, facets=None, intorder=2, quadrature=None):
+    def __init__(self, mesh, element, facets=None, intorder=2, quadrature=None, *, _side=0, _interior=False):
         intorder = 2 if intorder is None else int(intorder)
         if mesh.t.shape[0] not in (4, 10):
             raise NotImplementedError("the MI355X build supports tetrahedral facet bases")
@@ -66,62 +66,79 @@ class FacetBasis:
             except KeyError as error:
                 raise KeyError(f"unknown boundary {facets!r}") from error
         volume = Basis(mesh, element, intorder=intorder)
-        facet_ids = mesh.boundary_facets() if facets is None else np.asarray(facets, dtype=np.int64)
+        if facets is None:
+            facet_ids = mesh.interior_facets() if _interior else mesh.boundary_facets()
+        else:
+            facet_ids = np.asarray(facets, dtype=np.int64)
         if facet_ids.ndim != 1:
             raise NotImplementedError("the MI355X FacetBasis takes facet ids")
         F = facet_ids.size
-        parents = mesh.f2t[0, facet_ids]
+        # side 1 of an interior facet is its second neighbour (basis.py:2746-2752)
+        parents = mesh.f2t[_side, facet_ids]
         if np.any(parents < 0):
-            raise ValueError("facet does not have side 0")
+            bad = int(facet_ids[np.flatnonzero(parents < 0)[0]])
+            raise ValueError(f"facet {bad} does not have side {_side}")
         # local face of the parent that is this facet (first match)
         match = mesh.t2f[:, parents] == facet_ids[None, :]
         local = np.argmax(match, axis=0)
         face_points, face_weights = triangle_rule(intorder, quadrature)
         Q = len(face_weights)
         quadratic = isinstance(scalar, ElementTetP2)
-        # reference points (1-r-s) V[c0] + r V[c1] + s V[c2] per local face
-        corners = np.asarray(TET_FACES)                                  # (4, 3)
-        Vc = _REF_VERTICES[corners]                                      # (4, 3, 3)
+        # reference points (1-r-s) V[c0] + r V[c1] + s V[c2], with c0, c1, c2
+        # the parent's local corners of the facet's corners in the facet's own
+        # (sorted) order (basis.py:2634-2645), so both sides of an interior
+        # facet place a rule point at the same physical point.  The distinct
+        # corner orders are tabulated once each.
+        face_nodes = mesh.facets[:, facet_ids].T                         # (F, 3)
+        tt = np.asarray(mesh.t[:4, parents]).T                           # (F, 4)
+        corner_of = np.argmax(tt[:, None, :] == face_nodes[:, :, None], axis=2)  # (F, 3)
+        combos, orient = np.unique(corner_of, axis=0, return_inverse=True)
+        orient = np.asarray(orient).ravel()
+        K = combos.shape[0]
+        Vc = _REF_VERTICES[combos]                                       # (K, 3, 3)
         r, s_ = face_points[:, 0], face_points[:, 1]
         ref = ((1 - r - s_)[None, :, None] * Vc[:, None, 0] + r[None, :, None] * Vc[:, None, 1]
-               + s_[None, :, None] * Vc[:, None, 2])                     # (4, Q, 3)
-        derivatives = np.stack((Vc[:, 1] - Vc[:, 0], Vc[:, 2] - Vc[:, 0]), axis=2)  # (4, 3, 2)
+               + s_[None, :, None] * Vc[:, None, 2])                     # (K, Q, 3)
+        derivatives = np.stack((Vc[:, 1] - Vc[:, 0], Vc[:, 2] - Vc[:, 0]), axis=2)  # (K, 3, 2)
         shape, refgrad = tabulate(ref.reshape(-1, 3).T, quadratic)
         n = shape.shape[1]
-        shape = shape.reshape(4, Q, n)
-        refgrad = refgrad.reshape(4, Q, n, 3)
+        shape = shape.reshape(K, Q, n)
+        refgrad = refgrad.reshape(K, Q, n, 3)
         gn = mesh.t.shape[0]
         gshape, ggrad = tabulate(ref.reshape(-1, 3).T, gn == 10)
-        gshape, ggrad = gshape.reshape(4, Q, gn), ggrad.reshape(4, Q, gn, 3)
+        gshape, ggrad = gshape.reshape(K, Q, gn), ggrad.reshape(K, Q, gn, 3)
         x = mesh.p[:, mesh.t[:, parents]].transpose(2, 0, 1)            # (F, 3, gn)
-        J = np.einsum("fig,fqgj->fqij", x, ggrad[local])                 # (F, Q, 3, 3)
-        tangents = np.einsum("fqij,fjk->fqik", J, derivatives[local])    # (F, Q, 3, 2)
-        points = np.einsum("fqg,fig->fqi", gshape[local], x)             # (F, Q, 3)
+        J = np.einsum("fig,fqgj->fqij", x, ggrad[orient])                # (F, Q, 3, 3)
+        tangents = np.einsum("fqij,fjk->fqik", J, derivatives[orient])   # (F, Q, 3, 2)
+        points = np.einsum("fqg,fig->fqi", gshape[orient], x)            # (F, Q, 3)
         cross = np.cross(tangents[..., 0], tangents[..., 1])
         area = np.linalg.norm(cross, axis=-1)
         normal = cross / area[..., None]
         centre = x.mean(axis=2)
         outward = np.einsum("fqi,fqi->fq", normal, points - centre[:, None, :])
         normal = np.where((outward < 0.)[..., None], -normal, normal)
+        if _interior and _side == 1:
+            # both traces of an interior facet share side 0's normal (basis.py:2794-2795)
+            normal = -normal
         self.mesh, self.elem = mesh, element
         self.X, self.W = face_points.T.copy(), np.asarray(face_weights, dtype=np.float64)
         self.quadrature = (self.X.copy(), self.W.copy())
         self.N = volume.N
         self.doflocs = volume.doflocs
         self.element_dofs = volume.element_dofs[:, parents]
-        self.tabulated_shape = shape[local]                              # (F, Q, n)
-        self.tabulated_gradients = np.einsum("fqak,fqkj->fqaj", refgrad[local], np.linalg.inv(J))
+        self.tabulated_shape = shape[orient]                             # (F, Q, n)
+        self.tabulated_gradients = np.einsum("fqak,fqkj->fqaj", refgrad[orient], np.linalg.inv(J))
         self.dx = area * self.W[None, :]
         self.global_coordinates = points
         self.normals = normal
         self.parent_elements = parents
         self.facet_ids = facet_ids.copy()
-        self.facet_sides = np.zeros(F, dtype=np.int8)
+        self.facet_sides = np.full(F, _side, dtype=np.int8)
         self.normal_signs = np.ones(F, dtype=np.int8)
         self.local_faces = [tuple(TET_FACES[k]) for k in local]
         self.volume_basis = volume
-        self._local_face = local.astype(np.int32)
-        self._face_shape = shape                                         # (4, Q, n)
+        self._local_face = orient.astype(np.int32)                       # row of _face_shape
+        self._face_shape = shape                                         # (K, Q, n)
         self._device = None
 
     @property
@@ -160,6 +177,24 @@ class FacetBasis:
                 "face_shape": torch.from_numpy(np.ascontiguousarray(self._face_shape)).to(dev),
             }
         return self._device
+
+
+class InteriorFacetBasis(FacetBasis):
+    """Traces on one side of every selected interior facet
+    (basis.py:2993-3013): side 0 is the facet's first neighbour, side 1 its
+    second; both sides carry side 0's unit normal."""
+
+    def __init__(self, mesh, element, mapping=None, intorder=2, quadrature=None, facets=None, dofs=None, side=0,
+                 disable_doflocs=False):
+        if mapping is not None or dofs is not None:
+            raise NotImplementedError("custom mapping and dofs are not implemented")
+        if side not in (0, 1):
+            raise ValueError("side must be 0 or 1")
+        super().__init__(mesh, element, facets=facets, intorder=intorder, quadrature=quadrature, _side=side,
+                         _interior=True)
+        self.side = side
+        if disable_doflocs:
+            self.doflocs = None
 
 
 class NativeFacetLinearForm:
@@ -363,4 +398,5 @@ class NativeFacetBilinearForm:
         return self._matrix
 
 
-__all__ = ["FacetBasis", "NativeFacetBilinearForm", "NativeFacetLinearForm", "triangle_rule"]
+__all__ = ["FacetBasis", "InteriorFacetBasis", "NativeFacetBilinearForm", "NativeFacetLinearForm",
+           "triangle_rule"]

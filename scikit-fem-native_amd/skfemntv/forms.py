@@ -179,11 +179,29 @@ def _native_bilinear_assemble(form, basis, kwargs, num_threads=0, *, memory_limi
     return result
 
 
+def _native_interior_bilinear_assemble(form, trial_bases, test_bases, kwargs):
+    """forms.py:471-487: ``asm(form, [trial sides], [test sides])`` sums the
+    cross-basis form over every (trial side, test side) pair; the form reads
+    the pair as ``w.idx`` = (trial index, test index)."""
+    trial_bases, test_bases = tuple(trial_bases), tuple(test_bases)
+    if not trial_bases or not test_bases:
+        raise ValueError("interior facet basis lists cannot be empty")
+    result = None
+    for i, trial in enumerate(trial_bases):
+        for j, test in enumerate(test_bases):
+            matrix = _native_cross_bilinear_assemble(form, trial, test, kwargs, idx=(i, j))
+            result = matrix if result is None else result + matrix
+    return result
+
+
 def _native_cross_bilinear_assemble(form, trial_basis, test_basis, kwargs, *, memory_limit_bytes=None,
-                                    memory_safety_factor=1.25):
+                                    memory_safety_factor=1.25, idx=None):
     """forms.py:407-468: ``asm(form, trial_basis, test_basis)``; rows follow
     the test basis, columns the trial basis, one fresh CSR per call."""
-    expression = trace_expression(form.function, (TrialValue(), TestValue()), _geometry(test_basis), kwargs,
+    geometry = _geometry(test_basis)
+    if idx is not None:
+        geometry["idx"] = idx
+    expression = trace_expression(form.function, (TrialValue(), TestValue()), geometry, kwargs,
                                   context="cross-basis BilinearForm")
     terms = extract_terms(expression, BilinearTerm, BilinearSum,
                           message="cross-basis BilinearForm must reduce to value, gradient, or divergence "
@@ -290,9 +308,13 @@ def asm(form, *bases, num_threads=None, memory_limit_bytes=None, memory_safety_f
     if isinstance(form, _BilinearForm):
         if kwargs.pop("integration", None) is not None:
             raise UnsupportedNativeForm("interface assembly is outside the MI355X build")
+        if len(bases) == 2 and all(isinstance(b, (list, tuple)) for b in bases):
+            if num_threads is not None:
+                raise UnsupportedNativeForm("per-call threads are not yet supported for interior BilinearForm")
+            return _native_interior_bilinear_assemble(form, bases[0], bases[1], kwargs)
         if len(bases) == 2:
             if any(isinstance(b, (list, tuple)) for b in bases):
-                raise UnsupportedNativeForm("interior facet assembly is outside the MI355X build")
+                raise UnsupportedNativeForm("interior facet assembly takes two lists of facet bases")
             if num_threads is not None:
                 raise UnsupportedNativeForm("per-call threads are not yet supported for cross-basis BilinearForm")
             for b in bases:

@@ -174,3 +174,99 @@ def test_facet_gradient_and_normal_gradient_terms(quadratic, components):
         assert Kg.shape == nn.shape == (fb.N, fb.N)
         assert abs((Kg @ x) @ x - cut.cut_linear(dofs, r["offsets"], r["shape"], r["gradients"], r["weights"],
                                                  value=r["normals"][:, :1] ** 2).sum()) < 1e-12
+
+
+def _permuted_mesh(n, seed, quadratic):
+    """Jittered Kuhn cube whose elements list their vertices in a random even
+    permutation (orientation kept), so facet corner orders differ from the
+    parents' local face orders."""
+    ax = np.linspace(0., 1., n + 1)
+    p, t = host.kuhn_mesh(ax, ax, ax)
+    p = host.jitter_interior(p, 1. / n, 0.2, seed)
+    rng = np.random.default_rng(seed)
+    even = [pm for pm in __import__("itertools").permutations(range(4))
+            if sum(pm[i] > pm[j] for i in range(4) for j in range(i + 1, 4)) % 2 == 0]
+    choice = rng.integers(0, len(even), t.shape[1])
+    t = np.stack([t[list(even[c]), e] for e, c in enumerate(choice)], axis=1)
+    if quadratic:
+        return host.p2_mesh(p, t)
+    return p, t
+
+
+@pytest.mark.parametrize("quadratic", [False, True])
+def test_facet_basis_on_permuted_connectivity(quadratic):
+    """Facet rule points follow the facet's own corner order
+    (basis.py:2634-2645), not the parent's local face order."""
+    import skfemntv as sk
+    from skfemntv.helpers import dot
+    say(f"facet basis on permuted connectivity quadratic={quadratic}")
+    p, t = _permuted_mesh(3, 13, quadratic)
+    mesh = sk.MeshTet2(p, t) if quadratic else sk.MeshTet(p, t)
+    elem = sk.ElementVector(sk.ElementTetP2() if quadratic else sk.ElementTetP1(), dim=1)
+    fb = sk.FacetBasis(mesh, elem)
+    ref = host.facet_basis(p, t, quadratic, 1)
+    assert fb._face_shape.shape[0] > 4
+    for name, mine, theirs in (("dx", fb.dx, ref["dx"]), ("x", fb.global_coordinates, ref["x"]),
+                               ("normals", fb.normals, ref["normals"]), ("shape", fb.tabulated_shape, ref["shape"])):
+        np.testing.assert_allclose(mine, theirs, rtol=0, atol=1e-14, err_msg=name)
+    F, Q = fb.dx.shape
+    n = ref["shape"].shape[2]
+    b = sk.asm(sk.LinearForm(lambda v, w: dot(1. + w.x[0], v)), fb)
+    fx = (1. + ref["x"][..., 0])[..., None]
+    expect = oracle.assemble_linear(ref["shape"], np.zeros((F, Q, n, 3)), ref["dx"], ref["dofs"], ref["N"], 1,
+                                    value=np.ascontiguousarray(fx))
+    assert np.linalg.norm(b - expect) <= FP64_TOL * np.linalg.norm(expect)
+
+
+@pytest.mark.parametrize("quadratic", [False, True])
+def test_interior_facets_share_points_and_jump_penalty(quadratic):
+    """InteriorFacetBasis sides share points and normals (the reference's
+    test_3d_interior_sides_share_points_normals_and_continuous_trace), and
+    asm(form, [sides], [sides]) sums the (trial side, test side) cross blocks
+    with w.idx signs: against the oracle's restatement, symmetric, and zero
+    on continuous fields (their jumps vanish)."""
+    import skfemntv as sk
+    from skfemntv.helpers import dot, jump
+    from oracle import cut
+    say(f"interior facets quadratic={quadratic}")
+    p, t = _permuted_mesh(2, 17, quadratic)
+    mesh = sk.MeshTet2(p, t) if quadratic else sk.MeshTet(p, t)
+    elem = sk.ElementVector(sk.ElementTetP2() if quadratic else sk.ElementTetP1(), dim=1)
+    bases = [sk.InteriorFacetBasis(mesh, elem, side=s) for s in (0, 1)]
+    refs = [host.facet_basis(p, t, quadratic, 1, side=s, interior=True) for s in (0, 1)]
+    for b, r in zip(bases, refs):
+        np.testing.assert_array_equal(b.parent_elements, r["parents"])
+        for name in ("dx", "x", "normals", "shape"):
+            mine = {"dx": b.dx, "x": b.global_coordinates, "normals": b.normals, "shape": b.tabulated_shape}[name]
+            np.testing.assert_allclose(mine, r[name], rtol=0, atol=1e-14, err_msg=name)
+    np.testing.assert_allclose(bases[0].global_coordinates, bases[1].global_coordinates, rtol=0, atol=3e-14)
+    np.testing.assert_allclose(bases[0].normals, bases[1].normals, rtol=0, atol=3e-14)
+    u = bases[0].doflocs[0] + 2. * bases[0].doflocs[1] - bases[0].doflocs[2]
+    r0, r1 = bases[0].ragged(), bases[1].ragged()
+    tr = [np.einsum("pn,pn->p", r["shape"], u[r["dofs"][:, :, 0].repeat(len(bases[0].W), axis=0)])
+          for r in (r0, r1)]
+    assert np.abs(tr[0] - tr[1]).max() < 1e-13
+
+    # the four (trial side, test side) blocks of (1 + z) u v, summed
+    A = sk.asm(sk.BilinearForm(lambda u, v, w: (1. + w.x[2]) * dot(u, v)), bases, bases).toarray()
+    coef = (1. + bases[0].global_coordinates[..., 2]).ravel()
+    N = bases[0].N
+    want = np.zeros((N, N))
+    rag = [r0, r1]
+    for i in (0, 1):          # trial side
+        for j in (0, 1):      # test side
+            ip, ix, vals = cut.cut_cross(rag[j]["dofs"], rag[i]["dofs"], r0["offsets"], rag[j]["shape"],
+                                         rag[i]["shape"], r0["weights"], rag[j]["gradients"], rag[i]["gradients"],
+                                         coef, "value", "value")
+            for row in range(len(ip) - 1):
+                np.add.at(want[row], ix[ip[row]:ip[row + 1]], vals[ip[row]:ip[row + 1]])
+    assert np.linalg.norm(A - want) <= FP64_TOL * np.linalg.norm(want)
+    assert np.abs(A - A.T).max() <= 1e-14 * np.abs(A).max()
+
+    # with w.idx signs the blocks form a jump penalty: continuous basis
+    # functions have no jump across interior facets, so it vanishes
+    def penalty(u, v, w):
+        ju, jv = jump(w, u, v)
+        return (1. + w.x[2]) * dot(ju, jv)
+    J = sk.asm(sk.BilinearForm(penalty), bases, bases).toarray()
+    assert np.abs(J).max() <= 1e-13 * np.abs(A).max()
