@@ -10,7 +10,7 @@ PyTorch-ROCm only provides device buffers and the stream.
 from ._symbols import UnsupportedNativeForm
 from .basis import Basis, GeometryDiagnostics
 from .element import ElementComposite, ElementTetP1, ElementTetP2, ElementVector
-from .forms import BilinearForm, LinearForm, asm
+from .forms import BilinearForm, Functional, LinearForm, asm
 from ._assembly import (AssemblyMemoryBudgetError, AssemblyMemoryEstimate, LinearFormDiagnostics,
                         NativeBilinearForm, NativeCompositeBilinearForm, NativeCompositeLinearForm,
                         NativeCrossBilinearForm, NativeLinearForm)
@@ -35,6 +35,6 @@ __all__ = [
     "NativeFacetLinearForm", "NativeFacetBilinearForm",
     "CutLinearFormAssembler", "CutBilinearFormAssembler", "CutCrossAssembler", "CutCellBasis",
     "CutCellQuadrature", "DiscreteField",
-    "LinearForm", "BilinearForm", "Basis", "GeometryDiagnostics", "ElementTetP1", "ElementTetP2",
+    "LinearForm", "BilinearForm", "Functional", "Basis", "GeometryDiagnostics", "ElementTetP1", "ElementTetP2",
     "ElementVector", "MeshTet", "MeshTet2", "asm", "UnsupportedNativeForm", "NativeUnavailable", "NativeError",
 ]

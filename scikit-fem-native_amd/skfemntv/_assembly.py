@@ -921,6 +921,24 @@ def estimate_cross_memory(test_basis, trial_basis):
     return AssemblyMemoryEstimate("cross", Nr, Nc, nnz, int(b))
 
 
+def integrate_functional(values, weights):
+    """sum(values * weights) over matching (entities, quadrature) arrays,
+    reduced on the GPU (integrate_functional, functional_assembler.cpp:12-33)."""
+    import torch
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    if v.ndim != 2 or w.ndim != 2 or v.shape != w.shape:
+        raise ValueError("functional values and weights must have matching (entities, quadrature) shapes")
+    if v.size == 0:
+        return 0.0
+    dv, dw = upload(v), upload(w)
+    ptr_ = upload(np.array([0, v.size], dtype=np.int64))
+    out = torch.zeros(1, dtype=torch.float64, device=_native.device())
+    _native.check(_native.lib().skfa_sys_dot(_native.ptr(dv), _native.ptr(dw), _native.ptr(ptr_), 1,
+                                             _native.ptr(out), None, _native.stream()))
+    return float(out.item())
+
+
 def estimate_cut_memory(row_dofs, col_dofs, N_rows, N_cols, kind):
     """Upper bound of the device bytes of one cut assembler: pattern +
     values (nnz <= cells Lr Lc), one record per row incidence and the

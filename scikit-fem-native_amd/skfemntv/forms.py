@@ -43,10 +43,46 @@ class _BilinearForm:
         return asm(self, *bases, **kwargs)
 
 
+class _Functional:
+    def __init__(self, function: Callable):
+        self.function = function
+
+    def assemble(self, *bases, **kwargs):
+        return asm(self, *bases, **kwargs)
+
+
 def LinearForm(function=None, **kwargs):
     if kwargs:
         return lambda fn: _LinearForm(fn)
     return _LinearForm(function) if function is not None else _LinearForm
+
+
+def Functional(function=None, **kwargs):
+    """A scalar integrand of the geometry and coefficients (forms.py:77-106):
+    ``asm(functional, basis)`` returns its integral."""
+    if kwargs:
+        return lambda fn: _Functional(fn)
+    return _Functional(function) if function is not None else _Functional
+
+
+def _native_functional_assemble(form, basis, kwargs):
+    """forms.py (``_native_functional_assemble``): the integrand is evaluated
+    at the quadrature points, then sum(values * dx) is reduced on the GPU
+    (integrate_functional, functional_assembler.cpp:12-33)."""
+    from ._assembly import integrate_functional
+    from ._symbols import Parameters, parameter_values, resolve_coefficient
+    try:
+        expression = form.function(Parameters(parameter_values(_geometry(basis), kwargs)))
+        values = np.asarray(resolve_coefficient(expression, kwargs), dtype=np.float64)
+    except UnsupportedNativeForm:
+        raise
+    except Exception as error:
+        raise UnsupportedNativeForm(f"Functional contains an unsupported operation: {error}") from error
+    try:
+        values = np.broadcast_to(values, basis.dx.shape)
+    except ValueError as error:
+        raise UnsupportedNativeForm("Functional must evaluate to one scalar per quadrature point") from error
+    return integrate_functional(values, basis.dx)
 
 
 def BilinearForm(function=None, **kwargs):
@@ -293,6 +329,12 @@ def asm(form, *bases, num_threads=None, memory_limit_bytes=None, memory_safety_f
         if isinstance(num_threads, bool) or not isinstance(num_threads, int) or num_threads < 1:
             raise ValueError("num_threads must be a positive integer")
         requested = min(num_threads, available_num_threads())
+    if isinstance(form, _Functional):
+        if kwargs.pop("integration", None) is not None:
+            raise UnsupportedNativeForm("interface assembly is outside the MI355X build")
+        if len(bases) != 1:
+            raise UnsupportedNativeForm("native Functional requires one Basis or FacetBasis")
+        return _native_functional_assemble(form, bases[0], kwargs)
     if isinstance(form, _LinearForm):
         if kwargs.pop("integration", None) is not None:
             raise UnsupportedNativeForm("interface assembly is outside the MI355X build")

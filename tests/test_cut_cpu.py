@@ -159,3 +159,13 @@ def test_cut_host_validation_messages():
         sk.CutCrossAssembler(dofs, dofs[:-1], off, sh, sh, w, gr, gr)
     with pytest.raises(ValueError, match="cut cross offsets do not span"):
         sk.CutCrossAssembler(dofs, dofs, off - 1, sh, sh, w, gr, gr)
+
+
+def test_functional_validation_on_host():
+    import skfemntv as sk
+    from skfemntv._assembly import integrate_functional
+    with pytest.raises(ValueError, match="matching"):
+        integrate_functional(np.ones((2, 3)), np.ones((3, 2)))
+    assert integrate_functional(np.zeros((0, 4)), np.zeros((0, 4))) == 0.
+    with pytest.raises(sk.UnsupportedNativeForm, match="one Basis"):
+        sk.asm(sk.Functional(lambda w: 1.))

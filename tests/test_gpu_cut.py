@@ -219,3 +219,26 @@ def test_cut_basis_partial_domain_and_cross_blocks():
     # the cut pattern holds only the kept cells' DOF pairs
     kept_nodes = np.unique(p1.element_dofs[:, keep])
     assert set(np.flatnonzero(np.abs(M).sum(axis=1) > 0)) <= set(kept_nodes.tolist())
+
+
+def test_functional_on_volume_facet_and_cut_bases():
+    """asm(Functional, basis) integrates a scalar of the geometry: the
+    reference's cut-basis moment test, plus the unit cube's volume, first
+    moments and boundary area as closed forms."""
+    import skfemntv as sk
+    say("Functional on volume, facet and cut bases")
+    mesh = _mesh(False, n=3, seed=4)
+    basis = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP1(), dim=1))
+    cutb = _full_domain_cut(basis)
+    moment = sk.Functional(lambda w: w.x[0] + 2. * w.x[1])
+    got = sk.asm(moment, cutb)
+    want = cutb.weights @ (cutb.points[:, 0] + 2. * cutb.points[:, 1])
+    assert abs(got - want) <= 1e-13 * abs(want)
+    assert abs(got - 1.5) <= 1e-13
+    assert abs(sk.asm(moment, basis) - 1.5) <= 1e-13
+    unit = sk.Functional(lambda w: 1. + 0. * w.x[0])
+    assert abs(sk.asm(unit, basis) - 1.) <= 1e-13
+    fb = sk.FacetBasis(mesh, sk.ElementVector(sk.ElementTetP1(), dim=1))
+    assert abs(sk.asm(unit, fb) - 6.) <= 1e-12
+    flux = sk.Functional(lambda w: w.x[0] * w.n[0] + w.x[1] * w.n[1] + w.x[2] * w.n[2])
+    assert abs(sk.asm(flux, fb) - 3.) <= 1e-12
