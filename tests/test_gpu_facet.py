@@ -270,3 +270,28 @@ def test_interior_facets_share_points_and_jump_penalty(quadratic):
         return (1. + w.x[2]) * dot(ju, jv)
     J = sk.asm(sk.BilinearForm(penalty), bases, bases).toarray()
     assert np.abs(J).max() <= 1e-13 * np.abs(A).max()
+
+
+def test_facet_elasticity_kinds_through_asm_annihilate_rigid_modes():
+    """asm of 2 eps(u):eps(v) + div u div v on a vector-P2 FacetBasis (the
+    tracer's symmetric-gradient and divergence kinds): rigid motions have
+    zero strain and divergence, so K r = 0 for the six rigid modes, while K
+    itself is nonzero and symmetric."""
+    import skfemntv as sk
+    from skfemntv.helpers import ddot, div, sym_grad
+    say("facet elasticity kinds through asm")
+    p, t, (p2, t2) = p2_kuhn(2, 0.2, seed=21)
+    fb = sk.FacetBasis(sk.MeshTet2(p2, t2), sk.ElementVector(sk.ElementTetP2(), dim=3))
+    K = sk.asm(sk.BilinearForm(lambda u, v, w: 2. * ddot(sym_grad(u), sym_grad(v)) + div(u) * div(v)), fb)
+    K = K.toarray()
+    assert np.abs(K).max() > 0. and np.abs(K - K.T).max() <= 1e-13 * np.abs(K).max()
+    x = fb.doflocs
+    comp = np.arange(fb.N) % 3
+    modes = [(comp == k).astype(float) for k in range(3)]
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        r = np.zeros(fb.N)
+        r[comp == a] = -x[b, comp == a]
+        r[comp == b] = x[a, comp == b]
+        modes.append(r)
+    for r in modes:
+        assert np.abs(K @ r).max() <= 1e-12 * np.abs(K).max() * max(1., np.abs(r).max())
