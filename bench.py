@@ -217,7 +217,9 @@ def other_workload(name, steps, warmup, L, ev0, ev1, stream):
     the 64^3 jittered cube, stokes the Taylor-Hood Stokes operator on a
     composite basis at 48^3.  Algorithmic bytes per element: 106.7, 3255, 356,
     for th 24 V/E + 56 + 480 + 8 nnz/E (vertices, both connectivities, int32
-    element->CSR map, values), for stokes the sum of its three blocks."""
+    element->CSR map, values), for stokes the sum of its three blocks, for
+    cut (a spherical band of cut cells, 12 points each) the per-point tables,
+    cell offsets and connectivity, and the values."""
     import torch
     import skfemntv as sk
     from skfemntv._assembly import NativeBilinearForm, NativeLinearForm, upload
@@ -328,6 +330,48 @@ def other_workload(name, steps, warmup, L, ev0, ev1, stream):
                + 2 * ((56. + 4. * 120) * E + 8. * nnz_b)) / E
         desc = (f"Taylor-Hood Stokes (vector P2 x P1 composite): nu grad:grad + both divergence blocks, 48^3 "
                 f"Kuhn cube jittered 0.15h, {E} tets, FP64")
+    elif name == "cut":
+        # cut-cell (ragged quadrature) value + gradient form of a P2 field:
+        # the cells of the 64^3 P2 cube whose centroid lies within one cell
+        # width of the sphere |x - c| = 0.3, 12 random points each
+        from skfemntv import _native
+        from skfemntv.basis import tabulate
+        from skfemntv.cut import CUT_GRADIENT, CUT_VALUE
+        mesh = jittered_p2_cube(64, 0.2, 0)
+        c = mesh.p[:, mesh.t[:4]].mean(axis=1)
+        band = np.abs(np.linalg.norm(c - 0.5, axis=0) - 0.3) < 1. / 64
+        rng = np.random.default_rng(3)
+        Pc = 12
+        counts = np.where(band, Pc, 0)
+        off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        P = int(off[-1])
+        X = rng.dirichlet(np.ones(4), P)[:, 1:]
+        shape, ref_grad = tabulate(X.T, True)
+        corners = mesh.p[:, mesh.t[:4][:, band]]
+        Jinv = np.linalg.inv(np.moveaxis(corners[:, 1:4] - corners[:, [0]], -1, 0))
+        grads = np.einsum("pnk,pki->pni", ref_grad, np.repeat(Jinv, Pc, axis=0))
+        w = rng.uniform(0.1, 1., P) / (64. ** 3 * 6 * Pc)
+        dofs = mesh.t[:10].T[:, :, None].astype(np.int64)
+        asm_ = sk.CutBilinearFormAssembler(dofs, off, shape, grads, w)
+        pl = asm_._plan
+        dv, dg = upload(rng.uniform(0.5, 2., P)), upload(rng.uniform(0.5, 2., P))
+        vals = torch.zeros(max(pl.nnz, 1), dtype=torch.float64, device=_native.device())
+        dd = asm_._dev
+        Lb = _native.lib()
+
+        def step(precision="fp64"):
+            _native.check(Lb.skfa_assemble_cut_bilinear(
+                pl.handle, _native.ptr(pl.offsets), _native.ptr(dd["weights"]), _native.ptr(dd["shape"]),
+                _native.ptr(dd["gradients"]), 3, _native.ptr(dd["shape"]), _native.ptr(dd["gradients"]), 3,
+                CUT_VALUE | CUT_GRADIENT, _native.ptr(dv), _native.ptr(dg), 0, 0, None, 0, _native.ptr(vals),
+                _native.stream()))
+        E = int(band.sum())
+        nnz = pl.nnz
+        # per point: shape (10) + gradients (30) + weight + two coefficients,
+        # FP64; per cell: offsets, int32 connectivity; FP64 values written once
+        bpe = ((10 + 30 + 3) * 8. * P + (8. + 40.) * E + 8. * nnz) / E
+        desc = (f"cut cells: P2 value + gradient form on the {E} cells of a spherical band in the 64^3 cube, "
+                f"{Pc} points each ({P} points), FP64")
     else:
         raise SystemExit(f"unknown workload {name}")
     torch.cuda.synchronize()
@@ -372,7 +416,7 @@ def main():
     ap.add_argument("--cells", type=int, nargs=2, default=(16, 48), help="batched workload: cells per axis range")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (counter calibration)")
-    ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c3", "c4", "th", "stokes"),
+    ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c3", "c4", "th", "stokes", "cut"),
                     help="c2 (default): the headline line; c1/c3/c4: the other assembly configurations")
     args = ap.parse_args()
     if args.gpus != 1:
