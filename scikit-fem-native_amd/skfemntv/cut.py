@@ -75,8 +75,15 @@ class _CutPlan:
             self.offsets = None
             self.repeats_column = False
             return
-        rconn = np.ascontiguousarray(row_nodes[active], dtype=np.int32)
-        cconn = np.ascontiguousarray(col_nodes[active], dtype=np.int32)
+        rfull, cfull = row_nodes[active], col_nodes[active]
+        # The plan runs on the nodes that the cut cells touch, renumbered by
+        # increasing maps: rows keep their order and columns stay sorted, so
+        # the compact CSR values are the full ones, and no wave is spent on
+        # the (many) rows without cut cells.
+        self.row_used, rloc = np.unique(rfull, return_inverse=True)
+        self.col_used, cloc = np.unique(cfull, return_inverse=True)
+        rconn = np.ascontiguousarray(np.asarray(rloc).reshape(rfull.shape), dtype=np.int32)
+        cconn = np.ascontiguousarray(np.asarray(cloc).reshape(cfull.shape), dtype=np.int32)
         s = np.sort(cconn, axis=1)
         self.repeats_column = bool(np.any(s[:, 1:] == s[:, :-1]))
         self.offsets = upload(np.concatenate([offsets[:-1][active], offsets[-1:]]).astype(np.int64))
@@ -84,16 +91,27 @@ class _CutPlan:
         L = _native.lib()
         h = ctypes.c_void_p()
         _native.check(L.skfa_cross_plan_create(
-            _native.ptr(self._rconn), rconn.shape[1], nrow_nodes, Cr, _native.ptr(self._cconn), cconn.shape[1],
-            ncol_nodes, Cc, self.cells, _native.stream(), ctypes.byref(h)))
+            _native.ptr(self._rconn), rconn.shape[1], len(self.row_used), Cr, _native.ptr(self._cconn),
+            cconn.shape[1], len(self.col_used), Cc, self.cells, _native.stream(), ctypes.byref(h)))
         self._h = h
         info = (ctypes.c_int64 * 6)()
         _native.check(L.skfa_cross_plan_info(h, info))
         self.nnz = int(info[2])
-        self.indptr = torch.empty(self.nrows + 1, dtype=torch.int64, device=dev)
-        self.indices = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
-        _native.check(L.skfa_cross_plan_export_csr(h, _native.ptr(self.indptr), _native.ptr(self.indices),
-                                                   _native.stream()))
+        ip_c = torch.empty(len(self.row_used) * Cr + 1, dtype=torch.int64, device=dev)
+        ix_c = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
+        _native.check(L.skfa_cross_plan_export_csr(h, _native.ptr(ip_c), _native.ptr(ix_c), _native.stream()))
+        self.compact_rows = len(self.row_used) * Cr
+        # full numbering: row DOF used[i] * Cr + c holds compact row i * Cr + c
+        ip_c, ix_c = ip_c.cpu().numpy(), ix_c[:self.nnz].cpu().numpy().astype(np.int64)
+        self.row_dofs = (self.row_used[:, None] * Cr + np.arange(Cr)).ravel()
+        counts = np.zeros(self.nrows, dtype=np.int64)
+        counts[self.row_dofs] = np.diff(ip_c)
+        ip = np.zeros(self.nrows + 1, dtype=np.int64)
+        np.cumsum(counts, out=ip[1:])
+        ix = self.col_used[ix_c // Cc] * Cc + ix_c % Cc
+        self.indptr = torch.from_numpy(ip).to(dev)
+        self.indices = torch.from_numpy(np.ascontiguousarray(ix if len(ix) else np.zeros(1, np.int64),
+                                                             dtype=np.int32)).to(dev)
 
     @property
     def handle(self):
@@ -171,8 +189,8 @@ class CutLinearFormAssembler(_CutFormData):
             raise ValueError("at least one coefficient is required")
         import torch
         t0 = time.perf_counter()
-        out = torch.zeros(max(self._ndofs, 1), dtype=torch.float64, device=_native.device())
         p = self._plan
+        out = torch.zeros(max(p.compact_rows if p.cells else 0, 1), dtype=torch.float64, device=_native.device())
         if p.cells:
             dv = upload(value) if value is not None else None
             dg = upload(gradient) if gradient is not None else None
@@ -181,8 +199,10 @@ class CutLinearFormAssembler(_CutFormData):
                 p.handle, _native.ptr(p.offsets), _native.ptr(d["weights"]), _native.ptr(d["shape"]),
                 _native.ptr(d["gradients"]), D, _native.ptr(dv), vflag, _native.ptr(dg), gflag, _native.ptr(out),
                 _native.stream()))
-        self.result_device = out[:self._ndofs]
-        result = self.result_device.cpu().numpy()
+        result = np.zeros(self._ndofs)
+        if p.cells:
+            # compact row i * C + c is DOF used[i] * C + c
+            result[p.row_dofs] = out[:p.compact_rows].cpu().numpy()
         return result, time.perf_counter() - t0
 
     @property
