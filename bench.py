@@ -6,15 +6,19 @@ Workload (one "step" = one full assembly of the CSR values):
   A = Q diag(lambda) Q^T with Q a Haar rotation and lambda log-uniform in
   [1e-2, 1e2] (seed 1), 4-point quadrature, form dot(mul(A, grad u), grad v).
 Inputs (vertex coordinates, connectivity, A, gather plan) are resident in
-HBM before timing; the timed region is K back-to-back assemblies.
+HBM before timing; the timed region is K back-to-back assemblies, after an
+untimed clock ramp (--ramp seconds) and W warm-up steps.
 
 Prints one JSON line.  `value` = assembled elements/s of the FP64 path.
 `roofline` uses the algorithmic bytes of one assembly (825.7 B per element:
 vertices, int32 connectivity, 72 B coefficient, int32 element->CSR map,
 FP64 CSR values) over the kernel's mean duration measured with HIP events on
-the launch stream.  `cpu_baseline` times the reference's own compiled C++
-CrossBilinearAssembler (oracle/_ref) on a bounded 16^3 sample of the same
-workload with all host cores.
+the launch stream.  `configs` holds the C1 (P1 Laplace 32^3), C3 (vector-P2
+elasticity 48^3) and C4 (N1 96^3) assemblies measured the same way,
+`batched_fsai` the 256-system batch and its PCG solves, and `cpu_baseline`
+the C restatement of the reference's threaded CPU assembler (oracle/, built
+from this repository) on the full headline workload, with every host core
+and with one thread.
 """
 
 from __future__ import annotations
@@ -64,77 +68,83 @@ def jittered_p2_cube(n, amount, seed):
     return sk.MeshTet2.from_mesh(sk.MeshTet(p, mesh.t))
 
 
-def cpu_baseline(n_sample=16, repeats=7, warmup=2):
-    """Reference C++ (compiled from its sources into oracle/_ref) on a
-    bounded sample; falls back to the C restatement when _ref is absent."""
-    import oracle
-    from oracle import host, pipeline
-    ax = np.linspace(0., 1., n_sample + 1)
-    p, t = host.kuhn_mesh(ax, ax, ax)
-    p = host.jitter_interior(p, 1. / n_sample, 0.2, 0)
-    p2, t2 = host.p2_mesh(p, t)
-    X, W = host.tet_rule_4()
-    E = t.shape[1]
-    A = spd_tensors(E, 1)
-    A_eq = np.ascontiguousarray(np.broadcast_to(A[:, None], (E, 4, 3, 3)))
-    cores = len(os.sched_getaffinity(0))
-    ref = oracle.ref_module()
-    (shape, grads, dx, x, det, tol, cond), _ = pipeline.basis_geometry(p2, t2, X, W, True)
-    N, dofs = pipeline.dofs_of(t2, p2.shape[1], 10, 1)
-    out = {}
-    if ref is not None:
-        coef = np.ascontiguousarray(A_eq[:, :, None, :, None, :])
-        asm_ = ref.CrossBilinearAssembler(dofs, dofs, shape, shape, dx, grads, grads)
-        for threads in (cores, 1):
-            ref.set_num_threads(threads)
-            for _ in range(warmup):
-                asm_.assemble(coef, "gradient", "gradient", 0)
-            ts = []
-            for _ in range(repeats):
-                t0 = time.perf_counter()
-                asm_.assemble(coef, "gradient", "gradient", 0)
-                ts.append(time.perf_counter() - t0)
-            out[threads] = statistics.median(ts)
-        ref.set_num_threads(1)
-        kind = "reference"
-    else:
-        ip, ix, sc = oracle.pattern(dofs.reshape(E, -1), nrows=N)
-        colour, _ = oracle.colouring(dofs.reshape(E, -1), N)
-        for threads in (cores, 1):
-            for _ in range(warmup):
-                oracle.assemble_tensor(grads, dx, A_eq, sc, len(ix), threads=threads, colour=colour)
-            ts = []
-            for _ in range(repeats):
-                t0 = time.perf_counter()
-                oracle.assemble_tensor(grads, dx, A_eq, sc, len(ix), threads=threads, colour=colour)
-                ts.append(time.perf_counter() - t0)
-            out[threads] = statistics.median(ts)
-        kind = "port"
-    cpu_model = ""
+def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
+    return ""
+
+
+def cpu_baseline(mesh, A_host, repeats=7, warmup=2):
+    """The reference's threaded CPU algorithm, restated in C (oracle/fem_oracle.c,
+    gcc -O3), on the full headline workload.
+
+    Same data flow as the reference's CrossBilinearAssembler gradient x
+    gradient tensor path (cross_bilinear_assembler.cpp:19-107,207-247): the
+    geometry is tabulated once (basis_geometry.cpp:49-209), the constructor
+    builds the CSR pattern, the int64 element->CSR map and the greedy
+    first-fit element colouring over row DOFs (cross_bilinear_assembler.cpp:
+    307-341), and assemble() zeroes the values and runs the fused q x a x b
+    loop with scatter, colour by colour, each colour split statically over
+    the threads (parallel.hpp:21-45).  Timed warm, constructor excluded
+    (reported as cold setup), median of `repeats` after `warmup` calls
+    (benchmarks/hcurl_tet_n1_assembly.py:18-26), with every host core and
+    with 1 thread (the reference default, runtime.cpp:10)."""
+    import oracle
+    from oracle import host, pipeline
+    p2, t2 = mesh.p, mesh.t
+    E = t2.shape[1]
+    X, W = host.tet_rule_4()
+    t0 = time.perf_counter()
+    (shape, grads, dx, x, det, tol, cond), _ = pipeline.basis_geometry(p2, t2, X, W, True)
+    geometry_s = time.perf_counter() - t0
+    del shape, x, det, tol, cond
+    A_eq = np.ascontiguousarray(np.broadcast_to(A_host[:, None], (E, 4, 3, 3)))
+    t0 = time.perf_counter()
+    N, dofs = pipeline.dofs_of(t2, p2.shape[1], 10, 1)
+    rows = dofs.reshape(E, -1)
+    ip, ix, sc = oracle.pattern(rows, nrows=N)
+    colour, ncolours = oracle.colouring(rows, N)
+    constructor_s = time.perf_counter() - t0
+    nnz = len(ix)
+    del ip, ix
+    cores = len(os.sched_getaffinity(0))
+    med = {}
+    for threads in (cores, 1):
+        for _ in range(warmup):
+            oracle.assemble_tensor(grads, dx, A_eq, sc, nnz, threads=threads, colour=colour)
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            oracle.assemble_tensor(grads, dx, A_eq, sc, nnz, threads=threads, colour=colour)
+            ts.append(time.perf_counter() - t0)
+        med[threads] = statistics.median(ts)
+        log(f"cpu baseline: {threads} threads {med[threads]:.3f} s per assembly")
     return {
-        "value": E / out[cores],
+        "value": E / med[cores],
         "unit": "elements/s",
         "cores": cores,
-        "kind": kind,
-        "sample": (f"P2 anisotropic {n_sample}^3 Kuhn cube ({E} tets), 4-pt rule, warm median of {repeats} "
-                   f"assemble() calls; CrossBilinearAssembler; 1-thread {E / out[1]:.4g} elements/s; {cpu_model}"),
+        "kind": "port",
+        "label": "reference algorithm, CPU restatement",
+        "sample": (f"the full headline workload: P2 anisotropic 64^3 jittered Kuhn cube ({E} tets, nnz {nnz}), "
+                   f"4-pt rule, FP64; C restatement of the reference's threaded CrossBilinearAssembler "
+                   f"(greedy first-fit colouring, {ncolours} colours, static per-colour split), gcc -O3; "
+                   f"warm median of {repeats} after {warmup} warm-up calls; CPU: {cpu_model()}"),
+        "seconds_per_assembly": med[cores],
+        "one_thread": {"value": E / med[1], "seconds_per_assembly": med[1]},
+        "cold_setup_s": {"geometry": geometry_s, "pattern_map_colouring": constructor_s},
     }
 
 
-def batched_fsai(systems, lo, hi, steps, warmup, L, ev0, ev1, stream):
+def batched_fsai(systems, lo, hi, steps, warmup, timer):
     """256 P1 anisotropic systems on uneven boxes: batched assembly (one
     HIP-graph replay per step), then FP64 PCG to ||b - Ax||/||b|| <= 1e-8
     with Jacobi and with the pretrained learned FSAI."""
     import torch
-    from skfemntv import _native
     from skfemntv.batch import AnisotropicBatch, batch_shapes
     from skfemntv.fsai import FSAINetwork, FSAISystems, LearnedFSAI
     from skfemntv.solver import JacobiPreconditioner, pcg, relative_residuals
@@ -147,13 +157,13 @@ def batched_fsai(systems, lo, hi, steps, warmup, L, ev0, ev1, stream):
         f"nnz {batch.A.nnz}, setup {setup:.1f}s ({batch.timings})")
     for _ in range(min(max(warmup, 1), 20)):   # 5 ms each; the GPU is warm by now
         batch.assemble(graph=True)
-    _native.check(L.skfa_event_record(ev0, stream))
-    for _ in range(steps):
-        batch.assemble(graph=True)
-    _native.check(L.skfa_event_record(ev1, stream))
-    ms = ctypes.c_float()
-    _native.check(L.skfa_event_elapsed_ms(ev0, ev1, ctypes.byref(ms)))
-    asm_s = ms.value / 1e3 / steps
+    asm_s = timer.run(lambda: batch.assemble(graph=True), steps)
+    # algorithmic bytes of one batched assembly: vertices, int32 P1
+    # connectivity, the 72 B tensor, the int32 4 x 4 element->CSR map and the
+    # FP64 values of the full (uncondensed) matrices, plus the load vector
+    plan = batch.form.plan
+    E, V = batch.nelements, batch.mesh.p.shape[1]
+    alg = 24. * V + (16. + 72. + 64.) * E + 8. * plan.nnz + 8. * plan.nrows
     A, b = batch.A, batch.b
     net = FSAINetwork.pretrained()
     out = {}
@@ -177,9 +187,12 @@ def batched_fsai(systems, lo, hi, steps, warmup, L, ev0, ev1, stream):
     return {
         "workload": f"{systems} P1 anisotropic systems, cells/axis iid U{{{lo}..{hi}}} (seed 5), jitter 0.2h, "
                     f"per-element SPD A (seed 5000+s), f=1, boundary condensed",
-        "elements": batch.nelements, "unknowns": A.shape[0], "nnz": A.nnz,
-        "assembly_ms_per_batch": asm_s * 1e3, "assembly_elements_per_s": batch.nelements / asm_s,
+        "elements": E, "unknowns": A.shape[0], "nnz": A.nnz, "nnz_uncondensed": plan.nnz,
+        "assembly_ms_per_batch": asm_s * 1e3, "assembly_elements_per_s": E / asm_s,
         "assembly": "payload + row-gather + condensation gather + load vector, replayed as one HIP graph",
+        "assembly_roofline": {"bound": "hbm", "bytes": alg, "achieved": alg / asm_s / 1e9, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": alg / asm_s / 1e9 / HBM_PEAK_GBS,
+                              "bytes_definition": "24 V + (16 conn + 72 tensor + 64 int32 map) E + 8 nnz + 8 N"},
         "setup_s": setup,
         "jacobi_pcg": {"wall_s": out["jac_s"], "batch_iterations": rj.batch_iterations,
                        "iterations_mean": float(rj.iterations.mean()), "iterations_max": int(rj.iterations.max()),
@@ -194,37 +207,69 @@ def batched_fsai(systems, lo, hi, steps, warmup, L, ev0, ev1, stream):
     }
 
 
-def timed(fn, steps, warmup, L, ev0, ev1, stream):
-    """Mean seconds per call of fn over `steps` calls, HIP events on the launch stream."""
+class Timer:
+    """Mean seconds per call over `steps` back-to-back calls, timed with HIP
+    events on the stream the kernels are launched on."""
+
+    def __init__(self):
+        from skfemntv import _native
+        self.native = _native
+        self.L = _native.lib()
+        self.stream = _native.stream()
+        self.ev0, self.ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+        _native.check(self.L.skfa_event_create(ctypes.byref(self.ev0)))
+        _native.check(self.L.skfa_event_create(ctypes.byref(self.ev1)))
+
+    def run(self, fn, steps):
+        import torch
+        torch.cuda.synchronize()
+        self.native.check(self.L.skfa_event_record(self.ev0, self.stream))
+        for _ in range(steps):
+            fn()
+        self.native.check(self.L.skfa_event_record(self.ev1, self.stream))
+        ms = ctypes.c_float()
+        self.native.check(self.L.skfa_event_elapsed_ms(self.ev0, self.ev1, ctypes.byref(ms)))
+        torch.cuda.synchronize()
+        return ms.value / 1e3 / steps
+
+
+def ramp(fn, seconds):
+    """Untimed back-to-back calls until `seconds` of wall time have passed:
+    the GPU reaches steady clocks after ~0.1 s of continuous assembly (C2:
+    3 warm-up steps measure 0.38 ms per step, 30 measure 0.33, 200 measure
+    0.32).  Returns the number of calls."""
     import torch
-    from skfemntv import _native
-    for _ in range(warmup):
-        fn()
-    torch.cuda.synchronize()
-    _native.check(L.skfa_event_record(ev0, stream))
-    for _ in range(steps):
-        fn()
-    _native.check(L.skfa_event_record(ev1, stream))
-    ms = ctypes.c_float()
-    _native.check(L.skfa_event_elapsed_ms(ev0, ev1, ctypes.byref(ms)))
-    return ms.value / 1e3 / steps
+    t0, calls = time.perf_counter(), 0
+    while True:
+        for _ in range(10):
+            fn()
+        calls += 10
+        torch.cuda.synchronize()
+        if time.perf_counter() - t0 >= seconds:
+            return calls
 
 
-def other_workload(name, steps, warmup, L, ev0, ev1, stream):
-    """The other assembly configurations at full size (one JSON line each):
-    c1 P1 Laplace + load 32^3, c3 vector-P2 elasticity 48^3, c4 N1 curl-curl +
-    kappa mass 96^3, th the Taylor-Hood divergence block (vector P2 x P1) on
-    the 64^3 jittered cube, stokes the Taylor-Hood Stokes operator on a
-    composite basis at 48^3.  Algorithmic bytes per element: 106.7, 3255, 356,
-    for th 24 V/E + 56 + 480 + 8 nnz/E (vertices, both connectivities, int32
-    element->CSR map, values), for stokes the sum of its three blocks, for
-    cut (a spherical band of cut cells, 12 points each) the per-point tables,
-    cell offsets and connectivity, and the values."""
+def build_workload(name):
+    """The other assembly configurations at full size.  Returns a dict with
+    the step function (precision -> None), the FP64 values tensor (or None),
+    elements, nnz, algorithmic bytes per element and a description.
+
+    c1 P1 Laplace + load 32^3, c3 vector-P2 elasticity 48^3, c4 N1 curl-curl
+    + kappa mass 96^3, th the Taylor-Hood divergence block (vector P2 x P1)
+    on the 64^3 jittered cube, stokes the Taylor-Hood Stokes operator on a
+    composite basis at 48^3, cut a spherical band of cut cells (12 points
+    each).  Algorithmic bytes count unique vertex coordinates, int32
+    connectivity, coefficients as given, the int32 element->CSR map at node
+    level (vector fields interleave their components inside a node block,
+    so a 10 x 10 node pair is one map entry), and the FP64 values written
+    once."""
     import torch
     import skfemntv as sk
     from skfemntv._assembly import NativeBilinearForm, NativeLinearForm, upload
     from skfemntv.hcurl import AffineTetN1Basis, TetN1Assembler
     t0 = time.perf_counter()
+    values = None
+    fp32 = False
     if name == "c1":
         ax = np.linspace(0., 1., 33)
         mesh = sk.MeshTet.init_tensor(ax, ax, ax)
@@ -238,6 +283,7 @@ def other_workload(name, steps, warmup, L, ev0, ev1, stream):
             form.assemble_device(gradient=one, precision=precision)
             load.assemble_device(f, False, None, False)
         bpe, desc, nnz = 106.7, f"P1 Laplace ddot(grad u, grad v) + load f=1, 32^3 Kuhn cube, {E} tets, FP64", form.plan.nnz
+        values, fp32 = form._values, True
     elif name == "c3":
         n = 48
         ax = np.linspace(0., 1., n + 1)
@@ -262,6 +308,7 @@ def other_workload(name, steps, warmup, L, ev0, ev1, stream):
             form.assemble_device(symmetric_gradient=s_dev, divergence=l_dev, precision=precision)
         bpe, nnz = 3255.0, form.plan.nnz
         desc = f"vector-P2 elasticity 2 mu eps:eps + lam div div, 48^3 Kuhn cube jittered 0.15h, {E} tets, FP64"
+        values, fp32 = form._values, True
     elif name == "c4":
         ax = np.linspace(0., 1., 97)
         mesh = sk.MeshTet.init_tensor(ax, ax, ax)
@@ -293,8 +340,8 @@ def other_workload(name, steps, warmup, L, ev0, ev1, stream):
         nnz = plan.nnz
         V = int(np.unique(t2[:4]).size)
         # vertices (24 B each) + int32 connectivity of both fields (4 + 10
-        # nodes) + int32 element->CSR map (4 x 30 entries) + FP64 values
-        bpe = (24. * V + 4. * 14 * E + 4. * 4 * 30 * E + 8. * nnz) / E
+        # nodes) + int32 node-level element->CSR map (4 x 10 entries) + FP64 values
+        bpe = (24. * V + 4. * 14 * E + 4. * 4 * 10 * E + 8. * nnz) / E
         desc = f"Taylor-Hood divergence block div(u) q, vector P2 x P1, 64^3 Kuhn cube jittered 0.2h, {E} tets, FP64"
     elif name == "stokes":
         # Taylor-Hood Stokes operator on a composite basis (vector P2 x P1),
@@ -325,9 +372,10 @@ def other_workload(name, steps, warmup, L, ev0, ev1, stream):
         nnz = nnz_a + 2 * nnz_b
         V = int(np.unique(mesh.t[:4]).size)
         # per block: vertices + int32 connectivity + coefficient + int32
-        # element->CSR map + FP64 values (velocity 30 x 30, coupling 30 x 4)
-        bpe = (3 * 24. * V + (40. + 8. + 4. * 900) * E + 8. * nnz_a
-               + 2 * ((56. + 4. * 120) * E + 8. * nnz_b)) / E
+        # node-level element->CSR map + FP64 values (velocity 10 x 10 node
+        # pairs, coupling 10 x 4)
+        bpe = (3 * 24. * V + (40. + 8. + 4. * 100) * E + 8. * nnz_a
+               + 2 * ((56. + 4. * 40) * E + 8. * nnz_b)) / E
         desc = (f"Taylor-Hood Stokes (vector P2 x P1 composite): nu grad:grad + both divergence blocks, 48^3 "
                 f"Kuhn cube jittered 0.15h, {E} tets, FP64")
     elif name == "cut":
@@ -375,86 +423,105 @@ def other_workload(name, steps, warmup, L, ev0, ev1, stream):
     else:
         raise SystemExit(f"unknown workload {name}")
     torch.cuda.synchronize()
-    setup = time.perf_counter() - t0
-    log(f"{name}: setup {setup:.1f}s")
-    sec = timed(step, steps, warmup, L, ev0, ev1, stream)
+    return {"name": name, "step": step, "values": values, "fp32": fp32, "E": E, "nnz": nnz, "bpe": bpe,
+            "desc": desc, "setup_s": time.perf_counter() - t0}
+
+
+def measure(w, steps, warmup, ramp_s, timer):
+    """Time one workload: ramp, `warmup` steps, `steps` timed FP64 steps;
+    for workloads with an FP32-products form, FP32 and FP64 blocks are
+    timed alternately (FP64, FP32, FP64, FP32) after the same warm-up."""
+    import torch
+    step = w["step"]
+    fp32 = w["fp32"]
+    ramp(lambda: (step("fp64"), step("fp32") if fp32 else None), ramp_s)
+    for _ in range(warmup):
+        step("fp64")
+        if fp32:
+            step("fp32")
+    sec = timer.run(lambda: step("fp64"), steps)
+    out = {"ms_per_step": sec * 1e3, "elements_per_s": w["E"] / sec, "nnz_per_s": w["nnz"] / sec,
+           "bytes_per_element": w["bpe"], "achieved_gbs": w["bpe"] * w["E"] / sec / 1e9,
+           "roofline_frac": w["bpe"] * w["E"] / sec / 1e9 / HBM_PEAK_GBS, "setup_s": w["setup_s"]}
+    if fp32:
+        s32 = timer.run(lambda: step("fp32"), steps)
+        s64b = timer.run(lambda: step("fp64"), steps)
+        s32b = timer.run(lambda: step("fp32"), steps)
+        step("fp64")
+        v64 = w["values"].clone()
+        step("fp32")
+        v32 = w["values"].clone()
+        err = float(torch.linalg.norm(v32 - v64) / torch.linalg.norm(v64))
+        step("fp64")
+        out["fp32_products"] = {"ms_per_step": 0.5 * (s32 + s32b) * 1e3,
+                                "speedup_vs_fp64": (sec + s64b) / (s32 + s32b),
+                                "rel_frobenius_vs_fp64": err,
+                                "timing": "alternating FP64/FP32 blocks of `steps` each after one warm-up"}
+    return sec, out
+
+
+def other_workload(name, steps, warmup, ramp_s, timer):
+    """One JSON line for a single non-headline workload (--workload)."""
+    w = build_workload(name)
+    log(f"{name}: setup {w['setup_s']:.1f}s")
+    sec, summary = measure(w, steps, warmup, ramp_s, timer)
+    E, nnz, bpe = w["E"], w["nnz"], w["bpe"]
     out = {"metric": METRIC, "value": E / sec, "unit": "elements/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
            "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic",
-           "config": {"workload": desc, "elements": E, "nnz": nnz, "parallelism": "single GPU"},
+           "config": {"workload": w["desc"], "elements": E, "nnz": nnz, "parallelism": "single GPU"},
            "roofline": {"bound": "hbm", "achieved": bpe * E / sec / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": bpe * E / sec / 1e9 / HBM_PEAK_GBS, "traffic": None},
-           "nnz_per_s": nnz / sec, "setup_s": setup}
+           "nnz_per_s": nnz / sec, "setup_s": w["setup_s"]}
     # measured HBM bytes of one launch (FETCH_SIZE + WRITE_SIZE, calibrated
     # on a copy of known size), from the committed rocprofv3 summary
     traffic_file = os.path.join(ROOT, "profiles", f"{TRAFFIC_FILES.get(name, '')}")
     if name in TRAFFIC_FILES and os.path.exists(traffic_file):
         with open(traffic_file) as f:
             out["roofline"]["traffic"] = json.load(f)["per_assembly_bytes"]
-    if name in ("c1", "c3"):
-        sec32 = timed(lambda: step("fp32"), steps, warmup, L, ev0, ev1, stream)
-        step("fp64")
-        v64 = form._values.clone()
-        step("fp32")
-        v32 = form._values.clone()
-        err32 = float(torch.linalg.norm(v32 - v64) / torch.linalg.norm(v64))
-        out["fp32_products"] = {"ms_per_step": sec32 * 1e3, "speedup_vs_fp64": sec / sec32,
-                                "rel_frobenius_vs_fp64": err32}
+    if "fp32_products" in summary:
+        out["fp32_products"] = summary["fp32_products"]
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    # the GPU reaches steady clocks after ~0.1 s of back-to-back assemblies:
-    # measured on C2, 3 warmup steps give 0.38 ms/step, 30 give 0.33, 200 give 0.32
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--n", type=int, default=64, help="cells per cube edge")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--systems", type=int, default=256, help="batched workload: number of systems (0 = skip)")
-    ap.add_argument("--cells", type=int, nargs=2, default=(16, 48), help="batched workload: cells per axis range")
-    ap.add_argument("--copy-mib", type=int, default=0,
-                    help="run one device-to-device copy of this size first (counter calibration)")
-    ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c3", "c4", "th", "stokes", "cut"),
-                    help="c2 (default): the headline line; c1/c3/c4: the other assembly configurations")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build's assembly path runs on a single MI355X; "
-              "multi-GPU sharding is a design note only", file=sys.stderr)
-        sys.exit(2)
-
+def headline(args, timer):
+    """C2: FP64 P2 anisotropic diffusion on the 64^3 jittered cube."""
     import torch
     import skfemntv as sk
-    from skfemntv import _native
     from skfemntv._assembly import NativeCrossBilinearForm, upload
-
-    torch.cuda.set_device(0)
-    if args.workload != "c2":
-        L = _native.lib()
-        stream = _native.stream()
-        ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
-        _native.check(L.skfa_event_create(ctypes.byref(ev0)))
-        _native.check(L.skfa_event_create(ctypes.byref(ev1)))
-        print(json.dumps(other_workload(args.workload, args.steps, args.warmup, L, ev0, ev1, stream)), flush=True)
-        return
+    from skfemntv.helpers import dot, grad, mul
     t0 = time.perf_counter()
     mesh = jittered_p2_cube(args.n, 0.2, 0)
     E = mesh.nelements
     log(f"mesh {E} tets, {mesh.p.shape[1]} nodes ({time.perf_counter() - t0:.1f}s)")
-    X = np.array([[.5854101966249685, .1381966011250105, .1381966011250105, .1381966011250105],
-                  [.1381966011250105, .5854101966249685, .1381966011250105, .1381966011250105],
-                  [.1381966011250105, .1381966011250105, .5854101966249685, .1381966011250105]])
-    W = np.full(4, 1. / 24.)
+    X, W = sk.basis.tet_rule_4()
+    A_host = spd_tensors(E, 1)
+
+    @sk.BilinearForm
+    def aniso(u, v, w):
+        return dot(mul(w.diffusion, grad(u)), grad(v))
+
+    # the recommended (3,3,E,Q) layout, a broadcast view of the per-element tensors
+    diffusion = np.broadcast_to(np.moveaxis(A_host, 0, -1)[..., None], (3, 3, E, 4))
+    # cold first call: geometry check, device plan (pattern + gather map),
+    # coefficient upload, kernels and the CSR copied to the host
+    t1 = time.perf_counter()
+    basis_cold = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP2(), dim=1), quadrature=(X, W))
+    K = sk.asm(aniso, basis_cold, diffusion=diffusion)
+    cold_asm_s = time.perf_counter() - t1
+    del K, basis_cold
+    log(f"cold first asm (new basis) {cold_asm_s:.2f}s")
     t1 = time.perf_counter()
     basis = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP2(), dim=1), quadrature=(X, W))
-    log(f"basis + geometry check {time.perf_counter() - t1:.1f}s")
-    t1 = time.perf_counter()
+    basis_s = time.perf_counter() - t1
     form = NativeCrossBilinearForm(basis, basis)
     torch.cuda.synchronize()
-    setup_plan = time.perf_counter() - t1
-    log(f"plan (pattern + gather map on GPU) {setup_plan:.2f}s nnz={form.plan.nnz}")
-    A_host = spd_tensors(E, 1)
+    t1 = time.perf_counter()
+    plan = form.plan          # builds the pattern, incidences and gather records on the GPU
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t1
+    log(f"basis + geometry check {basis_s:.2f}s; plan (pattern + gather map on the GPU) {plan_s:.3f}s, "
+        f"nnz={plan.nnz}")
     A = upload(A_host)
     if args.copy_mib:
         src = torch.ones(args.copy_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
@@ -462,40 +529,26 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    L = _native.lib()
-    stream = _native.stream()
-    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
-    _native.check(L.skfa_event_create(ctypes.byref(ev0)))
-    _native.check(L.skfa_event_create(ctypes.byref(ev1)))
 
     def step(precision="fp64"):
         form.assemble_tensor_device(A, precision=precision, symmetric=True)
 
+    ramp_calls = ramp(lambda: (step("fp64"), step("fp32")), args.ramp)
     for _ in range(args.warmup):
-        step()
+        step("fp64")
+        step("fp32")
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    _native.check(L.skfa_event_record(ev0, stream))
-    for _ in range(args.steps):
-        step()
-    _native.check(L.skfa_event_record(ev1, stream))
-    ms = ctypes.c_float()
-    _native.check(L.skfa_event_elapsed_ms(ev0, ev1, ctypes.byref(ms)))
-    torch.cuda.synchronize()
+    kernel_s = timer.run(lambda: step("fp64"), args.steps)   # the headline: exactly `steps` steps
     wall = time.perf_counter() - t_start
-    kernel_s = ms.value / 1e3 / args.steps
-    # FP32-products form on the same inputs
-    for _ in range(args.warmup):
-        step("fp32")
-    _native.check(L.skfa_event_record(ev0, stream))
-    for _ in range(args.steps):
-        step("fp32")
-    _native.check(L.skfa_event_record(ev1, stream))
-    _native.check(L.skfa_event_elapsed_ms(ev0, ev1, ctypes.byref(ms)))
-    kernel32_s = ms.value / 1e3 / args.steps
+    # FP32-products form: blocks alternate with FP64 after the same warm-up
+    s32a = timer.run(lambda: step("fp32"), args.steps)
+    s64b = timer.run(lambda: step("fp64"), args.steps)
+    s32b = timer.run(lambda: step("fp32"), args.steps)
     v64 = form.assemble_tensor_device(A, precision="fp64", symmetric=True).clone()
     v32 = form.assemble_tensor_device(A, precision="fp32", symmetric=True).clone()
     err32 = float(torch.linalg.norm(v32 - v64) / torch.linalg.norm(v64))
+    del v32
     torch.cuda.synchronize()
 
     algorithmic = BYTES_PER_ELEMENT_C2 * E
@@ -523,28 +576,25 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"P2 tet anisotropic diffusion, {args.n}^3 Kuhn cube jittered 0.2h, "
                                f"{E} tets, 4-pt quadrature, FP64",
-                   "elements": E, "nnz": form.plan.nnz, "dofs": basis.N, "parallelism": "single GPU"},
+                   "elements": E, "nnz": plan.nnz, "dofs": basis.N, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_payload_h1 + k_gather_h1 (one assembly)",
                      "floor_bytes": FLOOR_BYTES_PER_ELEMENT_C2 * E,
                      "floor_frac": FLOOR_BYTES_PER_ELEMENT_C2 * E / kernel_s / 1e9 / HBM_PEAK_GBS},
-        "fp32_products": {"ms_per_step": kernel32_s * 1e3, "speedup_vs_fp64": kernel_s / kernel32_s,
-                          "rel_frobenius_vs_fp64": err32},
-        "nnz_per_s": form.plan.nnz / kernel_s,
+        "fp32_products": {"ms_per_step": 0.5 * (s32a + s32b) * 1e3,
+                          "speedup_vs_fp64": (kernel_s + s64b) / (s32a + s32b),
+                          "rel_frobenius_vs_fp64": err32,
+                          "timing": "alternating FP64/FP32 blocks of `steps` each after one warm-up"},
+        "nnz_per_s": plan.nnz / kernel_s,
         "wall_ms_per_step": wall / args.steps * 1e3,
-        "plan_setup_s": setup_plan,
+        "clock_ramp": {"seconds": args.ramp, "calls": ramp_calls,
+                       "note": "untimed FP64+FP32 assemblies before the warm-up, until the GPU is at steady clocks"},
+        "plan_setup_s": plan_s,
+        "cold_first_asm_s": cold_asm_s,
     }
-    # End-to-end asm on the same workload: form tracing, the coefficient in
-    # the recommended (3,3,E,Q) layout (a broadcast view, uploaded per call),
-    # payload + gather, and the scipy CSR copied back to the host.  Median of 3.
-    from skfemntv.helpers import dot, grad, mul
-
-    @sk.BilinearForm
-    def aniso(u, v, w):
-        return dot(mul(w.diffusion, grad(u)), grad(v))
-
-    diffusion = np.broadcast_to(np.moveaxis(A_host, 0, -1)[..., None], (3, 3, E, 4))
+    # warm end-to-end asm on the same workload: form tracing, the coefficient
+    # upload, payload + gather, and the scipy CSR copied back to the host
     asm_times = []
     for _ in range(3):
         t1 = time.perf_counter()
@@ -554,13 +604,64 @@ def main():
                                 "elements_per_s": E / statistics.median(asm_times),
                                 "includes": "form tracing, coefficient upload, kernels, CSR values to host"}
     del K
+    return result, mesh, A_host, (form, A)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--ramp", type=float, default=0.3, help="seconds of untimed assemblies before the warm-up")
+    ap.add_argument("--n", type=int, default=64, help="cells per cube edge")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the C1/C3/C4 lines of the default run")
+    ap.add_argument("--systems", type=int, default=256, help="batched workload: number of systems (0 = skip)")
+    ap.add_argument("--cells", type=int, nargs=2, default=(16, 48), help="batched workload: cells per axis range")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this size first (counter calibration)")
+    ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c3", "c4", "th", "stokes", "cut"),
+                    help="c2 (default): the headline line with the other configurations as extra keys; "
+                         "the others print their own line")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build's assembly path runs on a single MI355X; "
+              "multi-GPU sharding is a design note only", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from skfemntv import _native
+
+    torch.cuda.set_device(0)
+    torch.zeros(1, device="cuda")
+    _native.lib()
+    timer = Timer()
+    if args.workload != "c2":
+        print(json.dumps(other_workload(args.workload, args.steps, args.warmup, args.ramp, timer)), flush=True)
+        return
+    result, mesh, A_host, keep = headline(args, timer)
+    if not args.no_configs:
+        configs = {}
+        for name in ("c1", "c3", "c4"):
+            log(f"config {name} ...")
+            w = build_workload(name)
+            _, summary = measure(w, args.steps, args.warmup, 0.2, timer)
+            summary["workload"] = w["desc"]
+            summary["elements"], summary["nnz"] = w["E"], w["nnz"]
+            configs[name] = summary
+            del w
+            torch.cuda.empty_cache()
+        result["configs"] = configs
+    del keep
+    torch.cuda.empty_cache()
     if args.systems > 0:
         log("batched assembly + PCG ...")
         result["batched_fsai"] = batched_fsai(args.systems, args.cells[0], args.cells[1], args.steps, args.warmup,
-                                              L, ev0, ev1, stream)
+                                              timer)
     if not args.no_cpu_baseline:
-        log("cpu baseline ...")
-        result["cpu_baseline"] = cpu_baseline()
+        log("cpu baseline (C restatement of the reference's threaded assembler, full workload) ...")
+        result["cpu_baseline"] = cpu_baseline(mesh, A_host)
+        result["cpu_baseline"]["gpu_speedup"] = result["value"] / result["cpu_baseline"]["value"]
     print(json.dumps(result), flush=True)
 
 

@@ -57,6 +57,12 @@ int skfa_plan_create(const int32_t* node_conn, int64_t E, int32_t n, int64_t nno
  * column-split lanes.  skfa_plan_create uses 16 KiB. */
 int skfa_plan_create_budget(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
                             int32_t lds_budget, void* stream, skfa_plan** out);
+/* Same with the smallest launch size: while the gather would have fewer
+ * than min_waves waves at one row per lane, the rows are split over more
+ * lanes (skfa_plan_create_budget uses 2048; 0 keeps one row per lane at
+ * every size, the layout of large meshes). */
+int skfa_plan_create_ex(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
+                        int32_t lds_budget, int64_t min_waves, void* stream, skfa_plan** out);
 int skfa_plan_destroy(skfa_plan* plan);
 /* Launch tuning of an existing plan (results are bit-identical for every
  * setting of keys 1-4; key 5 selects between two formulations that differ

@@ -10,8 +10,9 @@ it.  It has three layers:
                  kernels (geometry, CSR pattern + map, element kernels,
                  edge topology, N1 integration, scatter, colouring);
 * ``_ref``       the reference's own C++ extension, compiled from its sources
-                 by ``oracle/Makefile`` where the reference tree exists.  Used
-                 to pin the C restatement and as the timed CPU baseline.
+                 by ``oracle/Makefile`` where the reference tree exists (this
+                 build container only; it never travels to the GPU host).
+                 Used to pin the C restatement.
 
 Parity status of each function is listed in ``PARITY`` below.
 """

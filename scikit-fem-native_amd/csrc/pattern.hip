@@ -313,11 +313,11 @@ static int plan_build(skfa_plan* P, const int32_t* conn, hipStream_t s) {
     std::vector<int32_t> r_stride(static_cast<size_t>(nn));
     int64_t rec_total = 0, steps_total = 0, lds_max = 0;
     // Small meshes: at one row per lane the launch would hold fewer than
-    // kMinWaves waves (8 per CU on 256 CUs); start from a larger split so
-    // that the S lanes of a row share its incidences instead.
-    constexpr int64_t kMinWaves = 2048;
+    // min_waves waves (default 2048: 8 per CU on 256 CUs); start from a
+    // larger split so that the S lanes of a row share its incidences instead.
+    const int64_t min_waves = P->min_waves;
     int s_min = 1;
-    while (s_min * 2 <= 64 / unit && (nn * unit * s_min + 63) / 64 < kMinWaves) s_min <<= 1;
+    while (s_min * 2 <= 64 / unit && (nn * unit * s_min + 63) / 64 < min_waves) s_min <<= 1;
     P->min_split = s_min;
     int64_t r = 0;
     while (r < nn) {
@@ -685,12 +685,13 @@ int skfa_plan_set_tuning(skfa_plan* P, int32_t key, int32_t value) {
     }
 }
 
-int skfa_plan_create_budget(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
-                            int32_t lds_budget, void* stream, skfa_plan** out) {
+int skfa_plan_create_ex(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
+                        int32_t lds_budget, int64_t min_waves, void* stream, skfa_plan** out) {
     SKFA_REQUIRE(out != nullptr, "plan output pointer is null");
     *out = nullptr;
     SKFA_REQUIRE(lds_budget >= 2048 && lds_budget <= int32_t(kMaxWaveLds), "LDS budget out of range");
     SKFA_REQUIRE(E >= 0 && nnodes >= 0, "negative sizes");
+    SKFA_REQUIRE(min_waves >= 0, "minimum wave count must be non-negative");
     SKFA_REQUIRE(n >= 1 && n <= kMaxLocal, "nodes per element must be in [1, 16]");
     SKFA_REQUIRE(E < kMaxElements, "element count exceeds 2^27");
     SKFA_REQUIRE(components == 1 || components == 2 || components == 3, "components must be 1, 2 or 3");
@@ -702,6 +703,7 @@ int skfa_plan_create_budget(const int32_t* node_conn, int64_t E, int32_t n, int6
     P->nnodes = nnodes;
     P->C = components;
     P->lds_budget = lds_budget;
+    P->min_waves = min_waves;
     int rc = plan_build(P, node_conn, s);
     if (rc != kOk) {
         delete P;
@@ -709,6 +711,11 @@ int skfa_plan_create_budget(const int32_t* node_conn, int64_t E, int32_t n, int6
     }
     *out = P;
     return kOk;
+}
+
+int skfa_plan_create_budget(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
+                            int32_t lds_budget, void* stream, skfa_plan** out) {
+    return skfa_plan_create_ex(node_conn, E, n, nnodes, components, lds_budget, 2048, stream, out);
 }
 
 int skfa_plan_destroy(skfa_plan* plan) {

@@ -61,6 +61,7 @@ struct skfa_plan {
     int32_t lds_budget = 16 * 1024;   // bytes of LDS per wave requested at plan time
     int32_t lds_bytes = 0;            // largest span of any wave, bytes
     int32_t min_split = 1;            // smallest S of the layout (> 1 for small meshes)
+    int64_t min_waves = 2048;         // small meshes: raise S until the launch has this many waves
     skfa::DevBuf<uint8_t> records;
 
     // per-element payload of the element-constant fast path (allocated on

@@ -29,14 +29,15 @@ _TUNE_KEYS = {"xcd_chunk": 1, "lds_atomic": 2, "placement": 3, "streaming": 4, "
 class Plan:
     """Device CSR pattern + gather plan of one (basis, components) pair."""
 
-    def __init__(self, node_conn_dev, E, n, nnodes, components, lds_budget=12 * 1024):
+    def __init__(self, node_conn_dev, E, n, nnodes, components, lds_budget=12 * 1024, min_waves=2048):
         import torch
         L = _native.lib()
         h = ctypes.c_void_p()
-        _native.check(L.skfa_plan_create_budget(_native.ptr(node_conn_dev), E, n, nnodes, components,
-                                                int(lds_budget), _native.stream(), ctypes.byref(h)))
+        _native.check(L.skfa_plan_create_ex(_native.ptr(node_conn_dev), E, n, nnodes, components,
+                                            int(lds_budget), int(min_waves), _native.stream(), ctypes.byref(h)))
         self._h = h
         self.lds_budget = int(lds_budget)
+        self.min_waves = int(min_waves)
         self.tuning = {"xcd_chunk": 16, "lds_atomic": 1, "placement": 2, "streaming": 1, "element_table": -1,
                        "row_split": 0, "mfma": -1}
         info = (ctypes.c_int64 * 9)()

@@ -209,7 +209,6 @@ def test_row_splits_agree_at_small_lds_budgets(components):
         rng = np.random.default_rng(3)
         mu, lam = np.exp(rng.standard_normal(E)), np.exp(rng.standard_normal(E))
 
-        @sk.BilinearForm
         def elasticity(u, v, w):
             return 2. * w.mu * ddot(sym_grad(u), sym_grad(v)) + w.lam * (div(u) * div(v))
 
@@ -229,7 +228,9 @@ def test_row_splits_agree_at_small_lds_budgets(components):
                 assert pipeline.relative_frobenius(raw[split], vals) <= FP64_TOL
             else:
                 basis._plan = plan
-                K = sk.asm(elasticity, basis, mu=mu[:, None], lam=lam[:, None])
+                # a new form object: asm caches its assembler (and plan) per form
+                K = sk.asm(sk.BilinearForm(elasticity), basis, mu=mu[:, None], lam=lam[:, None])
+                assert K.nnz == plan.nnz
                 check_csr(K, ip, ix, vals)
                 raw[split] = K.data.copy()
         say(f"budget {budget}: waves {plan.waves}")
