@@ -206,6 +206,24 @@ int skfa_assemble_bilinear(skfa_plan* plan, const double* xyz, int64_t npoints, 
  * 9 x the scalar plan's nnz. */
 #define SKFA_FLAG_VECTOR3_DIAGONAL 2
 
+/* skfa_assemble_bilinear without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as skfa_assemble_bilinear; the
+ * per-element payload and the element tables of the element-constant path
+ * live in a 96 MiB device array of the library (one per GPU, held until the
+ * process ends) instead of plan-owned buffers.  A plan whose payload does not
+ * fit (more than about 1.5 M elements with a mass term, 2 M without, 0.9 M
+ * for 3-component fields) is rejected with an error before any launch.
+ * Every call on one GPU uses the same array: calls that overlap on that GPU
+ * (other streams, other threads) race. */
+int skfa_assemble_bilinear_noalloc(skfa_plan* plan, const double* xyz, int64_t npoints, const int32_t* vconn,
+                                   const int32_t* gconn, int32_t gnodes, const double* tables, int32_t nq,
+                                   const double* rtab, const float* rtab32, int32_t nqg, int32_t nqm,
+                                   const double* c_value, const double* c_gradient, const double* c_symgrad,
+                                   const double* c_divergence, const double* c_tensor, int32_t per_qp,
+                                   int32_t precision, int32_t flags, double* values, void* stream);
+
 /* Linear form f.v + G:grad(v) into out (nrows FP64, overwritten).
  * Replaces LinearFormAssembler::assemble
  * (cpp/src/assembly/linear_form_assembler.cpp:41-137).  f: (C) constant

@@ -28,6 +28,7 @@
 // bit-reproducible run to run; there are no atomics and no colouring.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1812,8 +1813,17 @@ static int sym_layout(const skfa_plan* P, int nqg, int nqm, bool f32, bool mass)
     return mass ? PL_SYM : PL_SYM6;
 }
 
+// Scratch of the element-constant path: the per-element payload (E x
+// stride doubles) and the element tables of the table gather (FP64, FP32).
+struct Workspace {
+    double* payload = nullptr;
+    double* ntab = nullptr;
+    float* ntab32 = nullptr;
+};
+
 template <int NL, int C, bool F32>
-static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, size_t lds, hipStream_t s) {
+static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, const Workspace& W, size_t lds,
+                         hipStream_t s) {
     if constexpr (C == 3) {
         if constexpr (NL == 10) {
             if (A.Qg == 4) return launch_vec10<F32>(P, A, lds, s);
@@ -1839,16 +1849,12 @@ static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, size_t lds
     if (sym && table_gather(P)) {
         // element tables from the reduced quadrature tables, then the
         // table-driven gather
-        const size_t nt = size_t(NL) * NL * kNK;
-        if (P->ntab.n < nt)
-            if (int rc = P->ntab.alloc(nt)) return rc;
-        if (P->ntab32.n < nt)
-            if (int rc = P->ntab32.alloc(nt)) return rc;
-        k_build_ntab<NL><<<1, 128, 0, s>>>(A.tab, A.Qg, A.Qm, P->ntab.p, P->ntab32.p);
+        SKFA_REQUIRE(W.ntab != nullptr && W.ntab32 != nullptr, "element-table buffers are missing");
+        k_build_ntab<NL><<<1, 128, 0, s>>>(A.tab, A.Qg, A.Qm, W.ntab, W.ntab32);
         SKFA_LAUNCH_CHECK();
         GatherArgs T = A;
-        T.tab = P->ntab.p;
-        T.tab32 = P->ntab32.p;
+        T.tab = W.ntab;
+        T.tab32 = W.ntab32;
         return A.has_mass ? launch_table<NL, F32, true>(P, T, s) : launch_table<NL, F32, false>(P, T, s);
     }
     if (sym) {
@@ -1876,25 +1882,52 @@ static int launch_general(const skfa_plan* P, const H1Args& A, size_t lds, hipSt
 
 template <int NL, int C>
 static int dispatch_nl(skfa_plan* P, bool fast, bool iso, bool f32, bool sym, const GatherArgs& G,
-                       const H1Args& H, size_t lds_fast, size_t lds_general, hipStream_t s) {
+                       const H1Args& H, const Workspace& W, size_t lds_fast, size_t lds_general, hipStream_t s) {
     if (fast)
-        return f32 ? launch_gather<NL, C, true>(P, G, sym, lds_fast, s)
-                   : launch_gather<NL, C, false>(P, G, sym, lds_fast, s);
+        return f32 ? launch_gather<NL, C, true>(P, G, sym, W, lds_fast, s)
+                   : launch_gather<NL, C, false>(P, G, sym, W, lds_fast, s);
     return iso ? launch_general<NL, C, true>(P, H, lds_general, s) : launch_general<NL, C, false>(P, H, lds_general, s);
 }
 
-}  // namespace skfa
+// The bilinear entry points.  With `scratch` null the plan owns the
+// payload and element-table buffers (allocated on first use); otherwise
+// they are carved from `scratch` (scratch_bytes long) and nothing is
+// allocated.
+// ------------------------------------------------------------------ no-allocation scratch
+// Device memory of skfa_assemble_bilinear_noalloc: a file-scope array that
+// every GPU the library runs on holds until the process ends.  Host code
+// never passes the array's name to a kernel (on the host it is only a
+// stand-in for the symbol); it uses the device address that
+// hipGetSymbolAddress returns, looked up once per device with that device
+// current.  Calls that overlap on one GPU share the array and race.
+constexpr size_t kScratchBytes = size_t(96) << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
 
-using namespace skfa;
+static std::mutex g_scratch_mutex;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
 
-extern "C" {
+static int device_scratch(unsigned char** out) {
+    int dev = 0;
+    SKFA_HIP(hipGetDevice(&dev));
+    SKFA_REQUIRE(dev >= 0 && dev < kMaxDevices, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* ptr = nullptr;
+        SKFA_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        SKFA_REQUIRE(ptr != nullptr, "scratch symbol has no device address");
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return kOk;
+}
 
-int skfa_assemble_bilinear(skfa_plan* P, const double* xyz, int64_t npoints, const int32_t* vconn,
-                           const int32_t* gconn, int32_t gnodes, const double* tables, int32_t nq,
-                           const double* rtab, const float* rtab32, int32_t nqg, int32_t nqm,
-                           const double* c_value, const double* c_gradient, const double* c_symgrad,
-                           const double* c_divergence, const double* c_tensor, int32_t per_qp, int32_t precision,
-                           int32_t flags, double* values, void* stream) {
+static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* vconn, const int32_t* gconn,
+                                int32_t gnodes, const double* tables, int32_t nq, const double* rtab,
+                                const float* rtab32, int32_t nqg, int32_t nqm, const double* c_value,
+                                const double* c_gradient, const double* c_symgrad, const double* c_divergence,
+                                const double* c_tensor, int32_t per_qp, int32_t precision, int32_t flags,
+                                double* values, void* stream, unsigned char* scratch, size_t scratch_bytes) {
     SKFA_REQUIRE(P != nullptr, "plan is null");
     SKFA_REQUIRE(P->n == 4 || P->n == 10, "bilinear assembly supports 4- and 10-node tetrahedra");
     SKFA_REQUIRE(P->C == 1 || P->C == 3, "bilinear assembly supports 1 or 3 components on tetrahedra");
@@ -1913,7 +1946,6 @@ int skfa_assemble_bilinear(skfa_plan* P, const double* xyz, int64_t npoints, con
     SKFA_REQUIRE(!(precision == 1 && rtab32 == nullptr), "FP32 tables are null");
     SKFA_REQUIRE(fast || (tables != nullptr && nq >= 1 && nq <= 4096), "tables / quadrature size invalid");
     SKFA_REQUIRE(!iso || (gnodes >= 4 && gnodes <= 64), "geometry node count out of range");
-    (void)npoints;
     const bool diag3 = (flags & SKFA_FLAG_VECTOR3_DIAGONAL) != 0;
     SKFA_REQUIRE(!diag3 || (P->C == 1 && fast && !c_symgrad && !c_divergence && !c_tensor),
                  "diagonal 3-component output needs a scalar plan, affine elements, element-constant value / "
@@ -1925,6 +1957,7 @@ int skfa_assemble_bilinear(skfa_plan* P, const double* xyz, int64_t npoints, con
     }
     GatherArgs G{};
     H1Args H{};
+    Workspace W{};
     bool sym = false;
     size_t lds_fast = 0, lds_general = size_t(P->lds_bytes);
     if (fast) {
@@ -1932,13 +1965,42 @@ int skfa_assemble_bilinear(skfa_plan* P, const double* xyz, int64_t npoints, con
         // declares symmetric
         sym = P->C == 1 && (c_tensor == nullptr || (flags & SKFA_FLAG_SYMMETRIC_TENSOR));
         const int stride = P->C == 1 ? (sym ? 8 : 10) : 14;
-        if (int rc = ensure_payload(P, stride)) return rc;
+        // the table gather runs for symmetric scalar forms unless the MFMA
+        // gather takes them (launch_gather)
+        const bool table = P->C == 1 && sym && table_gather(P) && !(!diag3 && mfma_gather(P, nqg, nqm));
+        const size_t nt = size_t(P->n) * P->n * kNK;
+        if (scratch == nullptr) {
+            if (int rc = ensure_payload(P, stride)) return rc;
+            if (table && P->ntab.n < nt)
+                if (int rc = P->ntab.alloc(nt)) return rc;
+            if (table && P->ntab32.n < nt)
+                if (int rc = P->ntab32.alloc(nt)) return rc;
+            W.payload = P->payload.p;
+            W.ntab = P->ntab.p;
+            W.ntab32 = P->ntab32.p;
+        } else {
+            // 256-byte aligned pieces: payload | FP64 tables | FP32 tables
+            auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+            const size_t pay = up(size_t(P->E) * stride * sizeof(double));
+            const size_t need = pay + (table ? up(nt * sizeof(double)) + up(nt * sizeof(float)) : 0);
+            if (need > scratch_bytes) {
+                set_error("the element payload of " + std::to_string(P->E) + " elements needs " +
+                          std::to_string(need) + " bytes, more than the " + std::to_string(scratch_bytes) +
+                          "-byte scratch of the no-allocation entry point");
+                return kCapacity;
+            }
+            W.payload = reinterpret_cast<double*>(scratch);
+            if (table) {
+                W.ntab = reinterpret_cast<double*>(scratch + pay);
+                W.ntab32 = reinterpret_cast<float*>(scratch + pay + up(nt * sizeof(double)));
+            }
+        }
         PayloadArgs pa{};
         pa.xyz = reinterpret_cast<const double4*>(xyz);
         pa.vconn = vconn;
         pa.E = P->E;
         pa.cm = c_value; pa.cg = c_gradient; pa.cs = c_symgrad; pa.cl = c_divergence; pa.cA = c_tensor;
-        pa.out = P->payload.p;
+        pa.out = W.payload;
         const int lay = sym ? sym_layout(P, nqg, nqm, precision == 1, c_value != nullptr) : PL_FULL;
         if (P->C == 1 && lay == PL_SYM32)
             k_payload_h1<1, true, SYMOUT_F32><<<elem_grid(P->E), 256, 0, s>>>(pa);
@@ -1953,7 +2015,7 @@ int skfa_assemble_bilinear(skfa_plan* P, const double* xyz, int64_t npoints, con
         SKFA_LAUNCH_CHECK();
         G.plan = plan_args(P);
         G.plan.diag3 = diag3 ? 1 : 0;
-        G.payload = P->payload.p;
+        G.payload = W.payload;
         G.tab = rtab;
         G.tab32 = rtab32;
         G.Qg = nqg;
@@ -1977,10 +2039,42 @@ int skfa_assemble_bilinear(skfa_plan* P, const double* xyz, int64_t npoints, con
     SKFA_REQUIRE(lds_fast <= 160 * 1024 && lds_general <= 160 * 1024, "LDS request exceeds 160 KiB");
     const bool f32 = precision == 1;
     if (P->n == 4)
-        return P->C == 1 ? dispatch_nl<4, 1>(P, fast, iso, f32, sym, G, H, lds_fast, lds_general, s)
-                         : dispatch_nl<4, 3>(P, fast, iso, f32, sym, G, H, lds_fast, lds_general, s);
-    return P->C == 1 ? dispatch_nl<10, 1>(P, fast, iso, f32, sym, G, H, lds_fast, lds_general, s)
-                     : dispatch_nl<10, 3>(P, fast, iso, f32, sym, G, H, lds_fast, lds_general, s);
+        return P->C == 1 ? dispatch_nl<4, 1>(P, fast, iso, f32, sym, G, H, W, lds_fast, lds_general, s)
+                         : dispatch_nl<4, 3>(P, fast, iso, f32, sym, G, H, W, lds_fast, lds_general, s);
+    return P->C == 1 ? dispatch_nl<10, 1>(P, fast, iso, f32, sym, G, H, W, lds_fast, lds_general, s)
+                     : dispatch_nl<10, 3>(P, fast, iso, f32, sym, G, H, W, lds_fast, lds_general, s);
+}
+
+}  // namespace skfa
+
+using namespace skfa;
+
+extern "C" {
+
+int skfa_assemble_bilinear(skfa_plan* P, const double* xyz, int64_t npoints, const int32_t* vconn,
+                           const int32_t* gconn, int32_t gnodes, const double* tables, int32_t nq,
+                           const double* rtab, const float* rtab32, int32_t nqg, int32_t nqm,
+                           const double* c_value, const double* c_gradient, const double* c_symgrad,
+                           const double* c_divergence, const double* c_tensor, int32_t per_qp, int32_t precision,
+                           int32_t flags, double* values, void* stream) {
+    (void)npoints;
+    return assemble_bilinear_ws(P, xyz, vconn, gconn, gnodes, tables, nq, rtab, rtab32, nqg, nqm, c_value,
+                                c_gradient, c_symgrad, c_divergence, c_tensor, per_qp, precision, flags, values,
+                                stream, nullptr, 0);
+}
+
+int skfa_assemble_bilinear_noalloc(skfa_plan* P, const double* xyz, int64_t npoints, const int32_t* vconn,
+                                   const int32_t* gconn, int32_t gnodes, const double* tables, int32_t nq,
+                                   const double* rtab, const float* rtab32, int32_t nqg, int32_t nqm,
+                                   const double* c_value, const double* c_gradient, const double* c_symgrad,
+                                   const double* c_divergence, const double* c_tensor, int32_t per_qp,
+                                   int32_t precision, int32_t flags, double* values, void* stream) {
+    (void)npoints;
+    unsigned char* scratch_ptr = nullptr;
+    if (int rc = device_scratch(&scratch_ptr)) return rc;
+    return assemble_bilinear_ws(P, xyz, vconn, gconn, gnodes, tables, nq, rtab, rtab32, nqg, nqm, c_value,
+                                c_gradient, c_symgrad, c_divergence, c_tensor, per_qp, precision, flags, values,
+                                stream, scratch_ptr, kScratchBytes);
 }
 
 int skfa_assemble_tet_n1(skfa_plan* P, const double* xyz, const int32_t* vconn, const double* tables,

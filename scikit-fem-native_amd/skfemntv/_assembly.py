@@ -185,8 +185,11 @@ def basis_scalar_plan(basis):
 
 
 def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=None, symmetric_gradient=None,
-                             divergence=None, tensor=None, per_qp=False, precision="fp64", symmetric_tensor=False):
-    """Launch skfa_assemble_bilinear with device coefficient tensors."""
+                             divergence=None, tensor=None, per_qp=False, precision="fp64", symmetric_tensor=False,
+                             noalloc=False):
+    """Launch skfa_assemble_bilinear with device coefficient tensors
+    (``noalloc``: skfa_assemble_bilinear_noalloc, which keeps its scratch in
+    a fixed device array of the library instead of allocating)."""
     d = basis.device_data()
     L = _native.lib()
     iso = not basis._affine
@@ -202,7 +205,8 @@ def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=No
     if prec == 1 and (iso or per_qp):
         raise UnsupportedNativeForm("FP32 products are available for affine elements with element-constant "
                                     "coefficients")
-    _native.check(L.skfa_assemble_bilinear(
+    entry = L.skfa_assemble_bilinear_noalloc if noalloc else L.skfa_assemble_bilinear
+    _native.check(entry(
         plan.handle, _native.ptr(d["xyz"]), d["xyz"].shape[0], _native.ptr(d["vconn"]),
         _native.ptr(d["gconn"]) if iso else None, d["gnodes"] if iso else 4,
         _native.ptr(tables), len(basis.W), _native.ptr(d["rtab"]), _native.ptr(d["rtab32"]), d["nqg"], d["nqm"],
