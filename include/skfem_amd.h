@@ -77,11 +77,17 @@ int skfa_plan_destroy(skfa_plan* plan);
                                   the row's incidences p, p+S, ...; the S lanes add in turn; default),
                                   1 entry split (lane p computes entries p, p+S, ... of every element
                                   row), 2 column split (every lane computes the whole element row and
-                                  adds the columns it owns); bit-identical results */
+                                  adds the columns it owns); bit-identical results; 3 concurrent
+                                  incidence split (as 0, but the S lanes add in the same
+                                  instructions and the LDS atomic add serialises lanes that hit one
+                                  address: rounding may differ from 0-2) */
 #define SKFA_TUNE_MFMA 7       /* scalar fields, element-constant coefficients, <= 4-point reduced rules:
                                   element rows contracted on MFMA (16 rows x 4 points per instruction);
                                   0 off (default, -1 = default), 1 every wave, 2 waves of <= 16 rows
                                   only; rounding differs only */
+#define SKFA_TUNE_ROW_FORM 8   /* 10-node scalar gather with SKFA_FLAG_BARYCENTRIC_TABLE: 1 element rows
+                                  from the barycentric structure (64 FMAs per row), 0 from the full
+                                  gradient table (120 FMAs), -1 automatic (1); rounding differs only */
 int skfa_plan_set_tuning(skfa_plan* plan, int32_t key, int32_t value);
 /* host info[9]: nrows, nnz, nnz_node, max_row_len, max_incidence, waves,
  * gather records, record bytes, LDS bytes of the largest wave span. */
@@ -205,6 +211,14 @@ int skfa_assemble_bilinear(skfa_plan* plan, const double* xyz, int64_t npoints, 
  * 3-component DOF-level CSR (the C = 3 plan's pattern).  `values` then holds
  * 9 x the scalar plan's nnz. */
 #define SKFA_FLAG_VECTOR3_DIAGONAL 2
+/* SKFA_FLAG_BARYCENTRIC_TABLE: for 10-node elements, rtab (and rtab32)
+ * continue after pm with c[16][Qg]: the barycentric coefficients of the
+ * reference gradients at the Qg gradient points, grad phi_b = sum_m c_bm
+ * grad L_m, pairs in the order (0,0) (1,1) (2,2) (3,3), then for each edge
+ * e = (i,j) of ((0,1),(1,2),(2,0),(0,3),(1,3),(2,3)) the pairs (4+e, i) and
+ * (4+e, j).  The gather then forms element rows from these mostly-zero
+ * structures (tuning key SKFA_TUNE_ROW_FORM). */
+#define SKFA_FLAG_BARYCENTRIC_TABLE 4
 
 /* skfa_assemble_bilinear without any device allocation.  No counterpart in
  * the reference: it is for callers that cannot allocate device memory (code

@@ -482,6 +482,11 @@ struct GatherArgs {
     double* values;
     // only waves with rmin <= rows <= rmax are assembled by this launch
     int32_t rmin = 0, rmax = 1 << 30;
+    // 10-node (P2) barycentric coefficients of the gradient rule: for the 16
+    // (node b, barycentric index m) pairs with grad(phi_b) = sum_m c_bm(q) grad(L_m),
+    // c_bm at the Qg points (pair-major); null when not available
+    const double* ctab = nullptr;
+    const float* ctab32 = nullptr;
 };
 
 // K_ab += w_q (dphi_aq^T D) . dphi_bq for all b: the lane's row a reads the
@@ -562,9 +567,52 @@ __device__ inline void load_payloads(const Args& A, const Record<RB>* rec, PT (*
 //                 b = p, p+S, ... of its row
 //   SP_COLUMN     all S lanes read the same element and compute the whole
 //                 row; lane p adds the in-row offsets o with o % S == p
-constexpr int SP_INCIDENCE = 0, SP_ENTRY = 1, SP_COLUMN = 2;
+//   SP_CONCURRENT as SP_INCIDENCE, but the S lanes add their element rows
+//                 in the same instructions: the LDS atomic add serialises
+//                 the lanes that hit one address (requires ds_add; the
+//                 order among parts is the hardware's, so the bits may
+//                 differ from the other splits by rounding)
+constexpr int SP_INCIDENCE = 0, SP_ENTRY = 1, SP_COLUMN = 2, SP_CONCURRENT = 3;
 
-template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, int SP>
+// P2 element row from the barycentric structure of the reference gradients
+// (the rule enters only through c_bm(q)):
+//   grad(phi_b)(q) = sum_{m in supp b} c_bm(q) g_m,  g_0 = (-1,-1,-1), g_m = e_{m-1}
+//   K_ab = sum_q u_q . grad(phi_b)(q) = sum_{m in supp b} sum_q c_bm(q) t_q^m,
+//   t_q^m = u_q . g_m  (t^0 = -(u_0 + u_1 + u_2), t^m = u_{m-1})
+// Vertex b = k has one pair (k, k); edge b = 4 + e, e = (i, j), has the
+// pairs (i) and (j) at rows 4 + 2e and 5 + 2e of ctab.  4 + 6 x 2 pairs:
+// 64 FMAs for the row instead of 120 (the table entries are mostly zero).
+template <int QG, typename T>
+__device__ inline void struct_row(const T (*u)[3], const T* __restrict__ ct, T* Kr) {
+    T t[4][QG];
+#pragma unroll
+    for (int q = 0; q < QG; ++q) {
+        t[0][q] = -((u[q][0] + u[q][1]) + u[q][2]);
+        t[1][q] = u[q][0];
+        t[2][q] = u[q][1];
+        t[3][q] = u[q][2];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        T acc = T(0);
+#pragma unroll
+        for (int q = 0; q < QG; ++q) acc = fma(ct[k * QG + q], t[k][q], acc);
+        Kr[k] = acc;
+    }
+    constexpr int E0[6] = {0, 1, 2, 0, 1, 2}, E1[6] = {1, 2, 0, 3, 3, 3};
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        T acc = T(0);
+#pragma unroll
+        for (int q = 0; q < QG; ++q) {
+            acc = fma(ct[(4 + 2 * e) * QG + q], t[E0[e]][q], acc);
+            acc = fma(ct[(5 + 2 * e) * QG + q], t[E1[e]][q], acc);
+        }
+        Kr[4 + e] = acc;
+    }
+}
+
+template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, int SP, int KF = 0>
 __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Record<RB>& rec,
                                     const typename PayloadLayout<PL>::type* pl,
                                     double* span_lds, int Qg, int Qm, bool mass,
@@ -646,7 +694,20 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
     T Kr[NL];
 #pragma unroll
     for (int b = 0; b < NL; ++b) Kr[b] = T(0);
-    if constexpr (QG > 0) {
+    if constexpr (KF == 1 && NL == 10 && QG > 0) {
+        // u_q = D^T (w_q grad phi_a(q)), then the structural row
+        T u[QG][3];
+#pragma unroll
+        for (int q = 0; q < QG; ++q) {
+            const T* ga = dg_l + (q * NL + a) * 3;
+            const T g0 = ga[0], g1 = ga[1], g2 = ga[2];
+            u[q][0] = fma(g0, D[0], fma(g1, D[3], g2 * D[6]));
+            u[q][1] = fma(g0, D[1], fma(g1, D[4], g2 * D[7]));
+            u[q][2] = fma(g0, D[2], fma(g1, D[5], g2 * D[8]));
+        }
+        const T* ct = F32 ? reinterpret_cast<const T*>(A.ctab32) : reinterpret_cast<const T*>(A.ctab);
+        struct_row<QG, T>(u, ct, Kr);
+    } else if constexpr (QG > 0) {
 #pragma unroll
         for (int q = 0; q < QG; ++q) grad_row<NL, T>(q, a, D, dg, dg_l, Kr);
     } else {
@@ -673,7 +734,7 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
             for (int b = 0; b < NL; ++b)
                 lds_add(row + C * rec_offset<RB, OFFW>(rec, b), double(Kr[b]), A.plan.lds_atomic);
         };
-        if (L.S == 1) {
+        if (L.S == 1 || SP == SP_CONCURRENT) {
             add_row();
         } else {
             for (int turn = 0; turn < L.S; ++turn)
@@ -685,7 +746,7 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
 // Software-pipelined gather: while batch b is computed, the payloads of
 // batch b+1 and the records of batch b+2 are in flight, so the two
 // dependent loads of a step (record -> payload) overlap other work.
-template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, int SP>
+template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, int SP, int KF = 0>
 __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
     extern __shared__ double span_lds[];
     constexpr int P = PayloadLayout<PL>::stride;
@@ -699,7 +760,7 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
     lane_setup<C>(A.plan, wv, lane, L);
     if (L.R < A.rmin || L.R > A.rmax) return;
     int64_t nsteps = L.steps;
-    if constexpr (SP == SP_INCIDENCE) {
+    if constexpr (SP == SP_INCIDENCE || SP == SP_CONCURRENT) {
         L.lmul = L.S;
         L.ladd = L.p;
         nsteps = (L.steps + L.S - 1) / L.S;
@@ -744,8 +805,8 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
             if (r0[k].packed >= 0)
-                compute_step<NL, C, PL, F32, RB, OFFW, QG, SP>(A, L, r0[k], p0[k], span_lds, Qg, Qm, mass, dg, pm,
-                                                               dg_l, pm_l, dgu_l, pmu_l);
+                compute_step<NL, C, PL, F32, RB, OFFW, QG, SP, KF>(A, L, r0[k], p0[k], span_lds, Qg, Qm, mass, dg,
+                                                                   pm, dg_l, pm_l, dgu_l, pmu_l);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             r0[k] = r1[k];
@@ -1722,16 +1783,23 @@ static int with_record_layout(const skfa_plan* P, F&& launch) {
     return kUnsupported;
 }
 
-template <int NL, int C, int PL, bool F32, int QG, int SP>
+template <int NL, int C, int PL, bool F32, int QG, int SP, int KF = 0>
 static int launch_gather_k(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
     const dim3 grid = wave_grid(P), block(64);
     if (SP == SP_ENTRY) lds += (size_t(A.Qg) * NL * 3 + size_t(A.Qm) * NL) * (F32 ? 4 : 8);
     return with_record_layout<NL>(P, [&](auto rb, auto ow) {
         constexpr int RB = decltype(rb)::value, OFFW = decltype(ow)::value;
-        k_gather_h1<NL, C, PL, F32, RB, OFFW, QG, SP><<<grid, block, lds, s>>>(A);
+        k_gather_h1<NL, C, PL, F32, RB, OFFW, QG, SP, KF><<<grid, block, lds, s>>>(A);
         SKFA_LAUNCH_CHECK();
         return kOk;
     });
+}
+
+// structural P2 rows (tuning key 8): scalar 10-node fields with the 4-point
+// reduced gradient rule, when the host supplied the coefficient tables
+template <int NL, int C, int QG>
+static bool structural_rows(const skfa_plan* P, const GatherArgs& A) {
+    return NL == 10 && C == 1 && QG == 4 && A.ctab != nullptr && P->tune_kform != 0;
 }
 
 // how the S lanes of a row share its work, from the plan's tuning
@@ -1739,7 +1807,15 @@ template <int NL, int C, int PL, bool F32, int QG>
 static int launch_gather_q(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
     if (P->tune_split == SP_ENTRY) return launch_gather_k<NL, C, PL, F32, QG, SP_ENTRY>(P, A, lds, s);
     if (P->tune_split == SP_COLUMN) return launch_gather_k<NL, C, PL, F32, QG, SP_COLUMN>(P, A, lds, s);
-    return launch_gather_k<NL, C, PL, F32, QG, SP_INCIDENCE>(P, A, lds, s);
+    // concurrent parts need the atomic LDS add
+    const bool conc = P->tune_split == SP_CONCURRENT && P->tune_lds_atomic != 0;
+    if constexpr (NL == 10 && C == 1 && QG == 4 && (PL == PL_SYM6 || PL == PL_SYM || PL == PL_SYM32)) {
+        if (structural_rows<NL, C, QG>(P, A))
+            return conc ? launch_gather_k<NL, C, PL, F32, QG, SP_CONCURRENT, 1>(P, A, lds, s)
+                        : launch_gather_k<NL, C, PL, F32, QG, SP_INCIDENCE, 1>(P, A, lds, s);
+    }
+    return conc ? launch_gather_k<NL, C, PL, F32, QG, SP_CONCURRENT>(P, A, lds, s)
+                : launch_gather_k<NL, C, PL, F32, QG, SP_INCIDENCE>(P, A, lds, s);
 }
 
 template <int NL, int C, int PL, bool F32>
@@ -2020,6 +2096,12 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
         G.tab32 = rtab32;
         G.Qg = nqg;
         G.Qm = nqm;
+        if ((flags & SKFA_FLAG_BARYCENTRIC_TABLE) && P->n == 10) {
+            // the 16 x Qg barycentric coefficients follow wg | dg | wm | pm
+            const size_t off = size_t(nqg) + size_t(nqg) * 30 + nqm + size_t(nqm) * 10;
+            G.ctab = rtab + off;
+            G.ctab32 = rtab32 ? rtab32 + off : nullptr;
+        }
         G.has_mass = c_value != nullptr;
         G.values = values;
         const size_t tab_len = size_t(nqg) + size_t(nqg) * P->n * 3 + nqm + size_t(nqm) * P->n;

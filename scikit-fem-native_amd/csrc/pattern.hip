@@ -676,8 +676,13 @@ int skfa_plan_set_tuning(skfa_plan* P, int32_t key, int32_t value) {
             P->tune_mfma = value;
             return kOk;
         case SKFA_TUNE_ROW_SPLIT:
-            SKFA_REQUIRE(value >= 0 && value <= 2, "row split must be 0 (incidence), 1 (entry) or 2 (column)");
+            SKFA_REQUIRE(value >= 0 && value <= 3,
+                         "row split must be 0 (incidence), 1 (entry), 2 (column) or 3 (concurrent incidence)");
             P->tune_split = value;
+            return kOk;
+        case SKFA_TUNE_ROW_FORM:
+            SKFA_REQUIRE(value >= -1 && value <= 1, "row form must be -1 (auto), 0 (table) or 1 (barycentric)");
+            P->tune_kform = value;
             return kOk;
         default:
             set_error("unknown tuning key");

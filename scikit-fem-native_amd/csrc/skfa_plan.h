@@ -76,7 +76,9 @@ struct skfa_plan {
     // launch tuning (skfa_plan_set_tuning): XCD chunk of waves (0 = plain
     // order) and LDS accumulation by ds_add (1) or read-add-write (0)
     int32_t tune_table = -1;      // element-table gather for symmetric scalar forms: -1 auto, 0 off, 1 on
-    int32_t tune_split = 0;       // quadrature gather rows with S lanes: 0 incidence, 1 entry, 2 column split
+    int32_t tune_split = 0;       // quadrature gather rows with S lanes: 0 incidence, 1 entry, 2 column split,
+                                  // 3 concurrent incidence split
+    int32_t tune_kform = -1;      // 10-node scalar rows: 1 barycentric structure, 0 full table, -1 auto (1)
     int32_t tune_mfma = -1;       // MFMA row contraction for scalar fields: -1/0 off, 1 all waves, 2 waves <= 16 rows
     int32_t tune_placement = 2;   // 0 plain, 1 XCD segments, 2 XCD chunks
     int32_t tune_xcd_chunk = 16;

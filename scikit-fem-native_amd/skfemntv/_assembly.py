@@ -23,7 +23,7 @@ from ._symbols import UnsupportedNativeForm
 
 # ------------------------------------------------------------------ plans
 _TUNE_KEYS = {"xcd_chunk": 1, "lds_atomic": 2, "placement": 3, "streaming": 4, "element_table": 5,
-              "row_split": 6, "mfma": 7}
+              "row_split": 6, "mfma": 7, "row_form": 8}
 
 
 class Plan:
@@ -39,7 +39,7 @@ class Plan:
         self.lds_budget = int(lds_budget)
         self.min_waves = int(min_waves)
         self.tuning = {"xcd_chunk": 16, "lds_atomic": 1, "placement": 2, "streaming": 1, "element_table": -1,
-                       "row_split": 0, "mfma": -1}
+                       "row_split": 0, "mfma": -1, "row_form": -1}
         info = (ctypes.c_int64 * 9)()
         _native.check(L.skfa_plan_info(h, info))
         self.nrows, self.nnz, self.nnz_node = int(info[0]), int(info[1]), int(info[2])
@@ -201,6 +201,8 @@ def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=No
         plan = basis_scalar_plan(basis)
         flags |= _native.FLAG_VECTOR3_DIAGONAL
     tables = d["tables_iso"] if iso else d["tables"]
+    if d.get("rtab_barycentric"):
+        flags |= _native.FLAG_BARYCENTRIC_TABLE
     prec = {"fp64": 0, "fp32": 1}[precision]
     if prec == 1 and (iso or per_qp):
         raise UnsupportedNativeForm("FP32 products are available for affine elements with element-constant "

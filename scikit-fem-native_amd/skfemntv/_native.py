@@ -37,6 +37,7 @@ FSAI_FRAG = 34816
 
 FLAG_SYMMETRIC_TENSOR = 1
 FLAG_VECTOR3_DIAGONAL = 2
+FLAG_BARYCENTRIC_TABLE = 4
 TUNE_XCD_CHUNK = 1
 TUNE_LDS_ATOMIC = 2
 TUNE_PLACEMENT = 3
@@ -44,6 +45,7 @@ TUNE_STREAMING = 4
 TUNE_ELEMENT_TABLE = 5
 TUNE_ROW_SPLIT = 6
 TUNE_MFMA = 7
+TUNE_ROW_FORM = 8
 
 
 class NativeUnavailable(RuntimeError):

@@ -144,18 +144,43 @@ def reduced_tables(X, W, quadratic):
 
     G = gsum(W, g)
     if not quadratic and np.array_equal(g, np.broadcast_to(g[:1], g.shape)):
-        wg, dg = np.array([W.sum()]), g[:1]
+        wg, dg, Xg = np.array([W.sum()]), g[:1], None
     elif np.allclose(gsum(W4, g4), G, rtol=1e-14, atol=1e-15 * np.abs(G).max()):
-        wg, dg = W4, g4
+        wg, dg, Xg = W4, g4, X4
     else:
-        wg, dg = W, g
+        wg, dg, Xg = W, g, X
     M = msum(W, s)
     if np.allclose(msum(W4, s4), M, rtol=1e-14, atol=1e-15 * np.abs(M).max()):
         wm, pm = W4, s4
     else:
         wm, pm = W, s
     tab = np.concatenate([wg, dg.ravel(), wm, pm.ravel()]).astype(np.float64)
+    if quadratic:
+        tab = np.concatenate([tab, barycentric_coefficients(Xg, dg).ravel()])
     return tab, len(wg), len(wm)
+
+
+def barycentric_coefficients(X, dg):
+    """(16, Q) coefficients c_bm(q) of the P2 reference gradients on the
+    barycentric gradients, grad phi_b(q) = sum_m c_bm(q) grad L_m: vertex k
+    (pair k) 4 L_k - 1; edge e = (i, j) (pairs 4 + 2e, 5 + 2e) 4 L_j on
+    grad L_i and 4 L_i on grad L_j.  Checked against the tabulated gradients."""
+    X = np.asarray(X, dtype=np.float64)
+    bary = np.vstack((1. - X.sum(axis=0), X)).T  # (Q,4)
+    c = np.empty((16, X.shape[1]))
+    for k in range(4):
+        c[k] = 4. * bary[:, k] - 1.
+    for e, (i, j) in enumerate(_P2_PAIRS):
+        c[4 + 2 * e] = 4. * bary[:, j]
+        c[5 + 2 * e] = 4. * bary[:, i]
+    rec = np.empty_like(dg)
+    for k in range(4):
+        rec[:, k] = c[k][:, None] * _DL[k]
+    for e, (i, j) in enumerate(_P2_PAIRS):
+        rec[:, 4 + e] = c[4 + 2 * e][:, None] * _DL[i] + c[5 + 2 * e][:, None] * _DL[j]
+    if not np.allclose(rec, dg, rtol=0., atol=1e-14 * max(1., np.abs(dg).max())):
+        raise AssertionError("barycentric coefficients do not reproduce the P2 gradients")
+    return c
 
 
 def _is_affine_p2(p, t):
@@ -315,6 +340,8 @@ class Basis:
                 # tables for affine assembly (gphi over the 4 corners)
                 "tables": torch.from_numpy(affine_tables).to(dev),
                 "rtab": torch.from_numpy(rtab).to(dev),
+                # 10-node rtab carries the barycentric coefficients after pm
+                "rtab_barycentric": self._quadratic and len(rtab) == nqg * 31 + nqm * 11 + 16 * nqg,
                 "rtab32": torch.from_numpy(rtab.astype(np.float32)).to(dev),
                 "nqg": nqg,
                 "nqm": nqm,

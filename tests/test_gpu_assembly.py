@@ -166,18 +166,23 @@ def test_symmetric_tensor_formulations_match_oracle(quadratic):
     form = NativeCrossBilinearForm(basis, basis)
     Ad = upload(A)
     out, raw = {}, {}
-    for table, mfma, split in ((1, 0, 0), (0, 0, 0), (0, 0, 1), (0, 0, 2), (0, 1, 0), (0, 2, 0)):
-        form.plan.set_tuning(element_table=table, mfma=mfma, row_split=split)
+    # (element table, mfma, row split, row form): row form 1 (barycentric
+    # rows, P2 with the 4-point rule) and split 3 (concurrent parts) round
+    # differently; the others must agree bit for bit across splits
+    for table, mfma, split, form_ in ((1, 0, 0, 0), (0, 0, 0, 0), (0, 0, 1, 0), (0, 0, 2, 0), (0, 1, 0, 0),
+                                      (0, 2, 0, 0), (0, 0, 0, 1), (0, 0, 3, 1), (0, 0, 3, 0)):
+        form.plan.set_tuning(element_table=table, mfma=mfma, row_split=split, row_form=form_)
         for prec in ("fp64", "fp32"):
             v = form.assemble_tensor_device(Ad, precision=prec, symmetric=True).clone().cpu().numpy()[:len(vals)]
-            out[(table, mfma, split, prec)] = pipeline.relative_frobenius(v, vals)
-            raw[(table, mfma, split, prec)] = v
+            out[(table, mfma, split, form_, prec)] = pipeline.relative_frobenius(v, vals)
+            raw[(table, mfma, split, form_, prec)] = v
     # the row splits change only which lane computes or adds an entry: same bits
     for prec in ("fp64", "fp32"):
-        np.testing.assert_array_equal(raw[(0, 0, 1, prec)], raw[(0, 0, 0, prec)])
-        np.testing.assert_array_equal(raw[(0, 0, 2, prec)], raw[(0, 0, 0, prec)])
-    form.plan.set_tuning(element_table=-1, mfma=-1, row_split=0)
-    say("rel errors " + ", ".join(f"table={k[0]} mfma={k[1]} split={k[2]} {k[3]}: {e:.2e}" for k, e in out.items()))
+        np.testing.assert_array_equal(raw[(0, 0, 1, 0, prec)], raw[(0, 0, 0, 0, prec)])
+        np.testing.assert_array_equal(raw[(0, 0, 2, 0, prec)], raw[(0, 0, 0, 0, prec)])
+    form.plan.set_tuning(element_table=-1, mfma=-1, row_split=0, row_form=-1)
+    say("rel errors " + ", ".join(f"table={k[0]} mfma={k[1]} split={k[2]} form={k[3]} {k[4]}: {e:.2e}"
+                                  for k, e in out.items()))
     for key, e in out.items():
         assert e <= (FP64_TOL if key[-1] == "fp64" else FP32_TOL), (key, e)
 
@@ -220,7 +225,7 @@ def test_row_splits_agree_at_small_lds_budgets(components):
         plan = Plan(d["node_conn"], E, 10, d["nnodes"], components, lds_budget=budget)
         raw = {}
         for split in (0, 1, 2):
-            plan.set_tuning(row_split=split)
+            plan.set_tuning(row_split=split, row_form=0)
             if components == 1:
                 form.plan = plan
                 v = form.assemble_tensor_device(Ad, precision="fp64", symmetric=True).clone().cpu().numpy()
@@ -236,6 +241,12 @@ def test_row_splits_agree_at_small_lds_budgets(components):
         say(f"budget {budget}: waves {plan.waves}")
         np.testing.assert_array_equal(raw[1], raw[0])
         np.testing.assert_array_equal(raw[2], raw[0])
+        if components == 1:
+            # barycentric rows with concurrent parts: within the FP64 bound
+            plan.set_tuning(row_split=3, row_form=1)
+            form.plan = plan
+            v = form.assemble_tensor_device(Ad, precision="fp64", symmetric=True).clone().cpu().numpy()
+            assert pipeline.relative_frobenius(v[:len(vals)], vals) <= FP64_TOL
 
 
 def test_vector_p2_elasticity_matches_oracle():
@@ -364,6 +375,25 @@ def test_assembly_is_bitwise_reproducible():
     a = form.assemble_tensor_device(A).clone()
     b = form.assemble_tensor_device(A).clone()
     assert bool((a == b).all())
+
+
+def test_concurrent_row_split_is_reproducible():
+    """Row split 3 lets the S lanes of a row add in one instruction; the LDS
+    unit serialises lanes that hit one address.  Ten repeats must give the
+    same bits (a wave's lanes are applied in a fixed order)."""
+    import skfemntv as sk
+    from skfemntv._assembly import NativeCrossBilinearForm, upload
+    say("concurrent row split reproducibility")
+    p, t = kuhn(10, 0.2, seed=9)
+    p2, t2 = host.p2_mesh(p, t)
+    basis = sk.Basis(sk.MeshTet2(p2, t2), sk.ElementVector(sk.ElementTetP2(), dim=1), quadrature=host.tet_rule_4())
+    form = NativeCrossBilinearForm(basis, basis)
+    assert form.plan.waves > 0
+    form.plan.set_tuning(row_split=3, row_form=1)
+    A = upload(spd_tensors(t.shape[1], 4))
+    first = form.assemble_tensor_device(A, symmetric=True).clone()
+    for _ in range(9):
+        assert bool((form.assemble_tensor_device(A, symmetric=True) == first).all())
 
 
 def test_geometry_diagnostics_and_rejection():

@@ -86,7 +86,21 @@ def test_reduced_tables_reproduce_rule_sums(quadratic):
         dg = tab[qg:qg + qg * n * 3].reshape(qg, n, 3)
         o = qg + qg * n * 3
         wm = tab[o:o + qm]
-        pm = tab[o + qm:].reshape(qm, n)
+        pm = tab[o + qm:o + qm + qm * n].reshape(qm, n)
+        if quadratic:
+            # barycentric coefficients of the gradient rule follow: they
+            # rebuild the tabulated gradients from grad L_m
+            c = tab[o + qm + qm * n:].reshape(16, qg)
+            DL = np.array([[-1., -1., -1.], [1., 0., 0.], [0., 1., 0.], [0., 0., 1.]])
+            pairs = [(k, k) for k in range(4)]
+            for e, (i, j) in enumerate(((0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3))):
+                pairs += [(4 + e, i), (4 + e, j)]
+            rec = np.zeros_like(dg)
+            for p_, (b, m) in enumerate(pairs):
+                rec[:, b] += c[p_][:, None] * DL[m]
+            np.testing.assert_allclose(rec, dg, rtol=0., atol=1e-14)
+        else:
+            assert len(tab) == o + qm + qm * n
         G = np.einsum("q,qai,qbj->aibj", W, g, g)
         M = np.einsum("q,qa,qb->ab", W, s, s)
         np.testing.assert_allclose(np.einsum("q,qai,qbj->aibj", wg, dg, dg), G, rtol=1e-13, atol=1e-15)
