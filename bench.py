@@ -423,8 +423,24 @@ def build_workload(name):
     else:
         raise SystemExit(f"unknown workload {name}")
     torch.cuda.synchronize()
+    # first call (the autotuner times the launch shapes of a new mesh shape here)
+    step("fp64")
+    torch.cuda.synchronize()
+    plan = {"c1": lambda: form.plan, "c3": lambda: form.plan, "c4": lambda: asm_.plan}.get(name, lambda: None)()
     return {"name": name, "step": step, "values": values, "fp32": fp32, "E": E, "nnz": nnz, "bpe": bpe,
-            "desc": desc, "setup_s": time.perf_counter() - t0}
+            "desc": desc, "setup_s": time.perf_counter() - t0, "plan": plan}
+
+
+def autotune_summary(plan):
+    """The launch shape the autotuner chose for this plan's mesh signature
+    (timed on the first assembly of a plan of this shape)."""
+    from skfemntv import autotune
+    rec = autotune.LAST.get(autotune.signature(plan)) or autotune._CACHE.get(autotune.signature(plan))
+    if rec is None:
+        return None
+    return {"lds_budget": rec.budget, "settings": rec.settings, "best_ms": rec.seconds_per_call * 1e3,
+            "candidates": rec.candidates, "tuning_s": rec.tuning_seconds,
+            "current": {"lds_budget": plan.lds_budget, **{k: plan.tuning[k] for k in rec.settings}}}
 
 
 def measure(w, steps, warmup, ramp_s, timer):
@@ -504,7 +520,8 @@ def headline(args, timer):
     # the recommended (3,3,E,Q) layout, a broadcast view of the per-element tensors
     diffusion = np.broadcast_to(np.moveaxis(A_host, 0, -1)[..., None], (3, 3, E, 4))
     # cold first call: geometry check, device plan (pattern + gather map),
-    # coefficient upload, kernels and the CSR copied to the host
+    # the autotuner's timing of launch shapes (first plan of this mesh
+    # shape only), coefficient upload, kernels and the CSR copied to the host
     t1 = time.perf_counter()
     basis_cold = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP2(), dim=1), quadrature=(X, W))
     K = sk.asm(aniso, basis_cold, diffusion=diffusion)
@@ -592,6 +609,7 @@ def headline(args, timer):
                        "note": "untimed FP64+FP32 assemblies before the warm-up, until the GPU is at steady clocks"},
         "plan_setup_s": plan_s,
         "cold_first_asm_s": cold_asm_s,
+        "autotune": autotune_summary(plan),
     }
     # warm end-to-end asm on the same workload: form tracing, the coefficient
     # upload, payload + gather, and the scipy CSR copied back to the host
@@ -648,6 +666,8 @@ def main():
             _, summary = measure(w, args.steps, args.warmup, 0.2, timer)
             summary["workload"] = w["desc"]
             summary["elements"], summary["nnz"] = w["E"], w["nnz"]
+            if w["plan"] is not None:
+                summary["autotune"] = autotune_summary(w["plan"])
             configs[name] = summary
             del w
             torch.cuda.empty_cache()

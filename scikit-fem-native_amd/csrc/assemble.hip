@@ -124,6 +124,7 @@ struct PlanArgs {
     int32_t lds_atomic;
     int32_t streaming;       // non-temporal record loads and value stores
     int32_t diag3 = 0;       // scalar plan written as the 3x3 diagonal blocks of a 3-component field
+    int32_t concurrent = 0;  // row parts add in the same instructions (row split 3 with ds_add)
 };
 
 // Lane role inside a wave: node row r0+i, components (c,d) of the CxC node
@@ -974,9 +975,10 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
                     v[c][d] = double(e);
                 }
             double* col = span_lds + int(L.rb) + C * pick(r0[k]);
-            // the S parts of a row add in turn (one writer per address)
+            // the S parts of a row add in turn (one writer per address), or
+            // at once with the LDS atomic serialising shared addresses
             for (int turn = 0; turn < L.S; ++turn)
-                if (L.p == turn) {
+                if (L.p == turn || PA.concurrent) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -1343,8 +1345,14 @@ __device__ inline void n1_step(const GatherN1Args& A, const Lane& L, const Recor
     const double sa = ((bits >> a) & 1) ? -1. : 1.;
 #pragma unroll
     for (int b = 0; b < 6; ++b) Kr[b] *= sa * (((bits >> b) & 1) ? -1. : 1.);
-    // one writer per address: the S parts of a row add in turn
+    // one writer per address: the S parts of a row add in turn (or all at
+    // once, the LDS atomic serialising shared addresses)
     double* row = span_lds + int(L.rb);
+    if (L.S == 1 || A.plan.concurrent) {
+#pragma unroll
+        for (int b = 0; b < 6; ++b) lds_add(row + rec_offset<RB, OFFW>(rec, b), Kr[b], A.plan.lds_atomic);
+        return;
+    }
     for (int turn = 0; turn < L.S; ++turn)
         if (L.p == turn) {
 #pragma unroll
@@ -1746,6 +1754,7 @@ static PlanArgs plan_args(const skfa_plan* P) {
     A.xcd_chunk = P->tune_xcd_chunk;
     A.lds_atomic = P->tune_lds_atomic;
     A.streaming = P->tune_streaming;
+    A.concurrent = P->tune_split == SP_CONCURRENT && P->tune_lds_atomic != 0;
     return A;
 }
 

@@ -29,28 +29,57 @@ _TUNE_KEYS = {"xcd_chunk": 1, "lds_atomic": 2, "placement": 3, "streaming": 4, "
 class Plan:
     """Device CSR pattern + gather plan of one (basis, components) pair."""
 
-    def __init__(self, node_conn_dev, E, n, nnodes, components, lds_budget=12 * 1024, min_waves=2048):
+    def __init__(self, node_conn_dev, E, n, nnodes, components, lds_budget=None, min_waves=2048):
         import torch
-        L = _native.lib()
-        h = ctypes.c_void_p()
-        _native.check(L.skfa_plan_create_ex(_native.ptr(node_conn_dev), E, n, nnodes, components,
-                                            int(lds_budget), int(min_waves), _native.stream(), ctypes.byref(h)))
-        self._h = h
-        self.lds_budget = int(lds_budget)
+        from . import autotune
+        if lds_budget is None:
+            # the budget the autotuner chose for a plan of this shape, else
+            # the default (scalar fields 12 KiB, 3-component fields 16 KiB)
+            lds_budget = autotune.cached_budget(E, n, nnodes, components,
+                                                16 * 1024 if components == 3 else 12 * 1024)
+        self._conn = node_conn_dev
+        self._h = None
+        self.E, self.n, self.nnodes, self.components = E, n, nnodes, components
         self.min_waves = int(min_waves)
         self.tuning = {"xcd_chunk": 16, "lds_atomic": 1, "placement": 2, "streaming": 1, "element_table": -1,
                        "row_split": 0, "mfma": -1, "row_form": -1}
+        self._create(lds_budget)
+        L = _native.lib()
+        dev = _native.device()
+        self.indptr = torch.empty(self.nrows + 1, dtype=torch.int64, device=dev)
+        self.indices = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
+        _native.check(L.skfa_plan_export_csr(self._h, _native.ptr(self.indptr), _native.ptr(self.indices),
+                                             _native.stream()))
+        self._host = None
+
+    def _create(self, lds_budget):
+        L = _native.lib()
+        h = ctypes.c_void_p()
+        _native.check(L.skfa_plan_create_ex(_native.ptr(self._conn), self.E, self.n, self.nnodes, self.components,
+                                            int(lds_budget), self.min_waves, _native.stream(), ctypes.byref(h)))
+        old = self._h
+        self._h = h
+        self.lds_budget = int(lds_budget)
         info = (ctypes.c_int64 * 9)()
         _native.check(L.skfa_plan_info(h, info))
         self.nrows, self.nnz, self.nnz_node = int(info[0]), int(info[1]), int(info[2])
         self.max_row_len, self.max_inc = int(info[3]), int(info[4])
         self.waves, self.steps, self.rec_bytes, self.buckets = int(info[5]), int(info[6]), int(info[7]), int(info[8])
-        dev = _native.device()
-        self.indptr = torch.empty(self.nrows + 1, dtype=torch.int64, device=dev)
-        self.indices = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
-        _native.check(L.skfa_plan_export_csr(h, _native.ptr(self.indptr), _native.ptr(self.indices), _native.stream()))
-        self._host = None
-        self.E, self.n, self.nnodes, self.components = E, n, nnodes, components
+        for name, value in self.tuning.items():
+            _native.check(L.skfa_plan_set_tuning(h, _TUNE_KEYS[name], int(value)))
+        if old is not None and old.value:
+            import torch
+            torch.cuda.synchronize()   # no launch may still use the old plan
+            L.skfa_plan_destroy(old)
+
+    def rebuild(self, lds_budget):
+        """Re-layout the gather waves for another LDS span budget.  The
+        pattern (indptr, indices) is the same for every budget; the tuning
+        settings are kept."""
+        nnz = self.nnz
+        self._create(lds_budget)
+        if self.nnz != nnz:
+            raise RuntimeError("plan rebuild changed the pattern")
 
     @property
     def handle(self):
@@ -101,12 +130,9 @@ def basis_plan(basis):
     if plan is None:
         d = basis.device_data()
         C = basis.elem._dim
-        # vector fields carry 9-entry node blocks: a larger span per wave keeps
-        # more rows per wave (measured on 48^3 vector P2: 2.50 ms at 16 KiB,
-        # 2.72 ms at 32 KiB, 2.97 ms at 12 KiB; scalar P2 is fastest at 12 KiB)
-        budget = 16 * 1024 if C == 3 else 12 * 1024
-        plan = Plan(d["node_conn"], basis.mesh.nelements, d["node_conn"].shape[1], d["nnodes"], C,
-                    lds_budget=budget)
+        # the LDS span budget comes from the autotuner's cache for this mesh
+        # shape, else the default of Plan
+        plan = Plan(d["node_conn"], basis.mesh.nelements, d["node_conn"].shape[1], d["nnodes"], C)
         basis._plan = plan
     return plan
 
@@ -208,12 +234,21 @@ def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=No
         raise UnsupportedNativeForm("FP32 products are available for affine elements with element-constant "
                                     "coefficients")
     entry = L.skfa_assemble_bilinear_noalloc if noalloc else L.skfa_assemble_bilinear
-    _native.check(entry(
-        plan.handle, _native.ptr(d["xyz"]), d["xyz"].shape[0], _native.ptr(d["vconn"]),
-        _native.ptr(d["gconn"]) if iso else None, d["gnodes"] if iso else 4,
-        _native.ptr(tables), len(basis.W), _native.ptr(d["rtab"]), _native.ptr(d["rtab32"]), d["nqg"], d["nqm"],
-        _native.ptr(value), _native.ptr(gradient), _native.ptr(symmetric_gradient), _native.ptr(divergence),
-        _native.ptr(tensor), int(per_qp), prec, flags, _native.ptr(values_dev), _native.stream()))
+
+    def launch():
+        _native.check(entry(
+            plan.handle, _native.ptr(d["xyz"]), d["xyz"].shape[0], _native.ptr(d["vconn"]),
+            _native.ptr(d["gconn"]) if iso else None, d["gnodes"] if iso else 4,
+            _native.ptr(tables), len(basis.W), _native.ptr(d["rtab"]), _native.ptr(d["rtab32"]), d["nqg"],
+            d["nqm"], _native.ptr(value), _native.ptr(gradient), _native.ptr(symmetric_gradient),
+            _native.ptr(divergence), _native.ptr(tensor), int(per_qp), prec, flags, _native.ptr(values_dev),
+            _native.stream()))
+
+    if not noalloc and not iso and not per_qp:
+        # element-constant fast path: pick the launch shape on first use
+        from . import autotune
+        autotune.maybe_tune(plan, launch)
+    launch()
 
 
 def _prepare_scalar_kinds(basis, coefficients):

@@ -1,0 +1,186 @@
+"""Launch-shape autotuner of the row-gather kernels.
+
+The gather's speed depends on choices that no formula settles well across
+meshes and GPUs: the LDS budget of a wave's CSR span (it fixes how many
+consecutive node rows a 64-lane wave owns and how many lanes share a row),
+how the lanes of one row share its incidences (in turn, or concurrently
+with the LDS atomic add resolving shared addresses), the element-row
+formulation of P2 rows (full gradient table, or the barycentric structure
+of the reference gradients), the element-table gather of 4-node forms and
+the chunk of consecutive waves dealt to one XCD.
+
+On the first assembly of a plan with at least ``MIN_ELEMENTS`` elements
+(outside HIP-graph capture), the tuner times every candidate on the live
+inputs and keeps the fastest; the choice is cached per mesh signature
+(elements, local nodes, node rows, components), so later plans of the same
+shape start from it without timing.  Every candidate computes the same
+matrix: FP64 results differ by rounding only (the oracle tests hold each
+formulation to 1e-12), and the span budget and XCD chunk do not change the
+bits at all.
+
+Timing: a 0.15 s clock ramp, then per candidate two warm-up calls and three
+blocks of four calls timed with HIP events on the launch stream; the
+candidate's score is its median block.  The reference has no counterpart:
+its only shape choice is the static per-colour thread partition
+(cpp/include/native_fem/parallel.hpp:21-41).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import itertools
+import time
+from dataclasses import dataclass, field
+
+from . import _native
+
+MIN_ELEMENTS = 100_000
+_enabled = True
+_CACHE: dict = {}
+LAST: dict = {}   # signature -> TuningRecord of the last tuning run (reporting)
+
+
+@dataclass
+class TuningRecord:
+    budget: int
+    settings: dict
+    seconds_per_call: float
+    candidates: int
+    tuning_seconds: float
+    table: list = field(default_factory=list)
+
+
+def set_enabled(flag: bool) -> None:
+    """Switch the autotuner on or off (process-wide)."""
+    global _enabled
+    _enabled = bool(flag)
+
+
+def enabled() -> bool:
+    return _enabled
+
+
+def signature(plan):
+    return (plan.E, plan.n, plan.nnodes, plan.components)
+
+
+def cached_budget(E, n, nnodes, components, default):
+    """LDS budget a new plan of this shape should start from."""
+    rec = _CACHE.get((E, n, nnodes, components))
+    return rec.budget if rec is not None else default
+
+
+def candidates(plan):
+    """(budgets, setting dicts) to try for a plan's kernels."""
+    n, C = plan.n, plan.components
+    chunks = (8, 16, 32)
+    if C == 3:
+        budgets = (12 * 1024, 16 * 1024, 20 * 1024, 24 * 1024)
+        variants = [{"row_split": s} for s in (0, 3)]
+    elif n == 10:
+        budgets = (8 * 1024, 10 * 1024, 12 * 1024, 14 * 1024, 16 * 1024)
+        variants = [{"row_split": s, "row_form": f} for s, f in itertools.product((0, 3), (0, 1))]
+    elif n == 4:
+        budgets = (8 * 1024, 12 * 1024, 16 * 1024)
+        variants = [{"element_table": t} for t in (-1, 0)]
+    else:   # edge plans (N1) and other element types
+        budgets = (8 * 1024, 12 * 1024, 16 * 1024)
+        variants = [{"row_split": s} for s in (0, 3)]
+    return budgets, variants, chunks
+
+
+class _Timer:
+    def __init__(self):
+        L = _native.lib()
+        self.L = L
+        self.ev = (ctypes.c_void_p(), ctypes.c_void_p())
+        for e in self.ev:
+            _native.check(L.skfa_event_create(ctypes.byref(e)))
+
+    def per_call(self, run, calls):
+        L = self.L
+        s = _native.stream()
+        _native.check(L.skfa_event_record(self.ev[0], s))
+        for _ in range(calls):
+            run()
+        _native.check(L.skfa_event_record(self.ev[1], s))
+        ms = ctypes.c_float()
+        _native.check(L.skfa_event_elapsed_ms(self.ev[0], self.ev[1], ctypes.byref(ms)))
+        return ms.value * 1e-3 / calls
+
+    def close(self):
+        for e in self.ev:
+            self.L.skfa_event_destroy(e)
+
+
+def _score(timer, run):
+    run()
+    run()
+    blocks = sorted(timer.per_call(run, 4) for _ in range(3))
+    return blocks[1]
+
+
+def maybe_tune(plan, run):
+    """Tune `plan` on its first qualifying assembly.  `run()` performs one
+    assembly with the plan's current layout and settings."""
+    if getattr(plan, "_tuned", False) or not _enabled or plan.E < MIN_ELEMENTS:
+        return None
+    import torch
+    if torch.cuda.is_current_stream_capturing():
+        return None
+    plan._tuned = True
+    sig = signature(plan)
+    rec = _CACHE.get(sig)
+    if rec is not None:
+        apply(plan, rec)
+        return rec
+    rec = tune(plan, run)
+    _CACHE[sig] = rec
+    LAST[sig] = rec
+    return rec
+
+
+def apply(plan, rec):
+    if plan.lds_budget != rec.budget:
+        plan.rebuild(rec.budget)
+    plan.set_tuning(**rec.settings)
+
+
+def tune(plan, run):
+    """Time every candidate and leave `plan` at the fastest."""
+    import torch
+    t0 = time.perf_counter()
+    budgets, variants, chunks = candidates(plan)
+    timer = _Timer()
+    try:
+        ramp_end = time.perf_counter() + 0.15
+        while time.perf_counter() < ramp_end:
+            for _ in range(4):
+                run()
+            torch.cuda.synchronize()
+        table = []
+        # span budget x formulation at the default chunk, then the chunk
+        for budget in budgets:
+            plan.rebuild(budget)
+            for v in variants:
+                plan.set_tuning(**v)
+                table.append((budget, dict(v), _score(timer, run)))
+        budget, best_v, best_t = min(table, key=lambda r: r[2])
+        plan.rebuild(budget)
+        plan.set_tuning(**best_v)
+        best_chunk = plan.tuning["xcd_chunk"]
+        for c in chunks:
+            if c == best_chunk:
+                continue
+            plan.set_tuning(xcd_chunk=c)
+            t = _score(timer, run)
+            table.append((budget, {**best_v, "xcd_chunk": c}, t))
+            if t < best_t:
+                best_t, best_chunk = t, c
+        settings = {**best_v, "xcd_chunk": best_chunk}
+        plan.set_tuning(**settings)
+    finally:
+        timer.close()
+    return TuningRecord(budget=budget, settings=settings, seconds_per_call=best_t, candidates=len(table),
+                        tuning_seconds=time.perf_counter() - t0,
+                        table=[(b, s, t) for b, s, t in table])

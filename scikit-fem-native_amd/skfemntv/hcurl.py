@@ -290,10 +290,16 @@ class TetN1Assembler:
             tables, nq = self._tables4, 4
         else:
             tables, nq = self._tables_basis, len(self.basis.W)
-        _native.check(L.skfa_assemble_tet_n1(
-            self.plan.handle, _native.ptr(self._xyz), _native.ptr(self.basis._vconn_dev), _native.ptr(tables), nq,
-            _native.ptr(kappa), _native.ptr(mu), int(per_qp), int(include_mass), int(include_curl),
-            _native.ptr(vals), _native.stream()))
+        def launch():
+            _native.check(L.skfa_assemble_tet_n1(
+                self.plan.handle, _native.ptr(self._xyz), _native.ptr(self.basis._vconn_dev), _native.ptr(tables),
+                nq, _native.ptr(kappa), _native.ptr(mu), int(per_qp), int(include_mass), int(include_curl),
+                _native.ptr(vals), _native.stream()))
+
+        if not per_qp:
+            from . import autotune
+            autotune.maybe_tune(self.plan, launch)
+        launch()
         return vals
 
     def assemble_mass(self, coefficient=None):

@@ -1,0 +1,96 @@
+"""The launch-shape autotuner picks a candidate within 5% of the best one.
+
+On the C2 (P2 anisotropic, 64^3 jittered) and C3 (vector-P2 elasticity,
+48^3 jittered) workloads, the tuner's choice is re-timed against every
+candidate it considered, each with more repetitions than the tuner used;
+its choice must be within 5% of the fastest.  The tuned matrix must equal
+the default layout's to rounding (the oracle parity of every formulation is
+tested elsewhere).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def retime(plan, run, budget, settings, timer, blocks=5, calls=8):
+    plan.rebuild(budget)
+    plan.set_tuning(**settings)
+    run()
+    run()
+    return float(np.median([timer.per_call(run, calls) for _ in range(blocks)]))
+
+
+def check_pick(plan, run, rec):
+    from skfemntv import autotune
+    timer = autotune._Timer()
+    try:
+        chosen = retime(plan, run, rec.budget, rec.settings, timer)
+        best = min(retime(plan, run, b, {**rec.settings, **s}, timer) for b, s, _ in rec.table)
+    finally:
+        timer.close()
+    say(f"tuned {rec.budget} {rec.settings}: {chosen * 1e3:.4f} ms, best candidate {best * 1e3:.4f} ms")
+    assert chosen <= 1.05 * best, (chosen, best)
+
+
+def test_autotuner_pick_c2():
+    import torch
+    import skfemntv as sk
+    from skfemntv import autotune
+    from skfemntv._assembly import NativeCrossBilinearForm, upload
+    sys.path.insert(0, ".")
+    import bench
+    say("autotune C2 64^3")
+    mesh = bench.jittered_p2_cube(64, 0.2, 0)
+    basis = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP2(), dim=1), quadrature=sk.basis.tet_rule_4())
+    form = NativeCrossBilinearForm(basis, basis)
+    A = upload(bench.spd_tensors(mesh.nelements, 1))
+    autotune._CACHE.clear()
+    plan = form.plan
+    plan._tuned = True     # default layout first, untuned
+    ref = form.assemble_tensor_device(A, symmetric=True).clone()
+    plan._tuned = False
+    tuned = form.assemble_tensor_device(A, symmetric=True).clone()
+    rec = autotune.LAST[autotune.signature(plan)]
+    assert rec.candidates >= 20
+    err = float(torch.linalg.norm(tuned - ref) / torch.linalg.norm(ref))
+    assert err <= 1e-13, err
+    check_pick(plan, lambda: form.assemble_tensor_device(A, symmetric=True), rec)
+
+
+def test_autotuner_pick_c3():
+    import torch
+    import skfemntv as sk
+    from skfemntv import autotune
+    from skfemntv._assembly import NativeBilinearForm, upload
+    say("autotune C3 48^3")
+    n = 48
+    ax = np.linspace(0., 1., n + 1)
+    m = sk.MeshTet.init_tensor(ax, ax, ax)
+    p = m.p.copy()
+    interior = np.all((p > 1e-12) & (p < 1 - 1e-12), axis=0)
+    p[:, interior] += np.random.default_rng(0).uniform(-0.15 / n, 0.15 / n, size=(3, int(interior.sum())))
+    mesh = sk.MeshTet2.from_mesh(sk.MeshTet(p, m.t))
+    basis = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP2(), dim=3), quadrature=sk.basis.tet_rule_4())
+    E = mesh.nelements
+    rng = np.random.default_rng(1)
+    s_dev, l_dev = upload(rng.uniform(0.5, 2., E)), upload(rng.uniform(0.5, 2., E))
+    autotune._CACHE.clear()
+    form = NativeBilinearForm(basis)
+    plan = form.plan
+    plan._tuned = True
+    ref = form.assemble_device(symmetric_gradient=s_dev, divergence=l_dev).clone()
+    plan._tuned = False
+    tuned = form.assemble_device(symmetric_gradient=s_dev, divergence=l_dev).clone()
+    rec = autotune.LAST[autotune.signature(plan)]
+    err = float(torch.linalg.norm(tuned - ref) / torch.linalg.norm(ref))
+    assert err <= 1e-13, err
+    check_pick(plan, lambda: form.assemble_device(symmetric_gradient=s_dev, divergence=l_dev), rec)
