@@ -202,15 +202,37 @@ __device__ inline void lds_add(double* p, double v, bool atomic) {
         *p += v;
 }
 
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// The span starts at the (16-byte aligned) base of the dynamic LDS: zero it
+// two doubles per lane and instruction.
 __device__ inline void zero_span(double* lds, int64_t span, int lane) {
-    for (int64_t k = lane; k < span; k += 64) lds[k] = 0.;
+    f64x2* l2 = reinterpret_cast<f64x2*>(lds);
+    const int64_t npair = span >> 1;
+    for (int64_t k = lane; k < npair; k += 64) l2[k] = f64x2{0., 0.};
+    if ((span & 1) && lane == 0) lds[span - 1] = 0.;
 }
 
+// Span -> HBM with 16-byte stores: one leading element when `out` is not
+// 16-byte aligned, then pairs, then a trailing element.
 __device__ inline void write_span(const double* lds, int64_t span, int lane, double* out, bool streaming = false) {
-    if (streaming)
-        for (int64_t k = lane; k < span; k += 64) __builtin_nontemporal_store(lds[k], out + k);
-    else
-        for (int64_t k = lane; k < span; k += 64) out[k] = lds[k];
+    if (span <= 0) return;
+    const int head = (reinterpret_cast<uintptr_t>(out) & 15) ? 1 : 0;
+    const int64_t npair = (span - head) >> 1;
+    f64x2* o2 = reinterpret_cast<f64x2*>(out + head);
+    const double* l = lds + head;
+    if (streaming) {
+        if (head && lane == 0) __builtin_nontemporal_store(lds[0], out);
+        for (int64_t k = lane; k < npair; k += 64) {
+            const f64x2 v = {l[2 * k], l[2 * k + 1]};
+            __builtin_nontemporal_store(v, o2 + k);
+        }
+        if (((span - head) & 1) && lane == 0) __builtin_nontemporal_store(lds[span - 1], out + span - 1);
+    } else {
+        if (head && lane == 0) out[0] = lds[0];
+        for (int64_t k = lane; k < npair; k += 64) o2[k] = f64x2{l[2 * k], l[2 * k + 1]};
+        if (((span - head) & 1) && lane == 0) out[span - 1] = lds[span - 1];
+    }
 }
 
 // Write a scalar span as the 3x3 node blocks K_(a,c),(b,d) = [c == d] s_ab
