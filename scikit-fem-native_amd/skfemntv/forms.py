@@ -376,6 +376,53 @@ def asm(form, *bases, num_threads=None, memory_limit_bytes=None, memory_safety_f
     raise TypeError("skfemntv.asm accepts forms created by skfemntv.LinearForm or skfemntv.BilinearForm")
 
 
+def trace_terms(form, kwargs=None, *, geometry=None, fields=0):
+    """Trace a form without assembling it and list its terms as tuples
+    (kind, factor, coefficient, row field, column field); coefficients are
+    arrays, or the parameter name of a coefficient not supplied in kwargs;
+    fields > 0 traces a composite form with that many subfields (row =
+    test field, column = trial field; a linear term has its field as the
+    row).  Errors are those of asm."""
+    kwargs = dict(kwargs or {})
+    geometry = {k: (QuadratureValue(v) if isinstance(v, np.ndarray) else v) for k, v in (geometry or {}).items()}
+    bilinear = isinstance(form, _BilinearForm)
+    if fields:
+        trial = tuple(CompositeField("trial", k) for k in range(fields))
+        test = tuple(CompositeField("test", k) for k in range(fields))
+        args = trial + test if bilinear else test
+        expression = trace_expression(form.function, args, geometry, kwargs, context="composite form")
+        if not bilinear and isinstance(expression, CompositeWeightedField):
+            expression = expression._linear_term()
+        terms = extract_terms(expression, CompositeBilinearTerm if bilinear else CompositeLinearTerm,
+                              CompositeBilinearSum if bilinear else CompositeLinearSum,
+                              message="composite form must reduce to subfield value or gradient contractions")
+    else:
+        args = (TrialValue(), TestValue()) if bilinear else (TestValue(),)
+        expression = trace_expression(form.function, args, geometry, kwargs, context="form")
+        terms = extract_terms(expression, BilinearTerm if bilinear else LinearTerm,
+                              BilinearSum if bilinear else LinearSum,
+                              message="form must reduce to value, gradient or divergence contractions")
+
+    def coef(c):
+        if c is None:
+            return None
+        if isinstance(c, Coefficient):
+            return c.name
+        if isinstance(c, CoefficientComponent):
+            return (c.name, c.index)
+        return np.asarray(c)
+
+    out = []
+    for t in terms:
+        if fields and bilinear:
+            out.append((t.kind, t.factor, coef(t.coefficient), t.row_field, t.column_field))
+        elif fields:
+            out.append((t.kind, t.factor, coef(t.coefficient), t.field, None))
+        else:
+            out.append((t.kind, t.factor, coef(t.coefficient), None, None))
+    return out
+
+
 def _reject_composite_member(basis):
     # a sub-basis of a composite basis carries composite DOF numbers; its
     # blocks are assembled through the composite basis
