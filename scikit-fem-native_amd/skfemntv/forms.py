@@ -16,12 +16,8 @@ import numpy as np
 
 from ._assembly import (NativeBilinearForm, NativeCompositeBilinearForm, NativeCompositeLinearForm,
                         NativeCrossBilinearForm, NativeLinearForm)
-from ._symbols import (BilinearSum, BilinearTerm, Coefficient, CoefficientComponent, CompositeBilinearSum,
-                       CompositeBilinearTerm, CompositeDivergence, CompositeField, CompositeLinearSum,
-                       CompositeLinearTerm, CompositeWeightedField, Divergence, LinearSum, LinearTerm,
-                       QuadratureValue, SymmetricGradient, TensorGradient, TestGradient, TestValue, TrialGradient,
-                       TrialValue, UnsupportedNativeForm, evaluate_coefficient, extract_terms, is_symbolic,
-                       trace_expression)
+from ._symbols import (TEST, TRIAL, Field, PointValue, UnsupportedNativeForm, apply_div, apply_grad, apply_mul,
+                       coefficient_array, collect, contract, namespace, resolve, trace)
 
 
 class _LinearForm:
@@ -70,10 +66,9 @@ def _native_functional_assemble(form, basis, kwargs):
     at the quadrature points, then sum(values * dx) is reduced on the GPU
     (integrate_functional, functional_assembler.cpp:12-33)."""
     from ._assembly import integrate_functional
-    from ._symbols import Parameters, parameter_values, resolve_coefficient
     try:
-        expression = form.function(Parameters(parameter_values(_geometry(basis), kwargs)))
-        values = np.asarray(resolve_coefficient(expression, kwargs), dtype=np.float64)
+        expression = form.function(namespace(_geometry(basis), kwargs))
+        values = np.asarray(resolve(expression, kwargs), dtype=np.float64)
     except UnsupportedNativeForm:
         raise
     except Exception as error:
@@ -95,17 +90,20 @@ def _geometry(basis):
     # Physical quadrature coordinates, component first; computed on the GPU
     # only when the form reads w.x.  Facet bases add their outward normals
     # as w.n (forms.py:418-421).
-    geometry = {"x": QuadratureValue(lambda: np.moveaxis(basis.global_coordinates, -1, 0))}
+    geometry = {"x": PointValue(lambda: np.moveaxis(basis.global_coordinates, -1, 0))}
     if getattr(basis, "normals", None) is not None:
-        geometry["n"] = QuadratureValue(np.moveaxis(basis.normals, -1, 0))
+        geometry["n"] = PointValue(np.moveaxis(basis.normals, -1, 0))
     return geometry
 
 
+_LINEAR_MESSAGE = ("native LinearForm must reduce to dot(coefficient, v), ddot(coefficient, grad(v)), "
+                   "or a sum of those terms")
+_BILINEAR_MESSAGE = "native BilinearForm must reduce to dot(u, v), ddot(grad(u), grad(v)), or a sum of those terms"
+
+
 def _compile_linear(form, basis, kwargs):
-    expression = trace_expression(form.function, (TestValue(),), _geometry(basis), kwargs, context="LinearForm")
-    return extract_terms(expression, LinearTerm, LinearSum,
-                         message=("native LinearForm must reduce to dot(coefficient, v), "
-                                  "ddot(coefficient, grad(v)), or a sum of those terms"))
+    expression = trace(form.function, (Field(TEST),), _geometry(basis), kwargs, context="LinearForm")
+    return collect(expression, linear=True, message=_LINEAR_MESSAGE)
 
 
 def _native_linear_assemble(form, basis, kwargs, num_threads=0):
@@ -119,7 +117,7 @@ def _native_linear_assemble(form, basis, kwargs, num_threads=0):
     value = None
     gradient = None
     for term in terms:
-        coefficient = evaluate_coefficient(term.coefficient, kwargs, factor=term.factor)
+        coefficient = coefficient_array(term, kwargs)
         if term.kind == "value":
             if native._shape[-1] == 1 and np.shape(coefficient) == native._shape[:2]:
                 coefficient = np.asarray(coefficient)[..., None]
@@ -163,15 +161,12 @@ def anisotropic_tensor_coefficient(raw, coefficient_shape, dimension):
 def _native_bilinear_assemble(form, basis, kwargs, num_threads=0, *, memory_limit_bytes=None,
                               memory_safety_factor=1.25, precision="fp64"):
     """forms.py:276-374."""
-    expression = trace_expression(form.function, (TrialValue(), TestValue()), _geometry(basis), kwargs,
-                                  context="BilinearForm")
-    terms = extract_terms(expression, BilinearTerm, BilinearSum,
-                          message=("native BilinearForm must reduce to dot(u, v), "
-                                   "ddot(grad(u), grad(v)), or a sum of those terms"))
+    expression = trace(form.function, (Field(TRIAL), Field(TEST)), _geometry(basis), kwargs, context="BilinearForm")
+    terms = collect(expression, linear=False, message=_BILINEAR_MESSAGE)
     kinds = {"value": None, "gradient": None, "symmetric_gradient": None, "divergence": None}
     tensor_terms = []
     for term in terms:
-        coefficient = evaluate_coefficient(term.coefficient, kwargs, factor=term.factor, squeeze=True)
+        coefficient = coefficient_array(term, kwargs, squeeze=True)
         if term.kind in kinds:
             prev = kinds[term.kind]
             kinds[term.kind] = coefficient if prev is None else prev + coefficient
@@ -237,11 +232,10 @@ def _native_cross_bilinear_assemble(form, trial_basis, test_basis, kwargs, *, me
     geometry = _geometry(test_basis)
     if idx is not None:
         geometry["idx"] = idx
-    expression = trace_expression(form.function, (TrialValue(), TestValue()), geometry, kwargs,
-                                  context="cross-basis BilinearForm")
-    terms = extract_terms(expression, BilinearTerm, BilinearSum,
-                          message="cross-basis BilinearForm must reduce to value, gradient, or divergence "
-                                  "contractions")
+    expression = trace(form.function, (Field(TRIAL), Field(TEST)), geometry, kwargs,
+                       context="cross-basis BilinearForm")
+    terms = collect(expression, linear=False,
+                    message="cross-basis BilinearForm must reduce to value, gradient, or divergence contractions")
     by_trial = form._cross_cache.setdefault(trial_basis, WeakKeyDictionary())
     native = by_trial.get(test_basis)
     if native is None:
@@ -255,7 +249,7 @@ def _native_cross_bilinear_assemble(form, trial_basis, test_basis, kwargs, *, me
     for term in terms:
         if term.kind not in ("value", "gradient", "row_divergence", "column_divergence"):
             raise UnsupportedNativeForm(f"cross-basis BilinearForm does not support {term.kind!r} terms")
-        coefficient = evaluate_coefficient(term.coefficient, kwargs, factor=term.factor, squeeze=True)
+        coefficient = coefficient_array(term, kwargs, squeeze=True)
         matrix = native.assemble(term.kind, coefficient)
         result = matrix if result is None else result + matrix
     return result
@@ -264,40 +258,37 @@ def _native_cross_bilinear_assemble(form, trial_basis, test_basis, kwargs, *, me
 def _native_composite_bilinear_assemble(form, basis, kwargs):
     """forms.py:489-539: the form takes the trial subfields, then the test
     subfields; every term is one (test field, trial field) block."""
-    trial = tuple(CompositeField("trial", k) for k in range(len(basis.subbases)))
-    test = tuple(CompositeField("test", k) for k in range(len(basis.subbases)))
-    expression = trace_expression(form.function, trial + test, _geometry(basis), kwargs,
-                                  context="composite BilinearForm")
-    terms = extract_terms(expression, CompositeBilinearTerm, CompositeBilinearSum,
-                          message="composite BilinearForm must reduce to subfield value or gradient contractions")
+    fields = len(basis.subbases)
+    args = tuple(Field(TRIAL, k) for k in range(fields)) + tuple(Field(TEST, k) for k in range(fields))
+    expression = trace(form.function, args, _geometry(basis), kwargs, context="composite BilinearForm")
+    terms = collect(expression, linear=False,
+                    message="composite BilinearForm must reduce to subfield value or gradient contractions")
     native = form._native_cache.get(basis)
     if native is None:
         native = NativeCompositeBilinearForm(basis)
         form._native_cache[basis] = native
     result = None
     for term in terms:
-        coefficient = evaluate_coefficient(term.coefficient, kwargs, factor=term.factor, squeeze=True)
-        block = native.assemble(term.row_field, term.column_field, kind=term.kind, coefficient=coefficient)
+        coefficient = coefficient_array(term, kwargs, squeeze=True)
+        block = native.assemble(term.test_sub, term.trial_sub, kind=term.kind, coefficient=coefficient)
         result = block if result is None else result + block
     return result
 
 
 def _native_composite_linear_assemble(form, basis, kwargs):
     """forms.py:206-273: value / gradient terms per test subfield."""
-    test = tuple(CompositeField("test", k) for k in range(len(basis.subbases)))
-    expression = trace_expression(form.function, test, _geometry(basis), kwargs, context="composite LinearForm")
-    if isinstance(expression, CompositeWeightedField):
-        expression = expression._linear_term()
-    terms = extract_terms(expression, CompositeLinearTerm, CompositeLinearSum,
-                          message="composite LinearForm must reduce to subfield value or gradient contractions")
+    test = tuple(Field(TEST, k) for k in range(len(basis.subbases)))
+    expression = trace(form.function, test, _geometry(basis), kwargs, context="composite LinearForm")
+    terms = collect(expression, linear=True,
+                    message="composite LinearForm must reduce to subfield value or gradient contractions")
     native = form._native_cache.get(basis)
     if native is None:
         native = NativeCompositeLinearForm(basis)
         form._native_cache[basis] = native
     grouped = {}
     for term in terms:
-        coefficient = evaluate_coefficient(term.coefficient, kwargs, factor=term.factor)
-        field_native = native.assembler(term.field)
+        coefficient = coefficient_array(term, kwargs)
+        field_native = native.assembler(term.test_sub)
         if term.kind == "value":
             if field_native._shape[-1] == 1 and np.shape(coefficient) == field_native._shape[:2]:
                 coefficient = np.asarray(coefficient)[..., None]
@@ -308,7 +299,7 @@ def _native_composite_linear_assemble(form, basis, kwargs):
                 coefficient = np.moveaxis(coefficient, (0, 1), (-2, -1))
         else:
             raise UnsupportedNativeForm(f"unsupported composite linear kind {term.kind!r}")
-        values = grouped.setdefault(term.field, {"value": None, "gradient": None})
+        values = grouped.setdefault(term.test_sub, {"value": None, "gradient": None})
         previous = values[term.kind]
         values[term.kind] = coefficient if previous is None else previous + coefficient
     result = np.zeros(basis.N, dtype=np.float64)
@@ -384,43 +375,24 @@ def trace_terms(form, kwargs=None, *, geometry=None, fields=0):
     test field, column = trial field; a linear term has its field as the
     row).  Errors are those of asm."""
     kwargs = dict(kwargs or {})
-    geometry = {k: (QuadratureValue(v) if isinstance(v, np.ndarray) else v) for k, v in (geometry or {}).items()}
+    geometry = {k: (PointValue(v) if isinstance(v, np.ndarray) else v) for k, v in (geometry or {}).items()}
     bilinear = isinstance(form, _BilinearForm)
     if fields:
-        trial = tuple(CompositeField("trial", k) for k in range(fields))
-        test = tuple(CompositeField("test", k) for k in range(fields))
-        args = trial + test if bilinear else test
-        expression = trace_expression(form.function, args, geometry, kwargs, context="composite form")
-        if not bilinear and isinstance(expression, CompositeWeightedField):
-            expression = expression._linear_term()
-        terms = extract_terms(expression, CompositeBilinearTerm if bilinear else CompositeLinearTerm,
-                              CompositeBilinearSum if bilinear else CompositeLinearSum,
-                              message="composite form must reduce to subfield value or gradient contractions")
+        test = tuple(Field(TEST, k) for k in range(fields))
+        args = tuple(Field(TRIAL, k) for k in range(fields)) + test if bilinear else test
     else:
-        args = (TrialValue(), TestValue()) if bilinear else (TestValue(),)
-        expression = trace_expression(form.function, args, geometry, kwargs, context="form")
-        terms = extract_terms(expression, BilinearTerm if bilinear else LinearTerm,
-                              BilinearSum if bilinear else LinearSum,
-                              message="form must reduce to value, gradient or divergence contractions")
+        args = (Field(TRIAL), Field(TEST)) if bilinear else (Field(TEST),)
+    expression = trace(form.function, args, geometry, kwargs, context="form")
+    terms = collect(expression, linear=not bilinear, message=_BILINEAR_MESSAGE if bilinear else _LINEAR_MESSAGE)
 
     def coef(c):
         if c is None:
             return None
-        if isinstance(c, Coefficient):
-            return c.name
-        if isinstance(c, CoefficientComponent):
-            return (c.name, c.index)
+        if hasattr(c, "name") and hasattr(c, "index") and not isinstance(c, np.ndarray):
+            return c.name if c.index is None else (c.name, c.index)
         return np.asarray(c)
 
-    out = []
-    for t in terms:
-        if fields and bilinear:
-            out.append((t.kind, t.factor, coef(t.coefficient), t.row_field, t.column_field))
-        elif fields:
-            out.append((t.kind, t.factor, coef(t.coefficient), t.field, None))
-        else:
-            out.append((t.kind, t.factor, coef(t.coefficient), None, None))
-    return out
+    return [(t.kind, t.scale, coef(t.coef), t.test_sub, t.trial_sub) for t in terms]
 
 
 def _reject_composite_member(basis):
@@ -431,122 +403,33 @@ def _reject_composite_member(basis):
 
 
 # ------------------------------------------------------------------ operators
-def _is_value_like(x):
-    return isinstance(x, (np.ndarray, QuadratureValue)) or (hasattr(x, "value") and hasattr(x, "grad"))
-
-
+# With a symbolic trial or test field among the arguments these build the
+# form's terms; with numbers only they are the numeric contractions
+# (scikit-fem's helpers) over the leading component axes.
 def dot(left, right):
-    """forms.py:788-961 restricted to the volume H1 kinds."""
-    if isinstance(left, TensorGradient) and isinstance(right, (TrialGradient, TestGradient)):
-        if left.gradient.__class__ is right.__class__:
-            raise UnsupportedNativeForm("anisotropic contraction requires trial and test gradients")
-        return BilinearTerm("gradient_tensor", left.coefficient, left.gradient.factor * right.factor)
-    if isinstance(right, TensorGradient) and isinstance(left, (TrialGradient, TestGradient)):
-        if right.gradient.__class__ is left.__class__:
-            raise UnsupportedNativeForm("anisotropic contraction requires trial and test gradients")
-        return BilinearTerm("gradient_tensor", right.coefficient, right.gradient.factor * left.factor)
-    if isinstance(left, CompositeField) and isinstance(right, CompositeField):
-        if left.kind == "gradient" and right.kind == "gradient":
-            return _composite_block(left, right, "gradient")
-        if left.kind != "value" or right.kind != "value":
-            raise UnsupportedNativeForm("composite dot requires two values or two gradients")
-        return _composite_block(left, right, "value")
-    for a, b in ((left, right), (right, left)):
-        if isinstance(b, CompositeField) and b.role == "test" and b.kind == "value":
-            if is_symbolic(a):
-                return CompositeLinearTerm(b.field, "value", a)
-            if _is_value_like(a):
-                return CompositeLinearTerm(b.field, "value", np.asarray(a))
-    if is_symbolic(left) and isinstance(right, TestValue):
-        return LinearTerm("value", left)
-    if is_symbolic(right) and isinstance(left, TestValue):
-        return LinearTerm("value", right)
-    if isinstance(right, TestValue) and _is_value_like(left):
-        return LinearTerm("value", np.asarray(left))
-    if isinstance(left, TestValue) and _is_value_like(right):
-        return LinearTerm("value", np.asarray(right))
-    if (isinstance(left, TrialValue) and isinstance(right, TestValue)) or (
-            isinstance(right, TrialValue) and isinstance(left, TestValue)):
-        return BilinearTerm("value", factor=left.factor * right.factor)
-    # Superset of the reference: dot(grad(u), grad(v)) is the isotropic
-    # gradient kind (the reference evaluates it with numpy and fails).
-    if (isinstance(left, TrialGradient) and isinstance(right, TestGradient)) or (
-            isinstance(right, TrialGradient) and isinstance(left, TestGradient)):
-        return BilinearTerm("gradient", factor=left.factor * right.factor)
-    if isinstance(left, (TrialValue, TestValue, TrialGradient, TestGradient, SymmetricGradient, Divergence)) or \
-            isinstance(right, (TrialValue, TestValue, TrialGradient, TestGradient, SymmetricGradient, Divergence)):
-        raise UnsupportedNativeForm("unsupported dot() operands in a native form")
+    if isinstance(left, Field) or isinstance(right, Field):
+        return contract(left, right, "dot")
     return np.einsum("i...,i...->...", left, right)
 
 
-def mul(left, right):
-    if isinstance(right, (TrialGradient, TestGradient)) and (
-            isinstance(left, (Coefficient, CoefficientComponent, np.ndarray, QuadratureValue))):
-        coefficient = left if is_symbolic(left) else np.asarray(left)
-        return TensorGradient(right, coefficient)
-    raise UnsupportedNativeForm("native mul currently supports coefficient @ trial/test gradient")
-
-
 def ddot(left, right):
-    if isinstance(left, SymmetricGradient) and isinstance(right, SymmetricGradient) and left.role != right.role:
-        return BilinearTerm("symmetric_gradient", factor=left.factor * right.factor)
-    if isinstance(left, CompositeField) and isinstance(right, CompositeField):
-        if left.kind != "gradient" or right.kind != "gradient":
-            raise UnsupportedNativeForm("composite ddot requires two gradients")
-        return _composite_block(left, right, "gradient")
-    for a, b in ((left, right), (right, left)):
-        if isinstance(b, CompositeField) and b.role == "test" and b.kind == "gradient":
-            if is_symbolic(a):
-                return CompositeLinearTerm(b.field, "gradient", a)
-            if isinstance(a, (np.ndarray, QuadratureValue)):
-                return CompositeLinearTerm(b.field, "gradient", np.asarray(a))
-    if is_symbolic(left) and isinstance(right, TestGradient):
-        return LinearTerm("gradient", left)
-    if is_symbolic(right) and isinstance(left, TestGradient):
-        return LinearTerm("gradient", right)
-    if isinstance(left, (np.ndarray, QuadratureValue)) and isinstance(right, TestGradient):
-        return LinearTerm("gradient", np.asarray(left))
-    if isinstance(right, (np.ndarray, QuadratureValue)) and isinstance(left, TestGradient):
-        return LinearTerm("gradient", np.asarray(right))
-    if (isinstance(left, TrialGradient) and isinstance(right, TestGradient)) or (
-            isinstance(right, TrialGradient) and isinstance(left, TestGradient)):
-        return BilinearTerm("gradient", factor=left.factor * right.factor)
+    if isinstance(left, Field) or isinstance(right, Field):
+        return contract(left, right, "ddot")
     return np.einsum("ij...,ij...->...", left, right)
 
 
-def _composite_block(left, right, kind):
-    if left.role == right.role:
-        raise UnsupportedNativeForm("composite contraction requires trial and test subfields")
-    test, trial = (left, right) if left.role == "test" else (right, left)
-    return CompositeBilinearTerm(test.field, trial.field, kind)
+def mul(left, right):
+    if isinstance(left, Field) or isinstance(right, Field):
+        return apply_mul(left, right)
+    return np.einsum("ij...,j...->i...", left, right)
 
 
 def grad(value):
-    if isinstance(value, CompositeField):
-        return CompositeField(value.role, value.field, "gradient")
-    if isinstance(value, TestValue):
-        return TestGradient(value.factor)
-    if isinstance(value, TrialValue):
-        return TrialGradient(value.factor)
-    try:
-        return value.grad
-    except AttributeError as error:
-        raise UnsupportedNativeForm("grad() requires a form field") from error
+    return apply_grad(value)
 
 
 def div(value):
-    if isinstance(value, CompositeField):
-        if value.kind != "value":
-            raise UnsupportedNativeForm("div() expects a composite value")
-        return CompositeDivergence(value)
-    if isinstance(value, TrialValue):
-        return Divergence("trial", value.factor)
-    if isinstance(value, TestValue):
-        return Divergence("test", value.factor)
-    try:
-        return np.einsum("ii...->...", value.grad)
-    except AttributeError as error:
-        raise UnsupportedNativeForm("div() requires a vector field") from error
+    return apply_div(value)
 
 
 __all__ = ["BilinearForm", "LinearForm", "asm", "dot", "ddot", "mul", "grad", "div", "UnsupportedNativeForm",

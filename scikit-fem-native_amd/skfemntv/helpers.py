@@ -2,7 +2,7 @@
 
 import numpy as np
 
-from ._symbols import SymmetricGradient, TestGradient, TrialGradient
+from ._symbols import apply_symgrad
 from .forms import ddot, div, dot, grad, mul
 
 
@@ -15,11 +15,7 @@ def transpose(value):
 
 
 def sym_grad(value):
-    gradient = grad(value)
-    if isinstance(gradient, (TrialGradient, TestGradient)):
-        role = "trial" if isinstance(gradient, TrialGradient) else "test"
-        return SymmetricGradient(role, gradient.factor)
-    return 0.5 * (gradient + transpose(gradient))
+    return apply_symgrad(value)
 
 
 def jump(w, *args):

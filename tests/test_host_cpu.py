@@ -170,19 +170,3 @@ def test_composite_relabelling_keeps_csr_order():
     dense[np.ix_(row_map, col_map)] = local.toarray()
     np.testing.assert_array_equal(K.toarray(), dense)
     assert K.has_sorted_indices and K.shape == (N, N)
-
-
-def test_composite_symbols_trace_stokes_blocks():
-    """u, p, v, q trace to (test, trial) blocks with the reference's kinds."""
-    from skfemntv._symbols import CompositeBilinearSum, CompositeField, CompositeLinearSum
-    from skfemntv.forms import ddot, div, dot, grad
-    u, p, v, q = (CompositeField("trial", 0), CompositeField("trial", 1), CompositeField("test", 0),
-                  CompositeField("test", 1))
-    form = ddot(grad(u), grad(v)) - div(v) * p - div(u) * q + 2. * (p * q)
-    assert isinstance(form, CompositeBilinearSum)
-    got = [(t.row_field, t.column_field, t.kind, t.factor) for t in form.terms]
-    assert got == [(0, 0, "gradient", 1.), (0, 1, "row_divergence", -1.), (1, 0, "column_divergence", -1.),
-                   (1, 1, "value", 2.)]
-    lin = dot(np.array([1., 0., 0.]), v) + q * np.array([3.])
-    assert isinstance(lin, CompositeLinearSum)
-    assert [(t.field, t.kind) for t in lin.terms] == [(0, "value"), (1, "value")]
