@@ -44,7 +44,8 @@ BYTES_PER_ELEMENT_C2 = 825.7  # algorithmic bytes per tet of one FP64 assembly
 # the same without the int32 element->CSR map (10 x 10 entries per tet): the "floor"
 FLOOR_BYTES_PER_ELEMENT_C2 = BYTES_PER_ELEMENT_C2 - 4. * 10 * 10
 # rocprofv3 FETCH_SIZE/WRITE_SIZE summaries of the other workloads at their default sizes
-TRAFFIC_FILES = {"th": "th_divergence_block_64_hbm_traffic.json"}
+TRAFFIC_FILES = {"th": "th_divergence_block_64_hbm_traffic.json",
+                 "c3": "c3_vector_p2_elasticity_48_hbm_traffic.json"}
 
 
 def log(msg):
@@ -476,6 +477,17 @@ def measure(w, steps, warmup, ramp_s, timer):
     return sec, out
 
 
+def measured_traffic(name):
+    """Measured HBM bytes of one assembly (FETCH_SIZE + WRITE_SIZE, the write
+    counter calibrated on a fill of known size), from the committed
+    rocprofv3 summary of that workload, or None."""
+    path = os.path.join(ROOT, "profiles", TRAFFIC_FILES.get(name, ""))
+    if name not in TRAFFIC_FILES or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["per_assembly_bytes"]
+
+
 def other_workload(name, steps, warmup, ramp_s, timer):
     """One JSON line for a single non-headline workload (--workload)."""
     w = build_workload(name)
@@ -489,12 +501,7 @@ def other_workload(name, steps, warmup, ramp_s, timer):
            "roofline": {"bound": "hbm", "achieved": bpe * E / sec / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": bpe * E / sec / 1e9 / HBM_PEAK_GBS, "traffic": None},
            "nnz_per_s": nnz / sec, "setup_s": w["setup_s"]}
-    # measured HBM bytes of one launch (FETCH_SIZE + WRITE_SIZE, calibrated
-    # on a copy of known size), from the committed rocprofv3 summary
-    traffic_file = os.path.join(ROOT, "profiles", f"{TRAFFIC_FILES.get(name, '')}")
-    if name in TRAFFIC_FILES and os.path.exists(traffic_file):
-        with open(traffic_file) as f:
-            out["roofline"]["traffic"] = json.load(f)["per_assembly_bytes"]
+    out["roofline"]["traffic"] = measured_traffic(name)
     if "fp32_products" in summary:
         out["fp32_products"] = summary["fp32_products"]
     return out
@@ -666,6 +673,7 @@ def main():
             _, summary = measure(w, args.steps, args.warmup, 0.2, timer)
             summary["workload"] = w["desc"]
             summary["elements"], summary["nnz"] = w["E"], w["nnz"]
+            summary["traffic"] = measured_traffic(name)
             if w["plan"] is not None:
                 summary["autotune"] = autotune_summary(w["plan"])
             configs[name] = summary
