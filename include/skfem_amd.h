@@ -68,7 +68,7 @@ int skfa_plan_destroy(skfa_plan* plan);
  * setting of keys 1-4; key 5 selects between two formulations that differ
  * by rounding only).  Keys: */
 #define SKFA_TUNE_XCD_CHUNK 1  /* waves per XCD chunk, 0 = plain dispatch order */
-#define SKFA_TUNE_LDS_ATOMIC 2 /* 1: accumulate with ds_add_f64, 0: read-add-write */
+#define SKFA_TUNE_LDS_ATOMIC 2 /* LDS accumulation: only 1 (ds_add_f64, always used) is accepted */
 #define SKFA_TUNE_PLACEMENT 3  /* 0 plain order, 1 one contiguous segment per XCD, 2 XCD chunks */
 #define SKFA_TUNE_STREAMING 4  /* 1: non-temporal record loads and CSR value stores */
 #define SKFA_TUNE_ELEMENT_TABLE 5 /* symmetric scalar forms: 1 element-table gather, 0 quadrature gather,
@@ -85,9 +85,11 @@ int skfa_plan_destroy(skfa_plan* plan);
                                   element rows contracted on MFMA (16 rows x 4 points per instruction);
                                   0 off (default, -1 = default), 1 every wave, 2 waves of <= 16 rows
                                   only; rounding differs only */
-#define SKFA_TUNE_ROW_FORM 8   /* 10-node scalar gather with SKFA_FLAG_BARYCENTRIC_TABLE: 1 element rows
-                                  from the barycentric structure (64 FMAs per row), 0 from the full
-                                  gradient table (120 FMAs), -1 automatic (1); rounding differs only */
+#define SKFA_TUNE_ROW_FORM 8   /* 10-node gathers with SKFA_FLAG_BARYCENTRIC_TABLE: 1 scalar element rows
+                                  from the barycentric structure (64 FMAs per row, not 120) and
+                                  vector-P2 3x3 node blocks from the pair table (54 FMAs, not 108),
+                                  0 from the quadrature tables, -1 automatic (1); rounding differs
+                                  only */
 int skfa_plan_set_tuning(skfa_plan* plan, int32_t key, int32_t value);
 /* host info[9]: nrows, nnz, nnz_node, max_row_len, max_incidence, waves,
  * gather records, record bytes, LDS bytes of the largest wave span. */
@@ -216,8 +218,14 @@ int skfa_assemble_bilinear(skfa_plan* plan, const double* xyz, int64_t npoints, 
  * reference gradients at the Qg gradient points, grad phi_b = sum_m c_bm
  * grad L_m, pairs in the order (0,0) (1,1) (2,2) (3,3), then for each edge
  * e = (i,j) of ((0,1),(1,2),(2,0),(0,3),(1,3),(2,3)) the pairs (4+e, i) and
- * (4+e, j).  The gather then forms element rows from these mostly-zero
- * structures (tuning key SKFA_TUNE_ROW_FORM). */
+ * (4+e, j).  Then, from the next multiple of 4 entries, the pair table
+ * pt[10][10][8]: pt[a][b][3i:3i+3] = q_(ab,i) = sum_q wg_q c_(a,i)(q)
+ * grad phi_b(q) (i = 0, 1; the c row of a's i-th pair, zero for i = 1 at a
+ * vertex), pt[a][b][6] = sum_q wm_q phi_a phi_b, pt[a][b][7] = 0; so that
+ * sum_q wg_q grad phi_a grad phi_b^T = sum_i grad L_m(a,i) q_(ab,i)^T.  The
+ * scalar gather then forms element rows from the coefficient structure and
+ * the vector-P2 gather its 3x3 node blocks from the pair table (tuning key
+ * SKFA_TUNE_ROW_FORM). */
 #define SKFA_FLAG_BARYCENTRIC_TABLE 4
 
 /* skfa_assemble_bilinear without any device allocation.  No counterpart in

@@ -25,7 +25,8 @@
 // coordinates and adds it at the precomputed in-row offsets.  The span is
 // then written to HBM once, coalesced.  The summation order per entry is
 // fixed by the plan (elements in ascending order), so results are
-// bit-reproducible run to run; there are no atomics and no colouring.
+// bit-reproducible run to run; there are no global atomics and no colouring
+// (the span adds are LDS ds_add_f64 with one writing lane per address).
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -121,7 +122,6 @@ struct PlanArgs {
     const WaveMeta* meta;    // per-wave constants
     int32_t placement;
     int32_t xcd_chunk;
-    int32_t lds_atomic;
     int32_t streaming;       // non-temporal record loads and value stores
     int32_t diag3 = 0;       // scalar plan written as the 3x3 diagonal blocks of a 3-component field
     int32_t concurrent = 0;  // row parts add in the same instructions (row split 3 with ds_add)
@@ -191,16 +191,14 @@ __device__ inline void lane_setup(const PlanArgs& P, int w, int lane, Lane& L) {
     }
 }
 
-// Every LDS address of a span has exactly one owning lane, so this add has
-// no concurrent writer: the LDS atomic only saves the read-back (one ds_add
-// instead of ds_read + v_add + ds_write); the summation order per entry is
-// the lane's program order.
-__device__ inline void lds_add(double* p, double v, bool atomic) {
-    if (atomic)
-        (void)atomicAdd(p, v);
-    else
-        *p += v;
-}
+// Span accumulation: one ds_add_f64 (no return value) per entry.  Outside
+// the concurrent row split every LDS address has one writing lane per
+// instruction, so the atomic only saves the read-back (one LDS instruction
+// instead of ds_read + v_add + ds_write) and the per-entry summation order is
+// the lanes' program order.  The add is unconditional: a runtime choice of
+// accumulation put a uniform branch around every add and kept the compiler
+// from grouping a step's adds.
+__device__ inline void lds_add(double* p, double v) { (void)atomicAdd(p, v); }
 
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
@@ -510,6 +508,10 @@ struct GatherArgs {
     // c_bm at the Qg points (pair-major); null when not available
     const double* ctab = nullptr;
     const float* ctab32 = nullptr;
+    // 10-node pair table (10 x 10 x 8: q_(ab,0), q_(ab,1), M_ab, 0) of the
+    // vector-P2 gather's barycentric form; null when not available
+    const double* qtab = nullptr;
+    const float* qtab32 = nullptr;
 };
 
 // K_ab += w_q (dphi_aq^T D) . dphi_bq for all b: the lane's row a reads the
@@ -685,7 +687,7 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
         auto finish = [&](int b, T k) {
             if (mass)
                 for (int q = 0; q < Qm; ++q) k = fma(mf * pm_l[q * NL + a], pmu_l[q * NL + b], k);
-            lds_add(span_lds + L.rb + int64_t(C) * rec_offset_dyn<RB, OFFW>(rec, b), double(k), A.plan.lds_atomic);
+            lds_add(span_lds + L.rb + int64_t(C) * rec_offset_dyn<RB, OFFW>(rec, b), double(k));
         };
         if constexpr (QG > 0) {
             T u[QG][3];
@@ -747,7 +749,7 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
 #pragma unroll
         for (int b = 0; b < NL; ++b) {
             const int o = rec_offset<RB, OFFW>(rec, b);
-            if ((o & (L.S - 1)) == L.p) lds_add(span_lds + L.rb + int64_t(C) * o, double(Kr[b]), A.plan.lds_atomic);
+            if ((o & (L.S - 1)) == L.p) lds_add(span_lds + L.rb + int64_t(C) * o, double(Kr[b]));
         }
     } else {
         // LDS addresses fit 32 bits; one writer per address in every add
@@ -755,7 +757,7 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
         auto add_row = [&]() {
 #pragma unroll
             for (int b = 0; b < NL; ++b)
-                lds_add(row + C * rec_offset<RB, OFFW>(rec, b), double(Kr[b]), A.plan.lds_atomic);
+                lds_add(row + C * rec_offset<RB, OFFW>(rec, b), double(Kr[b]));
         };
         if (L.S == 1 || SP == SP_CONCURRENT) {
             add_row();
@@ -853,9 +855,22 @@ __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
 //   G_ab = sum_q w_q (B^T dphi_a(q)) (B^T dphi_b(q))^T
 //   K_(a,c),(b,d) = lam G_cd + hs G_dc + [c == d] (iso tr G_ab + |det| m M_ab)
 // (the form of the lane-per-(c,d) gather's D_cd, contracted once per pair).
+// Two ways to form G_ab (KF, the row form):
+//   KF = 0  the quadrature sum above, 4 points: B^T dphi_b(q) and
+//           B^T (w_q dphi_a(q)) per point, then their outer products;
+//   KF = 1  B is constant on the element, so G_ab = B^T N_ab B with the
+//           reference sum N_ab = sum_q w_q dphi_a dphi_b^T, and through the
+//           barycentric structure of the P2 gradients N_ab has rank two:
+//             N_ab = sum_i grad L_m(a,i) q_(ab,i)^T,  i = 0, 1
+//           (q from the pair table of the reduced tables), so
+//             G_ab = sum_i (B^T grad L_m(a,i)) (B^T q_(ab,i))^T
+//           and the mass sum M_ab is a table entry too.  54 FMAs for G in
+//           place of 108, no per-point loop.
 // Lane roles: node row i of the wave, column node b of the element row,
 // incidence part p (lanes per row = 10 S); the lane's b is fixed for the
-// wave, so its reference gradients and mass values live in registers.
+// wave, so its reference gradients and mass values (KF = 0) live in
+// registers; with KF = 1 the lane reads its pair row (a, b) of the table,
+// 64 bytes, two steps ahead with the payload.
 // In-row offset of a fixed local node b: the record word holding it is
 // picked with per-lane masks (all-ones for that word) rather than an index,
 // which keeps the record in registers.
@@ -882,13 +897,13 @@ struct OffsetPick {
     }
 };
 
-template <bool F32, int RB, int OFFW, int QG>
+template <bool F32, int RB, int OFFW, int QG, int KF>
 __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
     extern __shared__ double span_lds[];
     // two steps in flight: the PL_VEC payloads take 2 x 14 doubles per lane
-    // (~250 VGPRs, 2 waves/SIMD, below the ~2.75 the LDS span allows);
-    // measured on 48^3 elasticity: K = 2 0.976 ms, K = 1 1.045 ms
+    // (measured on 48^3 elasticity with KF = 0: K = 2 0.976 ms, K = 1 1.045 ms)
     constexpr int NL = 10, C = 3, P = 14, K = 2;
+    constexpr int NQ = KF == 1 ? 8 : 1;   // pair-table values held per step
     using T = typename std::conditional<F32, float, double>::type;
     const int lane = threadIdx.x;
     const int wv = placed_wave(A.plan.xcd_seg, int(A.plan.nwaves), A.plan.placement, A.plan.xcd_chunk);
@@ -921,36 +936,78 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
         L.rb = int64_t(C) * C * (rbeg - L.lo);
     }
     const int64_t nsteps = (L.steps + L.S - 1) / L.S;
+    const bool mass = A.has_mass != 0;
     Record<RB> r0[K], r1[K], r2[K];
     double p0[K][P], p1[K][P];
+    T q0[K][NQ], q1[K][NQ];
     load_records<K, RB>(A, L, 0, r0);
     zero_span(span_lds, L.span, lane);
-    // weighted rows w_q dphi_a(q) and w_q phi_a(q) (lane-varying a) behind the span
     const int Qm = A.Qm;
     const int lg = QG * NL * 3;
     const T* src = F32 ? reinterpret_cast<const T*>(A.tab32) : reinterpret_cast<const T*>(A.tab);
+    // tables behind the span: KF = 0 the weighted rows w_q dphi_a(q) and
+    // w_q phi_a(q) (lane-varying a); KF = 1 grad L_m(a,i) for the 20 (a, i)
     T* wdg = reinterpret_cast<T*>(span_lds + L.span);
     T* wpm = wdg + lg;
-    for (int k = lane; k < lg + Qm * NL; k += 64) {
-        if (k < lg)
-            wdg[k] = src[QG + k] * src[k / (NL * 3)];
-        else
-            wpm[k - lg] = src[QG + lg + Qm + (k - lg)] * src[QG + lg + (k - lg) / NL];
+    if constexpr (KF == 1) {
+        if (lane < 60) {
+            constexpr int E0[6] = {0, 1, 2, 0, 1, 2}, E1[6] = {1, 2, 0, 3, 3, 3};
+            const int a = lane / 6, i = (lane / 3) & 1, x = lane % 3;
+            const int m = a < 4 ? a : (i == 0 ? E0[a - 4] : E1[a - 4]);
+            wdg[lane] = m == 0 ? T(-1) : (x == m - 1 ? T(1) : T(0));
+        }
+    } else {
+        for (int k = lane; k < lg + Qm * NL; k += 64) {
+            if (k < lg)
+                wdg[k] = src[QG + k] * src[k / (NL * 3)];
+            else
+                wpm[k - lg] = src[QG + lg + Qm + (k - lg)] * src[QG + lg + (k - lg) / NL];
+        }
     }
-    // the lane's column node b: unweighted gradients and values
+    // KF = 0: the lane's column node b, unweighted gradients and values
     T gb[QG][3];
+    if constexpr (KF == 0) {
 #pragma unroll
-    for (int q = 0; q < QG; ++q)
+        for (int q = 0; q < QG; ++q)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) gb[q][j] = src[QG + (q * NL + b) * 3 + j];
+            for (int j = 0; j < 3; ++j) gb[q][j] = src[QG + (q * NL + b) * 3 + j];
+    }
+    // KF = 1: pair rows (a, b) of the table, 8 values each
+    const T* qt = F32 ? reinterpret_cast<const T*>(A.qtab32) : reinterpret_cast<const T*>(A.qtab);
+    auto load_pairs = [&](const Record<RB>* rec, T (*qv)[NQ]) {
+        if constexpr (KF == 1) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (rec[k].packed < 0) continue;
+                const T* row = qt + ((rec[k].packed & 15) * NL + b) * 8;
+                if constexpr (F32) {
+                    const float4 v0 = reinterpret_cast<const float4*>(row)[0];
+                    const float4 v1 = reinterpret_cast<const float4*>(row)[1];
+                    qv[k][0] = v0.x; qv[k][1] = v0.y; qv[k][2] = v0.z; qv[k][3] = v0.w;
+                    qv[k][4] = v1.x; qv[k][5] = v1.y; qv[k][6] = v1.z; qv[k][7] = v1.w;
+                } else {
+                    const double2* r2v = reinterpret_cast<const double2*>(row);
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        const double2 v = r2v[t];
+                        qv[k][2 * t] = v.x;
+                        qv[k][2 * t + 1] = v.y;
+                    }
+                    qv[k][6] = mass ? row[6] : 0.;
+                    qv[k][7] = 0.;
+                }
+            }
+        }
+    };
     load_payloads<K, RB, P>(A, r0, p0);
+    load_pairs(r0, q0);
     load_records<K, RB>(A, L, K, r1);
     __syncthreads();
-    const bool mass = A.has_mass != 0;
     const T* phb = src + QG + lg + Qm + b;   // phi_b(q) at stride NL
     const OffsetPick<RB, OFFW> pick(b);
     for (int64_t st = 0; st < nsteps; st += K) {
         load_payloads<K, RB, P>(A, r1, p1);
+        load_pairs(r1, q1);
         load_records<K, RB>(A, L, st + 2 * K, r2);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -961,32 +1018,49 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
 #pragma unroll
             for (int t = 0; t < 9; ++t) B[t] = T(pl[t]);
             T G[3][3];
+            T mab = T(0);
+            if constexpr (KF == 1) {
+                const T* pa = wdg + a * 6;
+                const T* qa = q0[k];
+                T u[2][3], w[2][3];
 #pragma unroll
-            for (int x = 0; x < 3; ++x)
+                for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int y = 0; y < 3; ++y) G[x][y] = T(0);
-#pragma unroll
-            for (int q = 0; q < QG; ++q) {
-                const T* ga = wdg + (q * NL + a) * 3;
-                const T g0 = ga[0], g1 = ga[1], g2 = ga[2];
-                T va[3], vb[3];
-#pragma unroll
-                for (int x = 0; x < 3; ++x) {
-                    va[x] = fma(B[0 * 3 + x], g0, fma(B[1 * 3 + x], g1, B[2 * 3 + x] * g2));
-                    vb[x] = fma(B[0 * 3 + x], gb[q][0], fma(B[1 * 3 + x], gb[q][1], B[2 * 3 + x] * gb[q][2]));
-                }
+                    for (int x = 0; x < 3; ++x) {
+                        u[i][x] = fma(pa[3 * i], B[x], fma(pa[3 * i + 1], B[3 + x], pa[3 * i + 2] * B[6 + x]));
+                        w[i][x] = fma(qa[3 * i], B[x], fma(qa[3 * i + 1], B[3 + x], qa[3 * i + 2] * B[6 + x]));
+                    }
 #pragma unroll
                 for (int x = 0; x < 3; ++x)
 #pragma unroll
-                    for (int y = 0; y < 3; ++y) G[x][y] = fma(va[x], vb[y], G[x][y]);
+                    for (int y = 0; y < 3; ++y) G[x][y] = fma(u[0][x], w[0][y], u[1][x] * w[1][y]);
+                mab = qa[6];
+            } else {
+#pragma unroll
+                for (int x = 0; x < 3; ++x)
+#pragma unroll
+                    for (int y = 0; y < 3; ++y) G[x][y] = T(0);
+#pragma unroll
+                for (int q = 0; q < QG; ++q) {
+                    const T* ga = wdg + (q * NL + a) * 3;
+                    const T g0 = ga[0], g1 = ga[1], g2 = ga[2];
+                    T va[3], vb[3];
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) {
+                        va[x] = fma(B[0 * 3 + x], g0, fma(B[1 * 3 + x], g1, B[2 * 3 + x] * g2));
+                        vb[x] = fma(B[0 * 3 + x], gb[q][0], fma(B[1 * 3 + x], gb[q][1], B[2 * 3 + x] * gb[q][2]));
+                    }
+#pragma unroll
+                    for (int x = 0; x < 3; ++x)
+#pragma unroll
+                        for (int y = 0; y < 3; ++y) G[x][y] = fma(va[x], vb[y], G[x][y]);
+                }
+                if (mass)
+                    for (int q = 0; q < Qm; ++q) mab = fma(wpm[q * NL + a], phb[q * NL], mab);
             }
             const T iso = T(pl[9]), hs = T(pl[10]), lam = T(pl[11]);
             T diag = iso * ((G[0][0] + G[1][1]) + G[2][2]);
-            if (mass) {
-                T mab = T(0);
-                for (int q = 0; q < Qm; ++q) mab = fma(wpm[q * NL + a], phb[q * NL], mab);
-                diag = fma(T(pl[12]), mab, diag);
-            }
+            if (mass) diag = fma(T(pl[12]), mab, diag);
             double v[3][3];
 #pragma unroll
             for (int c = 0; c < 3; ++c)
@@ -997,15 +1071,20 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
                     v[c][d] = double(e);
                 }
             double* col = span_lds + int(L.rb) + C * pick(r0[k]);
+            auto add_block = [&]() {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) lds_add(col + c * C * row_len + d, v[c][d]);
+            };
             // the S parts of a row add in turn (one writer per address), or
             // at once with the LDS atomic serialising shared addresses
-            for (int turn = 0; turn < L.S; ++turn)
-                if (L.p == turn || PA.concurrent) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c)
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) lds_add(col + c * C * row_len + d, v[c][d], A.plan.lds_atomic);
-                }
+            if (L.S == 1 || PA.concurrent) {
+                add_block();
+            } else {
+                for (int turn = 0; turn < L.S; ++turn)
+                    if (L.p == turn) add_block();
+            }
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -1013,6 +1092,8 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
             r1[k] = r2[k];
 #pragma unroll
             for (int t = 0; t < P; ++t) p0[k][t] = p1[k][t];
+#pragma unroll
+            for (int t = 0; t < NQ; ++t) q0[k][t] = q1[k][t];
         }
     }
     __syncthreads();
@@ -1160,8 +1241,7 @@ __global__ void __launch_bounds__(64) k_gather_mfma(GatherArgs A) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int b = M::entry(p, r);
-                    if (b < NL) lds_add(span_lds + rb + rec_offset_dyn<RB, OFFW>(rec, b), double(c[r]),
-                                        A.plan.lds_atomic);
+                    if (b < NL) lds_add(span_lds + rb + rec_offset_dyn<RB, OFFW>(rec, b), double(c[r]));
                 }
             }
         }
@@ -1234,7 +1314,7 @@ __device__ inline void table_step(const GatherArgs& A, const Lane& L, const Reco
         k = fma(d4, n[4], k);
         k = fma(d5, n[5], k);
         if (MASS) k = fma(d6, n[6], k);
-        lds_add(span_lds + L.rb + rec_offset_dyn<RB, OFFW>(rec, b), double(k), A.plan.lds_atomic);
+        lds_add(span_lds + L.rb + rec_offset_dyn<RB, OFFW>(rec, b), double(k));
     }
 }
 
@@ -1372,13 +1452,13 @@ __device__ inline void n1_step(const GatherN1Args& A, const Lane& L, const Recor
     double* row = span_lds + int(L.rb);
     if (L.S == 1 || A.plan.concurrent) {
 #pragma unroll
-        for (int b = 0; b < 6; ++b) lds_add(row + rec_offset<RB, OFFW>(rec, b), Kr[b], A.plan.lds_atomic);
+        for (int b = 0; b < 6; ++b) lds_add(row + rec_offset<RB, OFFW>(rec, b), Kr[b]);
         return;
     }
     for (int turn = 0; turn < L.S; ++turn)
         if (L.p == turn) {
 #pragma unroll
-            for (int b = 0; b < 6; ++b) lds_add(row + rec_offset<RB, OFFW>(rec, b), Kr[b], A.plan.lds_atomic);
+            for (int b = 0; b < 6; ++b) lds_add(row + rec_offset<RB, OFFW>(rec, b), Kr[b]);
         }
 }
 
@@ -1573,7 +1653,7 @@ __global__ void __launch_bounds__(64) k_assemble_h1_general(H1Args A) {
 #pragma unroll
                 for (int b = 0; b < NL; ++b) {
                     const int o = rec_offset<RB, OFFW>(rec, b);
-                    lds_add(span_lds + L.rb + int64_t(C) * o, Kr[b], A.plan.lds_atomic);
+                    lds_add(span_lds + L.rb + int64_t(C) * o, Kr[b]);
                 }
             }
         }
@@ -1774,9 +1854,8 @@ static PlanArgs plan_args(const skfa_plan* P) {
     A.meta = P->wave_meta.p;
     A.placement = P->tune_placement;
     A.xcd_chunk = P->tune_xcd_chunk;
-    A.lds_atomic = P->tune_lds_atomic;
     A.streaming = P->tune_streaming;
-    A.concurrent = P->tune_split == SP_CONCURRENT && P->tune_lds_atomic != 0;
+    A.concurrent = P->tune_split == SP_CONCURRENT;
     return A;
 }
 
@@ -1838,8 +1917,7 @@ template <int NL, int C, int PL, bool F32, int QG>
 static int launch_gather_q(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
     if (P->tune_split == SP_ENTRY) return launch_gather_k<NL, C, PL, F32, QG, SP_ENTRY>(P, A, lds, s);
     if (P->tune_split == SP_COLUMN) return launch_gather_k<NL, C, PL, F32, QG, SP_COLUMN>(P, A, lds, s);
-    // concurrent parts need the atomic LDS add
-    const bool conc = P->tune_split == SP_CONCURRENT && P->tune_lds_atomic != 0;
+    const bool conc = P->tune_split == SP_CONCURRENT;
     if constexpr (NL == 10 && C == 1 && QG == 4 && (PL == PL_SYM6 || PL == PL_SYM || PL == PL_SYM32)) {
         if (structural_rows<NL, C, QG>(P, A))
             return conc ? launch_gather_k<NL, C, PL, F32, QG, SP_CONCURRENT, 1>(P, A, lds, s)
@@ -1893,9 +1971,15 @@ static int launch_mfma(const skfa_plan* P, const GatherArgs& A, hipStream_t s) {
 template <bool F32>
 static int launch_vec10(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
     const dim3 grid = wave_grid(P), block(64);
+    // barycentric pair form when the pair table came with the reduced
+    // tables (row form -1 or 1)
+    const bool pairs = A.qtab != nullptr && (!F32 || A.qtab32 != nullptr) && P->tune_kform != 0;
     return with_record_layout<10>(P, [&](auto rb, auto ow) {
         constexpr int RB = decltype(rb)::value, OFFW = decltype(ow)::value;
-        k_gather_vec10<F32, RB, OFFW, 4><<<grid, block, lds, s>>>(A);
+        if (pairs)
+            k_gather_vec10<F32, RB, OFFW, 4, 1><<<grid, block, lds, s>>>(A);
+        else
+            k_gather_vec10<F32, RB, OFFW, 4, 0><<<grid, block, lds, s>>>(A);
         SKFA_LAUNCH_CHECK();
         return kOk;
     });
@@ -2132,6 +2216,10 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
             const size_t off = size_t(nqg) + size_t(nqg) * 30 + nqm + size_t(nqm) * 10;
             G.ctab = rtab + off;
             G.ctab32 = rtab32 ? rtab32 + off : nullptr;
+            // then the 10 x 10 x 8 pair table, at a multiple of 4 entries
+            const size_t poff = (off + size_t(16) * nqg + 3) & ~size_t(3);
+            G.qtab = rtab + poff;
+            G.qtab32 = rtab32 ? rtab32 + poff : nullptr;
         }
         G.has_mass = c_value != nullptr;
         G.values = values;

@@ -664,8 +664,9 @@ int skfa_plan_set_tuning(skfa_plan* P, int32_t key, int32_t value) {
             P->tune_streaming = value;
             return kOk;
         case SKFA_TUNE_LDS_ATOMIC:
-            SKFA_REQUIRE(value == 0 || value == 1, "lds atomic flag must be 0 or 1");
-            P->tune_lds_atomic = value;
+            // the gathers always accumulate with ds_add_f64; 1 is kept as
+            // the accepted value of this key
+            SKFA_REQUIRE(value == 1, "LDS accumulation is always ds_add_f64: lds atomic flag must be 1");
             return kOk;
         case SKFA_TUNE_ELEMENT_TABLE:
             SKFA_REQUIRE(value >= -1 && value <= 1, "element table flag must be -1 (auto), 0 or 1");

@@ -82,7 +82,6 @@ struct skfa_plan {
     int32_t tune_mfma = -1;       // MFMA row contraction for scalar fields: -1/0 off, 1 all waves, 2 waves <= 16 rows
     int32_t tune_placement = 2;   // 0 plain, 1 XCD segments, 2 XCD chunks
     int32_t tune_xcd_chunk = 16;
-    int32_t tune_lds_atomic = 1;
     int32_t tune_streaming = 1;   // non-temporal record loads / value stores
     // contiguous wave segments of equal gather work, one per XCD
     int64_t xcd_seg_host[9] = {0};

@@ -89,11 +89,15 @@ class Plan:
         """Launch tuning (library key in parentheses).
 
         Bit-identical results for every value: ``xcd_chunk`` (1, waves per
-        XCD chunk), ``lds_atomic`` (2), ``placement`` (3: 0 plain, 1 XCD
-        segments, 2 XCD chunks), ``streaming`` (4), ``row_split`` (6: how
-        the lanes of one row share it: 0 incidences, 1 entries, 2 columns).  Different rounding only:
-        ``element_table`` (5: -1 auto, 0, 1) and ``mfma`` (7: 0 off, 1 all
-        waves, 2 waves of at most 16 rows)."""
+        XCD chunk), ``placement`` (3: 0 plain, 1 XCD segments, 2 XCD
+        chunks), ``streaming`` (4), ``row_split`` (6: how the lanes of one
+        row share it: 0 incidences, 1 entries, 2 columns).  Different
+        rounding only: ``row_split`` 3 (concurrent incidences),
+        ``element_table`` (5: -1 auto, 0, 1), ``mfma`` (7: 0 off, 1 all
+        waves, 2 waves of at most 16 rows) and ``row_form`` (8: P2 rows and
+        vector-P2 blocks from the barycentric structure, 1, or from the
+        quadrature tables, 0; -1 auto).  ``lds_atomic`` (2) accepts only 1:
+        the span adds are always ds_add_f64."""
         L = _native.lib()
         for name, value in settings.items():
             if name not in _TUNE_KEYS:

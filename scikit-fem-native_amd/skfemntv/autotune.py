@@ -4,10 +4,10 @@ The gather's speed depends on choices that no formula settles well across
 meshes and GPUs: the LDS budget of a wave's CSR span (it fixes how many
 consecutive node rows a 64-lane wave owns and how many lanes share a row),
 how the lanes of one row share its incidences (in turn, or concurrently
-with the LDS atomic add resolving shared addresses), the element-row
-formulation of P2 rows (full gradient table, or the barycentric structure
-of the reference gradients), the element-table gather of 4-node forms and
-the chunk of consecutive waves dealt to one XCD.
+with the LDS atomic add resolving shared addresses), the formulation of P2
+element rows and vector-P2 node blocks (quadrature tables, or the
+barycentric structure of the reference gradients), the element-table gather
+of 4-node forms and the chunk of consecutive waves dealt to one XCD.
 
 On the first assembly of a plan with at least ``MIN_ELEMENTS`` elements
 (outside HIP-graph capture), the tuner times every candidate on the live
@@ -76,7 +76,8 @@ def candidates(plan):
     chunks = (8, 16, 32)
     if C == 3:
         budgets = (12 * 1024, 16 * 1024, 20 * 1024, 24 * 1024)
-        variants = [{"row_split": s} for s in (0, 3)]
+        forms = (0, 1) if n == 10 else (0,)
+        variants = [{"row_split": s, "row_form": f} for s, f in itertools.product((0, 3), forms)]
     elif n == 10:
         budgets = (8 * 1024, 10 * 1024, 12 * 1024, 14 * 1024, 16 * 1024)
         variants = [{"row_split": s, "row_form": f} for s, f in itertools.product((0, 3), (0, 1))]

@@ -156,7 +156,11 @@ def reduced_tables(X, W, quadratic):
         wm, pm = W, s
     tab = np.concatenate([wg, dg.ravel(), wm, pm.ravel()]).astype(np.float64)
     if quadratic:
-        tab = np.concatenate([tab, barycentric_coefficients(Xg, dg).ravel()])
+        c = barycentric_coefficients(Xg, dg)
+        tab = np.concatenate([tab, c.ravel()])
+        # the pair table starts at a multiple of 4 entries (16-byte rows in
+        # the FP32 copy, 32-byte in FP64)
+        tab = np.concatenate([tab, np.zeros(-len(tab) % 4), pair_tables(wg, dg, wm, pm, c).ravel()])
     return tab, len(wg), len(wm)
 
 
@@ -181,6 +185,49 @@ def barycentric_coefficients(X, dg):
     if not np.allclose(rec, dg, rtol=0., atol=1e-14 * max(1., np.abs(dg).max())):
         raise AssertionError("barycentric coefficients do not reproduce the P2 gradients")
     return c
+
+
+def _pair_table_offset(nqg, nqm):
+    """Offset of the pair table in the 10-node reduced tables: after
+    wg | dg | wm | pm | c, rounded up to a multiple of 4."""
+    n = nqg + nqg * 30 + nqm + nqm * 10 + 16 * nqg
+    return n + (-n % 4)
+
+
+def _p2_pair_slots(a):
+    """(barycentric index m, coefficient row of c) of the two terms of
+    grad phi_a = sum_i c_(a,i)(q) grad L_m(a,i); a vertex has one term (the
+    second row is None)."""
+    if a < 4:
+        return (a, a), (a, None)
+    i, j = _P2_PAIRS[a - 4]
+    return (i, 4 + 2 * (a - 4)), (j, 5 + 2 * (a - 4))
+
+
+def pair_tables(wg, dg, wm, pm, c):
+    """(10, 10, 8) per node pair (a, b) of the 10-node element, for gathers
+    that keep one 3x3 block per pair:
+      [0:6]  q_(ab,i) = sum_q wg_q c_(a,i)(q) grad phi_b(q), i = 0, 1, which
+             factor the gradient sum  sum_q wg_q grad phi_a grad phi_b^T =
+             sum_i grad L_m(a,i) q_(ab,i)^T  (rank two at most),
+      [6]    M_ab = sum_q wm_q phi_a(q) phi_b(q),
+      [7]    0 (16-byte rows).
+    Checked against the sums taken directly from the tabulated gradients."""
+    n = dg.shape[1]
+    out = np.zeros((n, n, 8))
+    for a in range(n):
+        for i, (_, row) in enumerate(_p2_pair_slots(a)):
+            if row is not None:
+                out[a, :, 3 * i:3 * i + 3] = np.einsum("q,q,qbk->bk", wg, c[row], dg)
+    out[:, :, 6] = np.einsum("q,qa,qb->ab", wm, pm, pm)
+    G = np.einsum("q,qai,qbj->abij", wg, dg, dg)
+    rec = np.zeros_like(G)
+    for a in range(n):
+        for i, (m, _) in enumerate(_p2_pair_slots(a)):
+            rec[a] += np.einsum("i,bj->bij", _DL[m], out[a, :, 3 * i:3 * i + 3])
+    if not np.allclose(rec, G, rtol=0., atol=1e-14 * max(1., np.abs(G).max())):
+        raise AssertionError("pair tables do not reproduce the gradient sums")
+    return out
 
 
 def _is_affine_p2(p, t):
@@ -341,7 +388,7 @@ class Basis:
                 "tables": torch.from_numpy(affine_tables).to(dev),
                 "rtab": torch.from_numpy(rtab).to(dev),
                 # 10-node rtab carries the barycentric coefficients after pm
-                "rtab_barycentric": self._quadratic and len(rtab) == nqg * 31 + nqm * 11 + 16 * nqg,
+                "rtab_barycentric": self._quadratic and len(rtab) == _pair_table_offset(nqg, nqm) + 800,
                 "rtab32": torch.from_numpy(rtab.astype(np.float32)).to(dev),
                 "nqg": nqg,
                 "nqm": nqm,

@@ -299,6 +299,45 @@ def test_vector_p2_elasticity_matches_oracle():
     say(f"elasticity + mass rel err {err2:.3e}")
 
 
+def test_vector_p2_block_forms_match_oracle():
+    """Vector-P2 gather, both 3x3 block forms (row form 0: 4-point sums;
+    1: barycentric pair table), incidence and concurrent splits, at a small
+    span budget (two and four lanes per node row) and the default one,
+    FP64 and FP32 products, with value, gradient, symmetric-gradient and
+    divergence terms: every combination against the oracle
+    (bilinear_form_assembler.cpp:79-105)."""
+    import skfemntv as sk
+    from skfemntv._assembly import NativeBilinearForm, Plan, upload
+    say("vector p2 block forms x splits x budgets")
+    p, t = kuhn(3, 0.15, seed=31)
+    p2, t2 = host.p2_mesh(p, t)
+    X, W = host.tet_rule_4()
+    basis = sk.Basis(sk.MeshTet2(p2, t2), sk.ElementVector(sk.ElementTetP2(), dim=3), quadrature=(X, W))
+    E, Q = t.shape[1], 4
+    rng = np.random.default_rng(32)
+    coef = {k: np.exp(rng.standard_normal(E)) for k in ("value", "gradient", "symmetric_gradient", "divergence")}
+    oracle_kw = {"value": "value", "gradient": "gradient", "symmetric_gradient": "symgrad", "divergence": "divergence"}
+    ip, ix, vals, N = pipeline.bilinear(p2, t2, X, W, True, components=3,
+                                        **{oracle_kw[k]: np.repeat(v[:, None], Q, 1) for k, v in coef.items()})
+    dev = {k: upload(v) for k, v in coef.items()}
+    d = basis.device_data()
+    assert d["rtab_barycentric"]
+    form = NativeBilinearForm(basis)
+    results = {}
+    for budget in (8192, None):
+        plan = Plan(d["node_conn"], E, 10, d["nnodes"], 3, lds_budget=budget) if budget else form.plan
+        form.plan = plan
+        for row_form in (0, 1):
+            for split in (0, 3):
+                plan.set_tuning(row_form=row_form, row_split=split)
+                for prec in ("fp64", "fp32"):
+                    v = form.assemble_device(precision=prec, **dev)[:len(vals)].cpu().numpy()
+                    err = pipeline.relative_frobenius(v, vals)
+                    results[(budget, row_form, split, prec)] = err
+                    assert err <= (FP64_TOL if prec == "fp64" else FP32_TOL), (budget, row_form, split, prec, err)
+    say("rel errors " + ", ".join(f"{k}: {e:.1e}" for k, e in results.items()))
+
+
 def test_p2_mass_and_perqp_tensor_match_oracle():
     import skfemntv as sk
     from skfemntv.helpers import dot, grad, mul

@@ -90,7 +90,8 @@ def test_reduced_tables_reproduce_rule_sums(quadratic):
         if quadratic:
             # barycentric coefficients of the gradient rule follow: they
             # rebuild the tabulated gradients from grad L_m
-            c = tab[o + qm + qm * n:].reshape(16, qg)
+            o2 = o + qm + qm * n
+            c = tab[o2:o2 + 16 * qg].reshape(16, qg)
             DL = np.array([[-1., -1., -1.], [1., 0., 0.], [0., 1., 0.], [0., 0., 1.]])
             pairs = [(k, k) for k in range(4)]
             for e, (i, j) in enumerate(((0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3))):
@@ -99,6 +100,24 @@ def test_reduced_tables_reproduce_rule_sums(quadratic):
             for p_, (b, m) in enumerate(pairs):
                 rec[:, b] += c[p_][:, None] * DL[m]
             np.testing.assert_allclose(rec, dg, rtol=0., atol=1e-14)
+            # then the per-pair factors of the gradient sums and the mass
+            # sums: sum_i grad L_m(a,i) q_(ab,i)^T must equal the basis
+            # rule's gradient sum, [6] its mass sum
+            po = o2 + 16 * qg
+            po += -po % 4   # 16-byte aligned in the FP32 copy
+            assert not tab[o2 + 16 * qg:po].any()
+            pt = tab[po:].reshape(n, n, 8)
+            assert pt.shape == (10, 10, 8) and not pt[:, :, 7].any()
+            Gb = np.einsum("q,qai,qbj->abij", W, g, g)
+            terms = {k: [(k, 0)] for k in range(4)}
+            for e, (i, j) in enumerate(((0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3))):
+                terms[4 + e] = [(i, 0), (j, 1)]
+            for a in range(n):
+                rebuilt = sum(np.einsum("i,bj->bij", DL[m], pt[a, :, 3 * k:3 * k + 3]) for m, k in terms[a])
+                np.testing.assert_allclose(rebuilt, Gb[a], rtol=0., atol=1e-14)
+                if a < 4:
+                    assert not pt[a, :, 3:6].any()
+            np.testing.assert_allclose(pt[:, :, 6], np.einsum("q,qa,qb->ab", W, s, s), rtol=1e-13, atol=1e-16)
         else:
             assert len(tab) == o + qm + qm * n
         G = np.einsum("q,qai,qbj->aibj", W, g, g)
