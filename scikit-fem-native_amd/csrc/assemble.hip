@@ -770,7 +770,9 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
 
 // Software-pipelined gather: while batch b is computed, the payloads of
 // batch b+1 and the records of batch b+2 are in flight, so the two
-// dependent loads of a step (record -> payload) overlap other work.
+// dependent loads of a step (record -> payload) overlap other work.  (The
+// vector-P2 gather below swaps two buffer sets instead of copying; here
+// that measured slower: C2 0.264 -> 0.273 ms, C4 0.699 -> 0.707 ms.)
 template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, int SP, int KF = 0>
 __global__ void __launch_bounds__(64) k_gather_h1(GatherArgs A) {
     extern __shared__ double span_lds[];
@@ -937,10 +939,15 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
     }
     const int64_t nsteps = (L.steps + L.S - 1) / L.S;
     const bool mass = A.has_mass != 0;
-    Record<RB> r0[K], r1[K], r2[K];
-    double p0[K][P], p1[K][P];
-    T q0[K][NQ], q1[K][NQ];
-    load_records<K, RB>(A, L, 0, r0);
+    // two buffer sets, used alternately (no register rotation): set X holds
+    // the records, payloads and pair rows of the batch being computed, set Y
+    // those of the next batch; a batch's records are reduced to what the
+    // step needs (packed element/row, in-row offset) before their registers
+    // take the records two batches ahead
+    Record<RB> rX[K], rY[K];
+    double pX[K][P], pY[K][P];
+    T qX[K][NQ], qY[K][NQ];
+    load_records<K, RB>(A, L, 0, rX);
     zero_span(span_lds, L.span, lane);
     const int Qm = A.Qm;
     const int lg = QG * NL * 3;
@@ -999,21 +1006,29 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
             }
         }
     };
-    load_payloads<K, RB, P>(A, r0, p0);
-    load_pairs(r0, q0);
-    load_records<K, RB>(A, L, K, r1);
+    load_payloads<K, RB, P>(A, rX, pX);
+    load_pairs(rX, qX);
+    load_records<K, RB>(A, L, K, rY);
     __syncthreads();
     const T* phb = src + QG + lg + Qm + b;   // phi_b(q) at stride NL
     const OffsetPick<RB, OFFW> pick(b);
-    for (int64_t st = 0; st < nsteps; st += K) {
-        load_payloads<K, RB, P>(A, r1, p1);
-        load_pairs(r1, q1);
-        load_records<K, RB>(A, L, st + 2 * K, r2);
+    struct StepRef {
+        int32_t packed;   // element << 4 | local row a, -1 for padding
+        int32_t off;      // in-row offset of the lane's column node b
+    };
+    auto reduce = [&](const Record<RB>* rec, StepRef* ex) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            if (r0[k].packed < 0) continue;
-            const double* pl = p0[k];
-            const int a = r0[k].packed & 15;
+            ex[k].packed = rec[k].packed;
+            ex[k].off = pick(rec[k]);
+        }
+    };
+    auto compute = [&](const StepRef* ex, const double (*pb)[P], const T (*qb)[NQ]) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (ex[k].packed < 0) continue;
+            const double* pl = pb[k];
+            const int a = ex[k].packed & 15;
             T B[9];
 #pragma unroll
             for (int t = 0; t < 9; ++t) B[t] = T(pl[t]);
@@ -1021,7 +1036,7 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
             T mab = T(0);
             if constexpr (KF == 1) {
                 const T* pa = wdg + a * 6;
-                const T* qa = q0[k];
+                const T* qa = qb[k];
                 T u[2][3], w[2][3];
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
@@ -1070,7 +1085,7 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
                     if (c == d) e += diag;
                     v[c][d] = double(e);
                 }
-            double* col = span_lds + int(L.rb) + C * pick(r0[k]);
+            double* col = span_lds + int(L.rb) + C * ex[k].off;
             auto add_block = [&]() {
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
@@ -1086,15 +1101,23 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
                     if (L.p == turn) add_block();
             }
         }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            r0[k] = r1[k];
-            r1[k] = r2[k];
-#pragma unroll
-            for (int t = 0; t < P; ++t) p0[k][t] = p1[k][t];
-#pragma unroll
-            for (int t = 0; t < NQ; ++t) q0[k][t] = q1[k][t];
-        }
+    };
+    for (int64_t st = 0; st < nsteps; st += 2 * K) {
+        StepRef ex[K];
+        // batch st from set X; set Y receives the payloads of batch st + K,
+        // set X the records of batch st + 2K
+        reduce(rX, ex);
+        load_payloads<K, RB, P>(A, rY, pY);
+        load_pairs(rY, qY);
+        load_records<K, RB>(A, L, st + 2 * K, rX);
+        compute(ex, pX, qX);
+        if (st + K >= nsteps) break;
+        // batch st + K from set Y; the roles swap
+        reduce(rY, ex);
+        load_payloads<K, RB, P>(A, rX, pX);
+        load_pairs(rX, qX);
+        load_records<K, RB>(A, L, st + 3 * K, rY);
+        compute(ex, pY, qY);
     }
     __syncthreads();
     write_span(span_lds, L.span, lane, A.values + L.lo * C * C, A.plan.streaming);
