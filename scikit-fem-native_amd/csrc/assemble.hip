@@ -335,7 +335,9 @@ __device__ inline int rec_offset_dyn(const Record<RB>& r, int b) {
 //   C == 1 (stride 10): D = adj X adj^T / |det| with
 //            X = (g + s/2) I + (s/2 + lam) e0 e0^T + A, then |det| m
 //   C == 3 (stride 14): B = adj / sqrt|det|, g + s/2, s/2, lam, |det| m
-//            (lane (c,d) builds D_cd = dcd (g+s/2) B B^T + s/2 B_d B_c^T + lam B_c B_d^T)
+//            (lane (c,d) builds D_cd = dcd (g+s/2) B B^T + s/2 B_d B_c^T + lam B_c B_d^T);
+//            the FP32-products vector-P2 gather reads the same 13 values as
+//            16 floats (64 B), which it rounds to FP32 anyway
 //   N1     (stride 14): kappa adj adj^T/|det| (6), wsum mu J^T J/|det| (6),
 //            edge signs (bits)
 
@@ -373,8 +375,9 @@ __device__ inline void flush_payload_block(const V* buf, V* out, int64_t e0, int
 
 template <int C, bool SYM, int SYMOUT>
 struct PayloadOut {
-    using V = typename std::conditional<C == 1 && SYM && SYMOUT == SYMOUT_F32, float, double>::type;
-    static constexpr int PO = C == 1 ? (SYM ? (SYMOUT == SYMOUT_F64_NOMASS ? 6 : 8) : 10) : 14;
+    using V = typename std::conditional<SYMOUT == SYMOUT_F32 && (C == 3 || SYM), float, double>::type;
+    static constexpr int PO = C == 1 ? (SYM ? (SYMOUT == SYMOUT_F64_NOMASS ? 6 : 8) : 10)
+                                     : (SYMOUT == SYMOUT_F32 ? 16 : 14);
 };
 
 template <int C, bool SYM, int SYMOUT = SYMOUT_F64>
@@ -433,12 +436,13 @@ __global__ void __launch_bounds__(kPayloadBlock) k_payload_h1(PayloadArgs A) {
             } else {
                 const double sc = 1. / sqrt(ad);
 #pragma unroll
-                for (int t = 0; t < 9; ++t) o[t] = a[t] * sc;
-                o[9] = g + 0.5 * s;
-                o[10] = 0.5 * s;
-                o[11] = lam;
-                o[12] = ad * m;
-                o[13] = 0.;
+                for (int t = 0; t < 9; ++t) o[t] = V(a[t] * sc);
+                o[9] = V(g + 0.5 * s);
+                o[10] = V(0.5 * s);
+                o[11] = V(lam);
+                o[12] = V(ad * m);
+#pragma unroll
+                for (int t = 13; t < PO; ++t) o[t] = V(0);
             }
         }
         __syncthreads();
@@ -904,7 +908,8 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
     extern __shared__ double span_lds[];
     // two steps in flight: the PL_VEC payloads take 2 x 14 doubles per lane
     // (measured on 48^3 elasticity with KF = 0: K = 2 0.976 ms, K = 1 1.045 ms)
-    constexpr int NL = 10, C = 3, P = 14, K = 2;
+    // payload: 14 doubles, or 16 floats for the FP32-products form
+    constexpr int NL = 10, C = 3, P = F32 ? 16 : 14, K = 2;
     constexpr int NQ = KF == 1 ? 8 : 1;   // pair-table values held per step
     using T = typename std::conditional<F32, float, double>::type;
     const int lane = threadIdx.x;
@@ -945,7 +950,7 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
     // step needs (packed element/row, in-row offset) before their registers
     // take the records two batches ahead
     Record<RB> rX[K], rY[K];
-    double pX[K][P], pY[K][P];
+    T pX[K][P], pY[K][P];
     T qX[K][NQ], qY[K][NQ];
     load_records<K, RB>(A, L, 0, rX);
     zero_span(span_lds, L.span, lane);
@@ -1023,11 +1028,11 @@ __global__ void __launch_bounds__(64) k_gather_vec10(GatherArgs A) {
             ex[k].off = pick(rec[k]);
         }
     };
-    auto compute = [&](const StepRef* ex, const double (*pb)[P], const T (*qb)[NQ]) {
+    auto compute = [&](const StepRef* ex, const T (*pb)[P], const T (*qb)[NQ]) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (ex[k].packed < 0) continue;
-            const double* pl = pb[k];
+            const T* pl = pb[k];
             const int a = ex[k].packed & 15;
             T B[9];
 #pragma unroll
@@ -2178,7 +2183,9 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
         // D is symmetric without a tensor term, or with a tensor the caller
         // declares symmetric
         sym = P->C == 1 && (c_tensor == nullptr || (flags & SKFA_FLAG_SYMMETRIC_TENSOR));
-        const int stride = P->C == 1 ? (sym ? 8 : 10) : 14;
+        // the FP32-products vector-P2 gather reads a float payload (16 floats)
+        const bool vec32 = P->C == 3 && P->n == 10 && nqg == 4 && precision == 1;
+        const int stride = P->C == 1 ? (sym ? 8 : 10) : (vec32 ? 8 : 14);
         // the table gather runs for symmetric scalar forms unless the MFMA
         // gather takes them (launch_gather)
         const bool table = P->C == 1 && sym && table_gather(P) && !(!diag3 && mfma_gather(P, nqg, nqm));
@@ -2224,6 +2231,8 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
             k_payload_h1<1, true><<<elem_grid(P->E), 256, 0, s>>>(pa);
         else if (P->C == 1)
             k_payload_h1<1, false><<<elem_grid(P->E), 256, 0, s>>>(pa);
+        else if (vec32)
+            k_payload_h1<3, false, SYMOUT_F32><<<elem_grid(P->E), 256, 0, s>>>(pa);
         else
             k_payload_h1<3, false><<<elem_grid(P->E), 256, 0, s>>>(pa);
         SKFA_LAUNCH_CHECK();
