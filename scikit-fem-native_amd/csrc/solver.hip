@@ -284,9 +284,15 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     return t;
 }
 
+// Column of nonzero k of a block whose first row is r0: int32 indices are
+// absolute, int16 ones are offsets from r0 (the block-relative copy that
+// k_block_relative writes for the preconditioner factors).
+__device__ __forceinline__ int32_t column_of(int32_t j, int64_t) { return j; }
+__device__ __forceinline__ int32_t column_of(int16_t d, int64_t r0) { return int32_t(r0) + int32_t(d); }
+
 // Row sums of A x over rows [r0, r1) with x given by `xv(j)`; thread t gets row r0 + t.
-template <typename V, typename XV>
-__device__ __forceinline__ double stream_rows(const int64_t* __restrict__ ip, const int32_t* __restrict__ ix,
+template <typename I, typename V, typename XV>
+__device__ __forceinline__ double stream_rows(const int64_t* __restrict__ ip, const I* __restrict__ ix,
                                               const V* __restrict__ av, int64_t r0, int64_t r1, double* prod,
                                               XV xv) {
     const int64_t k0 = ip[r0], k1 = ip[r1];
@@ -303,7 +309,7 @@ __device__ __forceinline__ double stream_rows(const int64_t* __restrict__ ip, co
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int k = base + u * nt;
-            j[u] = k < cnt ? ix[k0 + k] : 0;
+            j[u] = k < cnt ? column_of(ix[k0 + k], r0) : 0;
             v[u] = k < cnt ? double(av[k0 + k]) : 0.;
         }
 #pragma unroll
@@ -410,8 +416,9 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg_update_jacobi(
 // Factor step, first half: a = rz/pq; r_new = r_old - a q; x += a p;
 // t = G r_new (neighbours' r_new formed on the fly from r_old and q);
 // partial r.r.
+template <typename I>
 __global__ void __launch_bounds__(kPcgThreads) k_pcg_update_g(
-    BlockTable bt, const int64_t* __restrict__ gp, const int32_t* __restrict__ gi, const float* __restrict__ gv,
+    BlockTable bt, const int64_t* __restrict__ gp, const I* __restrict__ gi, const float* __restrict__ gv,
     double* __restrict__ x, const double* __restrict__ r_old, double* __restrict__ r_new, double* __restrict__ t,
     const double* __restrict__ p, const double* __restrict__ q, const double* __restrict__ rz,
     const double* __restrict__ pq, const int32_t* __restrict__ active, double* __restrict__ part_rr) {
@@ -437,8 +444,9 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg_update_g(
 }
 
 // Factor step, second half: z = G^T t; partial r.z.
+template <typename I>
 __global__ void __launch_bounds__(kPcgThreads) k_pcg_apply_gt(
-    BlockTable bt, const int64_t* __restrict__ tp, const int32_t* __restrict__ ti, const float* __restrict__ tv,
+    BlockTable bt, const int64_t* __restrict__ tp, const I* __restrict__ ti, const float* __restrict__ tv,
     const double* __restrict__ t, const double* __restrict__ r, double* __restrict__ z,
     const int32_t* __restrict__ active, double* __restrict__ part_rz) {
     extern __shared__ double prod[];
@@ -463,6 +471,24 @@ __global__ void k_block_nnz_max(BlockTable bt, int64_t nblk, const int64_t* __re
                                 unsigned long long* __restrict__ out) {
     for (int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; b < nblk; b += int64_t(gridDim.x) * blockDim.x)
         atomicMax(out, static_cast<unsigned long long>(ip[bt.row1[b]] - ip[bt.row0[b]]));
+}
+
+// Block-relative 16-bit column offsets of a factor: out[k] = col - r0 for
+// every nonzero k of block b (rows [r0, r1)); `wide` is set when an offset
+// does not fit, and the caller keeps the int32 indices.  One workgroup per
+// block (grid-stride over blocks).
+__global__ void __launch_bounds__(256) k_block_relative(BlockTable bt, int64_t nblk, const int64_t* __restrict__ ip,
+                                                        const int32_t* __restrict__ ix, int16_t* __restrict__ out,
+                                                        int32_t* __restrict__ wide) {
+    for (int64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const int64_t r0 = bt.row0[b];
+        const int64_t k1 = ip[bt.row1[b]];
+        for (int64_t k = ip[r0] + threadIdx.x; k < k1; k += blockDim.x) {
+            const int64_t d = int64_t(ix[k]) - r0;
+            if (d < -32768 || d > 32767) *wide = 1;
+            out[k] = int16_t(d);
+        }
+    }
 }
 
 // After a step: rz_old <- rz, rz <- rz_new, iteration count, convergence.
@@ -722,6 +748,30 @@ int skfa_pcg_solve(const int64_t* indptr, const int32_t* indices, const double* 
     if (int rc = parts.alloc(size_t(std::max<int64_t>(nblk, 1)) * 3)) return rc;
     SKFA_HIP(hipMemcpyAsync(d_bptr.p, bptr.data(), sizeof(int64_t) * (nsys + 1), hipMemcpyHostToDevice, s));
     const BlockTable bt{d_bsys.p, d_br0.p, d_br1.p};
+    // factor indices as 16-bit offsets from their block's first row when
+    // every offset fits (FEM factors are banded): half the index bytes of
+    // the two factor products per iteration, same arithmetic
+    DevBuf<int16_t> g16, gt16;
+    bool rel16 = false;
+    if (kind != 0 && nblk > 0) {
+        int64_t nnz_g = 0, nnz_gt = 0;
+        SKFA_HIP(hipMemcpyAsync(&nnz_g, gptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        SKFA_HIP(hipMemcpyAsync(&nnz_gt, gtptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        SKFA_HIP(hipStreamSynchronize(s));
+        DevBuf<int32_t> wide;
+        if (int rc = g16.alloc(size_t(std::max<int64_t>(nnz_g, 1)))) return rc;
+        if (int rc = gt16.alloc(size_t(std::max<int64_t>(nnz_gt, 1)))) return rc;
+        if (int rc = wide.alloc(1)) return rc;
+        SKFA_HIP(hipMemsetAsync(wide.p, 0, sizeof(int32_t), s));
+        const unsigned cg = unsigned(std::min<int64_t>(nblk, 65536));
+        k_block_relative<<<cg, 256, 0, s>>>(bt, nblk, gptr, gidx, g16.p, wide.p);
+        k_block_relative<<<cg, 256, 0, s>>>(bt, nblk, gtptr, gtidx, gt16.p, wide.p);
+        SKFA_LAUNCH_CHECK();
+        int32_t hw = 1;
+        SKFA_HIP(hipMemcpyAsync(&hw, wide.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        SKFA_HIP(hipStreamSynchronize(s));
+        rel16 = hw == 0;
+    }
     double* part_a = parts.p;
     double* part_b = parts.p + nblk;
     double* part_c = parts.p + 2 * nblk;
@@ -775,10 +825,17 @@ int skfa_pcg_solve(const int64_t* indptr, const int32_t* indices, const double* 
                 k_pcg_update_jacobi<<<nb, upd_threads, 0, s>>>(bt, x, r_cur, z, p_new, q, dinv, rz, pq, active,
                                                             part_b, part_c);
             } else {
-                k_pcg_update_g<<<nb, kPcgThreads, lds_g, s>>>(bt, gptr, gidx, gval, x, r_cur, r_nxt, t, p_new, q,
-                                                                rz, pq, active, part_c);
-                k_pcg_apply_gt<<<nb, kPcgThreads, lds_gt, s>>>(bt, gtptr, gtidx, gtval, t, r_nxt, z, active,
-                                                                part_b);
+                if (rel16) {
+                    k_pcg_update_g<int16_t><<<nb, kPcgThreads, lds_g, s>>>(bt, gptr, g16.p, gval, x, r_cur, r_nxt,
+                                                                          t, p_new, q, rz, pq, active, part_c);
+                    k_pcg_apply_gt<int16_t><<<nb, kPcgThreads, lds_gt, s>>>(bt, gtptr, gt16.p, gtval, t, r_nxt, z,
+                                                                           active, part_b);
+                } else {
+                    k_pcg_update_g<int32_t><<<nb, kPcgThreads, lds_g, s>>>(bt, gptr, gidx, gval, x, r_cur, r_nxt,
+                                                                          t, p_new, q, rz, pq, active, part_c);
+                    k_pcg_apply_gt<int32_t><<<nb, kPcgThreads, lds_gt, s>>>(bt, gtptr, gtidx, gtval, t, r_nxt, z,
+                                                                           active, part_b);
+                }
                 std::swap(r_cur, r_nxt);
             }
             k_sys_reduce<<<nsys, 256, 0, s>>>(d_bptr.p, part_b, part_c, active, rz_new, rr);
