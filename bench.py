@@ -439,9 +439,10 @@ def autotune_summary(plan):
     rec = autotune.LAST.get(autotune.signature(plan)) or autotune._CACHE.get(autotune.signature(plan))
     if rec is None:
         return None
-    return {"lds_budget": rec.budget, "settings": rec.settings, "best_ms": rec.seconds_per_call * 1e3,
-            "candidates": rec.candidates, "tuning_s": rec.tuning_seconds,
-            "current": {"lds_budget": plan.lds_budget, **{k: plan.tuning[k] for k in rec.settings}}}
+    return {"lds_budget": rec.budget, "row_lanes": rec.row_lanes, "settings": rec.settings,
+            "best_ms": rec.seconds_per_call * 1e3, "candidates": rec.candidates, "tuning_s": rec.tuning_seconds,
+            "current": {"lds_budget": plan.lds_budget, "row_lanes": plan.row_lanes,
+                        **{k: plan.tuning[k] for k in rec.settings}}}
 
 
 def measure(w, steps, warmup, ramp_s, timer):
@@ -488,10 +489,16 @@ def measured_traffic(name):
         return json.load(f)["per_assembly_bytes"]
 
 
-def other_workload(name, steps, warmup, ramp_s, timer):
-    """One JSON line for a single non-headline workload (--workload)."""
+def other_workload(name, steps, warmup, ramp_s, timer, vector_mfma=False):
+    """One JSON line for a single non-headline workload (--workload);
+    ``vector_mfma`` puts the C3 plan on the vector-P2 MFMA gather's layout."""
     w = build_workload(name)
     log(f"{name}: setup {w['setup_s']:.1f}s")
+    if vector_mfma:
+        if name != "c3":
+            raise SystemExit("--vector-mfma applies to --workload c3")
+        w["plan"].rebuild(40 * 1024, row_lanes=4)
+        w["plan"]._tuned = True
     sec, summary = measure(w, steps, warmup, ramp_s, timer)
     E, nnz, bpe = w["E"], w["nnz"], w["bpe"]
     out = {"metric": METRIC, "value": E / sec, "unit": "elements/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
@@ -648,6 +655,9 @@ def main():
     ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c3", "c4", "th", "stokes", "cut"),
                     help="c2 (default): the headline line with the other configurations as extra keys; "
                          "the others print their own line")
+    ap.add_argument("--vector-mfma", action="store_true",
+                    help="with --workload c3: run the 16-row layout of the vector-P2 MFMA gather (40 KiB spans) "
+                         "instead of the autotuned one")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's assembly path runs on a single MI355X; "
@@ -662,7 +672,8 @@ def main():
     _native.lib()
     timer = Timer()
     if args.workload != "c2":
-        print(json.dumps(other_workload(args.workload, args.steps, args.warmup, args.ramp, timer)), flush=True)
+        print(json.dumps(other_workload(args.workload, args.steps, args.warmup, args.ramp, timer,
+                                        vector_mfma=args.vector_mfma)), flush=True)
         return
     result, mesh, A_host, keep = headline(args, timer)
     if not args.no_configs:

@@ -63,6 +63,17 @@ int skfa_plan_create_budget(const int32_t* node_conn, int64_t E, int32_t n, int6
  * every size, the layout of large meshes). */
 int skfa_plan_create_ex(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
                         int32_t lds_budget, int64_t min_waves, void* stream, skfa_plan** out);
+/* Same with the lanes per node row of the gather waves: 0 = the default
+ * layout; 4 (10-node 3-component fields only) = the layout of the vector-P2
+ * MFMA gather, 16 rows per wave, each row one column of the MFMA tiles and
+ * its 4 lanes the 4 slices of the contraction.  A plan with this layout
+ * assembles affine elements with element-constant coefficients, a 4-point
+ * reduced gradient rule and at most 4 mass points (else status -1).  Same
+ * pattern as every other layout; no reference counterpart (a launch-shape
+ * choice of this library, like the LDS budget). */
+int skfa_plan_create_layout(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
+                            int32_t lds_budget, int64_t min_waves, int32_t row_lanes, void* stream,
+                            skfa_plan** out);
 int skfa_plan_destroy(skfa_plan* plan);
 /* Launch tuning of an existing plan (results are bit-identical for every
  * setting of keys 1-4; key 5 selects between two formulations that differ

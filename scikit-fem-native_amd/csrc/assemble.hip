@@ -1207,6 +1207,11 @@ __global__ void __launch_bounds__(64) k_gather_mfma(GatherArgs A) {
     }
     if (slot < NL && p < Qm) am = src[Qg + lg + Qm + p * NL + slot];
     const bool mass = A.has_mass != 0;
+    // in-row offsets of the lane's entries b = entry(p, r), picked by masks
+    const OffsetPick<RB, OFFW> pick[4] = {OffsetPick<RB, OFFW>(min(M::entry(p, 0), NL - 1)),
+                                          OffsetPick<RB, OFFW>(min(M::entry(p, 1), NL - 1)),
+                                          OffsetPick<RB, OFFW>(min(M::entry(p, 2), NL - 1)),
+                                          OffsetPick<RB, OFFW>(min(M::entry(p, 3), NL - 1))};
     // unit t = (step t / G, group t % G): the lane's row is 16 (t % G) + slot
     auto load_unit = [&](int64_t t, Record<RB>& rec) {
         rec.packed = -1;
@@ -1269,7 +1274,7 @@ __global__ void __launch_bounds__(64) k_gather_mfma(GatherArgs A) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int b = M::entry(p, r);
-                    if (b < NL) lds_add(span_lds + rb + rec_offset_dyn<RB, OFFW>(rec, b), double(c[r]));
+                    if (b < NL) lds_add(span_lds + rb + pick[r](rec), double(c[r]));
                 }
             }
         }
@@ -1283,6 +1288,154 @@ __global__ void __launch_bounds__(64) k_gather_mfma(GatherArgs A) {
     }
     __syncthreads();
     write_span(span_lds, span, lane, A.values + lo, A.plan.streaming);
+}
+
+// ------------------------------------------------------------------ vector-P2 MFMA gather
+// 10-node 3-component fields (elasticity: lam div div + s eps:eps + iso
+// grad:grad + m u.v), element-constant coefficients, 4-point reduced
+// gradient rule, at most 4 mass points, on the 16-row wave layout (plan
+// row_lanes = 4).  Lane l owns row slot n = l & 15 and contraction slice
+// p = l >> 4.  With g = B^T (w_q grad(phi_a)(q)) for the row's element
+// (payload B = adj/sqrt|det|) and local row a, every entry of the 3x3 node
+// block (a, b) is a contraction over (q, k) with the reference gradients:
+//   K_(a,c),(b,d) = sum_q sum_k grad(phi_b)(q)_k X_cd(q, k) + sum_q phi_b(q) X_cd(q, 3)
+//   X_cd(q, k) = lam g_c B_kd + s/2 g_d B_kc + [c == d] iso (g . B_k)   (k < 3)
+//   X_cd(q, 3) = [c == d] |det| m wm_q phi_a(q)
+// so for each component pair cd and point q one 16x16x4 MFMA with the
+// reference table as A (rows b, 10 of 16 used; k = p) and X as B (columns =
+// the 16 rows of the wave) adds that point's share of all 10 blocks of all
+// 16 rows: 36 MFMAs per incidence layer.  The accumulation over (q, k) runs
+// in the MFMA's precision (FP32 or FP64); the lane then holds entries
+// b = entry(p, r) of its own row and adds them to the FP64 span.  Each LDS
+// address is written by one lane per instruction and the layers run in
+// incidence order, so the per-address summation order is the row's element
+// order.
+template <bool F32, int RB, int OFFW>
+__global__ void __launch_bounds__(64) k_gather_vec10_mfma(GatherArgs A) {
+    extern __shared__ double span_lds[];
+    using M = MfmaT<F32>;
+    using T = typename M::T;
+    constexpr int NL = 10, QG = 4, P = F32 ? 16 : 14;
+    constexpr int lg = QG * NL * 3;
+    const int lane = threadIdx.x;
+    const int n = lane & 15, p = lane >> 4;
+    const int wv = placed_wave(A.plan.xcd_seg, int(A.plan.nwaves), A.plan.placement, A.plan.xcd_chunk);
+    if (wv < 0) return;
+    const WaveMeta wm = A.plan.meta[wv];
+    const int R = wm.R;
+    const int64_t span = int64_t(wm.span) * 9;
+    const int Qm = A.Qm;
+    const T* src = F32 ? reinterpret_cast<const T*>(A.tab32) : reinterpret_cast<const T*>(A.tab);
+    // weighted tables behind the span: w_q grad(phi_a)(q) and wm_q phi_a(q)
+    T* wdg = reinterpret_cast<T*>(span_lds + span);
+    T* wpm = wdg + lg;
+    for (int k = lane; k < lg + Qm * NL; k += 64) {
+        if (k < lg)
+            wdg[k] = src[QG + k] * src[k / (NL * 3)];
+        else
+            wpm[k - lg] = src[QG + lg + Qm + (k - lg)] * src[QG + lg + (k - lg) / NL];
+    }
+    zero_span(span_lds, span, lane);
+    // A operands: reference gradients (slices 0-2) and values (slice 3) of node b = n
+    T aop[QG];
+#pragma unroll
+    for (int q = 0; q < QG; ++q) {
+        aop[q] = T(0);
+        if (n < NL) {
+            if (p < 3) aop[q] = src[QG + (q * NL + n) * 3 + p];
+            else if (q < Qm) aop[q] = src[QG + lg + Qm + q * NL + n];
+        }
+    }
+    const bool live = n < R;
+    int rlen = 0, rb = 0;
+    if (live) {
+        const int64_t b0 = A.plan.indptr[wm.r0 + n];
+        rlen = int(A.plan.indptr[wm.r0 + n + 1] - b0);
+        rb = int(9 * (b0 - wm.lo));
+    }
+    const int64_t steps = wm.steps;
+    // in-row offsets of the lane's entries b = entry(p, r), picked by masks
+    // (a runtime index into the record would put it in scratch memory)
+    const OffsetPick<RB, OFFW> pick[4] = {OffsetPick<RB, OFFW>(min(M::entry(p, 0), NL - 1)),
+                                          OffsetPick<RB, OFFW>(min(M::entry(p, 1), NL - 1)),
+                                          OffsetPick<RB, OFFW>(min(M::entry(p, 2), NL - 1)),
+                                          OffsetPick<RB, OFFW>(min(M::entry(p, 3), NL - 1))};
+    auto load_rec = [&](int64_t layer, Record<RB>& r) {
+        r.packed = -1;
+        if (live && layer < steps)
+            load_record<RB>(A.plan.records + (wm.rec0 + layer * R + n) * RB, r, A.plan.streaming);
+    };
+    Record<RB> r0[1], r1[1], r2[1];
+    T p0[1][P], p1[1][P];
+    load_rec(0, r0[0]);
+    load_payloads<1, RB, P>(A, r0, p0);
+    load_rec(1, r1[0]);
+    __syncthreads();
+    const T slice = p < 3 ? T(1) : T(0);
+    for (int64_t st = 0; st < steps; ++st) {
+        load_payloads<1, RB, P>(A, r1, p1);
+        load_rec(st + 2, r2[0]);
+        const bool ok = r0[0].packed >= 0;
+        const int a = ok ? (r0[0].packed & 15) : 0;
+        T B[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) B[t] = ok ? T(p0[0][t]) : T(0);
+        const T iso = ok ? T(p0[0][9]) : T(0), hs = ok ? T(p0[0][10]) : T(0);
+        const T lam = ok ? T(p0[0][11]) : T(0), mm = ok ? T(p0[0][12]) : T(0);
+        // row p of B for the gradient slices; slice 3 carries the mass only
+        T Bp[3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) Bp[x] = slice * (p == 0 ? B[x] : p == 1 ? B[3 + x] : B[6 + x]);
+        T lB[3], hB[3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            lB[x] = lam * Bp[x];
+            hB[x] = hs * Bp[x];
+        }
+        const T isop = slice * iso, mp = (T(1) - slice) * mm;
+        typename M::V acc[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) acc[t] = typename M::V{T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int q = 0; q < QG; ++q) {
+            const T* ga = wdg + (q * NL + a) * 3;
+            const T h0 = ga[0], h1 = ga[1], h2 = ga[2];
+            T g[3];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) g[x] = fma(B[x], h0, fma(B[3 + x], h1, B[6 + x] * h2));
+            const T gB = fma(g[0], Bp[0], fma(g[1], Bp[1], g[2] * Bp[2]));
+            const T mt = q < Qm ? mp * wpm[q * NL + a] : T(0);
+            const T dg = fma(isop, gB, mt);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    T X = fma(g[c], lB[d], g[d] * hB[c]);
+                    if (c == d) X += dg;
+                    acc[c * 3 + d] = M::mma(aop[q], X, acc[c * 3 + d]);
+                }
+        }
+        if (ok) {
+            int off[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) off[r] = pick[r](r0[0]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double* rowc = span_lds + rb + c * 3 * rlen;
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (M::entry(p, r) < NL) lds_add(rowc + 3 * off[r] + d, double(acc[c * 3 + d][r]));
+            }
+        }
+        r0[0] = r1[0];
+        r1[0] = r2[0];
+#pragma unroll
+        for (int t = 0; t < P; ++t) p0[0][t] = p1[0][t];
+    }
+    __syncthreads();
+    write_span(span_lds, span, lane, A.values + wm.lo * 9, A.plan.streaming);
 }
 
 // ------------------------------------------------------------------ element-table gather
@@ -2100,6 +2253,22 @@ static int launch_vec10(const skfa_plan* P, const GatherArgs& A, size_t lds, hip
     });
 }
 
+template <bool F32>
+static int launch_vec10_mfma(const skfa_plan* P, const GatherArgs& A, size_t lds, hipStream_t s) {
+    const dim3 grid = wave_grid(P), block(64);
+    return with_record_layout<10>(P, [&](auto rb, auto ow) {
+        constexpr int RB = decltype(rb)::value, OFFW = decltype(ow)::value;
+        k_gather_vec10_mfma<F32, RB, OFFW><<<grid, block, lds, s>>>(A);
+        SKFA_LAUNCH_CHECK();
+        return kOk;
+    });
+}
+
+// the 16-row vector-P2 MFMA layout covers these forms only
+static bool vec_mfma_form(const skfa_plan* P, bool fast, int nqg, int nqm) {
+    return fast && P->C == 3 && P->n == 10 && nqg == 4 && nqm >= 1 && nqm <= 4;
+}
+
 static bool mfma_gather(const skfa_plan* P, int nqg, int nqm) {
     return P->tune_mfma > 0 && nqg <= 4 && nqm <= 4;
 }
@@ -2132,6 +2301,7 @@ static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, const Work
                          hipStream_t s) {
     if constexpr (C == 3) {
         if constexpr (NL == 10) {
+            if (P->row_lanes == 4) return launch_vec10_mfma<F32>(P, A, lds, s);
             if (A.Qg == 4) return launch_vec10<F32>(P, A, lds, s);
         }
         return launch_gather_pl<NL, 3, PL_VEC, F32>(P, A, lds, s);
@@ -2256,6 +2426,9 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
     SKFA_REQUIRE(!diag3 || (P->C == 1 && fast && !c_symgrad && !c_divergence && !c_tensor),
                  "diagonal 3-component output needs a scalar plan, affine elements, element-constant value / "
                  "gradient coefficients only");
+    SKFA_REQUIRE(P->row_lanes != 4 || vec_mfma_form(P, fast, nqg, nqm),
+                 "a plan with the 4-lane vector MFMA layout assembles affine elements with element-constant "
+                 "coefficients, a 4-point gradient rule and at most 4 mass points");
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (P->nnz() == 0 || P->E == 0) {
         if (P->nnz()) SKFA_HIP(hipMemsetAsync(values, 0, sizeof(double) * P->nnz() * (diag3 ? 9 : 1), s));

@@ -306,7 +306,9 @@ static int plan_build(skfa_plan* P, const int32_t* conn, hipStream_t s) {
     // lanes per node row and part: C*C (one lane per component pair), or 10
     // for 10-node 3-component fields, whose gather gives each column node
     // of an element row its own lane
-    const int unit = (P->C == 3 && P->n == 10) ? 10 : cc;
+    // (4 for the vector-P2 MFMA gather: a row is one MFMA column, its 4
+    // lanes the 4 slices of the contraction, so a wave holds 16 rows)
+    const int unit = (P->C == 3 && P->n == 10) ? (P->row_lanes == 4 ? 4 : 10) : cc;
     std::vector<int64_t> w_r0, w_rec;
     std::vector<int32_t> w_split;
     std::vector<int64_t> r_rec(static_cast<size_t>(nn));
@@ -693,7 +695,16 @@ int skfa_plan_set_tuning(skfa_plan* P, int32_t key, int32_t value) {
 
 int skfa_plan_create_ex(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
                         int32_t lds_budget, int64_t min_waves, void* stream, skfa_plan** out) {
+    return skfa_plan_create_layout(node_conn, E, n, nnodes, components, lds_budget, min_waves, 0, stream, out);
+}
+
+int skfa_plan_create_layout(const int32_t* node_conn, int64_t E, int32_t n, int64_t nnodes, int32_t components,
+                            int32_t lds_budget, int64_t min_waves, int32_t row_lanes, void* stream,
+                            skfa_plan** out) {
     SKFA_REQUIRE(out != nullptr, "plan output pointer is null");
+    SKFA_REQUIRE(row_lanes == 0 || row_lanes == 4, "row lanes must be 0 (default layout) or 4");
+    SKFA_REQUIRE(row_lanes == 0 || (components == 3 && n == 10),
+                 "the 4-lane row layout is for 10-node 3-component fields");
     *out = nullptr;
     SKFA_REQUIRE(lds_budget >= 2048 && lds_budget <= int32_t(kMaxWaveLds), "LDS budget out of range");
     SKFA_REQUIRE(E >= 0 && nnodes >= 0, "negative sizes");
@@ -710,6 +721,7 @@ int skfa_plan_create_ex(const int32_t* node_conn, int64_t E, int32_t n, int64_t 
     P->C = components;
     P->lds_budget = lds_budget;
     P->min_waves = min_waves;
+    P->row_lanes = row_lanes;
     int rc = plan_build(P, node_conn, s);
     if (rc != kOk) {
         delete P;

@@ -62,6 +62,8 @@ struct skfa_plan {
     int32_t lds_bytes = 0;            // largest span of any wave, bytes
     int32_t min_split = 1;            // smallest S of the layout (> 1 for small meshes)
     int64_t min_waves = 2048;         // small meshes: raise S until the launch has this many waves
+    int32_t row_lanes = 0;            // lanes per node row and part: 0 = the kernel default (C*C, or 10 for
+                                      // 10-node 3-component fields); 4 = the vector-P2 MFMA gather's 16-row waves
     skfa::DevBuf<uint8_t> records;
 
     // per-element payload of the element-constant fast path (allocated on

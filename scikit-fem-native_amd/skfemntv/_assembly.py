@@ -29,14 +29,17 @@ _TUNE_KEYS = {"xcd_chunk": 1, "lds_atomic": 2, "placement": 3, "streaming": 4, "
 class Plan:
     """Device CSR pattern + gather plan of one (basis, components) pair."""
 
-    def __init__(self, node_conn_dev, E, n, nnodes, components, lds_budget=None, min_waves=2048):
+    def __init__(self, node_conn_dev, E, n, nnodes, components, lds_budget=None, min_waves=2048, row_lanes=None):
         import torch
         from . import autotune
         if lds_budget is None:
-            # the budget the autotuner chose for a plan of this shape, else
+            # the layout the autotuner chose for a plan of this shape, else
             # the default (scalar fields 12 KiB, 3-component fields 16 KiB)
-            lds_budget = autotune.cached_budget(E, n, nnodes, components,
-                                                16 * 1024 if components == 3 else 12 * 1024)
+            lds_budget, cached_lanes = autotune.cached_layout(E, n, nnodes, components,
+                                                              16 * 1024 if components == 3 else 12 * 1024)
+            if row_lanes is None:
+                row_lanes = cached_lanes
+        self.row_lanes = int(row_lanes or 0)
         self._conn = node_conn_dev
         self._h = None
         self.E, self.n, self.nnodes, self.components = E, n, nnodes, components
@@ -55,8 +58,9 @@ class Plan:
     def _create(self, lds_budget):
         L = _native.lib()
         h = ctypes.c_void_p()
-        _native.check(L.skfa_plan_create_ex(_native.ptr(self._conn), self.E, self.n, self.nnodes, self.components,
-                                            int(lds_budget), self.min_waves, _native.stream(), ctypes.byref(h)))
+        _native.check(L.skfa_plan_create_layout(_native.ptr(self._conn), self.E, self.n, self.nnodes,
+                                                self.components, int(lds_budget), self.min_waves, self.row_lanes,
+                                                _native.stream(), ctypes.byref(h)))
         old = self._h
         self._h = h
         self.lds_budget = int(lds_budget)
@@ -72,11 +76,14 @@ class Plan:
             torch.cuda.synchronize()   # no launch may still use the old plan
             L.skfa_plan_destroy(old)
 
-    def rebuild(self, lds_budget):
-        """Re-layout the gather waves for another LDS span budget.  The
-        pattern (indptr, indices) is the same for every budget; the tuning
-        settings are kept."""
+    def rebuild(self, lds_budget, row_lanes=None):
+        """Re-layout the gather waves for another LDS span budget (and,
+        given, another number of lanes per node row: 0 default, 4 the
+        16-row waves of the vector-P2 MFMA gather).  The pattern (indptr,
+        indices) is the same for every layout; the tuning settings are kept."""
         nnz = self.nnz
+        if row_lanes is not None:
+            self.row_lanes = int(row_lanes)
         self._create(lds_budget)
         if self.nnz != nnz:
             raise RuntimeError("plan rebuild changed the pattern")
@@ -234,6 +241,10 @@ def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=No
     if d.get("rtab_barycentric"):
         flags |= _native.FLAG_BARYCENTRIC_TABLE
     prec = {"fp64": 0, "fp32": 1}[precision]
+    if plan.row_lanes == 4 and (iso or per_qp or d["nqg"] != 4 or d["nqm"] > 4):
+        # the 16-row MFMA layout covers element-constant forms on affine
+        # elements only: other forms run on the default layout
+        plan.rebuild(plan.lds_budget, row_lanes=0)
     if prec == 1 and (iso or per_qp):
         raise UnsupportedNativeForm("FP32 products are available for affine elements with element-constant "
                                     "coefficients")
@@ -251,7 +262,8 @@ def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=No
     if not noalloc and not iso and not per_qp:
         # element-constant fast path: pick the launch shape on first use
         from . import autotune
-        autotune.maybe_tune(plan, launch)
+        vector_mfma = plan.components == 3 and plan.n == 10 and d["nqg"] == 4 and d["nqm"] <= 4
+        autotune.maybe_tune(plan, launch, vector_mfma)
     launch()
 
 

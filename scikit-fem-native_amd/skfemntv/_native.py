@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("SKFEMNTV_LIBRARY") or os.path.join(_HERE, "_lib", "li
 # Exported symbols (kept in sync with include/skfem_amd.h by a test).
 SYMBOLS = (
     "skfa_last_error", "skfa_abi_version", "skfa_device_name",
-    "skfa_plan_create", "skfa_plan_create_budget", "skfa_plan_create_ex", "skfa_plan_set_tuning", "skfa_plan_destroy", "skfa_plan_info", "skfa_plan_export_csr", "skfa_plan_export_map",
+    "skfa_plan_create", "skfa_plan_create_budget", "skfa_plan_create_ex", "skfa_plan_create_layout", "skfa_plan_set_tuning", "skfa_plan_destroy", "skfa_plan_info", "skfa_plan_export_csr", "skfa_plan_export_map",
     "skfa_cross_plan_create", "skfa_cross_plan_destroy", "skfa_cross_plan_info", "skfa_cross_plan_export_csr",
     "skfa_assemble_cross", "skfa_assemble_facet_linear", "skfa_assemble_facet_bilinear",
     "skfa_assemble_cut_bilinear", "skfa_assemble_cut_linear",
@@ -72,6 +72,7 @@ def _declare(lib):
         "skfa_plan_destroy": (ctypes.c_int, [P]),
         "skfa_plan_create_budget": (ctypes.c_int, [P, I64, I32, I64, I32, I32, P, ctypes.POINTER(P)]),
         "skfa_plan_create_ex": (ctypes.c_int, [P, I64, I32, I64, I32, I32, I64, P, ctypes.POINTER(P)]),
+        "skfa_plan_create_layout": (ctypes.c_int, [P, I64, I32, I64, I32, I32, I64, I32, P, ctypes.POINTER(P)]),
         "skfa_plan_set_tuning": (ctypes.c_int, [P, I32, I32]),
         "skfa_plan_info": (ctypes.c_int, [P, ctypes.POINTER(I64)]),
         "skfa_plan_export_csr": (ctypes.c_int, [P, P, P, P]),

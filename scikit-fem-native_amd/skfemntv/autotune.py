@@ -7,7 +7,8 @@ how the lanes of one row share its incidences (in turn, or concurrently
 with the LDS atomic add resolving shared addresses), the formulation of P2
 element rows and vector-P2 node blocks (quadrature tables, or the
 barycentric structure of the reference gradients), the element-table gather
-of 4-node forms and the chunk of consecutive waves dealt to one XCD.
+of 4-node forms, the 16-row layout of the vector-P2 MFMA gather and the
+chunk of consecutive waves dealt to one XCD.
 
 On the first assembly of a plan with at least ``MIN_ELEMENTS`` elements
 (outside HIP-graph capture), the tuner times every candidate on the live
@@ -48,6 +49,7 @@ class TuningRecord:
     candidates: int
     tuning_seconds: float
     table: list = field(default_factory=list)
+    row_lanes: int = 0
 
 
 def set_enabled(flag: bool) -> None:
@@ -64,14 +66,22 @@ def signature(plan):
     return (plan.E, plan.n, plan.nnodes, plan.components)
 
 
-def cached_budget(E, n, nnodes, components, default):
-    """LDS budget a new plan of this shape should start from."""
+def cached_layout(E, n, nnodes, components, default):
+    """(LDS budget, lanes per row) a new plan of this shape should start from."""
     rec = _CACHE.get((E, n, nnodes, components))
-    return rec.budget if rec is not None else default
+    return (rec.budget, rec.row_lanes) if rec is not None else (default, 0)
 
 
-def candidates(plan):
-    """(budgets, setting dicts) to try for a plan's kernels."""
+# span budget of the 16-row layout of the vector-P2 MFMA gather (its best
+# of 24-48 KiB on the 48^3 elasticity workload; the layout measured 3x
+# slower than the default one there, LDS-bound at about 3 waves per CU)
+MFMA_BUDGETS = (40 * 1024,)
+
+
+def candidates(plan, vector_mfma=False):
+    """(layouts, setting dicts, chunks) to try for a plan's kernels; a
+    layout is (LDS budget, lanes per row), a setting dict applies to the
+    default layouts (the MFMA gather has no such settings)."""
     n, C = plan.n, plan.components
     chunks = (8, 16, 32)
     if C == 3:
@@ -87,7 +97,10 @@ def candidates(plan):
     else:   # edge plans (N1) and other element types
         budgets = (8 * 1024, 12 * 1024, 16 * 1024)
         variants = [{"row_split": s} for s in (0, 3)]
-    return budgets, variants, chunks
+    layouts = [(b, 0) for b in budgets]
+    if vector_mfma and C == 3 and n == 10:
+        layouts += [(b, 4) for b in MFMA_BUDGETS]
+    return layouts, variants, chunks
 
 
 class _Timer:
@@ -121,9 +134,10 @@ def _score(timer, run):
     return blocks[1]
 
 
-def maybe_tune(plan, run):
+def maybe_tune(plan, run, vector_mfma=False):
     """Tune `plan` on its first qualifying assembly.  `run()` performs one
-    assembly with the plan's current layout and settings."""
+    assembly with the plan's current layout and settings; ``vector_mfma``
+    says that the form can run on the 16-row vector-P2 MFMA layout."""
     if getattr(plan, "_tuned", False) or not _enabled or plan.E < MIN_ELEMENTS:
         return None
     import torch
@@ -135,23 +149,23 @@ def maybe_tune(plan, run):
     if rec is not None:
         apply(plan, rec)
         return rec
-    rec = tune(plan, run)
+    rec = tune(plan, run, vector_mfma)
     _CACHE[sig] = rec
     LAST[sig] = rec
     return rec
 
 
 def apply(plan, rec):
-    if plan.lds_budget != rec.budget:
-        plan.rebuild(rec.budget)
+    if plan.lds_budget != rec.budget or plan.row_lanes != rec.row_lanes:
+        plan.rebuild(rec.budget, row_lanes=rec.row_lanes)
     plan.set_tuning(**rec.settings)
 
 
-def tune(plan, run):
+def tune(plan, run, vector_mfma=False):
     """Time every candidate and leave `plan` at the fastest."""
     import torch
     t0 = time.perf_counter()
-    budgets, variants, chunks = candidates(plan)
+    layouts, variants, chunks = candidates(plan, vector_mfma)
     timer = _Timer()
     try:
         ramp_end = time.perf_counter() + 0.15
@@ -160,14 +174,14 @@ def tune(plan, run):
                 run()
             torch.cuda.synchronize()
         table = []
-        # span budget x formulation at the default chunk, then the chunk
-        for budget in budgets:
-            plan.rebuild(budget)
-            for v in variants:
+        # layout x formulation at the default chunk, then the chunk
+        for budget, lanes in layouts:
+            plan.rebuild(budget, row_lanes=lanes)
+            for v in (variants if lanes == 0 else ({},)):
                 plan.set_tuning(**v)
-                table.append((budget, dict(v), _score(timer, run)))
-        budget, best_v, best_t = min(table, key=lambda r: r[2])
-        plan.rebuild(budget)
+                table.append(((budget, lanes), dict(v), _score(timer, run)))
+        (budget, lanes), best_v, best_t = min(table, key=lambda r: r[2])
+        plan.rebuild(budget, row_lanes=lanes)
         plan.set_tuning(**best_v)
         best_chunk = plan.tuning["xcd_chunk"]
         for c in chunks:
@@ -175,7 +189,7 @@ def tune(plan, run):
                 continue
             plan.set_tuning(xcd_chunk=c)
             t = _score(timer, run)
-            table.append((budget, {**best_v, "xcd_chunk": c}, t))
+            table.append(((budget, lanes), {**best_v, "xcd_chunk": c}, t))
             if t < best_t:
                 best_t, best_chunk = t, c
         settings = {**best_v, "xcd_chunk": best_chunk}
@@ -184,4 +198,4 @@ def tune(plan, run):
         timer.close()
     return TuningRecord(budget=budget, settings=settings, seconds_per_call=best_t, candidates=len(table),
                         tuning_seconds=time.perf_counter() - t0,
-                        table=[(b, s, t) for b, s, t in table])
+                        table=[(b, s, t) for b, s, t in table], row_lanes=lanes)

@@ -338,6 +338,53 @@ def test_vector_p2_block_forms_match_oracle():
     say("rel errors " + ", ".join(f"{k}: {e:.1e}" for k, e in results.items()))
 
 
+def test_vector_p2_mfma_layout_matches_oracle():
+    """The vector-P2 MFMA gather (plan layout with 4 lanes per node row):
+    16-row waves (min_waves 0), the split layouts of a small mesh (fewer rows
+    per wave) and a small span budget, FP64 (f64 MFMA) and FP32 products
+    (f32 MFMA, FP32 accumulation), with value, gradient, symmetric-gradient
+    and divergence terms, and with one kind at a time; plus the fallback of
+    a per-point coefficient to the default layout
+    (bilinear_form_assembler.cpp:79-105)."""
+    import skfemntv as sk
+    from skfemntv._assembly import NativeBilinearForm, Plan, upload
+    say("vector p2 MFMA layout")
+    p, t = kuhn(4, 0.15, seed=41)
+    p2, t2 = host.p2_mesh(p, t)
+    X, W = host.tet_rule_4()
+    basis = sk.Basis(sk.MeshTet2(p2, t2), sk.ElementVector(sk.ElementTetP2(), dim=3), quadrature=(X, W))
+    E, Q = t.shape[1], 4
+    rng = np.random.default_rng(42)
+    coef = {k: np.exp(rng.standard_normal(E)) for k in ("value", "gradient", "symmetric_gradient", "divergence")}
+    oracle_kw = {"value": "value", "gradient": "gradient", "symmetric_gradient": "symgrad", "divergence": "divergence"}
+    d = basis.device_data()
+    form = NativeBilinearForm(basis)
+    cases = [tuple(coef)] + [(k,) for k in ("value", "symmetric_gradient", "divergence")]
+    results = {}
+    for kinds in cases:
+        ip, ix, vals, N = pipeline.bilinear(p2, t2, X, W, True, components=3,
+                                            **{oracle_kw[k]: np.repeat(coef[k][:, None], Q, 1) for k in kinds})
+        dev = {k: upload(coef[k]) for k in kinds}
+        for budget, min_waves in ((32768, 0), (32768, 2048), (8192, 0)):
+            plan = Plan(d["node_conn"], E, 10, d["nnodes"], 3, lds_budget=budget, min_waves=min_waves, row_lanes=4)
+            plan._tuned = True
+            form.plan = plan
+            for prec in ("fp64", "fp32"):
+                v = form.assemble_device(precision=prec, **dev)[:len(vals)].cpu().numpy()
+                err = pipeline.relative_frobenius(v, vals)
+                results[(kinds if len(kinds) == 1 else "all", budget, min_waves, prec)] = err
+                assert err <= (FP64_TOL if prec == "fp64" else FP32_TOL), (kinds, budget, min_waves, prec, err)
+            assert plan.row_lanes == 4
+    say("rel errors " + ", ".join(f"{k}: {e:.1e}" for k, e in results.items()))
+    # a per-point coefficient is outside the MFMA layout: the plan returns to
+    # the default layout and the result still matches the oracle
+    sg = np.exp(rng.standard_normal((E, Q)))
+    ip, ix, vals, N = pipeline.bilinear(p2, t2, X, W, True, components=3, symgrad=sg)
+    v = form.assemble_device(symmetric_gradient=upload(sg), per_qp=True)[:len(vals)].cpu().numpy()
+    assert form.plan.row_lanes == 0
+    assert pipeline.relative_frobenius(v, vals) <= FP64_TOL
+
+
 def test_p2_mass_and_perqp_tensor_match_oracle():
     import skfemntv as sk
     from skfemntv.helpers import dot, grad, mul

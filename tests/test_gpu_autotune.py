@@ -21,8 +21,9 @@ def say(msg):
     sys.stdout.flush()
 
 
-def retime(plan, run, budget, settings, timer, blocks=5, calls=8):
-    plan.rebuild(budget)
+def retime(plan, run, layout, settings, timer, blocks=5, calls=8):
+    budget, lanes = layout
+    plan.rebuild(budget, row_lanes=lanes)
     plan.set_tuning(**settings)
     run()
     run()
@@ -33,11 +34,12 @@ def check_pick(plan, run, rec):
     from skfemntv import autotune
     timer = autotune._Timer()
     try:
-        chosen = retime(plan, run, rec.budget, rec.settings, timer)
+        chosen = retime(plan, run, (rec.budget, rec.row_lanes), rec.settings, timer)
         best = min(retime(plan, run, b, {**rec.settings, **s}, timer) for b, s, _ in rec.table)
     finally:
         timer.close()
-    say(f"tuned {rec.budget} {rec.settings}: {chosen * 1e3:.4f} ms, best candidate {best * 1e3:.4f} ms")
+    say(f"tuned {rec.budget} lanes {rec.row_lanes} {rec.settings}: {chosen * 1e3:.4f} ms, "
+        f"best candidate {best * 1e3:.4f} ms")
     assert chosen <= 1.05 * best, (chosen, best)
 
 

@@ -211,6 +211,15 @@ def test_c3_vector_p2_elasticity_48_pattern_and_rows():
     e32 = compare_rows(K32, ips, ixs, vals, rows, FP32_TOL)
     say(f"C3 sampled rows {rows.size}: fp64 {e64:.2e} fp32 {e32:.2e}")
     assert e32 <= FP32_TOL
+    # the 16-row layout of the vector-P2 MFMA gather at full size
+    form = NativeBilinearForm(basis)
+    form.plan.rebuild(32768, row_lanes=4)
+    form.plan._tuned = True
+    for prec, tol in (("fp64", FP64_TOL), ("fp32", FP32_TOL)):
+        Km = form.assemble(symmetric_gradient=2 * mu[:, None], divergence=lam[:, None], precision=prec)
+        em = compare_rows(Km, ips, ixs, vals, rows, tol)
+        say(f"C3 MFMA layout ({form.plan.waves} waves) {prec}: {em:.2e}")
+        assert em <= tol
 
 
 def test_c4_n1_96_edges_pattern_and_rows():
