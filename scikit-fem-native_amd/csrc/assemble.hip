@@ -313,16 +313,20 @@ __device__ inline int rec_offset(const Record<RB>& r, int b) {
 }
 
 // In-row offset of local node b (runtime b) without dynamic register indexing.
+// The words are copied out one by one at compile-time positions: a runtime
+// index into the record's byte array would put the record in scratch memory.
 template <int RB, int OFFW>
 __device__ inline int rec_offset_dyn(const Record<RB>& r, int b) {
     constexpr int NW = (RB - 4) / 4;
-    const uint32_t* w = reinterpret_cast<const uint32_t*>(r.off);
     constexpr int per = 4 / OFFW;   // offsets per word
     const int wi = b / per;
-    uint32_t word = w[0];
+    uint32_t word = 0;
 #pragma unroll
-    for (int t = 1; t < NW; ++t)
-        if (wi == t) word = w[t];
+    for (int t = 0; t < NW; ++t) {
+        uint32_t wt;
+        __builtin_memcpy(&wt, r.off + 4 * t, 4);
+        if (wi == t) word = wt;
+    }
     const int sh = (b - wi * per) * 8 * OFFW;
     return int((word >> sh) & (OFFW == 1 ? 0xffu : 0xffffu));
 }
@@ -641,8 +645,10 @@ __device__ inline void struct_row(const T (*u)[3], const T* __restrict__ ct, T* 
     }
 }
 
+// Always inlined: a call would pass the payload registers by pointer and put
+// them in scratch memory (the 3-component instances were not inlined).
 template <int NL, int C, int PL, bool F32, int RB, int OFFW, int QG, int SP, int KF = 0>
-__device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Record<RB>& rec,
+__device__ __forceinline__ void compute_step(const GatherArgs& A, const Lane& L, const Record<RB>& rec,
                                     const typename PayloadLayout<PL>::type* pl,
                                     double* span_lds, int Qg, int Qm, bool mass,
                                     const typename std::conditional<F32, float, double>::type* dg,
@@ -669,12 +675,23 @@ __device__ inline void compute_step(const GatherArgs& A, const Lane& L, const Re
         const double iso = (L.c == L.d) ? pl[9] : 0.;
         const double hs = pl[10], lam = pl[11];
         const int c = L.c, d = L.d;
+        // columns c and d of B through 0/1 weights (B is finite, so the
+        // sums are exact): a runtime index into the payload registers, or a
+        // select between two of them, would move the payload to scratch memory
+        const double wc[3] = {c == 0 ? 1. : 0., c == 1 ? 1. : 0., c == 2 ? 1. : 0.};
+        const double wd[3] = {d == 0 ? 1. : 0., d == 1 ? 1. : 0., d == 2 ? 1. : 0.};
+        double Bc[3], Bd[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            Bc[i] = fma(B[i * 3], wc[0], fma(B[i * 3 + 1], wc[1], B[i * 3 + 2] * wc[2]));
+            Bd[i] = fma(B[i * 3], wd[0], fma(B[i * 3 + 1], wd[1], B[i * 3 + 2] * wd[2]));
+        }
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const double bb = B[i * 3 + 0] * B[j * 3 + 0] + B[i * 3 + 1] * B[j * 3 + 1] + B[i * 3 + 2] * B[j * 3 + 2];
-                D[i * 3 + j] = T(iso * bb + hs * B[i * 3 + d] * B[j * 3 + c] + lam * B[i * 3 + c] * B[j * 3 + d]);
+                D[i * 3 + j] = T(iso * bb + hs * Bd[i] * Bc[j] + lam * Bc[i] * Bd[j]);
             }
         mf = (L.c == L.d) ? T(pl[12]) : T(0);
     }
