@@ -239,6 +239,28 @@ int skfa_assemble_bilinear(skfa_plan* plan, const double* xyz, int64_t npoints, 
  * SKFA_TUNE_ROW_FORM). */
 #define SKFA_FLAG_BARYCENTRIC_TABLE 4
 
+/* skfa_assemble_bilinear followed by the load vector of a constant source
+ * f (one device double) on the same scalar (C = 1) plan: `load` (nnodes
+ * FP64, overwritten) receives what skfa_assemble_linear(plan, xyz, vconn,
+ * gconn, gnodes, tables, nq, f, 0, NULL, 0, load) would, to rounding (the
+ * row sums run in the row's incidence order).  Replaces the
+ * BilinearFormAssembler::assemble + LinearFormAssembler::assemble pair
+ * (cpp/src/assembly/bilinear_form_assembler.cpp:42-105,
+ * cpp/src/assembly/linear_form_assembler.cpp:41-137) that the reference's
+ * asm(bilinear) + asm(linear) calls run one after the other.  When the
+ * element-table gather of 4-node elements runs, the gather lane that owns
+ * node row i also sums |det J_e| w_a over the row's incidences
+ * (w_a = sum_q W_q phi_a(X_q)), so the load costs no pass of its own;
+ * otherwise the linear-form kernels run after the gather.  Vector fields
+ * (components 3, or SKFA_FLAG_VECTOR3_DIAGONAL) are rejected (-1). */
+int skfa_assemble_bilinear_load(skfa_plan* plan, const double* xyz, int64_t npoints, const int32_t* vconn,
+                                const int32_t* gconn, int32_t gnodes, const double* tables, int32_t nq,
+                                const double* rtab, const float* rtab32, int32_t nqg, int32_t nqm,
+                                const double* c_value, const double* c_gradient, const double* c_symgrad,
+                                const double* c_divergence, const double* c_tensor, int32_t per_qp,
+                                int32_t precision, int32_t flags, double* values, const double* f, double* load,
+                                void* stream);
+
 /* skfa_assemble_bilinear without any device allocation.  No counterpart in
  * the reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

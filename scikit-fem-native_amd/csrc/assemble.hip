@@ -430,7 +430,7 @@ __global__ void __launch_bounds__(kPayloadBlock) k_payload_h1(PayloadArgs A) {
                     o[3] = V(Dm[4]); o[4] = V(Dm[5]); o[5] = V(Dm[8]);
                     if constexpr (PO == 8) {
                         o[6] = V(ad * m);
-                        o[7] = V(0);
+                        o[7] = V(ad);   // |det J|: the load vector fused into the element-table gather
                     }
                 } else {
 #pragma unroll
@@ -520,6 +520,13 @@ struct GatherArgs {
     // vector-P2 gather's barycentric form; null when not available
     const double* qtab = nullptr;
     const float* qtab32 = nullptr;
+    // load vector fused into the element-table gather of 4-node scalar
+    // fields (skfa_assemble_bilinear_load): b_i = f sum_(e,a) |det J_e| w_a,
+    // w_a = sum_q W_q phi_a(X_q) from the basis tables (w | phi | ...)
+    const double* load_tables = nullptr;
+    int32_t load_q = 0;
+    const double* load_f = nullptr;
+    double* load_out = nullptr;
 };
 
 // K_ab += w_q (dphi_aq^T D) . dphi_bq for all b: the lane's row a reads the
@@ -1519,7 +1526,12 @@ __device__ inline void table_step(const GatherArgs& A, const Lane& L, const Reco
 // Workgroups of WPB waves share one copy of the element table; each wave
 // owns a span region of `span_stride` doubles behind it.  Block placement
 // follows the plan's XCD chunking at block granularity.
-template <int NL, bool F32, int RB, int OFFW, bool MASS, int WPB>
+// LOAD: lane 0 of each row's lanes (it walks every incidence of the row, as
+// all of them do) also sums |det J_e| w_a over the row's incidences, in
+// incidence order, and writes b_i = f x that sum (the constant-source load
+// vector of linear_form_assembler.cpp:41-137 on affine elements); |det J| is
+// the payload's last slot.
+template <int NL, bool F32, int RB, int OFFW, bool MASS, int WPB, bool LOAD = false>
 __global__ void __launch_bounds__(64 * WPB) k_gather_table(GatherArgs A, int32_t span_stride, int32_t nblocks) {
     extern __shared__ double lds[];
     constexpr int P = PayloadLayout<PL_SYM>::stride;
@@ -1555,12 +1567,31 @@ __global__ void __launch_bounds__(64 * WPB) k_gather_table(GatherArgs A, int32_t
     }
     __syncthreads();
     if (!live) return;
+    // LOAD: w_a = sum_q W_q phi_a(X_q) (the order of k_linear_const)
+    double lw0 = 0., lw1 = 0., lw2 = 0., lw3 = 0., lacc = 0.;
+    if constexpr (LOAD) {
+        static_assert(NL == 4, "the fused load vector is for 4-node elements");
+        const Tables Tb = tables_at(A.load_tables, A.load_q, 4, 4);
+        for (int q = 0; q < A.load_q; ++q) {
+            lw0 += Tb.w[q] * Tb.phi[q * 4 + 0];
+            lw1 += Tb.w[q] * Tb.phi[q * 4 + 1];
+            lw2 += Tb.w[q] * Tb.phi[q * 4 + 2];
+            lw3 += Tb.w[q] * Tb.phi[q * 4 + 3];
+        }
+    }
     for (int64_t st = 0; st < L.steps; st += K) {
         load_payloads<K, RB, P>(A, r1, p1);
         load_records<K, RB>(A, L, st + 2 * K, r2);
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            if (r0[k].packed >= 0) table_step<NL, F32, RB, OFFW, MASS>(A, L, r0[k], p0[k], span_lds, ntab);
+            if (r0[k].packed >= 0) {
+                table_step<NL, F32, RB, OFFW, MASS>(A, L, r0[k], p0[k], span_lds, ntab);
+                if constexpr (LOAD) {
+                    const int a = r0[k].packed & 15;
+                    const double wa = a == 0 ? lw0 : a == 1 ? lw1 : a == 2 ? lw2 : lw3;
+                    lacc += p0[k][7] * wa;
+                }
+            }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             r0[k] = r1[k];
@@ -1568,6 +1599,9 @@ __global__ void __launch_bounds__(64 * WPB) k_gather_table(GatherArgs A, int32_t
 #pragma unroll
             for (int t = 0; t < P; ++t) p0[k][t] = p1[k][t];
         }
+    }
+    if constexpr (LOAD) {
+        if (L.active && L.p == 0) A.load_out[A.plan.meta[wv].r0 + L.i] = lacc * A.load_f[0];
     }
     // the span is this wave's own: its LDS operations complete in order
     if (A.plan.diag3)
@@ -2221,7 +2255,7 @@ static int launch_gather_pl(const skfa_plan* P, const GatherArgs& A, size_t lds,
 
 constexpr int kTableWaves = 4;
 
-template <int NL, bool F32, bool MASS>
+template <int NL, bool F32, bool MASS, bool LOAD = false>
 static int launch_table(const skfa_plan* P, const GatherArgs& A, hipStream_t s) {
     using T = typename std::conditional<F32, float, double>::type;
     constexpr int W = kTableWaves;
@@ -2233,7 +2267,7 @@ static int launch_table(const skfa_plan* P, const GatherArgs& A, hipStream_t s) 
     const dim3 grid(unsigned(std::max(nblocks, 1))), block(64 * W);
     return with_record_layout<NL>(P, [&](auto rb, auto ow) {
         constexpr int RB = decltype(rb)::value, OFFW = decltype(ow)::value;
-        k_gather_table<NL, F32, RB, OFFW, MASS, W><<<grid, block, lds, s>>>(A, span_stride, nblocks);
+        k_gather_table<NL, F32, RB, OFFW, MASS, W, LOAD><<<grid, block, lds, s>>>(A, span_stride, nblocks);
         SKFA_LAUNCH_CHECK();
         return kOk;
     });
@@ -2348,6 +2382,11 @@ static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, const Work
         GatherArgs T = A;
         T.tab = W.ntab;
         T.tab32 = W.ntab32;
+        if constexpr (NL == 4) {
+            if (A.load_out != nullptr)
+                return A.has_mass ? launch_table<NL, F32, true, true>(P, T, s)
+                                  : launch_table<NL, F32, false, true>(P, T, s);
+        }
         return A.has_mass ? launch_table<NL, F32, true>(P, T, s) : launch_table<NL, F32, false>(P, T, s);
     }
     if (sym) {
@@ -2420,7 +2459,10 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
                                 const float* rtab32, int32_t nqg, int32_t nqm, const double* c_value,
                                 const double* c_gradient, const double* c_symgrad, const double* c_divergence,
                                 const double* c_tensor, int32_t per_qp, int32_t precision, int32_t flags,
-                                double* values, void* stream, unsigned char* scratch, size_t scratch_bytes) {
+                                double* values, void* stream, unsigned char* scratch, size_t scratch_bytes,
+                                const double* load_f = nullptr, double* load_out = nullptr,
+                                bool* load_fused = nullptr) {
+    if (load_fused) *load_fused = false;
     SKFA_REQUIRE(P != nullptr, "plan is null");
     SKFA_REQUIRE(P->n == 4 || P->n == 10, "bilinear assembly supports 4- and 10-node tetrahedra");
     SKFA_REQUIRE(P->C == 1 || P->C == 3, "bilinear assembly supports 1 or 3 components on tetrahedra");
@@ -2532,6 +2574,15 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
         }
         G.has_mass = c_value != nullptr;
         G.values = values;
+        // the constant-source load vector rides along with the element-table
+        // gather of 4-node scalar fields (|det J| in the payload's last slot)
+        if (load_out != nullptr && table && P->n == 4 && !diag3 && tables != nullptr && nq >= 1 && nq <= 64) {
+            G.load_tables = tables;
+            G.load_q = nq;
+            G.load_f = load_f;
+            G.load_out = load_out;
+            if (load_fused) *load_fused = true;
+        }
         const size_t tab_len = size_t(nqg) + size_t(nqg) * P->n * 3 + nqm + size_t(nqm) * P->n;
         lds_fast = size_t(P->lds_bytes) + tab_len * (precision == 1 ? 4 : 8);
     } else {
@@ -2571,6 +2622,28 @@ int skfa_assemble_bilinear(skfa_plan* P, const double* xyz, int64_t npoints, con
     return assemble_bilinear_ws(P, xyz, vconn, gconn, gnodes, tables, nq, rtab, rtab32, nqg, nqm, c_value,
                                 c_gradient, c_symgrad, c_divergence, c_tensor, per_qp, precision, flags, values,
                                 stream, nullptr, 0);
+}
+
+int skfa_assemble_bilinear_load(skfa_plan* P, const double* xyz, int64_t npoints, const int32_t* vconn,
+                                const int32_t* gconn, int32_t gnodes, const double* tables, int32_t nq,
+                                const double* rtab, const float* rtab32, int32_t nqg, int32_t nqm,
+                                const double* c_value, const double* c_gradient, const double* c_symgrad,
+                                const double* c_divergence, const double* c_tensor, int32_t per_qp,
+                                int32_t precision, int32_t flags, double* values, const double* f, double* load,
+                                void* stream) {
+    (void)npoints;
+    SKFA_REQUIRE(P != nullptr, "plan is null");
+    SKFA_REQUIRE(f != nullptr && load != nullptr, "load source or output is null");
+    SKFA_REQUIRE(P->C == 1 && !(flags & SKFA_FLAG_VECTOR3_DIAGONAL), "the fused load vector is for scalar fields");
+    SKFA_REQUIRE(tables != nullptr && nq >= 1, "basis tables are required for the load vector");
+    bool fused = false;
+    if (int rc = assemble_bilinear_ws(P, xyz, vconn, gconn, gnodes, tables, nq, rtab, rtab32, nqg, nqm, c_value,
+                                      c_gradient, c_symgrad, c_divergence, c_tensor, per_qp, precision, flags,
+                                      values, stream, nullptr, 0, f, load, &fused))
+        return rc;
+    if (fused) return kOk;
+    // every other gather: the load vector by the linear-form kernels
+    return skfa_assemble_linear(P, xyz, vconn, gconn, gnodes, tables, nq, f, 0, nullptr, 0, load, stream);
 }
 
 int skfa_assemble_bilinear_noalloc(skfa_plan* P, const double* xyz, int64_t npoints, const int32_t* vconn,

@@ -223,10 +223,13 @@ def basis_scalar_plan(basis):
 
 def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=None, symmetric_gradient=None,
                              divergence=None, tensor=None, per_qp=False, precision="fp64", symmetric_tensor=False,
-                             noalloc=False):
+                             noalloc=False, load=None):
     """Launch skfa_assemble_bilinear with device coefficient tensors
     (``noalloc``: skfa_assemble_bilinear_noalloc, which keeps its scratch in
-    a fixed device array of the library instead of allocating)."""
+    a fixed device array of the library instead of allocating; ``load`` =
+    (f, out): skfa_assemble_bilinear_load, which also writes the load vector
+    of the constant source f (one device double) into out, scalar fields
+    only)."""
     d = basis.device_data()
     L = _native.lib()
     iso = not basis._affine
@@ -249,6 +252,11 @@ def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=No
         raise UnsupportedNativeForm("FP32 products are available for affine elements with element-constant "
                                     "coefficients")
     entry = L.skfa_assemble_bilinear_noalloc if noalloc else L.skfa_assemble_bilinear
+    if load is not None:
+        if noalloc:
+            raise ValueError("the fused load vector has no no-allocation entry point")
+        entry = L.skfa_assemble_bilinear_load
+    extra = () if load is None else (_native.ptr(load[0]), _native.ptr(load[1]))
 
     def launch():
         _native.check(entry(
@@ -257,7 +265,7 @@ def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=No
             _native.ptr(tables), len(basis.W), _native.ptr(d["rtab"]), _native.ptr(d["rtab32"]), d["nqg"],
             d["nqm"], _native.ptr(value), _native.ptr(gradient), _native.ptr(symmetric_gradient),
             _native.ptr(divergence), _native.ptr(tensor), int(per_qp), prec, flags, _native.ptr(values_dev),
-            _native.stream()))
+            *extra, _native.stream()))
 
     if not noalloc and not iso and not per_qp:
         # element-constant fast path: pick the launch shape on first use
@@ -758,12 +766,16 @@ class NativeCrossBilinearForm:
         self.assemble_cross_device(row_kind, column_kind, "tensor", coef, strides)
         return self._cross_matrix()
 
-    def assemble_tensor_device(self, tensor, *, per_qp=False, precision="fp64", out=None, symmetric=False):
+    def assemble_tensor_device(self, tensor, *, per_qp=False, precision="fp64", out=None, symmetric=False,
+                               load=None):
         """Tensor (E,3,3) [(E,Q,3,3) when per_qp] already on the device;
-        ``symmetric`` declares A == A^T per element (smaller payload)."""
+        ``symmetric`` declares A == A^T per element (smaller payload);
+        ``load`` = (f, out) also writes the load vector of the constant
+        source f (a one-element device tensor) into out (nnodes FP64), fused
+        with the gather where it can be."""
         vals = self._values if out is None else out
         assemble_bilinear_device(self.test_basis, self.plan, vals, tensor=tensor, per_qp=per_qp,
-                                 precision=precision, symmetric_tensor=symmetric)
+                                 precision=precision, symmetric_tensor=symmetric, load=load)
         return vals
 
 

@@ -20,7 +20,7 @@ SYMBOLS = (
     "skfa_cross_plan_create", "skfa_cross_plan_destroy", "skfa_cross_plan_info", "skfa_cross_plan_export_csr",
     "skfa_assemble_cross", "skfa_assemble_facet_linear", "skfa_assemble_facet_bilinear",
     "skfa_assemble_cut_bilinear", "skfa_assemble_cut_linear",
-    "skfa_assemble_bilinear", "skfa_assemble_bilinear_noalloc", "skfa_assemble_linear", "skfa_tabulate_geometry",
+    "skfa_assemble_bilinear", "skfa_assemble_bilinear_noalloc", "skfa_assemble_bilinear_load", "skfa_assemble_linear", "skfa_tabulate_geometry",
     "skfa_edge_topology", "skfa_assemble_tet_n1",
     "skfa_event_create", "skfa_event_destroy", "skfa_event_record", "skfa_event_elapsed_ms",
     "skfa_stream_synchronize",
@@ -91,6 +91,8 @@ def _declare(lib):
                                                   P, P, P, P, P, I32, I32, I32, P, P]),
         "skfa_assemble_bilinear_noalloc": (ctypes.c_int, [P, P, I64, P, P, I32, P, I32, P, P, I32, I32,
                                                           P, P, P, P, P, I32, I32, I32, P, P]),
+        "skfa_assemble_bilinear_load": (ctypes.c_int, [P, P, I64, P, P, I32, P, I32, P, P, I32, I32,
+                                                       P, P, P, P, P, I32, I32, I32, P, P, P, P]),
         "skfa_assemble_linear": (ctypes.c_int, [P, P, P, P, I32, P, I32, P, I32, P, I32, P, P]),
         "skfa_tabulate_geometry": (ctypes.c_int, [P, P, P, I32, I64, P, I32, I32, P, P, P, P, P, P, P, P]),
         "skfa_edge_topology": (ctypes.c_int, [P, I64, I64, P, P, P, P, ctypes.POINTER(I64), ctypes.POINTER(I64), P]),

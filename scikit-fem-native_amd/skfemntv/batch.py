@@ -3,9 +3,10 @@ uneven Kuhn-split boxes, assembled together and condensed on the GPU.
 
 The systems are concatenated into one mesh whose components do not touch, so
 one gather plan and one assembly launch pair (element payload pass + CSR
-row-gather) produce every matrix at once as a block-diagonal CSR; the load
-vector is one more launch, and the Dirichlet condensation of the boundary
-vertices is a value gather through a precomputed position map.  After one
+row-gather) produce every matrix at once as a block-diagonal CSR; the row
+gather also sums the load vector (skfa_assemble_bilinear_load), and the
+Dirichlet condensation of the boundary vertices is a value gather through a
+precomputed position map.  After one
 eager call the whole sequence can be replayed as a HIP graph.
 
 Per system s (seeded by ``np.random.default_rng([seed, s])``):
@@ -97,7 +98,7 @@ class AnisotropicBatch:
 
     def __init__(self, shapes, seed=0, jitter=0.2):
         import torch
-        from ._assembly import NativeCrossBilinearForm, NativeLinearForm, upload
+        from ._assembly import NativeCrossBilinearForm, upload
         from .solver import DeviceCSR
         self.shapes = np.asarray(shapes, dtype=np.int64).reshape(-1, 3)
         self.seed, self.jitter = int(seed), float(jitter)
@@ -126,12 +127,12 @@ class AnisotropicBatch:
         self.timings.basis_s = time.perf_counter() - t0
         t0 = time.perf_counter()
         self.form = NativeCrossBilinearForm(self.basis, self.basis)
-        self.load = NativeLinearForm(self.basis)
         torch.cuda.synchronize()
         self.timings.plan_s = time.perf_counter() - t0
         dev = _native.device()
         self.tensors = upload(tensors)
         self._one = torch.ones(1, dtype=torch.float64, device=dev)
+        self._rhs = torch.empty(self.basis.N, dtype=torch.float64, device=dev)
         t0 = time.perf_counter()
         # condensation structure: condense positions once to get the value map
         plan = self.form.plan
@@ -161,11 +162,11 @@ class AnisotropicBatch:
 
     def _assemble(self):
         L = _native.lib()
-        vals = self.form.assemble_tensor_device(self.tensors, symmetric=True)
+        # matrix and load vector in one gather (skfa_assemble_bilinear_load)
+        vals = self.form.assemble_tensor_device(self.tensors, symmetric=True, load=(self._one, self._rhs))
         _native.check(L.skfa_gather_values(_native.ptr(vals), _native.ptr(self.value_map), self.A.nnz,
                                            _native.ptr(self.A.data), _native.stream()))
-        rhs = self.load.assemble_device(self._one, False, None, False)
-        _native.check(L.skfa_gather_values(_native.ptr(rhs), _native.ptr(self.free), self.A.shape[0],
+        _native.check(L.skfa_gather_values(_native.ptr(self._rhs), _native.ptr(self.free), self.A.shape[0],
                                            _native.ptr(self.b), _native.stream()))
 
     def assemble(self, graph=False):

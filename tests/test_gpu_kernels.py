@@ -162,3 +162,49 @@ def test_general_linear_form_per_point_f_and_G_match_oracle(quadratic, C):
     Gc = np.ascontiguousarray(np.broadcast_to(rng.standard_normal((C, 3)), (E, Q, C, 3)))
     b0, _ = form.assemble(gradient=Gc)
     assert np.abs(b0.reshape(-1, C).sum(axis=0)).max() <= 1e-12 * np.abs(b0).max() * N
+
+
+@pytest.mark.parametrize("quadratic,min_waves", [(False, 0), (False, 2048), (True, 0)])
+def test_bilinear_with_fused_load_matches_separate_calls(quadratic, min_waves):
+    """skfa_assemble_bilinear_load: the matrix equals skfa_assemble_bilinear's
+    bit for bit and the load vector equals the linear-form kernel's to
+    rounding, whether the load rides in the P1 element-table gather (min
+    waves 0: one row per lane, the table gather) or runs after the gather
+    (small-mesh split layout, P2); both against the oracle's load vector
+    (linear_form_assembler.cpp:41-137 with a constant f)."""
+    import torch
+    import skfemntv as sk
+    from skfemntv._assembly import NativeCrossBilinearForm, NativeLinearForm, Plan, upload
+    say(f"fused load, quadratic={quadratic}, min_waves={min_waves}")
+    n = 6
+    ax = np.linspace(0., 1., n + 1)
+    p, t = host.kuhn_mesh(ax, ax, ax)
+    p = host.jitter_interior(p, 1. / n, 0.2, 5)
+    X, W = host.tet_rule_4()
+    if quadratic:
+        p2, t2 = host.p2_mesh(p, t)
+        mesh, el, nl = sk.MeshTet2(p2, t2), sk.ElementTetP2(), 10
+    else:
+        p2, t2 = p, t
+        mesh, el, nl = sk.MeshTet(p, t), sk.ElementTetP1(), 4
+    basis = sk.Basis(mesh, sk.ElementVector(el, dim=1), quadrature=(X, W))
+    E = t.shape[1]
+    rng = np.random.default_rng(6)
+    g = rng.standard_normal((E, 3, 3))
+    A = np.einsum("eij,ekj->eik", g, g) + np.eye(3)
+    form = NativeCrossBilinearForm(basis, basis)
+    d = basis.device_data()
+    form.plan = Plan(d["node_conn"], E, nl, d["nnodes"], 1, min_waves=min_waves)
+    form.plan._tuned = True
+    A_dev = upload(A)
+    f = torch.full((1,), 2.5, dtype=torch.float64, device=A_dev.device)
+    ref_vals = form.assemble_tensor_device(A_dev, symmetric=True).clone()
+    load = torch.full((basis.N,), np.nan, dtype=torch.float64, device=A_dev.device)
+    vals = form.assemble_tensor_device(A_dev, symmetric=True, load=(f, load)).clone()
+    assert torch.equal(vals, ref_vals)
+    lin = NativeLinearForm(basis).assemble_device(f, False, None, False).clone()
+    b = load.cpu().numpy()
+    assert rel(b, lin.cpu().numpy()) <= 1e-14
+    bo = 2.5 * pipeline.load_vector(p2, t2, X, W, quadratic, [1.])
+    assert rel(b, bo) <= FP64_TOL, rel(b, bo)
+    say(f"load rel err vs oracle {rel(b, bo):.1e}")
