@@ -92,7 +92,7 @@ def candidates(plan, vector_mfma=False):
         budgets = (8 * 1024, 10 * 1024, 12 * 1024, 14 * 1024, 16 * 1024)
         variants = [{"row_split": s, "row_form": f} for s, f in itertools.product((0, 3), (0, 1))]
     elif n == 4:
-        budgets = (8 * 1024, 12 * 1024, 16 * 1024)
+        budgets = (4 * 1024, 8 * 1024, 12 * 1024, 16 * 1024)
         variants = [{"element_table": t} for t in (-1, 0)]
     else:   # edge plans (N1) and other element types
         budgets = (8 * 1024, 12 * 1024, 16 * 1024)
