@@ -266,7 +266,7 @@ def build_workload(name):
     once."""
     import torch
     import skfemntv as sk
-    from skfemntv._assembly import NativeBilinearForm, NativeLinearForm, upload
+    from skfemntv._assembly import NativeBilinearForm, upload
     from skfemntv.hcurl import AffineTetN1Basis, TetN1Assembler
     t0 = time.perf_counter()
     values = None
@@ -276,13 +276,14 @@ def build_workload(name):
         mesh = sk.MeshTet.init_tensor(ax, ax, ax)
         basis = sk.Basis(mesh, sk.ElementVector(sk.ElementTetP1(), dim=1))
         E = mesh.nelements
-        form, load = NativeBilinearForm(basis), NativeLinearForm(basis)
+        form = NativeBilinearForm(basis)
         one = upload(np.ones(E))
         f = upload(np.array([1.]))
+        rhs = torch.empty(basis.N, dtype=torch.float64, device=one.device)
 
         def step(precision="fp64"):
-            form.assemble_device(gradient=one, precision=precision)
-            load.assemble_device(f, False, None, False)
+            # matrix and load vector in one call (skfa_assemble_bilinear_load)
+            form.assemble_device(gradient=one, precision=precision, load=(f, rhs))
         bpe, desc, nnz = 106.7, f"P1 Laplace ddot(grad u, grad v) + load f=1, 32^3 Kuhn cube, {E} tets, FP64", form.plan.nnz
         values, fp32 = form._values, True
     elif name == "c3":

@@ -374,13 +374,16 @@ class NativeBilinearForm:
         return self._matrix
 
     def assemble_device(self, *, value=None, gradient=None, symmetric_gradient=None, divergence=None,
-                        tensor=None, per_qp=False, precision="fp64", out=None):
+                        tensor=None, per_qp=False, precision="fp64", out=None, load=None):
         """Assemble into a device tensor (default: this form's buffer) with
-        coefficient tensors already on the device; returns the values."""
+        coefficient tensors already on the device; returns the values.
+        ``load`` = (f, out): also the load vector of the constant source f
+        (a one-element device tensor) into out (scalar fields), fused with
+        the gather where it can be (skfa_assemble_bilinear_load)."""
         vals = self._values if out is None else out
         assemble_bilinear_device(self.basis, self.plan, vals, value=value, gradient=gradient,
                                  symmetric_gradient=symmetric_gradient, divergence=divergence, tensor=tensor,
-                                 per_qp=per_qp, precision=precision)
+                                 per_qp=per_qp, precision=precision, load=load)
         return vals
 
 

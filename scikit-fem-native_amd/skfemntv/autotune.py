@@ -93,7 +93,7 @@ def candidates(plan, vector_mfma=False):
         variants = [{"row_split": s, "row_form": f} for s, f in itertools.product((0, 3), (0, 1))]
     elif n == 4:
         budgets = (4 * 1024, 8 * 1024, 12 * 1024, 16 * 1024)
-        variants = [{"element_table": t} for t in (-1, 0)]
+        variants = [{"element_table": t} for t in (-1, 0, 1)]
     else:   # edge plans (N1) and other element types
         budgets = (8 * 1024, 12 * 1024, 16 * 1024)
         variants = [{"row_split": s} for s in (0, 3)]
