@@ -190,7 +190,8 @@ def batched_fsai(systems, lo, hi, steps, warmup, timer):
                     f"per-element SPD A (seed 5000+s), f=1, boundary condensed",
         "elements": E, "unknowns": A.shape[0], "nnz": A.nnz, "nnz_uncondensed": plan.nnz,
         "assembly_ms_per_batch": asm_s * 1e3, "assembly_elements_per_s": E / asm_s,
-        "assembly": "payload + row-gather + condensation gather + load vector, replayed as one HIP graph",
+        "assembly": "payload + row-gather with the load vector, matrix and load written condensed through "
+                    "position maps, replayed as one HIP graph",
         "assembly_roofline": {"bound": "hbm", "bytes": alg, "achieved": alg / asm_s / 1e9, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": alg / asm_s / 1e9 / HBM_PEAK_GBS,
                               "bytes_definition": "24 V + (16 conn + 72 tensor + 64 int32 map) E + 8 nnz + 8 N"},

@@ -252,14 +252,23 @@ int skfa_assemble_bilinear(skfa_plan* plan, const double* xyz, int64_t npoints, 
  * node row i also sums |det J_e| w_a over the row's incidences
  * (w_a = sum_q W_q phi_a(X_q)), so the load costs no pass of its own;
  * otherwise the linear-form kernels run after the gather.  Vector fields
- * (components 3, or SKFA_FLAG_VECTOR3_DIAGONAL) are rejected (-1). */
+ * (components 3, or SKFA_FLAG_VECTOR3_DIAGONAL) are rejected (-1).
+ * value_map / row_map (both or neither, int32 device arrays of the plan's
+ * nnz and nnodes): entry k of the plan's CSR is written to
+ * values[value_map[k]] and the load of node row i to load[row_map[i]],
+ * entries with a negative map value are dropped, e.g. the Dirichlet
+ * condensation K[I][:, I], b[I] of
+ * examples/scikit-fem-gallery/comparison.py:80-87 straight from the gather
+ * (the maps must be injective on the kept entries).  Without the fused
+ * path the full results go through plan-owned buffers (allocated on first
+ * use) and a scatter. */
 int skfa_assemble_bilinear_load(skfa_plan* plan, const double* xyz, int64_t npoints, const int32_t* vconn,
                                 const int32_t* gconn, int32_t gnodes, const double* tables, int32_t nq,
                                 const double* rtab, const float* rtab32, int32_t nqg, int32_t nqm,
                                 const double* c_value, const double* c_gradient, const double* c_symgrad,
                                 const double* c_divergence, const double* c_tensor, int32_t per_qp,
                                 int32_t precision, int32_t flags, double* values, const double* f, double* load,
-                                void* stream);
+                                const int32_t* value_map, const int32_t* row_map, void* stream);
 
 /* skfa_assemble_bilinear without any device allocation.  No counterpart in
  * the reference: it is for callers that cannot allocate device memory (code

@@ -233,6 +233,21 @@ __device__ inline void write_span(const double* lds, int64_t span, int lane, dou
     }
 }
 
+// Span -> HBM through a map: entry k goes to out[map[k]] unless map[k] < 0
+// (map points at the span's first entry).  Kept entries keep their order,
+// so a wave's stores stay nearly contiguous.
+__device__ inline void write_span_mapped(const double* lds, int64_t span, int lane, const int32_t* __restrict__ map,
+                                         double* out, bool streaming) {
+    for (int64_t k = lane; k < span; k += 64) {
+        const int32_t j = map[k];
+        if (j < 0) continue;
+        if (streaming)
+            __builtin_nontemporal_store(lds[k], out + j);
+        else
+            out[j] = lds[k];
+    }
+}
+
 // Write a scalar span as the 3x3 node blocks K_(a,c),(b,d) = [c == d] s_ab
 // of a 3-component field (value / gradient kinds of vector fields): node row
 // r with len columns becomes DOF rows 3r+c, whose entries (column node j,
@@ -527,6 +542,11 @@ struct GatherArgs {
     int32_t load_q = 0;
     const double* load_f = nullptr;
     double* load_out = nullptr;
+    // optional output maps of the fused path: entry k of the plan's CSR goes
+    // to values[value_map[k]] and node row i's load to load_out[row_map[i]]
+    // (-1: dropped), e.g. the Dirichlet-condensed system
+    const int32_t* value_map = nullptr;
+    const int32_t* row_map = nullptr;
 };
 
 // K_ab += w_q (dphi_aq^T D) . dphi_bq for all b: the lane's row a reads the
@@ -1601,7 +1621,15 @@ __global__ void __launch_bounds__(64 * WPB) k_gather_table(GatherArgs A, int32_t
         }
     }
     if constexpr (LOAD) {
-        if (L.active && L.p == 0) A.load_out[A.plan.meta[wv].r0 + L.i] = lacc * A.load_f[0];
+        if (L.active && L.p == 0) {
+            const int64_t row = A.plan.meta[wv].r0 + L.i;
+            const int64_t dst = A.row_map ? int64_t(A.row_map[row]) : row;
+            if (dst >= 0) A.load_out[dst] = lacc * A.load_f[0];
+        }
+        if (A.value_map) {
+            write_span_mapped(span_lds, L.span, lane, A.value_map + L.lo, A.values, A.plan.streaming);
+            return;
+        }
     }
     // the span is this wave's own: its LDS operations complete in order
     if (A.plan.diag3)
@@ -2339,6 +2367,26 @@ static int sym_layout(const skfa_plan* P, int nqg, int nqm, bool f32, bool mass)
     return mass ? PL_SYM : PL_SYM6;
 }
 
+// The load vector rides in the gather when the element-table gather of
+// 4-node scalar fields runs (the choice launch_gather makes).
+static bool fused_load_path(const skfa_plan* P, const int32_t* gconn, int32_t per_qp, const double* c_tensor,
+                            int32_t flags, int32_t nqg, int32_t nqm, const double* tables, int32_t nq) {
+    const bool fast = gconn == nullptr && per_qp == 0;
+    const bool diag3 = (flags & SKFA_FLAG_VECTOR3_DIAGONAL) != 0;
+    const bool sym = P->C == 1 && (c_tensor == nullptr || (flags & SKFA_FLAG_SYMMETRIC_TENSOR));
+    return fast && !diag3 && sym && P->n == 4 && table_gather(P) && !mfma_gather(P, nqg, nqm) && tables != nullptr &&
+           nq >= 1 && nq <= 64;
+}
+
+// out[map[k]] = src[k] for map[k] >= 0
+__global__ void k_scatter_mapped(const double* __restrict__ src, const int32_t* __restrict__ map, int64_t n,
+                                 double* __restrict__ out) {
+    for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x) {
+        const int32_t j = map[k];
+        if (j >= 0) out[j] = src[k];
+    }
+}
+
 // Scratch of the element-constant path: the per-element payload (E x
 // stride doubles) and the element tables of the table gather (FP64, FP32).
 struct Workspace {
@@ -2461,7 +2509,8 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
                                 const double* c_tensor, int32_t per_qp, int32_t precision, int32_t flags,
                                 double* values, void* stream, unsigned char* scratch, size_t scratch_bytes,
                                 const double* load_f = nullptr, double* load_out = nullptr,
-                                bool* load_fused = nullptr) {
+                                bool* load_fused = nullptr, const int32_t* value_map = nullptr,
+                                const int32_t* row_map = nullptr) {
     if (load_fused) *load_fused = false;
     SKFA_REQUIRE(P != nullptr, "plan is null");
     SKFA_REQUIRE(P->n == 4 || P->n == 10, "bilinear assembly supports 4- and 10-node tetrahedra");
@@ -2576,11 +2625,14 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
         G.values = values;
         // the constant-source load vector rides along with the element-table
         // gather of 4-node scalar fields (|det J| in the payload's last slot)
-        if (load_out != nullptr && table && P->n == 4 && !diag3 && tables != nullptr && nq >= 1 && nq <= 64) {
+        if (load_out != nullptr &&
+            fused_load_path(P, gconn, per_qp, c_tensor, flags, nqg, nqm, tables, nq)) {
             G.load_tables = tables;
             G.load_q = nq;
             G.load_f = load_f;
             G.load_out = load_out;
+            G.value_map = value_map;
+            G.row_map = row_map;
             if (load_fused) *load_fused = true;
         }
         const size_t tab_len = size_t(nqg) + size_t(nqg) * P->n * 3 + nqm + size_t(nqm) * P->n;
@@ -2630,16 +2682,42 @@ int skfa_assemble_bilinear_load(skfa_plan* P, const double* xyz, int64_t npoints
                                 const double* c_value, const double* c_gradient, const double* c_symgrad,
                                 const double* c_divergence, const double* c_tensor, int32_t per_qp,
                                 int32_t precision, int32_t flags, double* values, const double* f, double* load,
-                                void* stream) {
+                                const int32_t* value_map, const int32_t* row_map, void* stream) {
     (void)npoints;
     SKFA_REQUIRE(P != nullptr, "plan is null");
     SKFA_REQUIRE(f != nullptr && load != nullptr, "load source or output is null");
     SKFA_REQUIRE(P->C == 1 && !(flags & SKFA_FLAG_VECTOR3_DIAGONAL), "the fused load vector is for scalar fields");
     SKFA_REQUIRE(tables != nullptr && nq >= 1, "basis tables are required for the load vector");
+    SKFA_REQUIRE((value_map == nullptr) == (row_map == nullptr), "give both output maps or neither");
+    const bool mapped = value_map != nullptr;
+    const bool fusable = fused_load_path(P, gconn, per_qp, c_tensor, flags, nqg, nqm, tables, nq);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (mapped && (!fusable || P->nnz() == 0 || P->E == 0)) {
+        // full-size results in plan-owned buffers, then through the maps
+        if (P->full_values.n < size_t(std::max<int64_t>(P->nnz(), 1)))
+            if (int rc = P->full_values.alloc(size_t(std::max<int64_t>(P->nnz(), 1)))) return rc;
+        if (P->full_load.n < size_t(std::max<int64_t>(P->nnodes, 1)))
+            if (int rc = P->full_load.alloc(size_t(std::max<int64_t>(P->nnodes, 1)))) return rc;
+        if (int rc = assemble_bilinear_ws(P, xyz, vconn, gconn, gnodes, tables, nq, rtab, rtab32, nqg, nqm, c_value,
+                                          c_gradient, c_symgrad, c_divergence, c_tensor, per_qp, precision, flags,
+                                          P->full_values.p, stream, nullptr, 0))
+            return rc;
+        if (int rc = skfa_assemble_linear(P, xyz, vconn, gconn, gnodes, tables, nq, f, 0, nullptr, 0,
+                                          P->full_load.p, stream))
+            return rc;
+        if (P->nnz())
+            k_scatter_mapped<<<unsigned(std::min<int64_t>((P->nnz() + 255) / 256, 65536)), 256, 0, s>>>(
+                P->full_values.p, value_map, P->nnz(), values);
+        if (P->nnodes)
+            k_scatter_mapped<<<unsigned(std::min<int64_t>((P->nnodes + 255) / 256, 65536)), 256, 0, s>>>(
+                P->full_load.p, row_map, P->nnodes, load);
+        SKFA_LAUNCH_CHECK();
+        return kOk;
+    }
     bool fused = false;
     if (int rc = assemble_bilinear_ws(P, xyz, vconn, gconn, gnodes, tables, nq, rtab, rtab32, nqg, nqm, c_value,
                                       c_gradient, c_symgrad, c_divergence, c_tensor, per_qp, precision, flags,
-                                      values, stream, nullptr, 0, f, load, &fused))
+                                      values, stream, nullptr, 0, f, load, &fused, value_map, row_map))
         return rc;
     if (fused) return kOk;
     // every other gather: the load vector by the linear-form kernels

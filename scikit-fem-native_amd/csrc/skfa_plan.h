@@ -71,6 +71,10 @@ struct skfa_plan {
     skfa::DevBuf<double> payload;
     // |det J| per element for the constant-load linear form (first use)
     skfa::DevBuf<double> elem_det;
+    // full-size values and load of skfa_assemble_bilinear_load with output
+    // maps when the load is not fused into the gather (first use)
+    skfa::DevBuf<double> full_values;
+    skfa::DevBuf<double> full_load;
     // element tables of the symmetric scalar gather (FP64 and FP32 copies)
     skfa::DevBuf<double> ntab;
     skfa::DevBuf<float> ntab32;

@@ -227,9 +227,10 @@ def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=No
     """Launch skfa_assemble_bilinear with device coefficient tensors
     (``noalloc``: skfa_assemble_bilinear_noalloc, which keeps its scratch in
     a fixed device array of the library instead of allocating; ``load`` =
-    (f, out): skfa_assemble_bilinear_load, which also writes the load vector
-    of the constant source f (one device double) into out, scalar fields
-    only)."""
+    (f, out) or (f, out, value_map, row_map): skfa_assemble_bilinear_load,
+    which also writes the load vector of the constant source f (one device
+    double) into out, scalar fields only, and with the int32 maps writes
+    values and load through them, dropping entries mapped to -1)."""
     d = basis.device_data()
     L = _native.lib()
     iso = not basis._affine
@@ -256,7 +257,9 @@ def assemble_bilinear_device(basis, plan, values_dev, *, value=None, gradient=No
         if noalloc:
             raise ValueError("the fused load vector has no no-allocation entry point")
         entry = L.skfa_assemble_bilinear_load
-    extra = () if load is None else (_native.ptr(load[0]), _native.ptr(load[1]))
+    if load is not None and len(load) == 2:
+        load = (load[0], load[1], None, None)
+    extra = () if load is None else tuple(_native.ptr(a) for a in load)
 
     def launch():
         _native.check(entry(

@@ -92,7 +92,7 @@ def _declare(lib):
         "skfa_assemble_bilinear_noalloc": (ctypes.c_int, [P, P, I64, P, P, I32, P, I32, P, P, I32, I32,
                                                           P, P, P, P, P, I32, I32, I32, P, P]),
         "skfa_assemble_bilinear_load": (ctypes.c_int, [P, P, I64, P, P, I32, P, I32, P, P, I32, I32,
-                                                       P, P, P, P, P, I32, I32, I32, P, P, P, P]),
+                                                       P, P, P, P, P, I32, I32, I32, P, P, P, P, P, P]),
         "skfa_assemble_linear": (ctypes.c_int, [P, P, P, P, I32, P, I32, P, I32, P, I32, P, P]),
         "skfa_tabulate_geometry": (ctypes.c_int, [P, P, P, I32, I64, P, I32, I32, P, P, P, P, P, P, P, P]),
         "skfa_edge_topology": (ctypes.c_int, [P, I64, I64, P, P, P, P, ctypes.POINTER(I64), ctypes.POINTER(I64), P]),

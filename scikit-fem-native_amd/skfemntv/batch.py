@@ -4,9 +4,9 @@ uneven Kuhn-split boxes, assembled together and condensed on the GPU.
 The systems are concatenated into one mesh whose components do not touch, so
 one gather plan and one assembly launch pair (element payload pass + CSR
 row-gather) produce every matrix at once as a block-diagonal CSR; the row
-gather also sums the load vector (skfa_assemble_bilinear_load), and the
-Dirichlet condensation of the boundary vertices is a value gather through a
-precomputed position map.  After one
+gather also sums the load vector and writes both through precomputed
+full -> condensed position maps, so the Dirichlet condensation of the
+boundary vertices costs no pass of its own (skfa_assemble_bilinear_load).  After one
 eager call the whole sequence can be replayed as a HIP graph.
 
 Per system s (seeded by ``np.random.default_rng([seed, s])``):
@@ -132,7 +132,6 @@ class AnisotropicBatch:
         dev = _native.device()
         self.tensors = upload(tensors)
         self._one = torch.ones(1, dtype=torch.float64, device=dev)
-        self._rhs = torch.empty(self.basis.N, dtype=torch.float64, device=dev)
         t0 = time.perf_counter()
         # condensation structure: condense positions once to get the value map
         plan = self.form.plan
@@ -140,8 +139,13 @@ class AnisotropicBatch:
                          torch.arange(plan.nnz, dtype=torch.float64, device=dev), (plan.nrows, plan.nrows))
         keep_dev = torch.from_numpy(keep.astype(np.uint8)).to(dev)
         cond, _ = full.condense(keep_dev)
-        self.value_map = cond.data.to(torch.int64)
-        self.free = torch.nonzero(keep_dev, as_tuple=True)[0]
+        # output maps of the gather: full CSR entry -> condensed entry and
+        # node row -> condensed row (-1: a boundary row or column, dropped)
+        self.value_map = torch.full((plan.nnz,), -1, dtype=torch.int32, device=dev)
+        self.value_map[cond.data.to(torch.int64)] = torch.arange(cond.nnz, dtype=torch.int32, device=dev)
+        free = torch.nonzero(keep_dev, as_tuple=True)[0]
+        self.row_map = torch.full((plan.nrows,), -1, dtype=torch.int32, device=dev)
+        self.row_map[free] = torch.arange(free.numel(), dtype=torch.int32, device=dev)
         self.A = DeviceCSR(cond.indptr, cond.indices, torch.empty_like(cond.data), cond.shape)
         self.b = torch.empty(cond.shape[0], dtype=torch.float64, device=dev)
         kept_per_system = np.add.reduceat(keep.astype(np.int64), self.node_offsets[:-1])
@@ -161,13 +165,10 @@ class AnisotropicBatch:
         return int(self.element_offsets[-1])
 
     def _assemble(self):
-        L = _native.lib()
-        # matrix and load vector in one gather (skfa_assemble_bilinear_load)
-        vals = self.form.assemble_tensor_device(self.tensors, symmetric=True, load=(self._one, self._rhs))
-        _native.check(L.skfa_gather_values(_native.ptr(vals), _native.ptr(self.value_map), self.A.nnz,
-                                           _native.ptr(self.A.data), _native.stream()))
-        _native.check(L.skfa_gather_values(_native.ptr(self._rhs), _native.ptr(self.free), self.A.shape[0],
-                                           _native.ptr(self.b), _native.stream()))
+        # condensed matrix and load vector straight from one gather
+        # (skfa_assemble_bilinear_load with output maps)
+        self.form.assemble_tensor_device(self.tensors, symmetric=True, out=self.A.data,
+                                         load=(self._one, self.b, self.value_map, self.row_map))
 
     def assemble(self, graph=False):
         """Assemble every system (condensed A values and b, in place).  With

@@ -208,3 +208,54 @@ def test_bilinear_with_fused_load_matches_separate_calls(quadratic, min_waves):
     bo = 2.5 * pipeline.load_vector(p2, t2, X, W, quadratic, [1.])
     assert rel(b, bo) <= FP64_TOL, rel(b, bo)
     say(f"load rel err vs oracle {rel(b, bo):.1e}")
+
+
+@pytest.mark.parametrize("quadratic,min_waves", [(False, 0), (False, 2048), (True, 0)])
+def test_fused_load_with_output_maps_condenses(quadratic, min_waves):
+    """skfa_assemble_bilinear_load with output maps: the values and load land
+    at the condensed positions K[I][:, I], b[I] (the Dirichlet condensation
+    of comparison.py:80-87) with dropped entries untouched, equal bit for bit
+    to condensing the unmapped results, on the fused P1 table path (min
+    waves 0) and through the plan's full-size buffers otherwise."""
+    import torch
+    import skfemntv as sk
+    from skfemntv._assembly import NativeCrossBilinearForm, Plan, upload
+    say(f"fused load + output maps, quadratic={quadratic}, min_waves={min_waves}")
+    n = 6
+    ax = np.linspace(0., 1., n + 1)
+    p, t = host.kuhn_mesh(ax, ax, ax)
+    p = host.jitter_interior(p, 1. / n, 0.2, 7)
+    X, W = host.tet_rule_4()
+    if quadratic:
+        p2, t2 = host.p2_mesh(p, t)
+        mesh, el, nl = sk.MeshTet2(p2, t2), sk.ElementTetP2(), 10
+    else:
+        mesh, el, nl = sk.MeshTet(p, t), sk.ElementTetP1(), 4
+    basis = sk.Basis(mesh, sk.ElementVector(el, dim=1), quadrature=(X, W))
+    E, N = t.shape[1], basis.N
+    rng = np.random.default_rng(8)
+    g = rng.standard_normal((E, 3, 3))
+    A = upload(np.einsum("eij,ekj->eik", g, g) + np.eye(3))
+    form = NativeCrossBilinearForm(basis, basis)
+    d = basis.device_data()
+    plan = Plan(d["node_conn"], E, nl, d["nnodes"], 1, min_waves=min_waves)
+    plan._tuned = True
+    form.plan = plan
+    dev = A.device
+    f = torch.full((1,), 1.5, dtype=torch.float64, device=dev)
+    full_load = torch.empty(N, dtype=torch.float64, device=dev)
+    full = form.assemble_tensor_device(A, symmetric=True, load=(f, full_load)).clone()
+    ip, ix = plan.host_csr_arrays()
+    keep = rng.random(N) < 0.7
+    rows = np.repeat(np.arange(N), np.diff(ip))
+    kept = keep[rows] & keep[ix]
+    vmap = np.full(len(ix), -1, dtype=np.int32)
+    vmap[kept] = np.arange(int(kept.sum()), dtype=np.int32)
+    rmap = np.full(N, -1, dtype=np.int32)
+    rmap[keep] = np.arange(int(keep.sum()), dtype=np.int32)
+    vals = torch.full((int(kept.sum()),), np.nan, dtype=torch.float64, device=dev)
+    load = torch.full((int(keep.sum()),), np.nan, dtype=torch.float64, device=dev)
+    form.assemble_tensor_device(A, symmetric=True, out=vals,
+                                load=(f, load, upload(vmap), upload(rmap)))
+    np.testing.assert_array_equal(vals.cpu().numpy(), full.cpu().numpy()[:len(ix)][kept])
+    np.testing.assert_array_equal(load.cpu().numpy(), full_load.cpu().numpy()[keep])
