@@ -403,15 +403,44 @@ template <int C, bool SYM, int SYMOUT = SYMOUT_F64>
 __global__ void __launch_bounds__(kPayloadBlock) k_payload_h1(PayloadArgs A) {
     using V = typename PayloadOut<C, SYM, SYMOUT>::V;
     constexpr int PO = PayloadOut<C, SYM, SYMOUT>::PO;
-    __shared__ V buf[kPayloadBlock * PO];
+    // one LDS area: first the block's tensor rows (C == 1), read with
+    // coalesced 16-byte loads instead of nine 8-byte loads per lane at a
+    // 72-byte lane stride, then the block's payload rows
+    constexpr int kTab = C == 1 ? kPayloadBlock * 9 * 8 : 0;
+    constexpr int kBuf = kPayloadBlock * PO * int(sizeof(V));
+    __shared__ __align__(16) unsigned char smem[kTab > kBuf ? kTab : kBuf];
+    V* buf = reinterpret_cast<V*>(smem);
+    const bool stage = C == 1 && A.cA != nullptr && (reinterpret_cast<uintptr_t>(A.cA) & 15) == 0;
     for (int64_t e0 = int64_t(blockIdx.x) * kPayloadBlock; e0 < A.E; e0 += int64_t(gridDim.x) * kPayloadBlock) {
         const int n = int(A.E - e0 < kPayloadBlock ? A.E - e0 : kPayloadBlock);
+        double Xs[9];
+        if constexpr (C == 1) {
+            if (stage) {
+                // e0 is a multiple of 256, so the slab starts 16-byte aligned
+                double* tb = reinterpret_cast<double*>(smem);
+                const double2* src = reinterpret_cast<const double2*>(A.cA + e0 * 9);
+                const int nd = n * 9;
+                for (int k = threadIdx.x; k < nd / 2; k += kPayloadBlock)
+                    reinterpret_cast<double2*>(tb)[k] = src[k];
+                if ((nd & 1) && threadIdx.x == 0) tb[nd - 1] = A.cA[e0 * 9 + nd - 1];
+                __syncthreads();
+                if (int(threadIdx.x) < n) {
+#pragma unroll
+                    for (int t = 0; t < 9; ++t) Xs[t] = tb[threadIdx.x * 9 + t];
+                }
+                __syncthreads();   // the payload rows reuse the area
+            }
+        }
         if (int(threadIdx.x) < n) {
             const int64_t e = e0 + threadIdx.x;
             V* o = buf + threadIdx.x * PO;
             double J[9];
             Geo G;
-            affine_jacobian(A.xyz, A.vconn + e * 4, J);
+            {
+                const int4 c = reinterpret_cast<const int4*>(A.vconn)[e];
+                const int32_t c4[4] = {c.x, c.y, c.z, c.w};
+                affine_jacobian(A.xyz, c4, J);
+            }
             adjugate(J, G);
             const double ad = fabs(G.det);
             const double m = A.cm ? A.cm[e] : 0.;
@@ -422,7 +451,7 @@ __global__ void __launch_bounds__(kPayloadBlock) k_payload_h1(PayloadArgs A) {
             if constexpr (C == 1) {
                 double X[9];
 #pragma unroll
-                for (int t = 0; t < 9; ++t) X[t] = A.cA ? A.cA[e * 9 + t] : 0.;
+                for (int t = 0; t < 9; ++t) X[t] = stage ? Xs[t] : (A.cA ? A.cA[e * 9 + t] : 0.);
                 X[0] += g + 0.5 * s + 0.5 * s + lam;
                 X[4] += g + 0.5 * s;
                 X[8] += g + 0.5 * s;
