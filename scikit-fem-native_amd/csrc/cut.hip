@@ -128,6 +128,81 @@ __global__ void __launch_bounds__(64) k_cut_rows(CutArgs A) {
     for (int64_t k = lane; k < span_len; k += 64) A.values[lo * cc + k] = span[k];
 }
 
+// Point-parallel form of k_cut_rows for the value / gradient terms (cells
+// of at most 16 column nodes, no repeated node): one wave owns one row node;
+// lane (s, b), s = lane / 16, takes column node b and the cell's points
+// s, s + 4, ... of each incidence, so the four slots load a cell's points
+// together and each lane keeps one accumulator; two shuffle levels (masks
+// 16, 32, a fixed order) add the four slots' sums and slot 0 adds entry b
+// to the span.  Incidences run in cell order; the next incidence's cell,
+// point range and column offset are loaded while one computes.  Cut band of
+// the 64^3 P2 cube (55,571 cells, 12 points each): 0.605 ms, against 0.859
+// ms for k_cut_rows (lane per column node, the points of each incidence in
+// sequence) and 1.42 ms for a 16-lane point-parallel form with a 16-column
+// reduce-scatter (196 VGPRs).
+constexpr int kCutSlots = 4;
+
+__global__ void __launch_bounds__(64) k_cut_rows_points(CutArgs A) {
+    extern __shared__ double span[];
+    const int lane = threadIdx.x;
+    const int slot = lane >> 4, b = lane & 15;
+    const int64_t r = blockIdx.x;
+    const int64_t cc = int64_t(A.Cr) * A.Cc;
+    const int64_t lo = A.nptr[r];
+    const int64_t len = A.nptr[r + 1] - lo;
+    const int64_t span_len = len * cc;
+    for (int64_t k = lane; k < span_len; k += 64) span[k] = 0.;
+    __syncthreads();
+    const bool has_value = (A.terms & SKFA_CUT_VALUE) != 0, has_grad = (A.terms & SKFA_CUT_GRADIENT) != 0;
+    const bool col = b < A.nc;
+    const int64_t kb = A.inc_ptr[r], ke = A.inc_ptr[r + 1];
+    // incidence k + 1's cell, point range and column offset load while k computes
+    auto fetch = [&](int64_t k, int32_t& packed, int64_t& q0, int64_t& q1, int& o) {
+        if (k < ke) {
+            const uint8_t* rec = A.records + k * A.rec_bytes;
+            packed = *reinterpret_cast<const int32_t*>(rec);
+            q0 = A.offsets[packed >> 4];
+            q1 = A.offsets[(packed >> 4) + 1];
+            o = col ? rec_offset(rec, A.off_bytes, b) : 0;
+        }
+    };
+    int32_t pk_n = 0;
+    int64_t q0_n = 0, q1_n = 0;
+    int o_n = 0;
+    fetch(kb, pk_n, q0_n, q1_n, o_n);
+    for (int64_t k = kb; k < ke; ++k) {
+        const int32_t packed = pk_n;
+        const int64_t q0 = q0_n, q1 = q1_n;
+        const int o = o_n;
+        fetch(k + 1, pk_n, q0_n, q1_n, o_n);
+        const int a = packed & 15;
+        double acc = 0.;
+        if (col) {
+            for (int64_t q = q0 + slot; q < q1; q += kCutSlots) {
+                const double w = A.weights[q];
+                double t = 0.;
+                if (has_value) t = w * (A.value ? A.value[q] : 1.) * A.rshape[q * A.nr + a] * A.cshape[q * A.nc + b];
+                if (has_grad) {
+                    const double* ga = A.rgrad + (q * A.nr + a) * A.Dr;
+                    const double* gb = A.cgrad + (q * A.nc + b) * A.Dc;
+                    double dot = 0.;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        if (d < A.Dr) dot = fma(ga[d], gb[d], dot);
+                    t = fma(w * (A.gradient ? A.gradient[q] : 1.), dot, t);
+                }
+                acc += t;
+            }
+        }
+        acc += __shfl_xor(acc, 16);
+        acc += __shfl_xor(acc, 32);
+        if (slot == 0 && col)
+            for (int c = 0; c < A.Cr; ++c) span[(int64_t(c) * len + o) * A.Cc + c] += acc;
+    }
+    __syncthreads();
+    for (int64_t k = lane; k < span_len; k += 64) A.values[lo * cc + k] = span[k];
+}
+
 struct CutLinearArgs {
     const int64_t* inc_ptr;
     const uint8_t* records;
@@ -242,6 +317,14 @@ int skfa_assemble_cut_bilinear(const skfa_cross_plan* P, const int64_t* offsets,
     A.col_grad = col_kind == SKFA_CROSS_GRADIENT;
     A.tensor = tensor;
     A.values = values;
+    // value / gradient terms on cells of at most 16 column nodes without
+    // repeated nodes: the point-parallel kernel (one row node per wave)
+    if (terms != SKFA_CUT_TENSOR && !one_lane_per_row && P->n_col <= 16) {
+        SKFA_REQUIRE(P->row_nodes <= int64_t(INT32_MAX), "too many row nodes");
+        k_cut_rows_points<<<dim3(unsigned(P->row_nodes)), dim3(64), size_t(row_bytes), s>>>(A);
+        SKFA_LAUNCH_CHECK();
+        return kOk;
+    }
     const int64_t waves = (P->row_nodes + rows - 1) / rows;
     SKFA_REQUIRE(waves <= int64_t(INT32_MAX), "too many row nodes");
     k_cut_rows<<<dim3(unsigned(waves)), dim3(64), size_t(rows * row_bytes), s>>>(A);
