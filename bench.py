@@ -18,7 +18,8 @@ elasticity 48^3) and C4 (N1 96^3) assemblies measured the same way,
 `batched_fsai` the 256-system batch and its PCG solves, and `cpu_baseline`
 the C restatement of the reference's threaded CPU assembler (oracle/, built
 from this repository) on the full headline workload, with every host core
-and with one thread.
+and with one thread; `side_workloads` the Taylor-Hood divergence block, the
+Taylor-Hood Stokes operator and the cut-cell band.
 """
 
 from __future__ import annotations
@@ -649,7 +650,8 @@ def main():
     ap.add_argument("--ramp", type=float, default=0.3, help="seconds of untimed assemblies before the warm-up")
     ap.add_argument("--n", type=int, default=64, help="cells per cube edge")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-configs", action="store_true", help="skip the C1/C3/C4 lines of the default run")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the C1/C3/C4 and side-workload keys of the default run")
     ap.add_argument("--systems", type=int, default=256, help="batched workload: number of systems (0 = skip)")
     ap.add_argument("--cells", type=int, nargs=2, default=(16, 48), help="batched workload: cells per axis range")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -693,6 +695,20 @@ def main():
             del w
             torch.cuda.empty_cache()
         result["configs"] = configs
+        # side workloads of the next-step rows (cross blocks, composite
+        # Stokes operator, cut cells), FP64, same timing
+        side = {}
+        for name in ("th", "stokes", "cut"):
+            log(f"side workload {name} ...")
+            w = build_workload(name)
+            _, summary = measure(w, args.steps, args.warmup, 0.2, timer)
+            summary["workload"] = w["desc"]
+            summary["elements"], summary["nnz"] = w["E"], w["nnz"]
+            summary["traffic"] = measured_traffic(name)
+            side[name] = summary
+            del w
+            torch.cuda.empty_cache()
+        result["side_workloads"] = side
     del keep
     torch.cuda.empty_cache()
     if args.systems > 0:
