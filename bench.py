@@ -41,6 +41,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "assembled elements/s + % HBM roofline; learned-FSAI vs Jacobi PCG time to 1e-8"
 HBM_PEAK_GBS = 8000.0
+BF16_MFMA_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix peak
 BYTES_PER_ELEMENT_C2 = 825.7  # algorithmic bytes per tet of one FP64 assembly
 # the same without the int32 element->CSR map (10 x 10 entries per tet): the "floor"
 FLOOR_BYTES_PER_ELEMENT_C2 = BYTES_PER_ELEMENT_C2 - 4. * 10 * 10
@@ -184,6 +185,26 @@ def batched_fsai(systems, lo, hi, steps, warmup, timer):
         learned_s = time.perf_counter() - t0
         out = {"jacobi": rj, "jac_s": jac_s, "learned": rl, "learned_s": learned_s, "setup_s": setup_s}
     rj, rl = out["jacobi"], out["learned"]
+    # the MLP kernels on the batch's lower nonzeros: inference (part of the
+    # setup above) and one training backward pass (activation kernel, split-K
+    # weight gradients, reduction).  MFMA flops per nonzero: forward
+    # 2 (16*128 + 128*128); backward the forward again, W2^T dz2, and the
+    # two weight-gradient products.
+    _, go = fs.kaporin(net.forward(fs.features, fs.nnz))
+    _, ws = net.backward(fs.features, go, fs.nnz, None)
+    fwd_s = timer.run(lambda: net.forward(fs.features, fs.nnz), 10)
+    bwd_s = timer.run(lambda: net.backward(fs.features, go, fs.nnz, ws), 5)
+    f_fwd = 2. * (16 * 128 + 128 * 128)
+    f_bwd = f_fwd + 2. * 128 * 128 + 2. * (128 * 128 + 16 * 128)
+    mlp = {"rows": fs.nnz, "forward_ms": fwd_s * 1e3, "backward_ms": bwd_s * 1e3,
+           "forward_tflops": f_fwd * fs.nnz / fwd_s / 1e12, "backward_tflops": f_bwd * fs.nnz / bwd_s / 1e12,
+           "peak_tflops": BF16_MFMA_PEAK_TFLOPS, "dtype": "bf16 MFMA, fp32 accumulate",
+           "flops_definition": "per nonzero: forward 2(16*128 + 128*128); backward the forward, "
+                                   "2*128*128 for W2^T dz2 and 2(128*128 + 16*128) for the weight gradients"}
+    mlp["forward_frac"] = mlp["forward_tflops"] / BF16_MFMA_PEAK_TFLOPS
+    mlp["backward_frac"] = mlp["backward_tflops"] / BF16_MFMA_PEAK_TFLOPS
+    log(f"mlp: forward {mlp['forward_ms']:.3f} ms ({mlp['forward_tflops']:.0f} TFLOP/s), "
+        f"backward {mlp['backward_ms']:.3f} ms ({mlp['backward_tflops']:.0f} TFLOP/s)")
     res_j = relative_residuals(A, b, rj.x, batch.sys_ptr)
     res_l = relative_residuals(A, b, rl.x, batch.sys_ptr)
     return {
@@ -207,6 +228,7 @@ def batched_fsai(systems, lo, hi, steps, warmup, timer):
                              "iterations_max": int(rl.iterations.max()),
                              "true_residual_max": float(res_l.max()), "lower_nonzeros": fs.nnz},
         "speedup_learned_vs_jacobi": out["jac_s"] / out["learned_s"],
+        "mlp": mlp,
     }
 
 

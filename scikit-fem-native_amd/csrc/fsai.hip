@@ -358,49 +358,47 @@ __device__ __forceinline__ v4s x_bf16(float4 v) {
     u.y = uint32_t(f2bf(v.z)) | (uint32_t(f2bf(v.w)) << 16);
     return as_v4s(u);
 }
-__device__ __forceinline__ v4s load_x(const float* __restrict__ feat, int64_t row, int64_t nnz, int quad) {
-    return x_bf16(load_x_raw(feat, row, nnz, quad));
+
+// max(v, 0) as a signed-integer max of the bits: one instruction, where
+// fmaxf first quiets a possible signalling NaN of an MFMA result.  Equal to
+// fmaxf(v, 0) for every non-NaN v (-0 gives +0).
+__device__ __forceinline__ float relu(float v) {
+    return __builtin_bit_cast(float, max(__builtin_bit_cast(int32_t, v), 0));
 }
 
-// Forward of one tile.  Outputs per lane (units ob*16 + quad*4 + r, column
-// lane%16): z1[8] and z2[8] pre-activations (FP32), h1b[4] BF16 packs of
-// relu(z1) in the 32-deep B-operand order.
-struct TileFwd {
-    v4f z1[8];
-    v4f z2[8];
-    uint4 h1b[4];
-};
-
-__device__ __forceinline__ uint4 pack8(const v4f& a, const v4f& b, bool relu) {
+__device__ __forceinline__ uint4 pack8(const v4f& a, const v4f& b, bool act) {
     const float f[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
     v8bf h;
 #pragma unroll
-    for (int s = 0; s < 8; ++s) h[s] = static_cast<__bf16>(relu ? fmaxf(f[s], 0.f) : f[s]);
+    for (int s = 0; s < 8; ++s) h[s] = static_cast<__bf16>(act ? relu(f[s]) : f[s]);
     return __builtin_bit_cast(uint4, h);
 }
 
-__device__ __forceinline__ void tile_forward(const uint2* __restrict__ w1f, const uint4* __restrict__ w2f,
-                                             const float* __restrict__ params, v4s x, int lane, TileFwd& t) {
-    const int quad = lane >> 4;
-#pragma unroll
-    for (int ob = 0; ob < 8; ++ob) {
-        v4f c = {0.f, 0.f, 0.f, 0.f};
-        c = mfma16(as_v4s(w1f[ob * 64 + lane]), x, c);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) c[r] += params[kOffB1 + ob * 16 + quad * 4 + r];
-        t.z1[ob] = c;
-    }
-#pragma unroll
-    for (int kb2 = 0; kb2 < 4; ++kb2) t.h1b[kb2] = pack8(t.z1[2 * kb2], t.z1[2 * kb2 + 1], true);
-#pragma unroll
-    for (int ob = 0; ob < 8; ++ob) {
-        v4f c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kb2 = 0; kb2 < 4; ++kb2) c = mfma32(as_v8bf(w2f[(ob * 4 + kb2) * 64 + lane]), as_v8bf(t.h1b[kb2]), c);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) c[r] += params[kOffB2 + ob * 16 + quad * 4 + r];
-        t.z2[ob] = c;
-    }
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+// Two BF16 values in one 32-bit word, `lo` in the low half (one packed
+// round-to-nearest-even conversion).
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
+    const v2bf h = {static_cast<__bf16>(lo), static_cast<__bf16>(hi)};
+    return __builtin_bit_cast(uint32_t, h);
+}
+
+// Storage order of hidden units (and of input features) in the backward
+// activation buffers: inside each block of 16 the two 2-bit fields are
+// swapped (u = 4q + r -> 4r + q), so that the four lane groups q of one
+// store instruction write four adjacent units (256 contiguous bytes).  The
+// map is its own inverse.
+__host__ __device__ __forceinline__ int unit_store(int u) {
+    return (u & ~15) | ((u & 3) << 2) | ((u >> 2) & 3);
+}
+
+// Whether BF16 slot s (0..7) of a packed uint4 is nonzero.
+__device__ __forceinline__ bool slot_nonzero(const uint4& u, int s) {
+    const uint32_t w = s < 2 ? u.x : s < 4 ? u.y : s < 6 ? u.z : u.w;
+    return (s & 1) ? w > 0xffffu : (w & 0xffffu) != 0u;
+}
+
+__device__ __forceinline__ float comp(const float4& v, int r) {
+    return r == 0 ? v.x : r == 1 ? v.y : r == 2 ? v.z : v.w;
 }
 
 // Sum over the four lanes that share a column (lanes c, c+16, c+32, c+48).
@@ -429,56 +427,68 @@ __device__ __forceinline__ void stage_frags(const uint16_t* __restrict__ frag, i
     __syncthreads();
 }
 
-// Inference form of tile_forward: the output unit's dot product is taken
-// right after each 16-unit block of layer 2 (same summation order as
-// summing the full z2 afterwards), so only one accumulator tile is live.
-// b1, b2 and w3 come from LDS (float4 per lane: units ob*16 + quad*4 + 0..3).
-__device__ __forceinline__ float tile_forward_out(const uint2* __restrict__ w1f, const uint4* __restrict__ w2f,
+// Inference of T 16-row tiles at once: each BF16 weight fragment read from
+// LDS feeds T MFMAs (the fragments are the LDS traffic of this kernel).  The
+// output unit's dot product is taken right after each 16-unit block of
+// layer 2 (same summation order as summing the full z2 afterwards), so only
+// one accumulator tile per input tile is live.  b1, b2 and w3 come from LDS
+// (float4 per lane: units ob*16 + quad*4 + 0..3); the biases are the MFMAs'
+// initial accumulators, which keeps the VALU work per tile below the MFMA
+// time.  Per tile the arithmetic and its order do not depend on T.
+template <int T>
+__device__ __forceinline__ void tiles_forward_out(const uint2* __restrict__ w1f, const uint4* __restrict__ w2f,
                                                   const float4* __restrict__ b1s, const float4* __restrict__ b2s,
-                                                  const float4* __restrict__ w3s, v4s x, int lane) {
+                                                  const float4* __restrict__ w3s, const v4s (&x)[T], int lane,
+                                                  float (&o)[T]) {
     const int quad = lane >> 4;
-    uint4 h1b[4];
+    uint4 h1b[T][4];
 #pragma unroll
     for (int kb2 = 0; kb2 < 4; ++kb2) {
-        v4f z[2];
+        v4f z[T][2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int ob = 2 * kb2 + h;
-            v4f c = {0.f, 0.f, 0.f, 0.f};
-            c = mfma16(as_v4s(w1f[ob * 64 + lane]), x, c);
+            const v4s w = as_v4s(w1f[ob * 64 + lane]);
             const float4 b = b1s[ob * 4 + quad];
-            c[0] += b.x;
-            c[1] += b.y;
-            c[2] += b.z;
-            c[3] += b.w;
-            z[h] = c;
+            const v4f bias = {b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int t = 0; t < T; ++t) z[t][h] = mfma16(w, x[t], bias);
         }
-        h1b[kb2] = pack8(z[0], z[1], true);
+#pragma unroll
+        for (int t = 0; t < T; ++t) h1b[t][kb2] = pack8(z[t][0], z[t][1], true);
     }
-    float o = 0.f;
+#pragma unroll
+    for (int t = 0; t < T; ++t) o[t] = 0.f;
 #pragma unroll
     for (int ob = 0; ob < 8; ++ob) {
-        v4f c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kb2 = 0; kb2 < 4; ++kb2) c = mfma32(as_v8bf(w2f[(ob * 4 + kb2) * 64 + lane]), as_v8bf(h1b[kb2]), c);
         const float4 b = b2s[ob * 4 + quad];
+        v4f c[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) c[t] = v4f{b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int kb2 = 0; kb2 < 4; ++kb2) {
+            const v8bf w = as_v8bf(w2f[(ob * 4 + kb2) * 64 + lane]);
+#pragma unroll
+            for (int t = 0; t < T; ++t) c[t] = mfma32(w, as_v8bf(h1b[t][kb2]), c[t]);
+        }
         const float4 w = w3s[ob * 4 + quad];
-        c[0] += b.x;
-        c[1] += b.y;
-        c[2] += b.z;
-        c[3] += b.w;
-        o = fmaf(w.x, fmaxf(c[0], 0.f), o);
-        o = fmaf(w.y, fmaxf(c[1], 0.f), o);
-        o = fmaf(w.z, fmaxf(c[2], 0.f), o);
-        o = fmaf(w.w, fmaxf(c[3], 0.f), o);
-        // keep one block's fragments and accumulator live at a time
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            o[t] = fmaf(w.x, relu(c[t][0]), o[t]);
+            o[t] = fmaf(w.y, relu(c[t][1]), o[t]);
+            o[t] = fmaf(w.z, relu(c[t][2]), o[t]);
+            o[t] = fmaf(w.w, relu(c[t][3]), o[t]);
+        }
+        // keep one block's fragments and accumulators live at a time
         __builtin_amdgcn_sched_barrier(0);
     }
-    return o;
 }
+
+constexpr int kFwdTiles = 2;   // 16-row tiles per pass of the inference kernel
 
 __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp_forward(MlpArgs a, const float* __restrict__ feat,
                                                                 int64_t nnz, float* __restrict__ out) {
+    constexpr int T = kFwdTiles;
     __shared__ uint2 w1s[kFragW1 / 4];
     __shared__ uint4 w2s[kFragW2 / 8];
     __shared__ float4 b1s[kH / 4], b2s[kH / 4], w3s[kH / 4];
@@ -490,45 +500,67 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp_forward(MlpArgs a, const
     stage_frags(a.frag, kFragW1, w1s, w2s);
     const int lane = threadIdx.x & 63;
     const int quad = lane >> 4;
-    const int64_t tiles = (nnz + 15) / 16;
+    const int64_t groups = (nnz + 16 * T - 1) / (16 * T);
     const int64_t wave = int64_t(blockIdx.x) * kMlpWaves + (threadIdx.x >> 6);
     const int64_t nwaves = int64_t(gridDim.x) * kMlpWaves;
     const float b3 = a.params[kOffB3];
-    // the next tile's features are in flight while this tile is computed
-    float4 xn = load_x_raw(feat, wave * 16 + (lane & 15), nnz, quad);
-    for (int64_t tile = wave; tile < tiles; tile += nwaves) {
-        const int64_t row = tile * 16 + (lane & 15);
-        const float4 x = xn;
-        xn = load_x_raw(feat, (tile + nwaves) * 16 + (lane & 15), nnz, quad);
-        float o = tile_forward_out(w1s, w2s, b1s, b2s, w3s, x_bf16(x), lane);
-        o = col_sum(o) + b3;
-        if (quad == 0 && row < nnz) out[row] = o;
+    // the next group's features are in flight while this group is computed
+    float4 xn[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) xn[t] = load_x_raw(feat, (wave * T + t) * 16 + (lane & 15), nnz, quad);
+    for (int64_t grp = wave; grp < groups; grp += nwaves) {
+        v4s x[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            x[t] = x_bf16(xn[t]);
+            xn[t] = load_x_raw(feat, ((grp + nwaves) * T + t) * 16 + (lane & 15), nnz, quad);
+        }
+        float o[T];
+        tiles_forward_out<T>(w1s, w2s, b1s, b2s, w3s, x, lane, o);
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int64_t row = (grp * T + t) * 16 + (lane & 15);
+            const float v = col_sum(o[t]) + b3;
+            if (quad == 0 && row < nnz) out[row] = v;
+        }
     }
 }
 
-// Backward activations.  For each 32-row chunk c (two tiles) writes, as BF16
-// in unit-major order, X^T (16x32), relu(z1)^T, dz2^T and dz1^T (128x32
-// each); bias and w3 gradients are accumulated per wave.
-__global__ void __launch_bounds__(64 * kMlpWaves) k_mlp_backward(MlpArgs a, const float* __restrict__ feat,
+// Backward activations, one 32-row chunk c per pass: row c*32 + 16h + j
+// (tile h, column j) takes chunk slot 2j + h, so a lane holds two adjacent
+// slots of each of its units as one 32-bit word.  Writes, as BF16 in
+// unit-major order ([c][stored unit][32 slots], units in unit_store order),
+// X^T (16x32), dz2^T and dz1^T (128x32 each); the wgrad kernel sums over
+// slots, so their order is free.  Every BF16 weight fragment read from LDS feeds both tiles.
+// Bias and w3 gradients are accumulated per lane in FP32 and summed per
+// block in a fixed order.
+// Two waves per SIMD: the register budget is capped at 256 (no scratch
+// results), which with 77 KB of LDS per block gives two blocks per CU.
+__global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_per_eu(2, 2))) k_mlp_backward(MlpArgs a, const float* __restrict__ feat,
                                                                  const float* __restrict__ go, int64_t nnz,
-                                                                 uint16_t* __restrict__ xc, uint16_t* __restrict__ h1c,
-                                                                 uint16_t* __restrict__ dz2c, uint16_t* __restrict__ dz1c,
-                                                                 float* __restrict__ bias_part) {
+                                                                 uint32_t* __restrict__ xc, uint32_t* __restrict__ dz2c,
+                                                                 uint32_t* __restrict__ dz1c, float* __restrict__ bias_part) {
     __shared__ uint2 w1s[kFragW1 / 4];
     __shared__ uint4 w2s[kFragW2 / 8];
+    __shared__ uint4 w2t[kFragW2 / 8];
+    __shared__ float4 b1s[kH / 4], b2s[kH / 4], w3s[kH / 4];
+    __shared__ float red[kMlpWaves * kBiasGrads];
+    for (int k = threadIdx.x; k < kH; k += blockDim.x) {
+        reinterpret_cast<float*>(b1s)[k] = a.params[kOffB1 + k];
+        reinterpret_cast<float*>(b2s)[k] = a.params[kOffB2 + k];
+        reinterpret_cast<float*>(w3s)[k] = a.params[kOffW3 + k];
+    }
+    {
+        const uint4* g2t = reinterpret_cast<const uint4*>(a.frag + kFragW1 + kFragW2);
+        for (int k = threadIdx.x; k < kFragW2 / 8; k += blockDim.x) w2t[k] = g2t[k];
+    }
     stage_frags(a.frag, kFragW1, w1s, w2s);
-    const uint4* w2b = reinterpret_cast<const uint4*>(a.frag + kFragW1 + kFragW2);
     const int lane = threadIdx.x & 63;
     const int quad = lane >> 4;
     const int col = lane & 15;
     const int64_t chunks = (nnz + 31) / 32;
     const int64_t wave = int64_t(blockIdx.x) * kMlpWaves + (threadIdx.x >> 6);
     const int64_t nwaves = int64_t(gridDim.x) * kMlpWaves;
-    float w3[8][4];
-#pragma unroll
-    for (int ob = 0; ob < 8; ++ob)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) w3[ob][r] = a.params[kOffW3 + ob * 16 + quad * 4 + r];
     float gb1[8][4], gb2[8][4], gw3[8][4];
 #pragma unroll
     for (int ob = 0; ob < 8; ++ob)
@@ -536,62 +568,103 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp_backward(MlpArgs a, cons
         for (int r = 0; r < 4; ++r) gb1[ob][r] = gb2[ob][r] = gw3[ob][r] = 0.f;
     float gb3 = 0.f;
     for (int64_t c = wave; c < chunks; c += nwaves) {
-#pragma unroll 1
-        for (int half = 0; half < 2; ++half) {
-            const int64_t row = c * 32 + half * 16 + col;
-            const v4s x = load_x(feat, row, nnz, quad);
-            TileFwd t;
-            tile_forward(w1s, w2s, a.params, x, lane, t);
-            const float g = (row < nnz) ? go[row] : 0.f;
-            if (quad == 0) gb3 += g;
-            v4f dz2[8];
+        const int64_t row0 = c * 32 + col, row1 = row0 + 16;
+        const float4 xr0 = load_x_raw(feat, row0, nnz, quad);
+        const float4 xr1 = load_x_raw(feat, row1, nnz, quad);
+        const float g0 = row0 < nnz ? go[row0] : 0.f;
+        const float g1 = row1 < nnz ? go[row1] : 0.f;
+        if (quad == 0) {
+            gb3 += g0;
+            gb3 += g1;
+        }
+        // uint32 word of (stored unit s, slot pair col) in chunk c: ub + 16 s;
+        // unit ob*16 + quad*4 + r is stored as ob*16 + r*4 + quad
+        const int64_t ub = c * (kH * 16) + quad * 16 + col;
+        const int64_t xb = c * (kF * 16) + quad * 16 + col;
 #pragma unroll
-            for (int ob = 0; ob < 8; ++ob)
+        for (int r = 0; r < 4; ++r) xc[xb + r * 64] = pack2(comp(xr0, r), comp(xr1, r));
+        const v4s x0 = x_bf16(xr0), x1 = x_bf16(xr1);
+        // layer 1: relu(z1) packed as layer 2's B operand (not stored; the
+        // wgrad kernel recomputes it); the packs also give relu'(z1) for dz1 (unit ib*16 + quad*4 + r sits in
+        // slot (ib & 1) * 4 + r of pack ib / 2)
+        uint4 h1b0[4], h1b1[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const float h2 = fmaxf(t.z2[ob][r], 0.f);
-                    gw3[ob][r] = fmaf(g, h2, gw3[ob][r]);
-                    const float d = t.z2[ob][r] > 0.f ? g * w3[ob][r] : 0.f;
-                    dz2[ob][r] = d;
-                    gb2[ob][r] += d;
-                }
-            uint4 dz2b[4];
+        for (int kb2 = 0; kb2 < 4; ++kb2) {
+            v4f z0[2], z1[2];
 #pragma unroll
-            for (int kb2 = 0; kb2 < 4; ++kb2) dz2b[kb2] = pack8(dz2[2 * kb2], dz2[2 * kb2 + 1], false);
-            v4f dz1[8];
-#pragma unroll
-            for (int ib = 0; ib < 8; ++ib) {
-                v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int kb2 = 0; kb2 < 4; ++kb2) acc = mfma32(as_v8bf(w2b[(ib * 4 + kb2) * 64 + lane]), as_v8bf(dz2b[kb2]), acc);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const float d = t.z1[ib][r] > 0.f ? acc[r] : 0.f;
-                    dz1[ib][r] = d;
-                    gb1[ib][r] += d;
-                }
+            for (int h = 0; h < 2; ++h) {
+                const int ob = 2 * kb2 + h;
+                const v4s w = as_v4s(w1s[ob * 64 + lane]);
+                const float4 b = b1s[ob * 4 + quad];
+                const v4f bias = {b.x, b.y, b.z, b.w};
+                const v4f c0 = mfma16(w, x0, bias);
+                const v4f c1 = mfma16(w, x1, bias);
+                z0[h] = c0;
+                z1[h] = c1;
             }
-            // chunk-transposed BF16 stores: [c][unit][32]
-            const int64_t base = c * (kH * 32) + half * 16 + col;
+            h1b0[kb2] = pack8(z0[0], z0[1], true);
+            h1b1[kb2] = pack8(z1[0], z1[1], true);
+        }
+        // (keeps the packs formed here rather than next to their uses, which
+        // would hold the FP32 pre-activations live through layer 2)
+        __builtin_amdgcn_sched_barrier(0);
+        // layer 2 and dz2 = relu'(z2) * g * w3
+        uint4 d2b0[4], d2b1[4];
+        v4f dp0 = {0.f, 0.f, 0.f, 0.f}, dp1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ob = 0; ob < 8; ++ob)
+        for (int ob = 0; ob < 8; ++ob) {
+            const float4 b = b2s[ob * 4 + quad];
+            v4f c0 = {b.x, b.y, b.z, b.w}, c1 = c0;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int u = ob * 16 + quad * 4 + r;
-                    h1c[base + u * 32] = f2bf(fmaxf(t.z1[ob][r], 0.f));
-                    dz2c[base + u * 32] = f2bf(dz2[ob][r]);
-                    dz1c[base + u * 32] = f2bf(dz1[ob][r]);
-                }
-            const int64_t xbase = c * (kF * 32) + half * 16 + col;
-            const uint2 xu = __builtin_bit_cast(uint2, x);
-            xc[xbase + (quad * 4 + 0) * 32] = uint16_t(xu.x & 0xffffu);
-            xc[xbase + (quad * 4 + 1) * 32] = uint16_t(xu.x >> 16);
-            xc[xbase + (quad * 4 + 2) * 32] = uint16_t(xu.y & 0xffffu);
-            xc[xbase + (quad * 4 + 3) * 32] = uint16_t(xu.y >> 16);
+            for (int kb2 = 0; kb2 < 4; ++kb2) {
+                const v8bf w = as_v8bf(w2s[(ob * 4 + kb2) * 64 + lane]);
+                c0 = mfma32(w, as_v8bf(h1b0[kb2]), c0);
+                c1 = mfma32(w, as_v8bf(h1b1[kb2]), c1);
+            }
+            const float4 w3 = w3s[ob * 4 + quad];
+            v4f d0, d1;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                gw3[ob][r] = fmaf(g0, relu(c0[r]), gw3[ob][r]);
+                gw3[ob][r] = fmaf(g1, relu(c1[r]), gw3[ob][r]);
+                d0[r] = c0[r] > 0.f ? g0 * comp(w3, r) : 0.f;
+                d1[r] = c1[r] > 0.f ? g1 * comp(w3, r) : 0.f;
+                gb2[ob][r] += d0[r];
+                gb2[ob][r] += d1[r];
+                dz2c[ub + (ob * 16 + r * 4) * 16] = pack2(d0[r], d1[r]);
+            }
+            if (ob & 1) {
+                d2b0[ob >> 1] = pack8(dp0, d0, false);
+                d2b1[ob >> 1] = pack8(dp1, d1, false);
+            } else {
+                dp0 = d0;
+                dp1 = d1;
+            }
+        }
+        // dz1 = relu'(z1) * (W2 dz2)
+#pragma unroll
+        for (int ib = 0; ib < 8; ++ib) {
+            v4f a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb2 = 0; kb2 < 4; ++kb2) {
+                const v8bf w = as_v8bf(w2t[(ib * 4 + kb2) * 64 + lane]);
+                a0 = mfma32(w, as_v8bf(d2b0[kb2]), a0);
+                a1 = mfma32(w, as_v8bf(d2b1[kb2]), a1);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int slot = (ib & 1) * 4 + r;
+                const float d0 = slot_nonzero(h1b0[ib >> 1], slot) ? a0[r] : 0.f;
+                const float d1 = slot_nonzero(h1b1[ib >> 1], slot) ? a1[r] : 0.f;
+                gb1[ib][r] += d0;
+                gb1[ib][r] += d1;
+                dz1c[ub + (ib * 16 + r * 4) * 16] = pack2(d0, d1);
+            }
         }
     }
-    // per-wave partials: b1 | b2 | w3 | b3
-    float* out = bias_part + wave * kBiasGrads;
+    // per-block partials b1 | b2 | w3 | b3: per wave over its 16 columns,
+    // then the block's waves in order
+    float* rw = red + (threadIdx.x >> 6) * kBiasGrads;
 #pragma unroll
     for (int ob = 0; ob < 8; ++ob)
 #pragma unroll
@@ -601,18 +674,54 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp_backward(MlpArgs a, cons
             const float s3 = quad_sum(gw3[ob][r]);
             if (col == 0) {
                 const int u = ob * 16 + quad * 4 + r;
-                out[u] = s1;
-                out[kH + u] = s2;
-                out[2 * kH + u] = s3;
+                rw[u] = s1;
+                rw[kH + u] = s2;
+                rw[2 * kH + u] = s3;
             }
         }
     const float s = quad_sum(gb3);   // quad 0 lanes hold the column sums
-    if (lane == 0) out[3 * kH] = s;
+    if (lane == 0) rw[3 * kH] = s;
+    __syncthreads();
+    for (int k = threadIdx.x; k < kBiasGrads; k += blockDim.x) {
+        float t = 0.f;
+#pragma unroll
+        for (int w = 0; w < kMlpWaves; ++w) t += red[w * kBiasGrads + k];
+        bias_part[int64_t(blockIdx.x) * kBiasGrads + k] = t;
+    }
 }
 
 // Weight gradients by split-K MFMA over chunks: dW2 = H1^T dZ2, dW1 = X^T dZ1.
-// Wave w of a block owns dW2 rows [32w, 32w+32) and dW1 columns [32w, 32w+32).
-__global__ void __launch_bounds__(256) k_mlp_wgrad(const uint16_t* __restrict__ xc, const uint16_t* __restrict__ h1c,
+// Wave w of a block owns dW2 rows [32w, 32w+32) and dW1 columns [32w, 32w+32)
+// (stored-unit order).  Operand element (stored unit, k = quad*8..+7) of a
+// chunk sits at uint4 index unit*4 + quad.  H1 is not stored: each wave
+// recomputes relu(X W1 + b1) for its 32 units from the FP32 features with
+// X as the MFMA's A operand, which leaves lane (unit col, quad) holding rows
+// 4quad + t of both tiles h, i.e. chunk slots quad*8 + 2t + h in order: the
+// dW2 A operand as it stands.  The next chunk's loads are issued before this
+// chunk's MFMAs, so a wave always has one chunk of loads in flight (the
+// summation order over chunks is unchanged).
+struct WgradOps {
+    float4 x[2];
+    uint4 z1[2], z2[8], xc;
+};
+
+__device__ __forceinline__ void wgrad_load(const float* __restrict__ feat, int64_t nnz, const uint16_t* __restrict__ xc,
+                                           const uint16_t* __restrict__ dz2c, const uint16_t* __restrict__ dz1c,
+                                           int64_t c, int w, int col, int quad, WgradOps& o) {
+    const uint4* Z2 = reinterpret_cast<const uint4*>(dz2c + c * (kH * 32));
+    const uint4* Z1 = reinterpret_cast<const uint4*>(dz1c + c * (kH * 32));
+    const uint4* X = reinterpret_cast<const uint4*>(xc + c * (kF * 32));
+#pragma unroll
+    for (int h = 0; h < 2; ++h) o.x[h] = load_x_raw(feat, c * 32 + h * 16 + col, nnz, quad);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) o.z1[i] = Z1[((2 * w + i) * 16 + col) * 4 + quad];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o.z2[j] = Z2[(j * 16 + col) * 4 + quad];
+    o.xc = X[col * 4 + quad];
+}
+
+__global__ void __launch_bounds__(256) k_mlp_wgrad(MlpArgs a, const float* __restrict__ feat, int64_t nnz,
+                                                   const uint16_t* __restrict__ xc,
                                                    const uint16_t* __restrict__ dz2c,
                                                    const uint16_t* __restrict__ dz1c, int64_t chunks,
                                                    float* __restrict__ w_part) {
@@ -620,6 +729,18 @@ __global__ void __launch_bounds__(256) k_mlp_wgrad(const uint16_t* __restrict__ 
     const int w = threadIdx.x >> 6;
     const int quad = lane >> 4;
     const int col = lane & 15;
+    // W1 columns and b1 of this lane's stored unit in each of the wave's
+    // two blocks, as the B operand (features 4quad..4quad+3) and the initial
+    // accumulator of the recompute
+    v4s w1b[2];
+    v4f b1v[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int ob = 2 * w + i;
+        w1b[i] = as_v4s(reinterpret_cast<const uint2*>(a.frag)[ob * 64 + quad * 16 + unit_store(col)]);
+        const float b = a.params[kOffB1 + ob * 16 + unit_store(col)];
+        b1v[i] = v4f{b, b, b, b};
+    }
     v4f acc2[2][8];
     v4f acc1[2];
 #pragma unroll
@@ -628,35 +749,53 @@ __global__ void __launch_bounds__(256) k_mlp_wgrad(const uint16_t* __restrict__ 
 #pragma unroll
         for (int j = 0; j < 8; ++j) acc2[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
     }
-    for (int64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
-        const uint4* H = reinterpret_cast<const uint4*>(h1c + c * (kH * 32));
-        const uint4* Z2 = reinterpret_cast<const uint4*>(dz2c + c * (kH * 32));
-        const uint4* Z1 = reinterpret_cast<const uint4*>(dz1c + c * (kH * 32));
-        const uint4* Xc = reinterpret_cast<const uint4*>(xc + c * (kF * 32));
-        // operand element (unit, k=quad*8..+7) sits at uint4 index unit*4 + quad
-        v8bf a2[2];
+    auto accumulate = [&](const WgradOps& o) {
+        const v4s x0 = x_bf16(o.x[0]), x1 = x_bf16(o.x[1]);
 #pragma unroll
-        for (int i = 0; i < 2; ++i) a2[i] = as_v8bf(H[((2 * w + i) * 16 + col) * 4 + quad]);
+        for (int i = 0; i < 2; ++i) {
+            const v4f h0 = mfma16(x0, w1b[i], b1v[i]);
+            const v4f h1 = mfma16(x1, w1b[i], b1v[i]);
+            v8bf ha;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const v8bf b = as_v8bf(Z2[(j * 16 + col) * 4 + quad]);
+            for (int t = 0; t < 4; ++t) {
+                ha[2 * t] = static_cast<__bf16>(relu(h0[t]));
+                ha[2 * t + 1] = static_cast<__bf16>(relu(h1[t]));
+            }
 #pragma unroll
-            for (int i = 0; i < 2; ++i) acc2[i][j] = mfma32(a2[i], b, acc2[i][j]);
+            for (int j = 0; j < 8; ++j) acc2[i][j] = mfma32(ha, as_v8bf(o.z2[j]), acc2[i][j]);
         }
-        const v8bf ax = as_v8bf(Xc[col * 4 + quad]);
 #pragma unroll
-        for (int i = 0; i < 2; ++i) acc1[i] = mfma32(ax, as_v8bf(Z1[((2 * w + i) * 16 + col) * 4 + quad]), acc1[i]);
+        for (int i = 0; i < 2; ++i) acc1[i] = mfma32(as_v8bf(o.xc), as_v8bf(o.z1[i]), acc1[i]);
+    };
+    // two operand buffers used in turn; a load past the last chunk reads the
+    // last chunk again (in bounds, never accumulated)
+    const int64_t G = gridDim.x;
+    const int64_t last = chunks - 1;
+    WgradOps b0, b1;
+    if (int64_t(blockIdx.x) < chunks) wgrad_load(feat, nnz, xc, dz2c, dz1c, blockIdx.x, w, col, quad, b0);
+    // (the scheduling barriers keep each chunk's loads ahead of the MFMAs
+    // of the chunk before; the scheduler would otherwise sink every load
+    // next to its first use and wait on each in turn)
+    for (int64_t c = blockIdx.x; c < chunks; c += 2 * G) {
+        wgrad_load(feat, nnz, xc, dz2c, dz1c, min(c + G, last), w, col, quad, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        accumulate(b0);
+        if (c + G >= chunks) break;
+        wgrad_load(feat, nnz, xc, dz2c, dz1c, min(c + 2 * G, last), w, col, quad, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        accumulate(b1);
     }
     float* out = w_part + int64_t(blockIdx.x) * (kF * kH + kH * kH);
-    // dW1[f][u] at out[f*128 + u]; dW2[i][j] at out[2048 + i*128 + j]
+    // dW1[f][u] at out[f*128 + u]; dW2[i][j] at out[2048 + i*128 + j]; the
+    // operands' rows and columns are stored units and features, mapped back
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            out[(quad * 4 + r) * kH + (2 * w + i) * 16 + col] = acc1[i][r];
+            const int ui = unit_store((2 * w + i) * 16 + quad * 4 + r);
+            out[unit_store(quad * 4 + r) * kH + unit_store((2 * w + i) * 16 + col)] = acc1[i][r];
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                out[kF * kH + ((2 * w + i) * 16 + quad * 4 + r) * kH + j * 16 + col] = acc2[i][j][r];
+            for (int j = 0; j < 8; ++j) out[kF * kH + ui * kH + unit_store(j * 16 + col)] = acc2[i][j][r];
         }
 }
 
@@ -716,7 +855,8 @@ __global__ void k_adam(float* __restrict__ params, const float* __restrict__ gra
     params[p] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
 }
 
-constexpr int kGradBlocks = 256;
+constexpr int kGradBlocks = 512;   // split-K blocks of the weight-gradient kernel (two per CU)
+constexpr int kBackBlocks = 512;   // backward activation grid cap: two 4-wave blocks per CU on 256 CUs
 
 unsigned grid_of(int64_t items, int threads, int64_t cap = 65536) {
     int64_t g = (items + threads - 1) / threads;
@@ -804,8 +944,8 @@ int skfa_mlp_forward(const float* params, const uint16_t* frag, const float* fea
                      void* stream) {
     SKFA_REQUIRE(params && frag && features && out, "null argument");
     if (nnz == 0) return kOk;
-    const int64_t tiles = (nnz + 15) / 16;
-    const unsigned blocks = unsigned(std::min<int64_t>((tiles + kMlpWaves - 1) / kMlpWaves, 2048));
+    const int64_t groups = (nnz + 16 * kFwdTiles - 1) / (16 * kFwdTiles);
+    const unsigned blocks = unsigned(std::min<int64_t>((groups + kMlpWaves - 1) / kMlpWaves, 2048));
     k_mlp_forward<<<blocks, 64 * kMlpWaves, 0, static_cast<hipStream_t>(stream)>>>(MlpArgs{params, frag}, features,
                                                                                     nnz, out);
     SKFA_LAUNCH_CHECK();
@@ -814,9 +954,9 @@ int skfa_mlp_forward(const float* params, const uint16_t* frag, const float* fea
 
 size_t skfa_mlp_workspace_bytes(int64_t nnz) {
     const int64_t chunks = (nnz + 31) / 32;
-    return size_t(chunks) * 32 * (kF + 3 * kH) * sizeof(uint16_t) +
+    return size_t(chunks) * 32 * (kF + 2 * kH) * sizeof(uint16_t) +
            size_t(kGradBlocks) * (kF * kH + kH * kH) * sizeof(float) +
-           size_t(kGradBlocks) * kMlpWaves * kBiasGrads * sizeof(float) + 256;
+           size_t(kBackBlocks) * kBiasGrads * sizeof(float) + 256;
 }
 
 int skfa_mlp_backward(const float* params, const uint16_t* frag, const float* features, const float* grad_out,
@@ -826,21 +966,24 @@ int skfa_mlp_backward(const float* params, const uint16_t* frag, const float* fe
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int64_t chunks = (nnz + 31) / 32;
     uint16_t* xc = static_cast<uint16_t*>(workspace);
-    uint16_t* h1c = xc + chunks * 32 * kF;
-    uint16_t* dz2c = h1c + chunks * 32 * kH;
+    uint16_t* dz2c = xc + chunks * 32 * kF;
     uint16_t* dz1c = dz2c + chunks * 32 * kH;
     uintptr_t pw = reinterpret_cast<uintptr_t>(dz1c + chunks * 32 * kH);
     pw = (pw + 255) & ~uintptr_t(255);
     float* w_part = reinterpret_cast<float*>(pw);
     float* bias_part = w_part + size_t(kGradBlocks) * (kF * kH + kH * kH);
-    const int bblocks = kGradBlocks;   // backward grid: fixed, so bias partials are fixed too
-    k_mlp_backward<<<bblocks, 64 * kMlpWaves, 0, s>>>(MlpArgs{params, frag}, features, grad_out, nnz, xc, h1c,
-                                                      dz2c, dz1c, bias_part);
+    // one chunk per wave up to the cap; the grid depends on nnz only, so the
+    // bias partials (one per block) and their sum order are reproducible
+    const int bblocks = int(std::max<int64_t>(1, std::min<int64_t>((chunks + kMlpWaves - 1) / kMlpWaves, kBackBlocks)));
+    k_mlp_backward<<<bblocks, 64 * kMlpWaves, 0, s>>>(
+        MlpArgs{params, frag}, features, grad_out, nnz, reinterpret_cast<uint32_t*>(xc),
+        reinterpret_cast<uint32_t*>(dz2c), reinterpret_cast<uint32_t*>(dz1c), bias_part);
     SKFA_LAUNCH_CHECK();
-    const int wblocks = int(std::max<int64_t>(1, std::min<int64_t>(chunks, kGradBlocks)));
-    k_mlp_wgrad<<<wblocks, 256, 0, s>>>(xc, h1c, dz2c, dz1c, chunks, w_part);
+    // at least eight chunks per block, so small batches write few partials
+    const int wblocks = int(std::max<int64_t>(1, std::min<int64_t>((chunks + 7) / 8, kGradBlocks)));
+    k_mlp_wgrad<<<wblocks, 256, 0, s>>>(MlpArgs{params, frag}, features, nnz, xc, dz2c, dz1c, chunks, w_part);
     SKFA_LAUNCH_CHECK();
-    k_grad_reduce<<<(kParams + 255) / 256, 256, 0, s>>>(w_part, wblocks, bias_part, bblocks * kMlpWaves, grad);
+    k_grad_reduce<<<(kParams + 255) / 256, 256, 0, s>>>(w_part, wblocks, bias_part, bblocks, grad);
     SKFA_LAUNCH_CHECK();
     return kOk;
 }
