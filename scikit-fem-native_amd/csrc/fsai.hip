@@ -127,13 +127,18 @@ __global__ void __launch_bounds__(256) k_row_stats(const int64_t* __restrict__ i
     }
 }
 
-__global__ void k_rsqrt_diag(const int64_t* __restrict__ ip, const int32_t* __restrict__ ix,
-                             const double* __restrict__ av, int64_t n, double* __restrict__ rs) {
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+// 1/sqrt(A_ii); sixteen lanes per row, coalesced over the row's entries.
+__global__ void __launch_bounds__(256) k_rsqrt_diag(const int64_t* __restrict__ ip, const int32_t* __restrict__ ix,
+                                                    const double* __restrict__ av, int64_t n, double* __restrict__ rs) {
+    const int sub = threadIdx.x & 15;
+    const int64_t groups = (int64_t(gridDim.x) * blockDim.x) >> 4;
+    for (int64_t i = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4; i < n; i += groups) {
+        const int64_t b = ip[i], len = ip[i + 1] - b;
         double d = 0.;
-        for (int64_t k = ip[i]; k < ip[i + 1]; ++k)
-            if (ix[k] == i) d = av[k];
-        rs[i] = 1. / sqrt(d);
+        for (int64_t k = sub; k < len; k += 16)
+            if (ix[b + k] == i) d = av[b + k];
+        d = sum16(d);   // one lane holds the diagonal, the others 0
+        if (sub == 0) rs[i] = 1. / sqrt(d);
     }
 }
 
@@ -238,11 +243,16 @@ __global__ void __launch_bounds__(kFeatBlock) k_features(const int64_t* __restri
 
 // ------------------------------------------------------------------ Kaporin objective
 // Gt from network outputs; G = Gt D^{-1/2} as float; per-nonzero diag flag.
-__global__ void k_gtilde(const float* __restrict__ o, const int64_t* __restrict__ lptr,
-                         const int32_t* __restrict__ lidx, const double* __restrict__ rs, int64_t n,
-                         double* __restrict__ gt, float* __restrict__ g, double* __restrict__ logdiag) {
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-        for (int64_t q = lptr[i]; q < lptr[i + 1]; ++q) {
+// Sixteen lanes per row, each taking the row's lower entries sub, sub+16, ...
+__global__ void __launch_bounds__(256) k_gtilde(const float* __restrict__ o, const int64_t* __restrict__ lptr,
+                                                const int32_t* __restrict__ lidx, const double* __restrict__ rs,
+                                                int64_t n, double* __restrict__ gt, float* __restrict__ g,
+                                                double* __restrict__ logdiag) {
+    const int sub = threadIdx.x & 15;
+    const int64_t groups = (int64_t(gridDim.x) * blockDim.x) >> 4;
+    for (int64_t i = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4; i < n; i += groups) {
+        const int64_t b = lptr[i], e = lptr[i + 1];
+        for (int64_t q = b + sub; q < e; q += 16) {
             const int32_t j = lidx[q];
             double v = double(o[q]);
             if (j == i) {
@@ -876,8 +886,7 @@ int skfa_fsai_features(const int64_t* indptr, const int32_t* indices, const doub
     SKFA_REQUIRE(indptr && indices && vals && lptr && lidx && rsqrt_diag && row_stats && features, "null argument");
     if (n == 0) return kOk;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const unsigned g = grid_of(n, 256);
-    k_rsqrt_diag<<<g, 256, 0, s>>>(indptr, indices, vals, n, rsqrt_diag);
+    k_rsqrt_diag<<<grid_of(n * 16, 256), 256, 0, s>>>(indptr, indices, vals, n, rsqrt_diag);
     k_row_stats<<<grid_of(n * 16, 256), 256, 0, s>>>(indptr, indices, vals, rsqrt_diag, xyz, n, row_stats);
     int64_t nnz_l = 0;
     SKFA_HIP(hipMemcpyAsync(&nnz_l, lptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -893,7 +902,7 @@ int skfa_fsai_factor(const float* outputs, const int64_t* lptr, const int32_t* l
                      int64_t n, double* gtilde, float* g, double* logdiag, void* stream) {
     SKFA_REQUIRE(outputs && lptr && lidx && rsqrt_diag, "null argument");
     if (n == 0) return kOk;
-    k_gtilde<<<grid_of(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(outputs, lptr, lidx, rsqrt_diag, n,
+    k_gtilde<<<grid_of(n * 16, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(outputs, lptr, lidx, rsqrt_diag, n,
                                                                             gtilde, g, logdiag);
     SKFA_LAUNCH_CHECK();
     return kOk;
@@ -919,7 +928,7 @@ int skfa_fsai_kaporin(const int64_t* indptr, const int32_t* indices, const doubl
     double* logsum = tr + nsys;
     int64_t* seg = reinterpret_cast<int64_t*>(logsum + nsys);
     const unsigned g = grid_of(n, 256);
-    k_gtilde<<<g, 256, 0, s>>>(outputs, lptr, lidx, rsqrt_diag, n, gt, nullptr, logdiag);
+    k_gtilde<<<grid_of(n * 16, 256), 256, 0, s>>>(outputs, lptr, lidx, rsqrt_diag, n, gt, nullptr, logdiag);
     k_kaporin_p<<<g, 256, 0, s>>>(indptr, indices, vals, rsqrt_diag, lptr, lidx, gt, n, P, prod);
     k_seg_of<<<(nsys + 256) / 256, 256, 0, s>>>(sys_ptr, lptr, nsys, seg);
     k_segsum<<<nsys, kRed, 0, s>>>(prod, seg, tr);

@@ -97,62 +97,108 @@ __global__ void k_condense_fill(const int64_t* __restrict__ indptr, const int32_
 }
 
 // ------------------------------------------------------------------ lower pattern / transpose
-__global__ void k_lower_count(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, int64_t nrows,
-                              int32_t* __restrict__ count) {
-    for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nrows; r += int64_t(gridDim.x) * blockDim.x) {
+// Sixteen lanes per row (four rows per wave, blocks of 256 threads): lane
+// `sub` of a group takes the row's entries base + sub in chunks of 16, so a
+// row's index and value reads are coalesced; selected entries are compacted
+// in order with the group's ballot.  The chunk loop bound is uniform over a
+// group.
+__device__ __forceinline__ uint32_t group_ballot(bool p) {
+    return uint32_t(__ballot(p) >> (threadIdx.x & 48)) & 0xFFFFu;
+}
+__device__ __forceinline__ int lanes_before(uint32_t mask, int sub) {
+    return __popc(mask & ((1u << sub) - 1u));
+}
+
+__global__ void __launch_bounds__(256) k_lower_count(const int64_t* __restrict__ indptr,
+                                                     const int32_t* __restrict__ indices, int64_t nrows,
+                                                     int32_t* __restrict__ count) {
+    const int sub = threadIdx.x & 15;
+    const int64_t groups = (int64_t(gridDim.x) * blockDim.x) >> 4;
+    for (int64_t r = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4; r < nrows; r += groups) {
+        const int64_t b = indptr[r], len = indptr[r + 1] - b;
         int32_t c = 0;
-        for (int64_t k = indptr[r]; k < indptr[r + 1]; ++k) c += indices[k] <= r ? 1 : 0;
-        count[r] = c;
+        for (int64_t base = 0; base < len; base += 16) {
+            const int64_t k = base + sub;
+            c += __popc(group_ballot(k < len && indices[b + k] <= r));
+        }
+        if (sub == 0) count[r] = c;
     }
 }
 
-__global__ void k_lower_fill(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, int64_t nrows,
-                             const int64_t* __restrict__ lptr, int32_t* __restrict__ lidx, int64_t* __restrict__ lsrc) {
-    for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nrows; r += int64_t(gridDim.x) * blockDim.x) {
+__global__ void __launch_bounds__(256) k_lower_fill(const int64_t* __restrict__ indptr,
+                                                    const int32_t* __restrict__ indices, int64_t nrows,
+                                                    const int64_t* __restrict__ lptr, int32_t* __restrict__ lidx,
+                                                    int64_t* __restrict__ lsrc) {
+    const int sub = threadIdx.x & 15;
+    const int64_t groups = (int64_t(gridDim.x) * blockDim.x) >> 4;
+    for (int64_t r = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4; r < nrows; r += groups) {
+        const int64_t b = indptr[r], len = indptr[r + 1] - b;
         int64_t o = lptr[r];
-        for (int64_t k = indptr[r]; k < indptr[r + 1]; ++k)
-            if (indices[k] <= r) {
-                lidx[o] = indices[k];
-                lsrc[o] = k;
-                ++o;
+        for (int64_t base = 0; base < len; base += 16) {
+            const int64_t k = base + sub;
+            const int32_t c = k < len ? indices[b + k] : 0;
+            const bool take = k < len && c <= r;
+            const uint32_t m = group_ballot(take);
+            if (take) {
+                const int64_t at = o + lanes_before(m, sub);
+                lidx[at] = c;
+                lsrc[at] = b + k;
             }
+            o += __popc(m);
+        }
     }
 }
 
 // Transpose of the lower pattern of a matrix whose pattern is symmetric:
 // row i of L^T holds the columns j >= i of A's row i, and entry (i, j) of
 // L^T sits in row j of L, found by binary search.  No atomics, no sort.
-__global__ void k_upper_count(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, int64_t nrows,
-                              int32_t* __restrict__ count) {
-    for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nrows; r += int64_t(gridDim.x) * blockDim.x) {
+__global__ void __launch_bounds__(256) k_upper_count(const int64_t* __restrict__ indptr,
+                                                     const int32_t* __restrict__ indices, int64_t nrows,
+                                                     int32_t* __restrict__ count) {
+    const int sub = threadIdx.x & 15;
+    const int64_t groups = (int64_t(gridDim.x) * blockDim.x) >> 4;
+    for (int64_t r = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4; r < nrows; r += groups) {
+        const int64_t b = indptr[r], len = indptr[r + 1] - b;
         int32_t c = 0;
-        for (int64_t k = indptr[r]; k < indptr[r + 1]; ++k) c += indices[k] >= r ? 1 : 0;
-        count[r] = c;
+        for (int64_t base = 0; base < len; base += 16) {
+            const int64_t k = base + sub;
+            c += __popc(group_ballot(k < len && indices[b + k] >= r));
+        }
+        if (sub == 0) count[r] = c;
     }
 }
 
-__global__ void k_upper_fill(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, int64_t nrows,
-                             const int64_t* __restrict__ lptr, const int32_t* __restrict__ lidx,
-                             const int64_t* __restrict__ tptr, int32_t* __restrict__ tidx, int64_t* __restrict__ tsrc,
-                             int32_t* __restrict__ missing) {
-    for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nrows; r += int64_t(gridDim.x) * blockDim.x) {
+__global__ void __launch_bounds__(256) k_upper_fill(const int64_t* __restrict__ indptr,
+                                                    const int32_t* __restrict__ indices, int64_t nrows,
+                                                    const int64_t* __restrict__ lptr, const int32_t* __restrict__ lidx,
+                                                    const int64_t* __restrict__ tptr, int32_t* __restrict__ tidx,
+                                                    int64_t* __restrict__ tsrc, int32_t* __restrict__ missing) {
+    const int sub = threadIdx.x & 15;
+    const int64_t groups = (int64_t(gridDim.x) * blockDim.x) >> 4;
+    for (int64_t r = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4; r < nrows; r += groups) {
+        const int64_t b = indptr[r], len = indptr[r + 1] - b;
         int64_t o = tptr[r];
-        for (int64_t k = indptr[r]; k < indptr[r + 1]; ++k) {
-            const int32_t j = indices[k];
-            if (j < r) continue;
-            int64_t lo = lptr[j], hi = lptr[j + 1];
-            while (lo < hi) {
-                const int64_t mid = (lo + hi) >> 1;
-                if (lidx[mid] < r) lo = mid + 1;
-                else hi = mid;
+        for (int64_t base = 0; base < len; base += 16) {
+            const int64_t k = base + sub;
+            const int32_t j = k < len ? indices[b + k] : 0;
+            const bool take = k < len && j >= r;
+            const uint32_t m = group_ballot(take);
+            if (take) {
+                int64_t lo = lptr[j], hi = lptr[j + 1];
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (lidx[mid] < r) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo == lptr[j + 1] || lidx[lo] != r) {
+                    atomicAdd(missing, 1);
+                    lo = lptr[j];
+                }
+                const int64_t at = o + lanes_before(m, sub);
+                tidx[at] = j;
+                tsrc[at] = lo;
             }
-            if (lo == lptr[j + 1] || lidx[lo] != r) {
-                atomicAdd(missing, 1);
-                lo = lptr[j];
-            }
-            tidx[o] = j;
-            tsrc[o] = lo;
-            ++o;
+            o += __popc(m);
         }
     }
 }
@@ -564,12 +610,12 @@ int skfa_csr_lower(const int64_t* indptr, const int32_t* indices, int64_t nrows,
     hipStream_t s = static_cast<hipStream_t>(stream);
     DevBuf<int32_t> cnt;
     if (int rc = cnt.alloc(nrows + 1)) return rc;
-    k_lower_count<<<grid_of(nrows, 256), 256, 0, s>>>(indptr, indices, nrows, cnt.p);
+    k_lower_count<<<grid_of(nrows * 16, 256), 256, 0, s>>>(indptr, indices, nrows, cnt.p);
     SKFA_LAUNCH_CHECK();
     if (int rc = exclusive_scan_i32_to_i64(cnt.p, lptr, nrows, s)) return rc;
     SKFA_HIP(hipMemcpyAsync(nnz_lower, lptr + nrows, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     if (lidx && lsrc) {
-        k_lower_fill<<<grid_of(nrows, 256), 256, 0, s>>>(indptr, indices, nrows, lptr, lidx, lsrc);
+        k_lower_fill<<<grid_of(nrows * 16, 256), 256, 0, s>>>(indptr, indices, nrows, lptr, lidx, lsrc);
         SKFA_LAUNCH_CHECK();
     }
     SKFA_HIP(hipStreamSynchronize(s));
@@ -608,10 +654,11 @@ int skfa_csr_transpose_lower(const int64_t* indptr, const int32_t* indices, int6
     if (int rc = cnt.alloc(size_t(nrows) + 1)) return rc;
     if (int rc = missing.alloc(1)) return rc;
     SKFA_HIP(hipMemsetAsync(missing.p, 0, sizeof(int32_t), s));
-    k_upper_count<<<grid_of(nrows, 256), 256, 0, s>>>(indptr, indices, nrows, cnt.p);
+    k_upper_count<<<grid_of(nrows * 16, 256), 256, 0, s>>>(indptr, indices, nrows, cnt.p);
     SKFA_LAUNCH_CHECK();
     if (int rc = exclusive_scan_i32_to_i64(cnt.p, tptr, nrows, s)) return rc;
-    k_upper_fill<<<grid_of(nrows, 256), 256, 0, s>>>(indptr, indices, nrows, lptr, lidx, tptr, tidx, tsrc, missing.p);
+    k_upper_fill<<<grid_of(nrows * 16, 256), 256, 0, s>>>(indptr, indices, nrows, lptr, lidx, tptr, tidx, tsrc,
+                                                        missing.p);
     SKFA_LAUNCH_CHECK();
     int32_t h_missing = 0;
     SKFA_HIP(hipMemcpyAsync(&h_missing, missing.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
