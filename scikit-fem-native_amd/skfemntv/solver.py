@@ -105,10 +105,10 @@ class DeviceCSR:
         dev = self.data.device
         lptr = torch.empty(n + 1, dtype=torch.int64, device=dev)
         nl = ctypes.c_int64()
-        _native.check(L.skfa_csr_lower(_native.ptr(self.indptr), _native.ptr(self.indices), n, _native.ptr(lptr),
-                                       None, None, ctypes.byref(nl), _native.stream()))
-        lidx = torch.empty(max(nl.value, 1), dtype=torch.int32, device=dev)
-        lsrc = torch.empty(max(nl.value, 1), dtype=torch.int64, device=dev)
+        # the lower part has at most nnz entries: one pass with buffers of that size
+        cap = max(self.nnz, 1)
+        lidx = torch.empty(cap, dtype=torch.int32, device=dev)
+        lsrc = torch.empty(cap, dtype=torch.int64, device=dev)
         _native.check(L.skfa_csr_lower(_native.ptr(self.indptr), _native.ptr(self.indices), n, _native.ptr(lptr),
                                        _native.ptr(lidx), _native.ptr(lsrc), ctypes.byref(nl), _native.stream()))
         return LowerPattern(lptr, lidx[:nl.value], lsrc[:nl.value], n, source=self)
