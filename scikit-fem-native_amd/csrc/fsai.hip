@@ -161,7 +161,52 @@ __device__ __forceinline__ int64_t row_of(const int64_t* __restrict__ lptr, int6
 // Blocks take 256 consecutive lower nonzeros.  Every row holds at least its
 // diagonal, so they span at most 256 rows: one search finds the first, and
 // each thread then searches the block's window of row pointers in LDS.
+// Rows i of the block are consecutive, so their A entries form one
+// contiguous range, staged in LDS with coalesced loads (when it fits
+// kFeatStage entries; else row i is read from global memory).  Each thread
+// then walks row j of A in order, kFeatChunk entries at a time with their loads
+// (column, value, 1/sqrt of the column's diagonal) issued together, and
+// advances its pointer into row i as in a sorted merge; the matched columns
+// come in increasing order, so every sum has the merge's order.
 constexpr int kFeatBlock = 256;
+constexpr int kFeatStage = 2048;
+constexpr int kFeatChunk = 4;   // row-j entries whose loads are issued together
+
+template <typename ColI, typename ValI>
+__device__ __forceinline__ void feature_merge(int64_t a, int64_t ea, ColI col_i, ValI val_i,
+                                              const int32_t* __restrict__ ix, const double* __restrict__ av,
+                                              const double* __restrict__ rs, int64_t bj, int64_t ej, int64_t i,
+                                              int32_t j, double rsi, double rsj, double& aij, double& t_mid,
+                                              double& t_lo, double& t_all) {
+    for (int64_t b0 = bj; b0 < ej; b0 += kFeatChunk) {
+        int32_t d[kFeatChunk];
+        double v[kFeatChunk], rk[kFeatChunk];
+#pragma unroll
+        for (int u = 0; u < kFeatChunk; ++u) {
+            const bool ok = b0 + u < ej;
+            d[u] = ok ? ix[b0 + u] : 0x7fffffff;
+            v[u] = ok ? av[b0 + u] : 0.;
+        }
+#pragma unroll
+        for (int u = 0; u < kFeatChunk; ++u) rk[u] = d[u] != 0x7fffffff ? rs[d[u]] : 0.;
+#pragma unroll
+        for (int u = 0; u < kFeatChunk; ++u) {
+            // (a padding entry, column 0x7fffffff, runs a to the end and matches nothing)
+            while (a < ea && col_i(a) < d[u]) ++a;
+            if (a < ea && col_i(a) == d[u]) {
+                const double va = val_i(a);
+                const double p = (va * rsi * rk[u]) * (v[u] * rsj * rk[u]);
+                // (adds of 0 rather than branches: a branch between the two
+                // sums puts them in scratch memory)
+                t_all += p;
+                t_lo += d[u] < j ? p : 0.;
+                t_mid += (d[u] > j && d[u] < i) ? p : 0.;
+                if (d[u] == j) aij = va * rsi * rsj;
+                ++a;
+            }
+        }
+    }
+}
 
 __global__ void __launch_bounds__(kFeatBlock) k_features(const int64_t* __restrict__ ip, const int32_t* __restrict__ ix,
                                                          const double* __restrict__ av, const double* __restrict__ rs,
@@ -169,13 +214,20 @@ __global__ void __launch_bounds__(kFeatBlock) k_features(const int64_t* __restri
                                                          const int64_t* __restrict__ lptr, const int32_t* __restrict__ lidx,
                                                          int64_t n, int64_t nnz_l, float* __restrict__ feat) {
     __shared__ int64_t win[kFeatBlock + 1];
+    __shared__ int64_t ipw[kFeatBlock + 1];
+    __shared__ int32_t sc[kFeatStage];
+    __shared__ double sv[kFeatStage];
     __shared__ int64_t first_row;
+    __shared__ int last_lo;
     for (int64_t q0 = int64_t(blockIdx.x) * kFeatBlock; q0 < nnz_l; q0 += int64_t(gridDim.x) * kFeatBlock) {
         if (threadIdx.x == 0) first_row = row_of(lptr, n, q0);
         __syncthreads();
         const int64_t i0 = first_row;
         // win[k] = lptr[i0 + k]; rows past n read as nnz_l
-        for (int k = threadIdx.x; k <= kFeatBlock; k += blockDim.x) win[k] = i0 + k <= n ? lptr[i0 + k] : nnz_l;
+        for (int k = threadIdx.x; k <= kFeatBlock; k += blockDim.x) {
+            win[k] = i0 + k <= n ? lptr[i0 + k] : nnz_l;
+            ipw[k] = i0 + k <= n ? ip[i0 + k] : ip[n];
+        }
         __syncthreads();
         const int64_t q = q0 + threadIdx.x;
         int lo = 0, hi = kFeatBlock;   // largest k with win[k] <= q
@@ -184,60 +236,60 @@ __global__ void __launch_bounds__(kFeatBlock) k_features(const int64_t* __restri
             if (win[mid] <= q) lo = mid;
             else hi = mid;
         }
+        // the block's last nonzero (thread 255, or the last of the matrix) has the largest row
+        if (q == min(q0 + kFeatBlock, nnz_l) - 1) last_lo = lo;
         __syncthreads();
-        if (q >= nnz_l) continue;
-        const int64_t i = i0 + lo;
-        const int32_t j = lidx[q];
-        const int64_t bi = ip[i], ei = ip[i + 1];
-        const int64_t bj = ip[j], ej = ip[j + 1];
-        const double rsi = rs[i], rsj = rs[j];
-        double aij = 0., t_mid = 0., t_lo = 0., t_all = 0.;
-        // sorted merge of rows i and j
-        int64_t a = bi, b = bj;
-        while (a < ei && b < ej) {
-            const int32_t ca = ix[a], cb = ix[b];
-            if (ca == cb) {
-                const double rk = rs[ca];
-                const double p = (av[a] * rsi * rk) * (av[b] * rsj * rk);
-                t_all += p;
-                if (ca < j) t_lo += p;
-                else if (ca > j && ca < i) t_mid += p;
-                if (ca == j) aij = av[a] * rsi * rsj;
-                ++a;
-                ++b;
-            } else if (ca < cb) {
-                ++a;
-            } else {
-                ++b;
+        const int64_t base = ipw[0];
+        const int64_t count = ipw[last_lo + 1] - base;
+        const bool staged = count <= kFeatStage;
+        if (staged)
+            for (int64_t e = threadIdx.x; e < count; e += blockDim.x) {
+                sc[e] = ix[base + e];
+                sv[e] = av[base + e];
             }
-        }
-        float f[kF];
-        f[0] = float(aij);
-        f[1] = (i == j) ? 1.f : 0.f;
-        f[2] = float(stats[4 * i + 0]);
-        f[3] = float(stats[4 * j + 0]);
-        f[4] = float(stats[4 * i + 1]);
-        f[5] = float(stats[4 * j + 1]);
-        f[6] = float(stats[4 * i + 2]);
-        f[7] = float(stats[4 * j + 2]);
-        f[8] = float(t_mid);
-        f[9] = float(t_lo);
-        f[10] = float(t_all);
-        f[11] = float(0.5 * log(rsj * rsj / (rsi * rsi)));
-        if (xyz && i != j) {
-            const double h = stats[4 * i + 3];
-            const double dx = (xyz[3 * j] - xyz[3 * i]) / h, dy = (xyz[3 * j + 1] - xyz[3 * i + 1]) / h,
-                         dz = (xyz[3 * j + 2] - xyz[3 * i + 2]) / h;
-            f[12] = float(dx);
-            f[13] = float(dy);
-            f[14] = float(dz);
-            f[15] = float(sqrt(dx * dx + dy * dy + dz * dz));
-        } else {
-            f[12] = f[13] = f[14] = f[15] = 0.f;
-        }
-        float4* out = reinterpret_cast<float4*>(feat + q * kF);
+        __syncthreads();
+        if (q < nnz_l) {
+            const int64_t i = i0 + lo;
+            const int32_t j = lidx[q];
+            const int64_t bj = ip[j], ej = ip[j + 1];
+            const double rsi = rs[i], rsj = rs[j];
+            double aij = 0., t_mid = 0., t_lo = 0., t_all = 0.;
+            if (staged)
+                feature_merge(ipw[lo] - base, ipw[lo + 1] - base, [&](int64_t e) { return sc[e]; },
+                              [&](int64_t e) { return sv[e]; }, ix, av, rs, bj, ej, i, j, rsi, rsj, aij, t_mid, t_lo,
+                              t_all);
+            else
+                feature_merge(ipw[lo], ipw[lo + 1], [&](int64_t e) { return ix[e]; }, [&](int64_t e) { return av[e]; },
+                              ix, av, rs, bj, ej, i, j, rsi, rsj, aij, t_mid, t_lo, t_all);
+            float f[kF];
+            f[0] = float(aij);
+            f[1] = (i == j) ? 1.f : 0.f;
+            f[2] = float(stats[4 * i + 0]);
+            f[3] = float(stats[4 * j + 0]);
+            f[4] = float(stats[4 * i + 1]);
+            f[5] = float(stats[4 * j + 1]);
+            f[6] = float(stats[4 * i + 2]);
+            f[7] = float(stats[4 * j + 2]);
+            f[8] = float(t_mid);
+            f[9] = float(t_lo);
+            f[10] = float(t_all);
+            f[11] = float(0.5 * log(rsj * rsj / (rsi * rsi)));
+            if (xyz && i != j) {
+                const double h = stats[4 * i + 3];
+                const double dx = (xyz[3 * j] - xyz[3 * i]) / h, dy = (xyz[3 * j + 1] - xyz[3 * i + 1]) / h,
+                             dz = (xyz[3 * j + 2] - xyz[3 * i + 2]) / h;
+                f[12] = float(dx);
+                f[13] = float(dy);
+                f[14] = float(dz);
+                f[15] = float(sqrt(dx * dx + dy * dy + dz * dz));
+            } else {
+                f[12] = f[13] = f[14] = f[15] = 0.f;
+            }
+            float4* out = reinterpret_cast<float4*>(feat + q * kF);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) out[v] = make_float4(f[4 * v], f[4 * v + 1], f[4 * v + 2], f[4 * v + 3]);
+            for (int v = 0; v < 4; ++v) out[v] = make_float4(f[4 * v], f[4 * v + 1], f[4 * v + 2], f[4 * v + 3]);
+        }
+        __syncthreads();   // the staged rows and windows are rewritten by the next block step
     }
 }
 
