@@ -1527,11 +1527,12 @@ __global__ void __launch_bounds__(64) k_gather_vec10_mfma(GatherArgs A) {
 // lanes' random (a, b) rows over the LDS banks
 constexpr int kNK = 7;
 
-template <int NL>
-__global__ void k_build_ntab(const double* __restrict__ rtab, int32_t Qg, int32_t Qm, double* __restrict__ out,
-                             float* __restrict__ out32) {
-    const int t = threadIdx.x;
-    if (t >= NL * NL) return;
+// Element-table entry t = a * NL + b from the reduced quadrature tables
+// (wg | dg | wm | pm), kNK values: the six D slots and the mass.  Each
+// table-gather block builds its own copy in LDS.
+template <int NL, typename T>
+__device__ inline void build_table_entry(const double* __restrict__ rtab, int32_t Qg, int32_t Qm, int t,
+                                         T* __restrict__ out) {
     const int a = t / NL, b = t % NL;
     const double* wg = rtab;
     const double* dg = wg + Qg;
@@ -1539,15 +1540,19 @@ __global__ void k_build_ntab(const double* __restrict__ rtab, int32_t Qg, int32_
     const double* pm = wm + Qm;
     double M[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
     for (int q = 0; q < Qg; ++q)
+#pragma unroll
         for (int i = 0; i < 3; ++i)
+#pragma unroll
             for (int j = 0; j < 3; ++j) M[i][j] += wg[q] * dg[(q * NL + a) * 3 + i] * dg[(q * NL + b) * 3 + j];
     double mass = 0.;
     for (int q = 0; q < Qm; ++q) mass += wm[q] * pm[q * NL + a] * pm[q * NL + b];
-    const double v[kNK] = {M[0][0], M[0][1] + M[1][0], M[0][2] + M[2][0], M[1][1], M[1][2] + M[2][1], M[2][2], mass};
-    for (int k = 0; k < kNK; ++k) {
-        out[t * kNK + k] = v[k];
-        out32[t * kNK + k] = float(v[k]);
-    }
+    out[t * kNK + 0] = T(M[0][0]);
+    out[t * kNK + 1] = T(M[0][1] + M[1][0]);
+    out[t * kNK + 2] = T(M[0][2] + M[2][0]);
+    out[t * kNK + 3] = T(M[1][1]);
+    out[t * kNK + 4] = T(M[1][2] + M[2][1]);
+    out[t * kNK + 5] = T(M[2][2]);
+    out[t * kNK + 6] = T(mass);
 }
 
 template <int NL, bool F32, int RB, int OFFW, bool MASS>
@@ -1589,8 +1594,8 @@ __global__ void __launch_bounds__(64 * WPB) k_gather_table(GatherArgs A, int32_t
     constexpr int kTab = NL * NL * kNK;
     constexpr int kTabDoubles = ((kTab * int(sizeof(T)) + 15) / 16) * 2;
     T* ntab = reinterpret_cast<T*>(lds);
-    const T* src = F32 ? reinterpret_cast<const T*>(A.tab32) : reinterpret_cast<const T*>(A.tab);
-    for (int k = threadIdx.x; k < kTab; k += 64 * WPB) ntab[k] = src[k];
+    // the element table from the reduced tables (A.tab), rounded to T
+    for (int t = threadIdx.x; t < NL * NL; t += 64 * WPB) build_table_entry<NL>(A.tab, A.Qg, A.Qm, t, ntab);
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
     // block placement (XCD chunks of blocks when the plan asks for chunking)
@@ -2417,11 +2422,9 @@ __global__ void k_scatter_mapped(const double* __restrict__ src, const int32_t* 
 }
 
 // Scratch of the element-constant path: the per-element payload (E x
-// stride doubles) and the element tables of the table gather (FP64, FP32).
+// stride doubles).
 struct Workspace {
     double* payload = nullptr;
-    double* ntab = nullptr;
-    float* ntab32 = nullptr;
 };
 
 template <int NL, int C, bool F32>
@@ -2451,14 +2454,9 @@ static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, const Work
     // element tables win for 4-node elements (few table rows, long column
     // split); the quadrature gather with uniform table reads wins for P2
     if (sym && table_gather(P)) {
-        // element tables from the reduced quadrature tables, then the
-        // table-driven gather
-        SKFA_REQUIRE(W.ntab != nullptr && W.ntab32 != nullptr, "element-table buffers are missing");
-        k_build_ntab<NL><<<1, 128, 0, s>>>(A.tab, A.Qg, A.Qm, W.ntab, W.ntab32);
-        SKFA_LAUNCH_CHECK();
-        GatherArgs T = A;
-        T.tab = W.ntab;
-        T.tab32 = W.ntab32;
+        // the table-driven gather (each block builds the element tables from
+        // the reduced quadrature tables)
+        const GatherArgs& T = A;
         if constexpr (NL == 4) {
             if (A.load_out != nullptr)
                 return A.has_mass ? launch_table<NL, F32, true, true>(P, T, s)
@@ -2583,24 +2581,11 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
         // the FP32-products vector-P2 gather reads a float payload (16 floats)
         const bool vec32 = P->C == 3 && P->n == 10 && nqg == 4 && precision == 1;
         const int stride = P->C == 1 ? (sym ? 8 : 10) : (vec32 ? 8 : 14);
-        // the table gather runs for symmetric scalar forms unless the MFMA
-        // gather takes them (launch_gather)
-        const bool table = P->C == 1 && sym && table_gather(P) && !(!diag3 && mfma_gather(P, nqg, nqm));
-        const size_t nt = size_t(P->n) * P->n * kNK;
         if (scratch == nullptr) {
             if (int rc = ensure_payload(P, stride)) return rc;
-            if (table && P->ntab.n < nt)
-                if (int rc = P->ntab.alloc(nt)) return rc;
-            if (table && P->ntab32.n < nt)
-                if (int rc = P->ntab32.alloc(nt)) return rc;
             W.payload = P->payload.p;
-            W.ntab = P->ntab.p;
-            W.ntab32 = P->ntab32.p;
         } else {
-            // 256-byte aligned pieces: payload | FP64 tables | FP32 tables
-            auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-            const size_t pay = up(size_t(P->E) * stride * sizeof(double));
-            const size_t need = pay + (table ? up(nt * sizeof(double)) + up(nt * sizeof(float)) : 0);
+            const size_t need = size_t(P->E) * stride * sizeof(double);
             if (need > scratch_bytes) {
                 set_error("the element payload of " + std::to_string(P->E) + " elements needs " +
                           std::to_string(need) + " bytes, more than the " + std::to_string(scratch_bytes) +
@@ -2608,10 +2593,6 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
                 return kCapacity;
             }
             W.payload = reinterpret_cast<double*>(scratch);
-            if (table) {
-                W.ntab = reinterpret_cast<double*>(scratch + pay);
-                W.ntab32 = reinterpret_cast<float*>(scratch + pay + up(nt * sizeof(double)));
-            }
         }
         PayloadArgs pa{};
         pa.xyz = reinterpret_cast<const double4*>(xyz);
