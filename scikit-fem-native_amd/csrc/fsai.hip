@@ -127,18 +127,16 @@ __global__ void __launch_bounds__(256) k_row_stats(const int64_t* __restrict__ i
     }
 }
 
-// 1/sqrt(A_ii); sixteen lanes per row, coalesced over the row's entries.
+// 1/sqrt(A_ii), one thread per row.  A's rows are sorted, so the lower
+// pattern's count of row i says where the diagonal sits: the last of the
+// row's entries j <= i.  A row without a diagonal gives 1/sqrt(0).
 __global__ void __launch_bounds__(256) k_rsqrt_diag(const int64_t* __restrict__ ip, const int32_t* __restrict__ ix,
-                                                    const double* __restrict__ av, int64_t n, double* __restrict__ rs) {
-    const int sub = threadIdx.x & 15;
-    const int64_t groups = (int64_t(gridDim.x) * blockDim.x) >> 4;
-    for (int64_t i = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4; i < n; i += groups) {
-        const int64_t b = ip[i], len = ip[i + 1] - b;
-        double d = 0.;
-        for (int64_t k = sub; k < len; k += 16)
-            if (ix[b + k] == i) d = av[b + k];
-        d = sum16(d);   // one lane holds the diagonal, the others 0
-        if (sub == 0) rs[i] = 1. / sqrt(d);
+                                                    const double* __restrict__ av, const int64_t* __restrict__ lptr,
+                                                    int64_t n, double* __restrict__ rs) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t k = ip[i] + (lptr[i + 1] - lptr[i]) - 1;
+        const double d = (k >= ip[i] && ix[k] == i) ? av[k] : 0.;
+        rs[i] = 1. / sqrt(d);
     }
 }
 
@@ -938,7 +936,7 @@ int skfa_fsai_features(const int64_t* indptr, const int32_t* indices, const doub
     SKFA_REQUIRE(indptr && indices && vals && lptr && lidx && rsqrt_diag && row_stats && features, "null argument");
     if (n == 0) return kOk;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    k_rsqrt_diag<<<grid_of(n * 16, 256), 256, 0, s>>>(indptr, indices, vals, n, rsqrt_diag);
+    k_rsqrt_diag<<<grid_of(n, 256), 256, 0, s>>>(indptr, indices, vals, lptr, n, rsqrt_diag);
     k_row_stats<<<grid_of(n * 16, 256), 256, 0, s>>>(indptr, indices, vals, rsqrt_diag, xyz, n, row_stats);
     int64_t nnz_l = 0;
     SKFA_HIP(hipMemcpyAsync(&nnz_l, lptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
