@@ -163,22 +163,25 @@ __device__ __forceinline__ int64_t row_of(const int64_t* __restrict__ lptr, int6
 // contiguous range, staged in LDS with coalesced loads (when it fits
 // kFeatStage entries; else row i is read from global memory).  Each thread
 // then walks row j of A in order, kFeatChunk entries at a time with their loads
-// (column, value, 1/sqrt of the column's diagonal) issued together, and
+// (column, value) issued together, and
 // advances its pointer into row i as in a sorted merge; the matched columns
 // come in increasing order, so every sum has the merge's order.
 constexpr int kFeatBlock = 256;
 constexpr int kFeatStage = 2048;
 constexpr int kFeatChunk = 4;   // row-j entries whose loads are issued together
 
+// val_i(a) = A_ik rs_k^2 for column k of row i, so the product of the two
+// scaled entries At_ik At_jk = val_i(a) A_jk rs_i rs_j needs no gather of
+// rs_k while row j is walked; At_ij = val_i(a) rs_i / rs_j at k = j.
 template <typename ColI, typename ValI>
 __device__ __forceinline__ void feature_merge(int64_t a, int64_t ea, ColI col_i, ValI val_i,
                                               const int32_t* __restrict__ ix, const double* __restrict__ av,
-                                              const double* __restrict__ rs, int64_t bj, int64_t ej, int64_t i,
-                                              int32_t j, double rsi, double rsj, double& aij, double& t_mid,
-                                              double& t_lo, double& t_all) {
+                                              int64_t bj, int64_t ej, int64_t i, int32_t j, double rsi, double rsj,
+                                              double& aij, double& t_mid, double& t_lo, double& t_all) {
+    const double rij = rsi * rsj;
     for (int64_t b0 = bj; b0 < ej; b0 += kFeatChunk) {
         int32_t d[kFeatChunk];
-        double v[kFeatChunk], rk[kFeatChunk];
+        double v[kFeatChunk];
 #pragma unroll
         for (int u = 0; u < kFeatChunk; ++u) {
             const bool ok = b0 + u < ej;
@@ -186,20 +189,18 @@ __device__ __forceinline__ void feature_merge(int64_t a, int64_t ea, ColI col_i,
             v[u] = ok ? av[b0 + u] : 0.;
         }
 #pragma unroll
-        for (int u = 0; u < kFeatChunk; ++u) rk[u] = d[u] != 0x7fffffff ? rs[d[u]] : 0.;
-#pragma unroll
         for (int u = 0; u < kFeatChunk; ++u) {
             // (a padding entry, column 0x7fffffff, runs a to the end and matches nothing)
             while (a < ea && col_i(a) < d[u]) ++a;
             if (a < ea && col_i(a) == d[u]) {
                 const double va = val_i(a);
-                const double p = (va * rsi * rk[u]) * (v[u] * rsj * rk[u]);
+                const double p = (va * v[u]) * rij;
                 // (adds of 0 rather than branches: a branch between the two
                 // sums puts them in scratch memory)
                 t_all += p;
                 t_lo += d[u] < j ? p : 0.;
                 t_mid += (d[u] > j && d[u] < i) ? p : 0.;
-                if (d[u] == j) aij = va * rsi * rsj;
+                if (d[u] == j) aij = va * rsi / rsj;
                 ++a;
             }
         }
@@ -242,8 +243,10 @@ __global__ void __launch_bounds__(kFeatBlock) k_features(const int64_t* __restri
         const bool staged = count <= kFeatStage;
         if (staged)
             for (int64_t e = threadIdx.x; e < count; e += blockDim.x) {
-                sc[e] = ix[base + e];
-                sv[e] = av[base + e];
+                const int32_t c = ix[base + e];
+                const double r = rs[c];
+                sc[e] = c;
+                sv[e] = av[base + e] * (r * r);
             }
         __syncthreads();
         if (q < nnz_l) {
@@ -254,11 +257,16 @@ __global__ void __launch_bounds__(kFeatBlock) k_features(const int64_t* __restri
             double aij = 0., t_mid = 0., t_lo = 0., t_all = 0.;
             if (staged)
                 feature_merge(ipw[lo] - base, ipw[lo + 1] - base, [&](int64_t e) { return sc[e]; },
-                              [&](int64_t e) { return sv[e]; }, ix, av, rs, bj, ej, i, j, rsi, rsj, aij, t_mid, t_lo,
+                              [&](int64_t e) { return sv[e]; }, ix, av, bj, ej, i, j, rsi, rsj, aij, t_mid, t_lo,
                               t_all);
             else
-                feature_merge(ipw[lo], ipw[lo + 1], [&](int64_t e) { return ix[e]; }, [&](int64_t e) { return av[e]; },
-                              ix, av, rs, bj, ej, i, j, rsi, rsj, aij, t_mid, t_lo, t_all);
+                feature_merge(
+                    ipw[lo], ipw[lo + 1], [&](int64_t e) { return ix[e]; },
+                    [&](int64_t e) {
+                        const double r = rs[ix[e]];
+                        return av[e] * (r * r);
+                    },
+                    ix, av, bj, ej, i, j, rsi, rsj, aij, t_mid, t_lo, t_all);
             float f[kF];
             f[0] = float(aij);
             f[1] = (i == j) ? 1.f : 0.f;
