@@ -434,21 +434,32 @@ __device__ __forceinline__ float relu(float v) {
     return __builtin_bit_cast(float, max(__builtin_bit_cast(int32_t, v), 0));
 }
 
-__device__ __forceinline__ uint4 pack8(const v4f& a, const v4f& b, bool act) {
-    const float f[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
-    v8bf h;
-#pragma unroll
-    for (int s = 0; s < 8; ++s) h[s] = static_cast<__bf16>(act ? relu(f[s]) : f[s]);
-    return __builtin_bit_cast(uint4, h);
-}
-
 typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 // Two BF16 values in one 32-bit word, `lo` in the low half (one packed
 // round-to-nearest-even conversion).
+typedef float v2f __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t pack2(float lo, float hi) {
-    const v2bf h = {static_cast<__bf16>(lo), static_cast<__bf16>(hi)};
-    return __builtin_bit_cast(uint32_t, h);
+    const v2f f = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, v2bf));
 }
+
+typedef short v2s __attribute__((ext_vector_type(2)));
+
+// Eight values as BF16 (round to nearest even); with `act`, ReLU is applied
+// to the packed BF16 pairs as a signed 16-bit max with 0 (sign bit set =
+// negative), which equals converting relu(v).
+__device__ __forceinline__ uint4 pack8(const v4f& a, const v4f& b, bool act) {
+    uint4 u = make_uint4(pack2(a[0], a[1]), pack2(a[2], a[3]), pack2(b[0], b[1]), pack2(b[2], b[3]));
+    if (act) {
+        const v2s z = {0, 0};
+        u.x = __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(v2s, u.x), z));
+        u.y = __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(v2s, u.y), z));
+        u.z = __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(v2s, u.z), z));
+        u.w = __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(v2s, u.w), z));
+    }
+    return u;
+}
+
 
 // Storage order of hidden units (and of input features) in the backward
 // activation buffers: inside each block of 16 the two 2-bit fields are
@@ -530,14 +541,14 @@ __device__ __forceinline__ void tiles_forward_out(const uint2* __restrict__ w1f,
 #pragma unroll
     for (int ob = 0; ob < 8; ++ob) {
         const float4 b = b2s[ob * 4 + quad];
+        const v4f bias = {b.x, b.y, b.z, b.w};
         v4f c[T];
-#pragma unroll
-        for (int t = 0; t < T; ++t) c[t] = v4f{b.x, b.y, b.z, b.w};
 #pragma unroll
         for (int kb2 = 0; kb2 < 4; ++kb2) {
             const v8bf w = as_v8bf(w2f[(ob * 4 + kb2) * 64 + lane]);
+            // (the first step reads the bias as its accumulator, no copies)
 #pragma unroll
-            for (int t = 0; t < T; ++t) c[t] = mfma32(w, as_v8bf(h1b[t][kb2]), c[t]);
+            for (int t = 0; t < T; ++t) c[t] = mfma32(w, as_v8bf(h1b[t][kb2]), kb2 == 0 ? bias : c[t]);
         }
         const float4 w = w3s[ob * 4 + quad];
 #pragma unroll
