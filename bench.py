@@ -47,7 +47,9 @@ BYTES_PER_ELEMENT_C2 = 825.7  # algorithmic bytes per tet of one FP64 assembly
 FLOOR_BYTES_PER_ELEMENT_C2 = BYTES_PER_ELEMENT_C2 - 4. * 10 * 10
 # rocprofv3 FETCH_SIZE/WRITE_SIZE summaries of the other workloads at their default sizes
 TRAFFIC_FILES = {"th": "th_divergence_block_64_hbm_traffic.json",
-                 "c3": "c3_vector_p2_elasticity_48_hbm_traffic.json"}
+                 "c1": "c1_p1_laplace_32_hbm_traffic.json",
+                 "c3": "c3_vector_p2_elasticity_48_hbm_traffic.json",
+                 "c4": "c4_n1_curlcurl_96_hbm_traffic.json"}
 
 
 def log(msg):
