@@ -187,6 +187,12 @@ def batched_fsai(systems, lo, hi, steps, warmup, timer):
         learned_s = time.perf_counter() - t0
         out = {"jacobi": rj, "jac_s": jac_s, "learned": rl, "learned_s": learned_s, "setup_s": setup_s}
     rj, rl = out["jacobi"], out["learned"]
+    # measured HBM bytes of one batched assembly (committed rocprofv3 summary)
+    batch_traffic = None
+    tpath = os.path.join(ROOT, "profiles", "c5_batched_p1_256_hbm_traffic.json")
+    if systems == 256 and (lo, hi) == (16, 48) and os.path.exists(tpath):
+        with open(tpath) as f:
+            batch_traffic = json.load(f)["per_assembly_bytes"]
     # the MLP kernels on the batch's lower nonzeros: inference (part of the
     # setup above) and one training backward pass (activation kernel, split-K
     # weight gradients, reduction).  MFMA flops per nonzero: forward
@@ -218,7 +224,8 @@ def batched_fsai(systems, lo, hi, steps, warmup, timer):
                     "position maps, replayed as one HIP graph",
         "assembly_roofline": {"bound": "hbm", "bytes": alg, "achieved": alg / asm_s / 1e9, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": alg / asm_s / 1e9 / HBM_PEAK_GBS,
-                              "bytes_definition": "24 V + (16 conn + 72 tensor + 64 int32 map) E + 8 nnz + 8 N"},
+                              "bytes_definition": "24 V + (16 conn + 72 tensor + 64 int32 map) E + 8 nnz + 8 N",
+                              "traffic": batch_traffic},
         "setup_s": setup,
         "jacobi_pcg": {"wall_s": out["jac_s"], "batch_iterations": rj.batch_iterations,
                        "iterations_mean": float(rj.iterations.mean()), "iterations_max": int(rj.iterations.max()),
