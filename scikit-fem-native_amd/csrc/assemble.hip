@@ -368,7 +368,18 @@ struct PayloadArgs {
     const double* wtab;  // N1: quadrature weights (their sum scales the curl term)
     int32_t Q;
     double* out;
+    // element tables of the table gather (built by block 0 when ntab is set)
+    const double* rtab;
+    int32_t tab_n, tab_qg, tab_qm;
+    double* ntab;
+    float* ntab32;
 };
+
+// Element-table entry t = a * n + b of n-node elements from the reduced
+// quadrature tables (wg | dg | wm | pm): the six D slots and the mass, FP64
+// and FP32.  kNK values per entry (defined with the table gather below).
+__device__ inline void build_table_entry(const double* __restrict__ rtab, int n, int32_t Qg, int32_t Qm, int t,
+                                         double* __restrict__ out, float* __restrict__ out32);
 
 // Storage of the symmetric scalar payload (C == 1, SYM): 8 doubles (D upper
 // triangle, |det| m, pad), 6 doubles (D only: forms without a mass term) or
@@ -411,6 +422,11 @@ __global__ void __launch_bounds__(kPayloadBlock) k_payload_h1(PayloadArgs A) {
     __shared__ __align__(16) unsigned char smem[kTab > kBuf ? kTab : kBuf];
     V* buf = reinterpret_cast<V*>(smem);
     const bool stage = C == 1 && A.cA != nullptr && (reinterpret_cast<uintptr_t>(A.cA) & 15) == 0;
+    // the table gather's element tables ride along in block 0 (no kernel of
+    // their own); the gather runs after this kernel in stream order
+    if (A.ntab != nullptr && blockIdx.x == 0)
+        for (int t = threadIdx.x; t < A.tab_n * A.tab_n; t += kPayloadBlock)
+            build_table_entry(A.rtab, A.tab_n, A.tab_qg, A.tab_qm, t, A.ntab, A.ntab32);
     for (int64_t e0 = int64_t(blockIdx.x) * kPayloadBlock; e0 < A.E; e0 += int64_t(gridDim.x) * kPayloadBlock) {
         const int n = int(A.E - e0 < kPayloadBlock ? A.E - e0 : kPayloadBlock);
         double Xs[9];
@@ -1527,32 +1543,24 @@ __global__ void __launch_bounds__(64) k_gather_vec10_mfma(GatherArgs A) {
 // lanes' random (a, b) rows over the LDS banks
 constexpr int kNK = 7;
 
-// Element-table entry t = a * NL + b from the reduced quadrature tables
-// (wg | dg | wm | pm), kNK values: the six D slots and the mass.  Each
-// table-gather block builds its own copy in LDS.
-template <int NL, typename T>
-__device__ inline void build_table_entry(const double* __restrict__ rtab, int32_t Qg, int32_t Qm, int t,
-                                         T* __restrict__ out) {
-    const int a = t / NL, b = t % NL;
+__device__ inline void build_table_entry(const double* __restrict__ rtab, int n, int32_t Qg, int32_t Qm, int t,
+                                         double* __restrict__ out, float* __restrict__ out32) {
+    const int a = t / n, b = t % n;
     const double* wg = rtab;
     const double* dg = wg + Qg;
-    const double* wm = dg + Qg * NL * 3;
+    const double* wm = dg + Qg * n * 3;
     const double* pm = wm + Qm;
     double M[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
     for (int q = 0; q < Qg; ++q)
-#pragma unroll
         for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) M[i][j] += wg[q] * dg[(q * NL + a) * 3 + i] * dg[(q * NL + b) * 3 + j];
+            for (int j = 0; j < 3; ++j) M[i][j] += wg[q] * dg[(q * n + a) * 3 + i] * dg[(q * n + b) * 3 + j];
     double mass = 0.;
-    for (int q = 0; q < Qm; ++q) mass += wm[q] * pm[q * NL + a] * pm[q * NL + b];
-    out[t * kNK + 0] = T(M[0][0]);
-    out[t * kNK + 1] = T(M[0][1] + M[1][0]);
-    out[t * kNK + 2] = T(M[0][2] + M[2][0]);
-    out[t * kNK + 3] = T(M[1][1]);
-    out[t * kNK + 4] = T(M[1][2] + M[2][1]);
-    out[t * kNK + 5] = T(M[2][2]);
-    out[t * kNK + 6] = T(mass);
+    for (int q = 0; q < Qm; ++q) mass += wm[q] * pm[q * n + a] * pm[q * n + b];
+    const double v[kNK] = {M[0][0], M[0][1] + M[1][0], M[0][2] + M[2][0], M[1][1], M[1][2] + M[2][1], M[2][2], mass};
+    for (int k = 0; k < kNK; ++k) {
+        out[t * kNK + k] = v[k];
+        out32[t * kNK + k] = float(v[k]);
+    }
 }
 
 template <int NL, bool F32, int RB, int OFFW, bool MASS>
@@ -1594,8 +1602,8 @@ __global__ void __launch_bounds__(64 * WPB) k_gather_table(GatherArgs A, int32_t
     constexpr int kTab = NL * NL * kNK;
     constexpr int kTabDoubles = ((kTab * int(sizeof(T)) + 15) / 16) * 2;
     T* ntab = reinterpret_cast<T*>(lds);
-    // the element table from the reduced tables (A.tab), rounded to T
-    for (int t = threadIdx.x; t < NL * NL; t += 64 * WPB) build_table_entry<NL>(A.tab, A.Qg, A.Qm, t, ntab);
+    const T* src = F32 ? reinterpret_cast<const T*>(A.tab32) : reinterpret_cast<const T*>(A.tab);
+    for (int k = threadIdx.x; k < kTab; k += 64 * WPB) ntab[k] = src[k];
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
     // block placement (XCD chunks of blocks when the plan asks for chunking)
@@ -2422,9 +2430,11 @@ __global__ void k_scatter_mapped(const double* __restrict__ src, const int32_t* 
 }
 
 // Scratch of the element-constant path: the per-element payload (E x
-// stride doubles).
+// stride doubles) and the element tables of the table gather (FP64, FP32).
 struct Workspace {
     double* payload = nullptr;
+    double* ntab = nullptr;
+    float* ntab32 = nullptr;
 };
 
 template <int NL, int C, bool F32>
@@ -2454,9 +2464,14 @@ static int launch_gather(skfa_plan* P, const GatherArgs& A, bool sym, const Work
     // element tables win for 4-node elements (few table rows, long column
     // split); the quadrature gather with uniform table reads wins for P2
     if (sym && table_gather(P)) {
-        // the table-driven gather (each block builds the element tables from
-        // the reduced quadrature tables)
-        const GatherArgs& T = A;
+        // element tables from the reduced quadrature tables, then the
+        // table-driven gather
+        // (the payload pass built the element tables from the reduced
+        // quadrature tables into W.ntab / W.ntab32)
+        SKFA_REQUIRE(W.ntab != nullptr && W.ntab32 != nullptr, "element-table buffers are missing");
+        GatherArgs T = A;
+        T.tab = W.ntab;
+        T.tab32 = W.ntab32;
         if constexpr (NL == 4) {
             if (A.load_out != nullptr)
                 return A.has_mass ? launch_table<NL, F32, true, true>(P, T, s)
@@ -2581,11 +2596,24 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
         // the FP32-products vector-P2 gather reads a float payload (16 floats)
         const bool vec32 = P->C == 3 && P->n == 10 && nqg == 4 && precision == 1;
         const int stride = P->C == 1 ? (sym ? 8 : 10) : (vec32 ? 8 : 14);
+        // the table gather runs for symmetric scalar forms unless the MFMA
+        // gather takes them (launch_gather)
+        const bool table = P->C == 1 && sym && table_gather(P) && !(!diag3 && mfma_gather(P, nqg, nqm));
+        const size_t nt = size_t(P->n) * P->n * kNK;
         if (scratch == nullptr) {
             if (int rc = ensure_payload(P, stride)) return rc;
+            if (table && P->ntab.n < nt)
+                if (int rc = P->ntab.alloc(nt)) return rc;
+            if (table && P->ntab32.n < nt)
+                if (int rc = P->ntab32.alloc(nt)) return rc;
             W.payload = P->payload.p;
+            W.ntab = P->ntab.p;
+            W.ntab32 = P->ntab32.p;
         } else {
-            const size_t need = size_t(P->E) * stride * sizeof(double);
+            // 256-byte aligned pieces: payload | FP64 tables | FP32 tables
+            auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+            const size_t pay = up(size_t(P->E) * stride * sizeof(double));
+            const size_t need = pay + (table ? up(nt * sizeof(double)) + up(nt * sizeof(float)) : 0);
             if (need > scratch_bytes) {
                 set_error("the element payload of " + std::to_string(P->E) + " elements needs " +
                           std::to_string(need) + " bytes, more than the " + std::to_string(scratch_bytes) +
@@ -2593,6 +2621,10 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
                 return kCapacity;
             }
             W.payload = reinterpret_cast<double*>(scratch);
+            if (table) {
+                W.ntab = reinterpret_cast<double*>(scratch + pay);
+                W.ntab32 = reinterpret_cast<float*>(scratch + pay + up(nt * sizeof(double)));
+            }
         }
         PayloadArgs pa{};
         pa.xyz = reinterpret_cast<const double4*>(xyz);
@@ -2600,6 +2632,14 @@ static int assemble_bilinear_ws(skfa_plan* P, const double* xyz, const int32_t* 
         pa.E = P->E;
         pa.cm = c_value; pa.cg = c_gradient; pa.cs = c_symgrad; pa.cl = c_divergence; pa.cA = c_tensor;
         pa.out = W.payload;
+        if (table) {
+            pa.rtab = rtab;
+            pa.tab_n = P->n;
+            pa.tab_qg = nqg;
+            pa.tab_qm = nqm;
+            pa.ntab = W.ntab;
+            pa.ntab32 = W.ntab32;
+        }
         const int lay = sym ? sym_layout(P, nqg, nqm, precision == 1, c_value != nullptr) : PL_FULL;
         if (P->C == 1 && lay == PL_SYM32)
             k_payload_h1<1, true, SYMOUT_F32><<<elem_grid(P->E), 256, 0, s>>>(pa);

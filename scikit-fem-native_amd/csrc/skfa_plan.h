@@ -75,6 +75,9 @@ struct skfa_plan {
     // maps when the load is not fused into the gather (first use)
     skfa::DevBuf<double> full_values;
     skfa::DevBuf<double> full_load;
+    // element tables of the symmetric scalar gather (FP64 and FP32 copies)
+    skfa::DevBuf<double> ntab;
+    skfa::DevBuf<float> ntab32;
 
     // launch tuning (skfa_plan_set_tuning): XCD chunk of waves (0 = plain
     // order) and LDS accumulation by ds_add (1) or read-add-write (0)
