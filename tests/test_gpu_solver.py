@@ -221,6 +221,32 @@ def test_mlp_forward_backward_match_oracle():
             assert rel_frob(g[a:b], gref[a:b]) <= 5e-3, (nnz, name, rel_frob(g[a:b], gref[a:b]))
 
 
+def test_mlp_large_batch_matches_oracle():
+    """300,001 rows: past the inference grid's 2,048 blocks (262,144 rows per
+    pass, so the grid-stride loop runs twice) and the backward kernels' block
+    caps (512 activation blocks = 65,536 rows, 512 weight-gradient blocks of
+    eight chunks), with a ragged last chunk."""
+    import torch
+    from skfemntv.fsai import FSAINetwork
+    print("case: BF16 MFMA MLP forward/backward, 300001 rows", flush=True)
+    flat = _random_params(7)
+    net = FSAINetwork(params=flat)
+    nnz = 300_001
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((nnz, 16)).astype(np.float32)
+    go = (rng.standard_normal(nnz) / nnz).astype(np.float32)
+    Xd = torch.from_numpy(X).cuda()
+    out = net.forward(Xd, nnz).cpu().numpy()[:nnz].astype(np.float64)
+    ref, _ = of.mlp_forward(flat.astype(np.float64), X.astype(np.float64))
+    assert rel_frob(out, ref) <= 1e-3
+    grad, _ = net.backward(Xd, torch.from_numpy(go).cuda(), nnz)
+    g = grad.cpu().numpy().astype(np.float64)
+    gref = of.mlp_backward(flat.astype(np.float64), X.astype(np.float64), go.astype(np.float64))
+    for name, a, b in (("W1", 0, 2048), ("b1", 2048, 2176), ("W2", 2176, 18560), ("b2", 18560, 18688),
+                       ("w3", 18688, 18816), ("b3", 18816, 18817)):
+        assert rel_frob(g[a:b], gref[a:b]) <= 5e-3, (name, rel_frob(g[a:b], gref[a:b]))
+
+
 def test_mlp_backward_deterministic():
     import torch
     from skfemntv.fsai import FSAINetwork
