@@ -186,6 +186,35 @@ def test_features_and_kaporin_match_oracle(oracle_blocks):
     np.testing.assert_allclose(g, of.factor_values(o.astype(np.float64), lptr, lidx, A), rtol=1e-6)
 
 
+def test_features_and_factor_transpose_on_dense_rows():
+    """A dense 120 x 120 SPD matrix: rows of 120 entries take several chunks
+    of the 16-lane pattern kernels, and the first feature blocks span more A
+    entries than the kernel stages in LDS, so their rows i are read from
+    global memory (the other path)."""
+    import torch
+    from skfemntv.fsai import FSAISystems
+    from skfemntv.solver import DeviceCSR, FactorPreconditioner
+    print("case: FSAI features and factor transpose, dense 120 x 120", flush=True)
+    rng = np.random.default_rng(21)
+    n = 120
+    M = rng.standard_normal((n, n))
+    A = sp.csr_matrix(M @ M.T + n * np.eye(n))
+    xyz = rng.random((n, 3))
+    fs = FSAISystems(DeviceCSR.from_scipy(A), [0, n], torch.from_numpy(xyz).cuda())
+    lptr, lidx, F = of.features(A, xyz)
+    np.testing.assert_array_equal(fs.pattern.lptr.cpu().numpy(), lptr)
+    np.testing.assert_array_equal(fs.pattern.lidx.cpu().numpy(), lidx)
+    Fg = fs.features[:fs.nnz].cpu().numpy().astype(np.float64)
+    np.testing.assert_allclose(Fg, F, rtol=2e-6, atol=2e-7)
+    o = (0.3 * rng.standard_normal(len(lidx))).astype(np.float32)
+    g = fs.factor(torch.from_numpy(o).cuda())
+    P = FactorPreconditioner(fs.pattern, g)
+    G = sp.csr_matrix((g.cpu().numpy().astype(np.float64), lidx, lptr), shape=(n, n)).toarray()
+    r = rng.standard_normal(n)
+    z = P.apply(torch.from_numpy(r).cuda()).cpu().numpy()
+    np.testing.assert_allclose(z, G.T @ (G @ r), rtol=1e-12, atol=1e-12 * np.abs(G.T @ (G @ r)).max())
+
+
 def _random_params(seed):
     from skfemntv.fsai import init_params
     flat = init_params(seed).astype(np.float64)
