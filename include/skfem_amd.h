@@ -355,7 +355,10 @@ int skfa_assemble_facet_bilinear(const skfa_plan* plan, const int32_t* conn, con
  *           (P, Cr, [row_dim], Cc, [col_dim]) tensor.
  *   one_lane_per_row  1 when a cell repeats a column node (one lane then adds
  *           all of a row's entries), else 0.
- * Each entry adds its cells in cell order; results are deterministic. */
+ * Each entry adds its cells in cell order; results are deterministic.  Value /
+ * gradient terms with one_lane_per_row = 0 form per-cell node matrices
+ * (cells x n_row x n_col FP64) in a buffer the plan keeps, allocated on the
+ * first such call: calls on one plan must not overlap. */
 #define SKFA_CUT_VALUE 1
 #define SKFA_CUT_GRADIENT 2
 #define SKFA_CUT_TENSOR 4

@@ -13,8 +13,10 @@
 // node-level pattern and, per incidence, the in-row offsets of the cell's
 // column nodes.  One 64-lane wave owns G consecutive row nodes; each has a
 // lane group of 64/G lanes and lane j of a group owns the cell's column
-// nodes j, j + 64/G, ...  For each incidence the lane sums its entries over
-// the cell's points and adds them into the wave's CSR span in LDS.  A
+// nodes j, j + 64/G, ...  For each incidence the lane adds its entries (for
+// value / gradient terms the cell's node matrix, formed beforehand one
+// wave per cell on the matrix cores; for tensor terms and cells that repeat a node, its sum
+// over the cell's points) into the wave's CSR span in LDS.  A
 // lane's entries have distinct addresses (distinct column nodes; cells that
 // repeat a node run with one lane per row), groups own disjoint spans, and
 // every entry adds its incidences in cell order, so results are
@@ -128,76 +130,131 @@ __global__ void __launch_bounds__(64) k_cut_rows(CutArgs A) {
     for (int64_t k = lane; k < span_len; k += 64) A.values[lo * cc + k] = span[k];
 }
 
-// Point-parallel form of k_cut_rows for the value / gradient terms (cells
-// of at most 16 column nodes, no repeated node): one wave owns one row node;
-// lane (s, b), s = lane / 16, takes column node b and the cell's points
-// s, s + 4, ... of each incidence, so the four slots load a cell's points
-// together and each lane keeps one accumulator; two shuffle levels (masks
-// 16, 32, a fixed order) add the four slots' sums and slot 0 adds entry b
-// to the span.  Incidences run in cell order; the next incidence's cell,
-// point range and column offset are loaded while one computes.  Cut band of
-// the 64^3 P2 cube (55,571 cells, 12 points each): 0.605 ms, against 0.859
-// ms for k_cut_rows (lane per column node, the points of each incidence in
-// sequence) and 1.42 ms for a 16-lane point-parallel form with a 16-column
-// reduce-scatter (196 VGPRs).
-constexpr int kCutSlots = 4;
+// Value / gradient terms on cells without repeated nodes, in two passes.
+// k_cut_cell_matrices: one wave per cell forms the cell's node matrix
+//   M_e[a][b] = sum_q w_q (value_q phi_a(q) psi_b(q)
+//                          + gradient_q grad phi_a(q) . grad psi_b(q))
+// (the per-point entry of cut_form_assembler.cpp:182-190, summed per cell)
+// as a product of a (n_row x 4Q) and a (4Q x n_col) matrix, one FP64
+// 16x16x4 MFMA per point: lane l supplies slice k = l / 16 (0 the value
+// term, 1..3 a gradient component) of row a = l % 16 of the left operand,
+//   w_q value_q phi_a(q)  or  w_q gradient_q d_k phi_a(q),
+// and of column b = l % 16 of the right operand, psi_b(q) or d_k psi_b(q),
+// and receives entries M[l / 16 + 4r][l % 16].  Each lane loads one value
+// of each operand per point, so a cell's point rows pass through the cache
+// once instead of once per entry.  NP points issue their loads together.
+// k_cut_rows_cells: a lane group of `group` lanes per row node, lane b
+// owning column node b; each incidence adds M_e[a][b] to the row's span
+// entry (diagonal component pairs).  kCutBatch incidences of a row issue
+// their record loads together and then their matrix loads together, so a
+// row pays two memory round trips per batch instead of one per point.  The
+// adds of a row run in incidence (cell) order, one lane per address.
+typedef double cut_v4d __attribute__((ext_vector_type(4)));
+constexpr int kCutBatch = 8;
 
-__global__ void __launch_bounds__(64) k_cut_rows_points(CutArgs A) {
-    extern __shared__ double span[];
-    const int lane = threadIdx.x;
-    const int slot = lane >> 4, b = lane & 15;
-    const int64_t r = blockIdx.x;
-    const int64_t cc = int64_t(A.Cr) * A.Cc;
-    const int64_t lo = A.nptr[r];
-    const int64_t len = A.nptr[r + 1] - lo;
-    const int64_t span_len = len * cc;
-    for (int64_t k = lane; k < span_len; k += 64) span[k] = 0.;
-    __syncthreads();
+__device__ inline double read_lane(double x, int l) {
+    const int64_t bits = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane(int(bits), l);
+    const int hi = __builtin_amdgcn_readlane(int(bits >> 32), l);
+    return __longlong_as_double((int64_t(hi) << 32) | uint32_t(lo));
+}
+
+// SAME: row and column tables are the same arrays (square forms), so the
+// right operand is the left one before its point factor.
+template <bool SAME, int NP>
+__global__ void __launch_bounds__(256) k_cut_cell_matrices(CutArgs A, double* cell_mat, int64_t cells) {
+    const int lane = threadIdx.x & 63;
+    const int64_t e = int64_t(blockIdx.x) * 4 + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    if (e >= cells) return;
+    const int i = lane & 15, k = lane >> 4;
     const bool has_value = (A.terms & SKFA_CUT_VALUE) != 0, has_grad = (A.terms & SKFA_CUT_GRADIENT) != 0;
-    const bool col = b < A.nc;
-    const int64_t kb = A.inc_ptr[r], ke = A.inc_ptr[r + 1];
-    // incidence k + 1's cell, point range and column offset load while k computes
-    auto fetch = [&](int64_t k, int32_t& packed, int64_t& q0, int64_t& q1, int& o) {
-        if (k < ke) {
-            const uint8_t* rec = A.records + k * A.rec_bytes;
-            packed = *reinterpret_cast<const int32_t*>(rec);
-            q0 = A.offsets[packed >> 4];
-            q1 = A.offsets[(packed >> 4) + 1];
-            o = col ? rec_offset(rec, A.off_bytes, b) : 0;
+    const bool a_on = i < A.nr && (k == 0 ? has_value : (has_grad && k <= A.Dr));
+    const bool b_on = i < A.nc && (k == 0 ? has_value : (has_grad && k <= A.Dc));
+    const double* ah = k == 0 ? A.rshape + i : A.rgrad + i * A.Dr + (k - 1);
+    const int64_t as = k == 0 ? A.nr : int64_t(A.nr) * A.Dr;
+    const double* bh = k == 0 ? A.cshape + i : A.cgrad + i * A.Dc + (k - 1);
+    const int64_t bs = k == 0 ? A.nc : int64_t(A.nc) * A.Dc;
+    const int64_t q0 = A.offsets[e], q1 = A.offsets[e + 1];
+    cut_v4d acc = {0., 0., 0., 0.};
+    for (int64_t qc = q0; qc < q1; qc += 64) {
+        // point qc + lane's factors w value and w gradient, read by every
+        // lane below through readlane
+        const int64_t ql = qc + lane;
+        double fv = 0., fg = 0.;
+        if (ql < q1) {
+            const double w = A.weights[ql];
+            fv = w * (A.value ? A.value[ql] : 1.);
+            fg = w * (A.gradient ? A.gradient[ql] : 1.);
         }
-    };
-    int32_t pk_n = 0;
-    int64_t q0_n = 0, q1_n = 0;
-    int o_n = 0;
-    fetch(kb, pk_n, q0_n, q1_n, o_n);
-    for (int64_t k = kb; k < ke; ++k) {
-        const int32_t packed = pk_n;
-        const int64_t q0 = q0_n, q1 = q1_n;
-        const int o = o_n;
-        fetch(k + 1, pk_n, q0_n, q1_n, o_n);
-        const int a = packed & 15;
-        double acc = 0.;
-        if (col) {
-            for (int64_t q = q0 + slot; q < q1; q += kCutSlots) {
-                const double w = A.weights[q];
-                double t = 0.;
-                if (has_value) t = w * (A.value ? A.value[q] : 1.) * A.rshape[q * A.nr + a] * A.cshape[q * A.nc + b];
-                if (has_grad) {
-                    const double* ga = A.rgrad + (q * A.nr + a) * A.Dr;
-                    const double* gb = A.cgrad + (q * A.nc + b) * A.Dc;
-                    double dot = 0.;
+        const int64_t qe = min(qc + 64, q1);
+        for (int64_t qb = qc; qb < qe; qb += NP) {
+            double hv[NP], bv[SAME ? 1 : NP];
 #pragma unroll
-                    for (int d = 0; d < 3; ++d)
-                        if (d < A.Dr) dot = fma(ga[d], gb[d], dot);
-                    t = fma(w * (A.gradient ? A.gradient[q] : 1.), dot, t);
+            for (int j = 0; j < NP; ++j) {
+                const int64_t q = qb + j;
+                hv[j] = 0.;
+                if (!SAME) bv[j] = 0.;
+                if (q < qe) {
+                    if (a_on) hv[j] = ah[q * as];
+                    if (!SAME && b_on) bv[j] = bh[q * bs];
                 }
-                acc += t;
+            }
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                const int src = int(qb - qc) + j;   // uniform; beyond qe the operands are 0
+                const double f = k == 0 ? read_lane(fv, src & 63) : read_lane(fg, src & 63);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(f * hv[j], SAME ? hv[j] : bv[SAME ? 0 : j], acc, 0, 0, 0);
             }
         }
-        acc += __shfl_xor(acc, 16);
-        acc += __shfl_xor(acc, 32);
-        if (slot == 0 && col)
-            for (int c = 0; c < A.Cr; ++c) span[(int64_t(c) * len + o) * A.Cc + c] += acc;
+    }
+    double* out = cell_mat + e * (int64_t(A.nr) * A.nc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int a = k + 4 * r;
+        if (a < A.nr && i < A.nc) out[a * A.nc + i] = acc[r];
+    }
+}
+
+__global__ void __launch_bounds__(64) k_cut_rows_cells(CutArgs A, const double* cell_mat) {
+    extern __shared__ double span[];
+    const int lane = threadIdx.x;
+    const int slot = lane / A.group;
+    const int b = lane - slot * A.group;
+    const int64_t r0 = int64_t(blockIdx.x) * A.rows_per_wave;
+    const int64_t r1 = min(r0 + int64_t(A.rows_per_wave), A.row_nodes);
+    const int64_t cc = int64_t(A.Cr) * A.Cc;
+    const int64_t lo = A.nptr[r0];
+    const int64_t span_len = (A.nptr[r1] - lo) * cc;
+    for (int64_t k = lane; k < span_len; k += 64) span[k] = 0.;
+    __syncthreads();
+    const int64_t r = r0 + slot;
+    if (r < r1 && b < A.nc) {
+        const int64_t len = A.nptr[r + 1] - A.nptr[r];
+        double* rs = span + (A.nptr[r] - lo) * cc;
+        const int64_t nn = int64_t(A.nr) * A.nc;
+        const int64_t kb = A.inc_ptr[r], ke = A.inc_ptr[r + 1];
+        for (int64_t k = kb; k < ke; k += kCutBatch) {
+            int32_t pk[kCutBatch];
+            int o[kCutBatch];
+            double m[kCutBatch];
+#pragma unroll
+            for (int j = 0; j < kCutBatch; ++j) {
+                pk[j] = 0;
+                o[j] = 0;
+                if (k + j < ke) {
+                    const uint8_t* rec = A.records + (k + j) * A.rec_bytes;
+                    pk[j] = *reinterpret_cast<const int32_t*>(rec);
+                    o[j] = rec_offset(rec, A.off_bytes, b);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kCutBatch; ++j)
+                m[j] = k + j < ke ? cell_mat[int64_t(pk[j] >> 4) * nn + (pk[j] & 15) * A.nc + b] : 0.;
+#pragma unroll
+            for (int j = 0; j < kCutBatch; ++j)
+                if (k + j < ke)
+                    for (int c = 0; c < A.Cr; ++c) rs[(int64_t(c) * len + o[j]) * A.Cc + c] += m[j];
+        }
     }
     __syncthreads();
     for (int64_t k = lane; k < span_len; k += 64) A.values[lo * cc + k] = span[k];
@@ -317,16 +374,28 @@ int skfa_assemble_cut_bilinear(const skfa_cross_plan* P, const int64_t* offsets,
     A.col_grad = col_kind == SKFA_CROSS_GRADIENT;
     A.tensor = tensor;
     A.values = values;
-    // value / gradient terms on cells of at most 16 column nodes without
-    // repeated nodes: the point-parallel kernel (one row node per wave)
-    if (terms != SKFA_CUT_TENSOR && !one_lane_per_row && P->n_col <= 16) {
-        SKFA_REQUIRE(P->row_nodes <= int64_t(INT32_MAX), "too many row nodes");
-        k_cut_rows_points<<<dim3(unsigned(P->row_nodes)), dim3(64), size_t(row_bytes), s>>>(A);
+    const int64_t waves = (P->row_nodes + rows - 1) / rows;
+    SKFA_REQUIRE(waves <= int64_t(INT32_MAX), "too many row nodes");
+    // value / gradient terms on cells without repeated nodes: per-cell node
+    // matrices, then the row gather (a lane per column node)
+    if (terms != SKFA_CUT_TENSOR && !one_lane_per_row) {
+        const int64_t total = P->E * int64_t(P->n_row) * P->n_col;
+        const int64_t blocks = (P->E + 3) / 4;   // a wave per cell
+        SKFA_REQUIRE(blocks <= int64_t(INT32_MAX), "too many cut cells");
+        if (P->cell_matrices.n < size_t(total)) {
+            const int rc = P->cell_matrices.alloc(size_t(total));
+            if (rc != kOk) return rc;
+        }
+        const bool same = row_shape == col_shape && row_grad == col_grad && P->n_row == P->n_col && row_dim == col_dim;
+        if (same)
+            k_cut_cell_matrices<true, 8><<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(A, P->cell_matrices.p, P->E);
+        else
+            k_cut_cell_matrices<false, 8><<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(A, P->cell_matrices.p, P->E);
+        SKFA_LAUNCH_CHECK();
+        k_cut_rows_cells<<<dim3(unsigned(waves)), dim3(64), size_t(rows * row_bytes), s>>>(A, P->cell_matrices.p);
         SKFA_LAUNCH_CHECK();
         return kOk;
     }
-    const int64_t waves = (P->row_nodes + rows - 1) / rows;
-    SKFA_REQUIRE(waves <= int64_t(INT32_MAX), "too many row nodes");
     k_cut_rows<<<dim3(unsigned(waves)), dim3(64), size_t(rows * row_bytes), s>>>(A);
     SKFA_LAUNCH_CHECK();
     return kOk;

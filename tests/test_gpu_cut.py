@@ -95,6 +95,40 @@ def test_cut_cross_matches_oracle(rk, ck, Cr, Cc, tensor):
         assert rel(A.values, ref1) <= FP64_TOL
 
 
+@pytest.mark.parametrize("quadratic,dim", [(True, 3), (False, 2)])
+def test_cut_cell_matrices_ragged_point_counts(quadratic, dim):
+    """Point counts 0..140 per cell cross the node-matrix pass's 8-point
+    batches and 64-point chunks; the square form (one table for rows and
+    columns) and the cross form (two tables) take different kernels."""
+    import skfemntv as sk
+    say(f"cut node matrices, ragged counts quadratic={quadratic} dim={dim}")
+    dofs, off, sh, gr, w = cut.random_case(quadratic, 1, seed=50 + dim, n=2, dim=dim, max_points=140)
+    counts = np.diff(off)
+    special = [1, 7, 8, 9, 15, 16, 17, 63, 64, 65, 128, 129]
+    counts[:len(special)] = special
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    rng = np.random.default_rng(5)
+    P = int(off[-1])
+    nodes = dofs.shape[1]
+    sh, gr, w = rng.standard_normal((P, nodes)), rng.standard_normal((P, nodes, dim)), rng.uniform(0.1, 1., P)
+    v, g = rng.uniform(0.5, 2., P), rng.uniform(0.5, 2., P)
+    A = sk.CutBilinearFormAssembler(dofs, off, sh, gr, w)
+    for value, gradient in ((v, None), (None, g), (v, g)):
+        A.assemble(value, gradient)
+        _, _, ref = cut.cut_bilinear(dofs, off, sh, gr, w, value, gradient)
+        assert rel(A.values, ref) <= FP64_TOL
+    # cross form: P1 rows against the same cells' P2 columns, two tables
+    rd, _, _, _, _ = cut.random_case(False, 1, seed=60, n=2)
+    cd, _, _, _, _ = cut.random_case(True, 1, seed=60, n=2)
+    rs, rgr = rng.standard_normal((P, 4)), rng.standard_normal((P, 4, dim))
+    cs, cgr = rng.standard_normal((P, 10)), rng.standard_normal((P, 10, dim))
+    X = sk.CutCrossAssembler(rd, cd, off, rs, cs, w, rgr, cgr)
+    for kind in ("value", "gradient"):
+        X.assemble(v, kind, kind)
+        _, _, ref = cut.cut_cross(rd, cd, off, rs, cs, w, rgr, cgr, v, kind, kind)
+        assert rel(X.values, ref) <= FP64_TOL
+
+
 def test_cut_edge_cases():
     import skfemntv as sk
     say("cut edge cases: many points, repeated node, no points")
