@@ -348,15 +348,17 @@ __device__ __forceinline__ double stream_rows(const int64_t* __restrict__ ip, co
     const int64_t rb = has_row ? ip[row] : 0, re = has_row ? ip[row + 1] : 0;
     const int cnt = int(k1 - k0);
     const int nt = int(blockDim.x);
-    // eight independent loads per thread in flight
+    // eight independent loads per thread in flight; the matrix is read once
+    // per product, so its loads are non-temporal and leave the caches to the
+    // vectors the products gather
     for (int base = int(threadIdx.x); base < cnt; base += 8 * nt) {
         int32_t j[8];
         double v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int k = base + u * nt;
-            j[u] = k < cnt ? column_of(ix[k0 + k], r0) : 0;
-            v[u] = k < cnt ? double(av[k0 + k]) : 0.;
+            j[u] = k < cnt ? column_of(__builtin_nontemporal_load(ix + k0 + k), r0) : 0;
+            v[u] = k < cnt ? double(__builtin_nontemporal_load(av + k0 + k)) : 0.;
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
