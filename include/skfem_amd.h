@@ -373,7 +373,10 @@ int skfa_assemble_cut_bilinear(const skfa_cross_plan* plan, const int64_t* offse
  * grad phi_a(q) . G_c(q)).  Replaces CutLinearFormAssembler::assemble
  * (cpp/src/assembly/cut_form_assembler.cpp:67-134).
  *   value     (C) constant or (P, C) per point (value_per_point = 1); NULL = no term
- *   gradient  (C, dim) constant or (P, C, dim) per point; NULL = no term */
+ *   gradient  (C, dim) constant or (P, C, dim) per point; NULL = no term
+ * Per-cell node vectors (cells x n x C FP64) go through the plan's cut-cell
+ * buffer (allocated or grown on first use): calls on one plan must not
+ * overlap. */
 int skfa_assemble_cut_linear(const skfa_cross_plan* plan, const int64_t* offsets, const double* weights,
                              const double* shape, const double* grad, int32_t dim, const double* value,
                              int32_t value_per_point, const double* gradient, int32_t gradient_per_point,

@@ -277,30 +277,58 @@ struct CutLinearArgs {
     double* out;             // (row_nodes * C)
 };
 
-// One lane per DOF row (node * C + c): the sum over the node's incidences in
-// cell order of sum_q w_q (phi_a(q) f_c(q) + grad phi_a(q) . G_c(q)).
-__global__ void __launch_bounds__(256) k_cut_linear(CutLinearArgs A) {
+// Load vector in two passes, as the bilinear value / gradient terms.
+// k_cut_cell_vectors: one thread per (cell, node a, component c) sums
+//   b_e[a][c] = sum_q w_q (phi_a(q) f_c(q) + grad phi_a(q) . G_c(q))
+// over the cell's points in order (the per-point contribution of
+// cut_form_assembler.cpp:124-137, summed per cell); threads of one cell
+// read different shape rows, so the point tables pass through once.
+// k_cut_linear_rows: one lane per DOF row (node * C + c) adds b_e[a][c]
+// over the node's incidences in cell order, kCutBatch incidences' records
+// and cell values loaded together.
+__global__ void __launch_bounds__(256) k_cut_cell_vectors(CutLinearArgs A, double* cell_vec, int64_t total) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int nC = A.n * A.C;
+    const int64_t e = t / nC;
+    const int r = int(t - e * nC);
+    const int a = r / A.C, c = r - a * A.C;
+    double s = 0.;
+#pragma unroll 4
+    for (int64_t q = A.offsets[e]; q < A.offsets[e + 1]; ++q) {
+        double con = 0.;
+        if (A.value) con += A.shape[q * A.n + a] * A.value[A.value_per_point ? q * A.C + c : c];
+        if (A.gradient) {
+            const double* g = A.grad + (q * A.n + a) * A.D;
+            const double* G = A.gradient + (A.gradient_per_point ? (q * A.C + c) * A.D : int64_t(c) * A.D);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if (d < A.D) con += g[d] * G[d];
+        }
+        s += A.weights[q] * con;
+    }
+    cell_vec[t] = s;
+}
+
+__global__ void __launch_bounds__(256) k_cut_linear_rows(CutLinearArgs A, const double* cell_vec) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= A.row_nodes * A.C) return;
     const int64_t node = t / A.C;
     const int c = int(t - node * A.C);
+    const int64_t kb = A.inc_ptr[node], ke = A.inc_ptr[node + 1];
     double acc = 0.;
-    for (int64_t k = A.inc_ptr[node]; k < A.inc_ptr[node + 1]; ++k) {
-        const int32_t packed = *reinterpret_cast<const int32_t*>(A.records + k * A.rec_bytes);
-        const int64_t e = packed >> 4;
-        const int a = packed & 15;
-        double s = 0.;
-        for (int64_t q = A.offsets[e]; q < A.offsets[e + 1]; ++q) {
-            double v = 0.;
-            if (A.value) v += A.shape[q * A.n + a] * A.value[A.value_per_point ? q * A.C + c : c];
-            if (A.gradient) {
-                const double* g = A.grad + (q * A.n + a) * A.D;
-                const double* G = A.gradient + (A.gradient_per_point ? (q * A.C + c) * A.D : int64_t(c) * A.D);
-                for (int d = 0; d < A.D; ++d) v += g[d] * G[d];
-            }
-            s += A.weights[q] * v;
-        }
-        acc += s;
+    for (int64_t k = kb; k < ke; k += kCutBatch) {
+        int32_t pk[kCutBatch];
+        double v[kCutBatch];
+#pragma unroll
+        for (int j = 0; j < kCutBatch; ++j)
+            pk[j] = k + j < ke ? *reinterpret_cast<const int32_t*>(A.records + (k + j) * A.rec_bytes) : 0;
+#pragma unroll
+        for (int j = 0; j < kCutBatch; ++j)
+            v[j] = k + j < ke ? cell_vec[(int64_t(pk[j] >> 4) * A.n + (pk[j] & 15)) * A.C + c] : 0.;
+#pragma unroll
+        for (int j = 0; j < kCutBatch; ++j)
+            if (k + j < ke) acc += v[j];
     }
     A.out[t] = acc;
 }
@@ -382,17 +410,17 @@ int skfa_assemble_cut_bilinear(const skfa_cross_plan* P, const int64_t* offsets,
         const int64_t total = P->E * int64_t(P->n_row) * P->n_col;
         const int64_t blocks = (P->E + 3) / 4;   // a wave per cell
         SKFA_REQUIRE(blocks <= int64_t(INT32_MAX), "too many cut cells");
-        if (P->cell_matrices.n < size_t(total)) {
-            const int rc = P->cell_matrices.alloc(size_t(total));
+        if (P->cell_buffer.n < size_t(total)) {
+            const int rc = P->cell_buffer.alloc(size_t(total));
             if (rc != kOk) return rc;
         }
         const bool same = row_shape == col_shape && row_grad == col_grad && P->n_row == P->n_col && row_dim == col_dim;
         if (same)
-            k_cut_cell_matrices<true, 8><<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(A, P->cell_matrices.p, P->E);
+            k_cut_cell_matrices<true, 8><<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(A, P->cell_buffer.p, P->E);
         else
-            k_cut_cell_matrices<false, 8><<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(A, P->cell_matrices.p, P->E);
+            k_cut_cell_matrices<false, 8><<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(A, P->cell_buffer.p, P->E);
         SKFA_LAUNCH_CHECK();
-        k_cut_rows_cells<<<dim3(unsigned(waves)), dim3(64), size_t(rows * row_bytes), s>>>(A, P->cell_matrices.p);
+        k_cut_rows_cells<<<dim3(unsigned(waves)), dim3(64), size_t(rows * row_bytes), s>>>(A, P->cell_buffer.p);
         SKFA_LAUNCH_CHECK();
         return kOk;
     }
@@ -430,9 +458,21 @@ int skfa_assemble_cut_linear(const skfa_cross_plan* P, const int64_t* offsets, c
     A.gradient = gradient;
     A.gradient_per_point = gradient_per_point;
     A.out = out;
+    hipStream_t s = static_cast<hipStream_t>(stream);
     const int64_t blocks = (total + 255) / 256;
     SKFA_REQUIRE(blocks <= int64_t(INT32_MAX), "too many rows");
-    k_cut_linear<<<dim3(unsigned(blocks)), dim3(256), 0, static_cast<hipStream_t>(stream)>>>(A);
+    const int64_t cells = P->E * int64_t(P->n_row) * P->C_row;
+    const int64_t cblocks = (cells + 255) / 256;
+    SKFA_REQUIRE(cblocks <= int64_t(INT32_MAX), "too many cut cells");
+    if (cells > 0) {
+        if (P->cell_buffer.n < size_t(cells)) {
+            const int rc = P->cell_buffer.alloc(size_t(cells));
+            if (rc != kOk) return rc;
+        }
+        k_cut_cell_vectors<<<dim3(unsigned(cblocks)), dim3(256), 0, s>>>(A, P->cell_buffer.p, cells);
+        SKFA_LAUNCH_CHECK();
+    }
+    k_cut_linear_rows<<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(A, P->cell_buffer.p);
     SKFA_LAUNCH_CHECK();
     return kOk;
 }

@@ -119,10 +119,11 @@ struct skfa_cross_plan {
     // then the in-row node offsets of the element's n_col column nodes
     skfa::DevBuf<uint8_t> records;
     int32_t rec_bytes = 0, off_bytes = 1;
-    // cut-cell forms: per-cell node matrices (E, n_row, n_col) FP64, allocated
-    // by the first cut assembly on the plan and reused (calls on one plan
-    // share it, so they must not overlap)
-    mutable skfa::DevBuf<double> cell_matrices;
+    // cut-cell forms: per-cell node matrices (E, n_row, n_col) or node
+    // vectors (E, n_row, C_row) FP64, allocated by the first cut assembly on
+    // the plan and grown as needed (calls on one plan share it, so they must
+    // not overlap)
+    mutable skfa::DevBuf<double> cell_buffer;
 
     int64_t nrows() const { return row_nodes * C_row; }
     int64_t ncols() const { return col_nodes * C_col; }
