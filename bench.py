@@ -49,7 +49,9 @@ FLOOR_BYTES_PER_ELEMENT_C2 = BYTES_PER_ELEMENT_C2 - 4. * 10 * 10
 TRAFFIC_FILES = {"th": "th_divergence_block_64_hbm_traffic.json",
                  "c1": "c1_p1_laplace_32_hbm_traffic.json",
                  "c3": "c3_vector_p2_elasticity_48_hbm_traffic.json",
-                 "c4": "c4_n1_curlcurl_96_hbm_traffic.json"}
+                 "c4": "c4_n1_curlcurl_96_hbm_traffic.json",
+                 "stokes": "stokes_taylor_hood_48_hbm_traffic.json",
+                 "cut": "cut_band_p2_64_hbm_traffic.json"}
 
 
 def log(msg):
