@@ -1920,8 +1920,33 @@ __device__ inline void load_coef(const H1Args& A, int64_t idx, Coef& k) {
     k.g = A.cg ? A.cg[idx] : 0.;
     k.s = A.cs ? A.cs[idx] : 0.;
     k.lam = A.cl ? A.cl[idx] : 0.;
+    if (A.cA == nullptr) {
 #pragma unroll
-    for (int t = 0; t < 9; ++t) k.A[t] = A.cA ? A.cA[idx * 9 + t] : 0.;
+        for (int t = 0; t < 9; ++t) k.A[t] = 0.;
+        return;
+    }
+    // 72-byte rows: 16-byte aligned at even idx (uniform over a wave for
+    // per-point rows of an even Q), so four 16-byte loads and one 8-byte load
+    const double* src = A.cA + idx * 9;
+    if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+        const double2* v = reinterpret_cast<const double2*>(src);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const double2 x = v[t];
+            k.A[2 * t] = x.x;
+            k.A[2 * t + 1] = x.y;
+        }
+        k.A[8] = src[8];
+    } else {
+        k.A[0] = src[0];
+        const double2* v = reinterpret_cast<const double2*>(src + 1);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const double2 x = v[t];
+            k.A[2 * t + 1] = x.x;
+            k.A[2 * t + 2] = x.y;
+        }
+    }
 }
 
 template <int C>
