@@ -152,3 +152,66 @@ def random_case(quadratic, components, seed, n=3, dim=3, empty_fraction=0.4, max
     gradients = rng.standard_normal((P, nodes, dim))
     weights = rng.uniform(0.05, 1., P)
     return dofs, offsets, shape, gradients, weights
+
+
+def _cell_sums(per_point, offsets, cells):
+    """Sum per-point arrays over each cell's point range, in point order per
+    cell (cells without points give zeros)."""
+    offsets = np.asarray(offsets, dtype=np.int64)
+    active = np.diff(offsets) > 0
+    out = np.zeros((cells,) + per_point.shape[1:])
+    if active.any():
+        out[active] = np.add.reduceat(per_point, offsets[:-1][active], axis=0)
+    return out, active
+
+
+def cut_bilinear_vectorized(dofs, offsets, shape, gradients, weights, value=None, gradient=None):
+    """cut_bilinear with numpy array operations, for meshes where the loops
+    are too slow: the per-point entries w_q (v_q phi_a phi_b + g_q grad phi_a
+    . grad phi_b) are summed per cell, then each cell's node matrix is added
+    at its CSR positions (component-diagonal pairs).  Summation order differs
+    from cut_bilinear by rounding only (pinned against it on small cases)."""
+    dofs = np.asarray(dofs, dtype=np.int64)
+    cells, nodes, C = dofs.shape
+    indptr, indices = cut_pattern(dofs, dofs, offsets)
+    values = np.zeros(len(indices))
+    P = len(weights)
+    e = np.zeros((P, nodes, nodes))
+    if value is not None:
+        e += np.asarray(value)[:, None, None] * shape[:, :, None] * shape[:, None, :]
+    if gradient is not None:
+        e += np.asarray(gradient)[:, None, None] * np.einsum("pad,pbd->pab", gradients, gradients)
+    e *= np.asarray(weights)[:, None, None]
+    M, active = _cell_sums(e, offsets, cells)
+    del e
+    ncols = int(dofs.max()) + 1 if dofs.size else 1
+    rows_of = np.repeat(np.arange(len(indptr) - 1, dtype=np.int64), np.diff(indptr))
+    keys = rows_of * ncols + indices
+    for k in range(C):
+        d = dofs[active, :, k]
+        key = d[:, :, None] * ncols + d[:, None, :]
+        np.add.at(values, np.searchsorted(keys, key.ravel()), M[active].ravel())
+    return indptr, indices, values
+
+
+def cut_linear_vectorized(dofs, offsets, shape, gradients, weights, value=None, gradient=None):
+    """cut_linear with numpy array operations (per-point contributions summed
+    per cell, then added at the cells' DOFs); pinned against cut_linear on
+    small cases."""
+    dofs = np.asarray(dofs, dtype=np.int64)
+    cells, nodes, C = dofs.shape
+    ndofs = int(dofs.max()) + 1 if dofs.size else 0
+    P = len(weights)
+    s = np.zeros((P, nodes, C))
+    if value is not None:
+        f = np.broadcast_to(np.asarray(value), (P, C)) if np.ndim(value) == 1 else np.asarray(value)
+        s += shape[:, :, None] * f[:, None, :]
+    if gradient is not None:
+        G = np.broadcast_to(np.asarray(gradient), (P,) + np.shape(gradient)) if np.ndim(gradient) == 2 \
+            else np.asarray(gradient)
+        s += np.einsum("pad,pkd->pak", gradients, G)
+    s *= np.asarray(weights)[:, None, None]
+    b, _ = _cell_sums(s, offsets, cells)
+    out = np.zeros(ndofs)
+    np.add.at(out, dofs.ravel(), b.ravel())
+    return out

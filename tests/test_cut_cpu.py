@@ -169,3 +169,24 @@ def test_functional_validation_on_host():
     assert integrate_functional(np.zeros((0, 4)), np.zeros((0, 4))) == 0.
     with pytest.raises(sk.UnsupportedNativeForm, match="one Basis"):
         sk.asm(sk.Functional(lambda w: 1.))
+
+
+@pytest.mark.parametrize("quadratic,components,dim", [(False, 1, 3), (True, 1, 3), (True, 3, 3), (False, 1, 2)])
+def test_cut_vectorized_oracle_matches_loops(quadratic, components, dim):
+    """The array forms used for full-size GPU checks give the loop forms'
+    results (cells without points, 1..12 points per cell)."""
+    dofs, off, sh, gr, w = cut.random_case(quadratic, components, seed=70 + components, dim=dim, max_points=12)
+    rng = np.random.default_rng(7)
+    P = len(w)
+    v, g = rng.uniform(0.5, 2., P), rng.uniform(0.5, 2., P)
+    for value, gradient in ((v, None), (None, g), (v, g)):
+        ip, ix, ref = cut.cut_bilinear(dofs, off, sh, gr, w, value, gradient)
+        ip2, ix2, mine = cut.cut_bilinear_vectorized(dofs, off, sh, gr, w, value, gradient)
+        np.testing.assert_array_equal(ip2, ip)
+        np.testing.assert_array_equal(ix2, ix)
+        _close(mine, ref)
+    cases = [(rng.standard_normal(components), None), (None, rng.standard_normal((P, components, dim))),
+             (rng.standard_normal((P, components)), rng.standard_normal((components, dim)))]
+    for value, gradient in cases:
+        _close(cut.cut_linear_vectorized(dofs, off, sh, gr, w, value, gradient),
+               cut.cut_linear(dofs, off, sh, gr, w, value, gradient))

@@ -276,3 +276,47 @@ def test_functional_on_volume_facet_and_cut_bases():
     assert abs(sk.asm(unit, fb) - 6.) <= 1e-12
     flux = sk.Functional(lambda w: w.x[0] * w.n[0] + w.x[1] * w.n[1] + w.x[2] * w.n[2])
     assert abs(sk.asm(flux, fb) - 3.) <= 1e-12
+
+
+def test_cut_band_full_size_matches_oracle():
+    """The benchmark's cut band at full size: the cells of the jittered 64^3
+    P2 cube whose centroid lies within one cell width of the sphere
+    |x - (0.5, 0.5, 0.5)| = 0.3 (55,571 cells), 12 random points each.  The
+    square value + gradient form (pattern bit-exact) and the load vector
+    against the oracle's array forms."""
+    import skfemntv as sk
+    from skfemntv.basis import tabulate
+    say("cut band 64^3 P2, 55,571 cells x 12 points, bilinear + linear")
+    n = 64
+    ax = np.linspace(0., 1., n + 1)
+    base = sk.MeshTet.init_tensor(ax, ax, ax)
+    p = base.p.copy()
+    interior = np.all((p > 1e-12) & (p < 1 - 1e-12), axis=0)
+    p[:, interior] += np.random.default_rng(0).uniform(-0.2 / n, 0.2 / n, size=(3, int(interior.sum())))
+    mesh = sk.MeshTet2.from_mesh(sk.MeshTet(p, base.t))
+    c = mesh.p[:, mesh.t[:4]].mean(axis=1)
+    band = np.abs(np.linalg.norm(c - 0.5, axis=0) - 0.3) < 1. / n
+    E = int(band.sum())
+    assert E == 55571
+    rng = np.random.default_rng(3)
+    Pc = 12
+    off = np.arange(E + 1, dtype=np.int64) * Pc
+    P = E * Pc
+    X = rng.dirichlet(np.ones(4), P)[:, 1:]
+    shape, ref_grad = tabulate(X.T, True)
+    corners = mesh.p[:, mesh.t[:4][:, band]]
+    Jinv = np.linalg.inv(np.moveaxis(corners[:, 1:4] - corners[:, [0]], -1, 0))
+    grads = np.einsum("pnk,pki->pni", ref_grad, np.repeat(Jinv, Pc, axis=0))
+    w = rng.uniform(0.1, 1., P) / (n ** 3 * 6 * Pc)
+    dofs = mesh.t[:10, band].T[:, :, None].astype(np.int64)
+    v, g = rng.uniform(0.5, 2., P), rng.uniform(0.5, 2., P)
+    A = sk.CutBilinearFormAssembler(dofs, off, shape, grads, w)
+    A.assemble(v, g)
+    ip, ix, ref = cut.cut_bilinear_vectorized(dofs, off, shape, grads, w, v, g)
+    np.testing.assert_array_equal(A.indptr, ip)
+    np.testing.assert_array_equal(A.indices, ix)
+    assert rel(A.values, ref) <= FP64_TOL
+    f, G = rng.standard_normal((P, 1)), rng.standard_normal((P, 1, 3))
+    L = sk.CutLinearFormAssembler(dofs, off, shape, grads, w)
+    out, _ = L.assemble(f, G)
+    assert rel(out, cut.cut_linear_vectorized(dofs, off, shape, grads, w, f, G)) <= FP64_TOL
